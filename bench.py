@@ -1,0 +1,195 @@
+"""Benchmark of the Wigner split-operator hot path on one MI355X.
+
+Workload (default): the damped Wigner2D propagator at 1024 x 1024,
+x,p in [-12,12), Caldeira-Leggett friction gamma = 0.1 with D_theta = 2 m gamma kT
+= 0.1 (kT = 0.5) and sigma(p) = p, double well V = 0.1 x^4 - x^2, dt = 0.005,
+cat-state initial W, complex128, Lie step of GPU_Wigner2D_FFT (absorbing mask on,
+renormalisation and the Ehrenfest observables every step).  A step is one full
+dt of the whole grid; value = cell-steps/s (cells x steps / wall time).
+
+Also reported (extra keys): the same workload in the FP32-bulk form and its
+relative L2 error against the FP64 run, and the undamped quartic-oscillator
+step at 1024^2 (FP64) -- the plain step without the friction transforms.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def cfg2(precision="fp64", X=1024):
+    """1024^2 CL-damped double well (the Lindblad/Caldeira-Leggett damped workload)."""
+    return dict(X=X, P=X, X_amp=12.0, P_amp=12.0, hbar=1.0, mass=1.0, dt=0.005,
+                potential="0.1*pow(x,4) - pow(x,2)", dpotential="0.4*pow(x,3) - 2.0*x",
+                scheme="lie", D_theta=2.0 * 1.0 * 0.1 * 0.5, gamma=0.1, damping="CaldeiraLeggett",
+                fp_damping="fabs(p)", mask=True, precision=precision)
+
+
+def plain(precision="fp64", X=1024):
+    """Undamped quartic oscillator at 1024^2, x,p in [-10,10), no mask."""
+    return dict(X=X, P=X, X_amp=10.0, P_amp=10.0, hbar=1.0, mass=1.0, dt=0.01,
+                potential="0.5*pow(x,2) + 0.1*pow(x,4)", dpotential="x + 0.4*pow(x,3)",
+                scheme="lie", mask=False, precision=precision)
+
+
+def make(cfg, device):
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    c = dict(cfg)
+    args = [c.pop(k) for k in ("X", "P", "X_amp", "P_amp", "hbar", "mass", "dt")]
+    return Wigner2D(*args, device=device, **c)
+
+
+def initial_state(prop, kind):
+    import numpy as np
+
+    x = prop.x_host[None, :]
+    p = prop.p_host[:, None]
+    if kind == "cat":
+        g = lambda xx: np.exp(-(xx ** 2) - p ** 2)  # noqa: E731
+        W = g(x - 2.0) + g(x + 2.0) + 2.0 * g(x) * np.cos(4.0 * p)
+    else:
+        W = np.exp(-((x - 2.0) ** 2) - p ** 2) / np.pi
+    return W / (W.sum() * prop.dX * prop.dP)
+
+
+def timed_run(prop, W, steps, warmup):
+    import torch
+
+    prop.run(W, warmup, use_graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    obs = prop.run(W, steps, first_step=warmup, normalize_initial=False, use_graph=True)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, obs
+
+
+def kernel_time(fn, reps):
+    """Average duration of fn()'s launches on torch's current stream (HIP events)."""
+    import torch
+
+    fn()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    e.synchronize()
+    return s.elapsed_time(e) / reps * 1e-3
+
+
+def cpu_baseline(target_seconds=15.0):
+    """The oracle (NumPy complex128, single thread) on the same workload, bounded sample."""
+    import numpy as np
+
+    from oracle import wigner2d as ow
+
+    c = cfg2()
+    orc = ow.Wigner2DOracle(c["X"], c["P"], c["X_amp"], c["P_amp"], c["hbar"], c["mass"], c["dt"],
+                            D_theta=c["D_theta"], gamma=c["gamma"],
+                            potential=lambda t, x: 0.1 * x ** 4 - x ** 2,
+                            dpotential=lambda t, x: 0.4 * x ** 3 - 2.0 * x,
+                            fp_damping=lambda p: np.abs(p), damping="CaldeiraLeggett", scheme="lie",
+                            mask=True)
+    W = ow.cat_state_wigner(orc.g, a=2.0)
+    orc.run(W, 0)  # multipliers cached, one iteration
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        orc.run(W, 0)  # one Lie iteration (observables + K + V + CL + renorm)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > target_seconds:
+            break
+    return {"value": c["X"] * c["P"] * n / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} Lie+CL iterations of the 1024^2 damped workload with the NumPy "
+                      f"complex128 oracle (numpy.fft, 1 thread) in {el:.1f} s on "
+                      f"{platform.processor() or platform.machine()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=200)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the hot path would shard by ensemble member, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    dev = torch.device("cuda:0")
+    prop = make(cfg2("fp64"), dev)
+    W = prop.to_device(initial_state(prop, "cat"))
+    wall, obs = timed_run(prop, W, a.steps, a.warmup)
+    cells = prop.X * prop.P
+    value = cells * a.steps / wall
+    # dominant kernel: the fused p sweep (V step + CL friction + column sums)
+    Wk = W.clone()
+    t_p = kernel_time(lambda: prop.potential(Wk, 0, fused=True), a.kernel_reps)
+    t_x = kernel_time(lambda: prop.kinetic(Wk), a.kernel_reps)
+    bytes_per_launch = cells * 16 * 2  # read + write the complex128 state once
+    achieved = bytes_per_launch / t_p / 1e9
+    out = {
+        "metric": "cell-steps/s at 1024^2: plain FP64 vs batched FP32 @ stated error; HBM GB/s",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {"workload": "Wigner2D Lie step + Caldeira-Leggett order-3 friction, 1024x1024, "
+                               "x,p in [-12,12), gamma=0.1, kT=0.5 (D_theta=0.1), V=0.1x^4-x^2, "
+                               "dt=0.005, cat state, observables + renormalisation every step",
+                   "grid": [prop.P, prop.X], "precision": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "psweep_kernel<double,1024,16,4,CL>", "kernel_us": t_p * 1e6,
+                     "xsweep_us": t_x * 1e6,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+    if not a.no_extra:
+        import numpy as np
+
+        p32 = make(cfg2("fp32"), dev)
+        W32 = p32.to_device(initial_state(p32, "cat"))
+        wall32, _ = timed_run(p32, W32, a.steps, a.warmup)
+        ref = W.cpu().numpy()
+        err32 = float(np.linalg.norm(W32.cpu().numpy() - ref) / np.linalg.norm(ref))
+        pp = make(plain("fp64"), dev)
+        Wp = pp.to_device(initial_state(pp, "gauss"))
+        wallp, _ = timed_run(pp, Wp, a.steps, a.warmup)
+        out["extra"] = {
+            "fp32_bulk_cell_steps_per_s": cells * a.steps / wall32,
+            "fp32_rel_l2_vs_fp64": err32,
+            "plain_fp64_cell_steps_per_s": cells * a.steps / wallp,
+            "plain_fp64_hbm_frac": cells * 64 * a.steps / wallp / 1e9 / HBM_PEAK_GBS,
+        }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,185 @@
+/*
+ * wignerhip.h -- C ABI of libwignerhip.so, the MI355X (gfx950) Wigner
+ * split-operator library.
+ *
+ * Conventions (all entry points):
+ *   - return an int status, 0 = success; codes 1-9 mirror the cuFFT status set
+ *     that cufft_wrapper.py:15-73 maps to Python exceptions, 10+ are new;
+ *     wh_last_error() gives a message for the calling thread's last failure;
+ *   - device memory is borrowed (plain device pointers, e.g. torch
+ *     tensor.data_ptr()); the library never frees caller memory;
+ *   - every launching call takes an explicit stream (hipStream_t as void*,
+ *     NULL = the null stream) and is asynchronous with respect to the host;
+ *   - layouts: complex (re,im interleaved) FP64 (WH_C128) or FP32 (WH_C64),
+ *     W[p][x] row-major, both axes FFT ordered (index 0 = origin), exactly the
+ *     layout of the reference's W_GPU (GPU_Wigner2D_FFT.py:110-114, 571-581).
+ */
+#ifndef WIGNERHIP_H
+#define WIGNERHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+/* 1-9: cufftExceptions of cufft_wrapper.py:55-65 */
+#define WH_SUCCESS 0
+#define WH_INVALID_PLAN 1
+#define WH_ALLOC_FAILED 2
+#define WH_INVALID_TYPE 3
+#define WH_INVALID_VALUE 4
+#define WH_INTERNAL_ERROR 5
+#define WH_EXEC_FAILED 6
+#define WH_SETUP_FAILED 7
+#define WH_INVALID_SIZE 8
+#define WH_UNALIGNED_DATA 9
+/* new */
+#define WH_INVALID_DESCRIPTOR 10
+#define WH_LAUNCH_FAILED 11
+#define WH_NO_DEVICE 12
+
+/* precisions: CUFFT_Z2Z / CUFFT_C2C of cufft_wrapper.py:8-9 */
+#define WH_C128 0
+#define WH_C64 1
+/* directions: CUFFT_FORWARD / CUFFT_INVERSE of cufft_wrapper.py:11-12 */
+#define WH_FORWARD (-1)
+#define WH_INVERSE 1
+
+/* split schemes */
+#define WH_SCHEME_LIE 0    /* K(dt); V(dt)        GPU_Wigner2D_FFT.Run, GPU_Wigner2D_FFT.py:1884-1899 */
+#define WH_SCHEME_STRANG 1 /* V(dt/2); K; V(dt/2) GPU_Wigner2D_GPitaevskii.Run, GP.py:1911-1956 */
+
+#define WH_MAX_DEG 8 /* degree bound of the polynomial V(t,x) and K(p) */
+#define WH_NOBS 16   /* doubles per observable record */
+
+/* observable record columns (one row per recorded step) */
+#define WH_OBS_X 0
+#define WH_OBS_X2 1
+#define WH_OBS_P 2
+#define WH_OBS_P2 3
+#define WH_OBS_XP 4
+#define WH_OBS_DV 5
+#define WH_OBS_PDV 6
+#define WH_OBS_XDV 7
+#define WH_OBS_H 8
+#define WH_OBS_NEG 9
+#define WH_OBS_NORM 10
+#define WH_OBS_RAWNORM_RE 11
+#define WH_OBS_RAWNORM_IM 12
+#define WH_OBS_PROBSQ 13
+
+/* ---- library ---------------------------------------------------------- */
+/* Message of the calling thread's last failed call ("" if none). */
+const char* wh_last_error(void);
+/* Static text for a status code. */
+const char* wh_status_string(int status);
+/* Library version, (major << 16) | minor. */
+int wh_version(void);
+
+/* ---- plain axis transforms: replaces cufft_wrapper.py ----------------- */
+typedef struct wh_fft_plan wh_fft_plan;
+
+/*
+ * Replaces cufftPlanMany (cufft_wrapper.py:118-145) for the plans the Wigner
+ * path builds: rank 1, length n (a power of two, 64..2048), either
+ *   istride == 1, idist == n      batch contiguous rows  (Plan_Z2Z_2D_Axis1, :218-250;
+ *                                                         Plan_Z2Z 1-D, :147-180)
+ *   istride == batch, idist == 1  batch interleaved columns of an [n][batch] array
+ *                                                         (Plan_Z2Z_2D_Axis0, :183-215)
+ * type: WH_C128 (Z2Z) or WH_C64 (C2C).  Allocates the twiddle table on the
+ * current device.
+ */
+int wh_fft_plan_many(wh_fft_plan** plan, int n, int istride, int idist, int type, int batch);
+/*
+ * Replaces cufftExecZ2Z / cufftExecC2C (cufft_wrapper.py:81-103): in place
+ * (idata == odata), direction WH_FORWARD or WH_INVERSE, inverse unnormalised.
+ */
+int wh_fft_exec(wh_fft_plan* plan, void* idata, void* odata, int direction, void* stream);
+/* As wh_fft_exec, with the 1/n of the callers' "*= 1./n" (GPU_Wigner2D_FFT.py:790-801) folded in. */
+int wh_fft_exec_normalized(wh_fft_plan* plan, void* idata, void* odata, int direction, void* stream);
+/* Replaces cufftDestroy (cufft_wrapper.py:109-115). */
+int wh_fft_destroy(wh_fft_plan* plan);
+
+/* ---- fused Wigner2D propagator ---------------------------------------- */
+/*
+ * Replaces, for one P x X Wigner function, the per-instance set-up of the
+ * reference: SetCUDA_Constants + SetFFT_Plans + SetCUDA_Functions and the
+ * device grids of SetPhaseSpaceGrid2D (GPU_Wigner2D_FFT.py:569-596, 833-906;
+ * GPU_Wigner2D_GPitaevskii.py:854-937).  All pointers in the descriptor are
+ * HOST arrays, copied at creation.  Constants are the kernel-side values (the
+ * caller applies any '%f' truncation the reference performs).
+ */
+typedef struct {
+  int struct_size; /* sizeof(wh_wigner2d_desc) */
+  int precision;   /* WH_C128 or WH_C64 */
+  int x_dim, p_dim; /* powers of two, 64..2048 */
+  int scheme;      /* WH_SCHEME_LIE or WH_SCHEME_STRANG */
+  /* kernel constants (expPotential / expPLambda sources, GPU_Wigner2D_FFT.py:80-200) */
+  double dt, dx, dp, dtheta, dlambda, hbar, mass;
+  /* kinetic phase: 0 = dt*lambda*p/mass (expPLambda, FFT.py:153);
+     1 = dt*(K(p+hbar*lambda/2)-K(p-hbar*lambda/2))/hbar, K = sum kin_coef[n] p^n */
+  int kinetic_poly;
+  int kinetic_deg;
+  double kin_coef[WH_MAX_DEG + 1];
+  /* potential V(t,x) = sum_n pot_coef[row*(WH_MAX_DEG+1)+n] x^n, row = time index
+     (clamped to pot_rows-1); pot_rows = 1 for a time-independent V */
+  int pot_deg;
+  int pot_rows;
+  const double* pot_coef;
+  /* 1-D tables (NULL = all ones) */
+  const double* theta_decay;  /* [P] exp(-frac*D_theta*theta^2*dt) (FFT.py:117; GP.py:80) */
+  const double* lambda_decay; /* [X] exp(-D_lambda*lambda^2) (FFT.py:154) */
+  const double* x_mask;       /* [X] absorbing mask (FFT.py:121-123, times :514-516 with GP) */
+  /* Caldeira-Leggett order-3 dissipator (FFT.py:1145-1157), applied after the V step */
+  int damping;                /* 0 none, 1 Caldeira-Leggett order 3 */
+  double gamma_dt;            /* dt*gammaDamping */
+  const double* sigma_p;      /* [P] sign(p) f(p) at kernel p (theta_fp, FFT.py:399-431) */
+  const double* theta_host;   /* [P] host Theta grid (FFT.py:578) */
+  int renormalize;            /* divide by dXdP*sum(W) after each step (FFT.py:1912-1913) */
+  /* Gross-Pitaevskii (FFT.py:1687-1705, 479-519; GP.py:466-496, 1743-1748) */
+  double gp_coeff;            /* a_GP (kernel constant); 0 = off */
+  double gp_rho_factor;       /* 1 (GPU_Wigner2D_FFT) or dP (GPU_Wigner2D_GPitaevskii) */
+  const double* lambda_host;  /* [X] host Lambda grid, for exp(i lambda theta/2) */
+  /* observables (FFT.py:1857-1878): host grids and dV/dx, V at t=0 */
+  const double* x_host;       /* [X] */
+  const double* p_host;       /* [P] */
+  const double* dv_host;      /* [X] dPotential(0,x) */
+  const double* v0_host;      /* [X] Potential(0,x) */
+  double dxdp;                /* dX*dP */
+  double inv_2m;              /* 1/(2 mass) (host mass) */
+  double dx_host;             /* dX (ProbabilitySquare record) */
+} wh_wigner2d_desc;
+
+typedef struct wh_wigner2d wh_wigner2d;
+
+int wh_wigner2d_create(const wh_wigner2d_desc* desc, wh_wigner2d** out);
+int wh_wigner2d_destroy(wh_wigner2d* plan);
+
+/*
+ * Replaces the time loop of GPU_Wigner2D_FFT.Run (GPU_Wigner2D_FFT.py:1843-1913)
+ * and GPU_Wigner2D_GPitaevskii.Run (GP.py:1898-1999): advances W (device,
+ * in place) by n_steps iterations with time indices first_step .. first_step+n_steps-1
+ * and writes (n_steps+1) observable rows of WH_NOBS doubles to obs (device):
+ *   Lie:    row k = state before iteration first_step+k (row n_steps = final state);
+ *   Strang: row k = state after iteration first_step+k   (row n_steps unused).
+ * normalize_initial: divide the incoming W by dXdP*sum(W) first (Run's set-up,
+ * FFT.py:1778-1780).  On return W holds the (renormalised) final state.
+ * use_graph: replay the per-step launch sequence from a captured hipGraph.
+ */
+int wh_wigner2d_run(wh_wigner2d* plan, void* W, int first_step, int n_steps, double* obs,
+                    int normalize_initial, int use_graph, void* stream);
+
+/* Single sweeps, for callers composing their own loop (no renormalisation). */
+/* K step: FFT_x, kinetic multiplier, inverse (Fourier_X_To_Lambda_GPU; expPLambda; Fourier_Lambda_To_X_GPU, FFT.py:1885-1888) */
+int wh_wigner2d_kinetic(wh_wigner2d* plan, void* W, void* stream);
+/* V step at time index `step` (Fourier_P_To_Theta_GPU; expPotential; Fourier_Theta_To_P_GPU, FFT.py:1891-1899).
+   fused = 1: the p sweep exactly as configured inside wh_wigner2d_run (damping and the
+   observable column sums fused in; Lie scheme); fused = 0: the plain V step. */
+int wh_wigner2d_potential(wh_wigner2d* plan, void* W, int step, int fused, void* stream);
+/* Observables of W (FFT.py:1857-1878) into one row of WH_NOBS doubles (device). */
+int wh_wigner2d_observe(wh_wigner2d* plan, const void* W, double* obs_row, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WIGNERHIP_H */

@@ -1,0 +1,240 @@
+// Device-side FFT building blocks for gfx950 (CDNA4), wave64.
+//
+// A "team" of TT = N/R lanes computes one length-N complex transform.  Lane t
+// holds R elements in registers in the strided layout  v[j] = a[t + j*TT].
+// The transform is a Stockham autosort sequence of radix passes [R, R, ..., r]
+// (r = N / R^k).  The first pass reads the registers directly, every later pass
+// exchanges through LDS, and the last pass writes back into the same strided
+// register layout, so a forward transform, a pointwise multiply and an inverse
+// transform chain in registers without extra LDS traffic, and global loads and
+// stores of the strided layout are coalesced for both row and column teams.
+//
+// Pass with radix r after the spans Ns = prod(previous radices):
+//   butterfly b in [0, N/r) reads a[b + j*N/r], twiddles by w^(j*(b mod Ns)),
+//   w = exp(dir*2*pi*i/(Ns*r)), applies an r-point DFT and writes
+//   a'[(b - b mod Ns)*r + (b mod Ns) + q*Ns].
+// Twiddles come from a table tw[n] = exp(-2*pi*i*n/N) computed in FP64 on the
+// host.  Forward = exponent -i (dir = -1), inverse = +i and unnormalised, as
+// cufftExecZ2Z (cufft_wrapper.py:11-12, 81-103).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace wh {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cplx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {
+  return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) { return {a.x * s, a.y * s}; }
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+// number of Stockham passes for N with maximal radix R
+constexpr int num_passes(int N, int R) { return N <= R ? 1 : 1 + num_passes(N / R, R); }
+// radix of pass p (all R except possibly the last)
+constexpr int pass_radix(int N, int R, int p) {
+  return (p == num_passes(N, R) - 1) ? N / ipow(R, p) : R;
+}
+constexpr int bitrev(int v, int bits) {
+  return bits == 0 ? 0 : ((v & 1) << (bits - 1)) | bitrev(v >> 1, bits - 1);
+}
+
+// cos(2*pi*k/16), k = 0..15
+constexpr double cos16(int k) {
+  constexpr double c1 = 0.92387953251128675613;  // cos(pi/8)
+  constexpr double c2 = 0.70710678118654752440;  // cos(pi/4)
+  constexpr double c3 = 0.38268343236508977173;  // cos(3pi/8)
+  const double t[16] = {1.0, c1, c2, c3, 0.0, -c3, -c2, -c1, -1.0, -c1, -c2, -c3, 0.0, c3, c2, c1};
+  return t[k & 15];
+}
+
+// a * exp(DIR * 2*pi*i * K / M), M | 16; all special angles are exact.
+template <int DIR, int K, int M, typename T>
+__device__ __forceinline__ cplx<T> rot(cplx<T> a) {
+  constexpr int k = ((K % M) * (16 / M)) & 15;
+  if constexpr (k == 0) {
+    return a;
+  } else if constexpr (k == 8) {
+    return {-a.x, -a.y};
+  } else if constexpr ((k == 4 && DIR < 0) || (k == 12 && DIR > 0)) {
+    return {a.y, -a.x};  // * (-i)
+  } else if constexpr (k == 4 || k == 12) {
+    return {-a.y, a.x};  // * (+i)
+  } else {
+    constexpr double c = cos16(k);
+    constexpr double s0 = cos16(k + 12);  // sin(2 pi k/16) = cos(2 pi (k-4)/16)
+    constexpr double s = DIR < 0 ? -s0 : s0;
+    const T cc = T(c), ss = T(s);
+    return {a.x * cc - a.y * ss, a.x * ss + a.y * cc};
+  }
+}
+
+// Radix-2 decimation-in-frequency butterflies of one stage of length L at offset OFF.
+template <int DIR, int L, int OFF, int I, typename T>
+struct DifStage {
+  __device__ __forceinline__ static void run(cplx<T>* v) {
+    if constexpr (I < L / 2) {
+      cplx<T> a = v[OFF + I], b = v[OFF + I + L / 2];
+      v[OFF + I] = cadd(a, b);
+      v[OFF + I + L / 2] = rot<DIR, I, L>(csub(a, b));
+      DifStage<DIR, L, OFF, I + 1, T>::run(v);
+    }
+  }
+};
+template <int DIR, int L, int OFF, typename T>
+struct Dif {
+  __device__ __forceinline__ static void run(cplx<T>* v) {
+    if constexpr (L > 1) {
+      DifStage<DIR, L, OFF, 0, T>::run(v);
+      Dif<DIR, L / 2, OFF, T>::run(v);
+      Dif<DIR, L / 2, OFF + L / 2, T>::run(v);
+    }
+  }
+};
+
+// In-register r-point DFT:  y_q = sum_j x_j exp(DIR*2*pi*i*j*q/r), r | 16.
+template <int DIR, int r, typename T>
+__device__ __forceinline__ void dft(cplx<T>* x) {
+  Dif<DIR, r, 0, T>::run(x);
+  cplx<T> y[r];
+#pragma unroll
+  for (int q = 0; q < r; ++q) y[q] = x[bitrev(q, ilog2(r))];
+#pragma unroll
+  for (int q = 0; q < r; ++q) x[q] = y[q];
+}
+
+// LDS index with one pad slot every 16 entries (spreads the radix-16 strided
+// writes over the banks).
+__device__ __forceinline__ int pad16(int i) { return i + (i >> 4); }
+template <int N>
+struct Padded {
+  static constexpr int size = N + (N >> 4) + 1;
+};
+
+// LDS views of one team's working array.  ptr(pb) is the address of padded
+// index pb; S is the element stride between consecutive padded indices.
+template <typename T>
+struct LdsRow {  // contiguous, padded: a[i] at base[pad16(i)]
+  static constexpr int S = 1;
+  cplx<T>* base;
+  __device__ __forceinline__ cplx<T>* ptr(int pb) const { return base + pb; }
+};
+template <typename T, int LS>
+struct LdsStrided {  // interleaved columns: a[i] at base[pad16(i) * LS]
+  static constexpr int S = LS;
+  cplx<T>* base;
+  __device__ __forceinline__ cplx<T>* ptr(int pb) const { return base + pb * LS; }
+};
+
+// Hides the value of an expression from the optimiser, so that the per-lane
+// addresses of one transform are not merged with (and kept live until) those of
+// the next transform in the same kernel.
+template <typename P>
+__device__ __forceinline__ P opaque_s(P p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+__device__ __forceinline__ int opaque_v(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> twiddle(const cplx<T>* __restrict__ tw, int idx, int dir) {
+  cplx<T> w = tw[idx];
+  if (dir > 0) w.y = -w.y;
+  return w;
+}
+
+// One Stockham pass P >= 1 (LDS-sourced) through the team's LDS view.
+//
+// Every LDS address is a per-lane base plus a compile-time offset: all spans
+// after the first pass are multiples of 16, so pad16(i + 16m) = pad16(i) + 17m,
+// and the first pass (radix 16, span 1) writes 16 consecutive indices of one
+// aligned group of 16, so pad16(16b + q) = pad16(16b) + q.
+template <typename T, int N, int R, int DIR, int P, class Lds>
+struct TeamPass {
+  static constexpr int NP = num_passes(N, R);
+  static constexpr int TT = N / R;
+  __device__ __forceinline__ static void run(cplx<T> (&v)[R], int t, Lds lds,
+                                             const cplx<T>* __restrict__ tw) {
+    if constexpr (P < NP) {
+      constexpr int rp = pass_radix(N, R, P - 1);  // radix of the previous pass
+      constexpr int Nsp = ipow(R, P - 1);          // span before the previous pass
+      constexpr int r = pass_radix(N, R, P);
+      constexpr int Ns = ipow(R, P);
+      constexpr int nbp = R / rp, nb = R / r;
+      constexpr int S = Lds::S;
+      static_assert(Nsp % 16 == 0 || (Nsp == 1 && rp == 16), "separable LDS offsets");
+      static_assert((N / r) % 16 == 0, "separable LDS offsets");
+      const int tl = opaque_v(t);
+      // write outputs of the previous pass
+      __syncthreads();
+#pragma unroll
+      for (int mm = 0; mm < nbp; ++mm) {
+        const int b = tl + mm * TT;
+        const int k = b % Nsp;
+        cplx<T>* p = lds.ptr(pad16((b - k) * rp + k));
+#pragma unroll
+        for (int q = 0; q < rp; ++q) {
+          constexpr int unit = Nsp == 1 ? 1 : Nsp * 17 / 16;
+          p[S * q * unit] = v[mm * rp + q];
+        }
+      }
+      __syncthreads();
+      cplx<T> o[R];
+#pragma unroll
+      for (int mm = 0; mm < nb; ++mm) {
+        const int b = tl + mm * TT;
+        const int k = b % Ns;
+        const cplx<T>* p = lds.ptr(pad16(b));
+        cplx<T> x[r];
+#pragma unroll
+        for (int j = 0; j < r; ++j) x[j] = p[S * j * ((N / r) * 17 / 16)];
+        const int tstep = k * (N / (Ns * r));
+#pragma unroll
+        for (int j = 1; j < r; ++j) x[j] = cmul(x[j], twiddle(tw, (j * tstep) & (N - 1), DIR));
+        dft<DIR, r>(x);
+        if constexpr (P == NP - 1) {
+          // last pass: outputs land on t + TT*(mm + q*nb) -> strided register layout
+#pragma unroll
+          for (int q = 0; q < r; ++q) o[mm + q * nb] = x[q];
+        } else {
+#pragma unroll
+          for (int q = 0; q < r; ++q) o[mm * r + q] = x[q];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[j] = o[j];
+      TeamPass<T, N, R, DIR, P + 1, Lds>::run(v, t, lds, tw);
+    }
+  }
+};
+
+// Full length-N transform by a team.  In/out: v[j] = a[t + j*TT] (natural order).
+// lds: view of the team's private LDS array (Padded<N>::size entries).
+// Contains __syncthreads(): every lane of the block must call it.
+template <typename T, int N, int R, int DIR, class Lds>
+__device__ __forceinline__ void team_fft(cplx<T> (&v)[R], int t, Lds lds,
+                                         const cplx<T>* __restrict__ tw) {
+  static_assert(N % R == 0 && R <= 16 && (R & (R - 1)) == 0, "radix");
+  dft<DIR, R>(v);  // pass 0: butterfly b = t reads v[j] = a[t + j*TT], no twiddle
+  TeamPass<T, N, R, DIR, 1, Lds>::run(v, t, lds, opaque_s(tw));
+}
+
+}  // namespace wh
