@@ -176,8 +176,75 @@ int wh_wigner2d_kinetic(wh_wigner2d* plan, void* W, void* stream);
    fused = 1: the p sweep exactly as configured inside wh_wigner2d_run (damping and the
    observable column sums fused in; Lie scheme); fused = 0: the plain V step. */
 int wh_wigner2d_potential(wh_wigner2d* plan, void* W, int step, int fused, void* stream);
+/*
+ * Adjoint of one Lie step at time index `step` (no counterpart in the reference,
+ * which has no gradients): Lambda <- U_step^dagger Lambda, i.e. the p sweep with
+ * the conjugated potential multiplier followed by the x sweep with the conjugated
+ * kinetic multiplier, and grad[k] (device, WH_MAX_DEG+1 doubles) =
+ * dL/da_k(step) = Re <Lambda_in, dU_step/da_k W_step>, where W_next = U_step W_step
+ * is the forward state after the step and a_k(step) the coefficients of
+ * V(t_step, x).  Lie scheme without damping or Gross-Pitaevskii only.
+ */
+int wh_wigner2d_adjoint_step(wh_wigner2d* plan, void* Lambda, const void* W_next, int step, double* grad,
+                             void* stream);
 /* Observables of W (FFT.py:1857-1878) into one row of WH_NOBS doubles (device). */
 int wh_wigner2d_observe(wh_wigner2d* plan, const void* W, double* obs_row, void* stream);
+
+/* ---- ensembles: many Wigner functions, uneven sizes, one launch per sweep --- */
+/*
+ * No counterpart in the reference (one grid per class instance there): the
+ * reference's per-instance set-up (the wh_wigner2d_desc fields) split into the
+ * part shared by all members and the per-member part.
+ */
+typedef struct {
+  int struct_size; /* sizeof(wh_ensemble_desc) */
+  int precision;   /* WH_C128 or WH_C64, all members */
+  int scheme;      /* WH_SCHEME_LIE or WH_SCHEME_STRANG */
+  double dt, hbar, mass;
+  int kinetic_poly, kinetic_deg;
+  double kin_coef[WH_MAX_DEG + 1];
+  int damping;     /* 0 none, 1 Caldeira-Leggett order 3 */
+  double gamma_dt;
+  int renormalize;
+  double gp_coeff;
+} wh_ensemble_desc;
+
+typedef struct {
+  int struct_size; /* sizeof(wh_member_desc) */
+  int x_dim, p_dim;
+  double dx, dp, dtheta, dlambda;
+  int pot_deg, pot_rows;
+  const double* pot_coef;
+  const double *theta_decay, *lambda_decay, *x_mask, *sigma_p, *theta_host, *lambda_host;
+  const double *x_host, *p_host, *dv_host, *v0_host;
+  double dxdp, inv_2m, dx_host, gp_rho_factor;
+} wh_member_desc;
+
+typedef struct wh_ensemble wh_ensemble;
+
+int wh_ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members, int n_members,
+                       wh_ensemble** out);
+int wh_ensemble_destroy(wh_ensemble* e);
+/*
+ * As wh_wigner2d_run for every member at once: states[m] is member m's device
+ * state, obs is a device array [n_members][n_steps+1][WH_NOBS].
+ */
+int wh_ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_steps, double* obs,
+                    int normalize_initial, int use_graph, void* stream);
+/*
+ * Launch variants (index into the kernel variant tables; -1 keeps the current
+ * choice): x sweep, p sweep, p sweep with Caldeira-Leggett.  Rejected with
+ * WH_INVALID_VALUE if a member's grid does not fit the variant.
+ */
+int wh_ensemble_set_variants(wh_ensemble* e, int xvar, int pvar, int pclvar);
+/* Number of variants of each table and the (radix, lanes) of variant i. */
+int wh_variant_info(int table, int i, int* radix, int* lanes);
+/* Replaces member `member`'s potential coefficient rows ([rows][WH_MAX_DEG+1]
+   doubles, host or device memory, copied on `stream`) -- e.g. between the runs of
+   an optimisation loop.  No counterpart in the reference (which recompiles). */
+int wh_ensemble_set_potential(wh_ensemble* e, int member, const double* coef, int rows, int deg, void* stream);
+/* The single-grid plan's underlying ensemble (one member). */
+wh_ensemble* wh_wigner2d_ensemble(wh_wigner2d* plan);
 
 #ifdef __cplusplus
 }

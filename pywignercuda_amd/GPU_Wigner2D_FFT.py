@@ -1,0 +1,363 @@
+"""Drop-in for GPU_Wigner2D_FFT.py: Propagator_Base and GPU_Wigner2D_FFT.
+
+Usage is the reference's (instances/Wigner2D/*.ipynb): subclass, set
+``dampingFunction``, ``grossPitaevskiiCoefficient``, ``fp_Damping_String``, call
+``SetTimeTrack(...)`` and ``GPU_Wigner2D_FFT.__init__(self, X_gridDIM, P_gridDIM,
+X_amplitude, P_amplitude, hBar, mass, D_Theta, D_Lambda, gammaDamping,
+potentialString, dPotentialString, kinematicString)``, set ``W_init`` (a P x X
+array in FFT order) and call ``Run()``.  ``Run`` returns 0 and sets the same
+result attributes (timeRange, X_average, ..., negativeArea, W_end) and writes the
+same datasets (HDF5 when h5py is importable, else ``fileName + '.npz'``).
+
+The time loop runs on the GPU through libwignerhip (fused sweeps, observables
+reduced on the device, one host copy per saved frame).  Options beyond the
+reference, set before ``__init__``:
+
+    precision          'fp64' (default, complex128) or 'fp32' (complex64 bulk)
+    truncateConstants  True (default): kernel constants spliced as '%f', exactly
+                       as the reference's CUDA sources receive them
+                       (GPU_Wigner2D_FFT.py:833-842); False: full precision
+    absorbingMask      True (default): the potential kernel's edge mask
+                       (GPU_Wigner2D_FFT.py:121-123); False: no mask
+    useGraph           True (default): replay the step loop from a hipGraph
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.fftpack as fftpack
+
+from . import _io
+from .physics import host_eval
+from .wigner2d import Wigner2D
+
+
+class Propagator_Base:
+    """Host-side part of Propagator_Base (GPU_Wigner2D_FFT.py:529-1705) kept for drop-in use."""
+
+    precision = "fp64"
+    truncateConstants = True
+    absorbingMask = True
+    useGraph = True
+    grossPitaevskiiCoefficient = 0.0
+    dampingFunction = "CaldeiraLeggett"
+    fp_Damping_String = "p"
+
+    # ---------------------------------------------------------------- set-up
+    def SetTimeTrack(self, dt, timeSteps=128, skipFrames=1, fileName="", compression=None):
+        """GPU_Wigner2D_FFT.py:530-546 (no kernels are compiled here)."""
+        self.runTime = float(dt) * timeSteps
+        self.timeSteps = timeSteps
+        self.dt = dt
+        self.skipFrames = skipFrames
+        self.fileName = fileName
+        self.compression = compression
+        self.__x_p_representation__ = "x_p"
+
+    def SetPhaseSpaceBox2D(self, X_gridDIM, P_gridDIM, X_amplitude, P_amplitude):
+        """GPU_Wigner2D_FFT.py:549-567."""
+        self.X_gridDIM = X_gridDIM
+        self.P_gridDIM = P_gridDIM
+        self.X_amplitude = X_amplitude
+        self.P_amplitude = P_amplitude
+        self.dX = 2.0 * X_amplitude / float(X_gridDIM)
+        self.dP = 2.0 * P_amplitude / float(P_gridDIM)
+        self.dTheta = 2.0 * np.pi / (2.0 * P_amplitude)
+        self.Theta_amplitude = self.dTheta * P_gridDIM / 2.0
+        self.dLambda = 2.0 * np.pi / (2.0 * X_amplitude)
+        self.Lambda_amplitude = self.dLambda * X_gridDIM / 2.0
+        self.SetPhaseSpaceGrid2D()
+
+    def SetPhaseSpaceGrid2D(self):
+        """GPU_Wigner2D_FFT.py:569-596; the device grids are computed in the kernels."""
+        self.X_range = np.linspace(-self.X_amplitude, self.X_amplitude - self.dX, self.X_gridDIM)
+        self.Lambda_range = np.linspace(-self.Lambda_amplitude, self.Lambda_amplitude - self.dLambda, self.X_gridDIM)
+        self.Theta_range = np.linspace(-self.Theta_amplitude, self.Theta_amplitude - self.dTheta, self.P_gridDIM)
+        self.P_range = np.linspace(-self.P_amplitude, self.P_amplitude - self.dP, self.P_gridDIM)
+        self.X = fftpack.fftshift(self.X_range)[np.newaxis, :]
+        self.Theta = fftpack.fftshift(self.Theta_range)[:, np.newaxis]
+        self.Lambda = fftpack.fftshift(self.Lambda_range)[np.newaxis, :]
+        self.P = fftpack.fftshift(self.P_range)[:, np.newaxis]
+
+    # ---------------------------------------------------------------- physics on the host
+    def Potential(self, t, x):
+        return host_eval(self.potentialString, t=t, x=x)
+
+    def dPotential(self, t, x):
+        return host_eval(self.dPotentialString, t=t, x=x)
+
+    def fp_Damping(self, p):
+        return host_eval(self.fp_Damping_String, p=p)
+
+    # ---------------------------------------------------------------- classical reference trajectories
+    def symplecticExpK(self, mass, dt, c, Z):
+        return Z + np.array([dt * c * Z[1] / mass, 0])
+
+    def symplecticExpV(self, mass, dt, d, dVdx, Z):
+        return Z + np.array([0, -dt * d * dVdx(0.0, Z[0])])
+
+    def symplecticExpKExpV2(self, mass, dt, dVdx, Z0):
+        """Second-order classical symplectic step (GPU_Wigner2D_FFT.py:609-622)."""
+        Z = self.symplecticExpK(mass, dt, 0.5, self.symplecticExpV(mass, dt, 1.0, dVdx, Z0))
+        return self.symplecticExpK(mass, dt, 0.5, self.symplecticExpV(mass, dt, 0.0, dVdx, Z))
+
+    def SymplecticPropagator(self, dt, n_iterations, Z0, gammaDamping):
+        """Classical trajectory with linear friction (GPU_Wigner2D_FFT.py:624-640)."""
+        Z = np.asarray(Z0, dtype=float)
+        traj = []
+        for i in range(n_iterations):
+            traj.append(np.append(Z, dt * (i + 1)))
+            Z = self.symplecticExpKExpV2(self.mass, dt, self.dPotential, Z)
+            Z[1] = np.exp(-2.0 * gammaDamping * dt) * Z[1]
+        return np.array(traj)
+
+    def SymplecticPropagator_SmoothedP(self, dt, n_iterations, Z0, gammaDamping):
+        Z = np.asarray(Z0, dtype=float)
+        traj = []
+        for i in range(n_iterations):
+            traj.append(np.append(Z, dt * (i + 1)))
+            Z = self.symplecticExpKExpV2(self.mass, dt, self.dPotential, Z)
+            Z[1] = Z[1] - 2.0 * gammaDamping * dt * np.sign(Z[1]) * self.fp_Damping(Z[1])
+        return np.array(traj)
+
+    # ---------------------------------------------------------------- initial states and helpers
+    def fft_shift2D(self, X):
+        return fftpack.fftshift(fftpack.fftshift(X, axes=1), axes=0)
+
+    def Wigner_HarmonicOscillator(self, n, omega, x, p):
+        """GPU_Wigner2D_FFT.py:715-728."""
+        from scipy.special import laguerre
+
+        self.representation = "x_p"
+        r2 = self.mass * omega ** 2 * (self.X - x) ** 2 + (self.P - p) ** 2 / self.mass
+        W = (-1) ** n * laguerre(n)(2 * r2) * np.exp(-r2)
+        return W / (np.sum(W) * self.dX * self.dP)
+
+    def Heaviside(self, x):
+        return 0.5 * (np.sign(x) + 1.0)
+
+    def Psi_Half_HarmonicOscillator(self, n, omega, x0, p0, X):
+        from scipy.special import legendre
+
+        k = np.sqrt(self.mass * omega / self.hBar)
+        return (np.exp(1j * p0) * np.exp(-0.5 * k ** 2 * (X - x0) ** 2) * legendre(n)(k * (X - x0))
+                * self.Heaviside(-(X - x0)))
+
+    def Wigner_Half_HarmonicOscillator(self, n, omega, x0, p0):
+        """GPU_Wigner2D_FFT.py:743-771: W from psi(x - hbar theta/2) psi*(x + hbar theta/2)."""
+        Xm = self.X_range[np.newaxis, :] - 0.5 * self.hBar * self.Theta_range[:, np.newaxis]
+        Xp = self.X_range[np.newaxis, :] + 0.5 * self.hBar * self.Theta_range[:, np.newaxis]
+        W = self.Psi_Half_HarmonicOscillator(n, omega, x0, p0, Xm) * \
+            self.Psi_Half_HarmonicOscillator(n, omega, x0, p0, Xp).conj()
+        W = self.Fourier_Theta_To_P_CPU(fftpack.fftshift(W))
+        return W / (np.sum(W) * self.dX * self.dP)
+
+    def Fourier_XTheta_to_XP(self, M):
+        return fftpack.ifft(M, axis=0)
+
+    def Fourier_Theta_To_P_CPU(self, W):
+        return fftpack.ifft(W, axis=0)
+
+    def Fourier_P_To_Theta_CPU(self, W):
+        return fftpack.fft(W, axis=0)
+
+    def Fourier_X_To_Lambda_CPU(self, W):
+        return fftpack.fft(W, axis=1)
+
+    def Fourier_Lambda_To_X_CPU(self, W):
+        return fftpack.ifft(W, axis=1)
+
+    # ---------------------------------------------------------------- device transforms (torch tensors)
+    def _axis_plans(self, dtype):
+        from . import cufft_wrapper as cw
+
+        key = "_plans_" + str(dtype)
+        if not hasattr(self, key):
+            t = cw.CUFFT_Z2Z if "128" in str(dtype) else cw.CUFFT_C2C
+            shape = (self.P_gridDIM, self.X_gridDIM)
+            setattr(self, key, (cw.Plan_Z2Z_2D_Axis0(shape, t), cw.Plan_Z2Z_2D_Axis1(shape, t)))
+        return getattr(self, key)
+
+    def Fourier_X_To_Lambda_GPU(self, W_GPU):
+        from . import cufft_wrapper as cw
+
+        cw.fft_Z2Z(W_GPU, W_GPU, self._axis_plans(W_GPU.dtype)[1])
+
+    def Fourier_Lambda_To_X_GPU(self, W_GPU):
+        from . import cufft_wrapper as cw
+
+        cw.ifft_Z2Z_normalized(W_GPU, W_GPU, self._axis_plans(W_GPU.dtype)[1])
+
+    def Fourier_P_To_Theta_GPU(self, W_GPU):
+        from . import cufft_wrapper as cw
+
+        cw.fft_Z2Z(W_GPU, W_GPU, self._axis_plans(W_GPU.dtype)[0])
+
+    def Fourier_Theta_To_P_GPU(self, W_GPU):
+        from . import cufft_wrapper as cw
+
+        cw.ifft_Z2Z_normalized(W_GPU, W_GPU, self._axis_plans(W_GPU.dtype)[0])
+
+    def Fourier_XTheta_To_LambdaP_GPU(self, W_GPU):
+        self.Fourier_X_To_Lambda_GPU(W_GPU)
+        self.Fourier_Theta_To_P_GPU(W_GPU)
+
+    def Fourier_LambdaP_To_XTheta_GPU(self, W_GPU):
+        self.Fourier_Lambda_To_X_GPU(W_GPU)
+        self.Fourier_P_To_Theta_GPU(W_GPU)
+
+    # ---------------------------------------------------------------- output files
+    def WriteHDF5_variables(self):
+        """GPU_Wigner2D_FFT.py:844-857."""
+        f = self.file
+        f["dx"] = self.dX
+        f["dtheta"] = self.dTheta
+        f["dp"] = self.dP
+        f["dlambda"] = self.dLambda
+        f["x_gridDIM"] = self.X_gridDIM
+        f["p_gridDIM"] = self.P_gridDIM
+        f["x_min"] = -self.X_amplitude
+        f["x_max"] = self.X_amplitude - self.dX
+        f["p_min"] = -self.P_amplitude
+        f["p_max"] = self.P_amplitude - self.dP
+        f["lambda_min"] = self.Lambda_range.min()
+        f["lambda_max"] = self.Lambda_range.max()
+        f["dt"] = self.dt
+        f["timeSteps"] = self.timeSteps
+        f["skipFrames"] = self.skipFrames
+
+    def save_Frame(self, t, W):
+        """GPU_Wigner2D_FFT.py:660-663 (W a device tensor or an array)."""
+        arr = W.cpu().numpy() if hasattr(W, "cpu") else np.asarray(W)
+        self.file.create_dataset(str(t), data=arr)
+
+    def WignerFunctionFromFile(self, n, fileName=None):
+        return _io.read(fileName or self.fileName)[str(n)]
+
+    def LoadEhrenfestFromFile(self):
+        """GPU_Wigner2D_FFT.py:919-942."""
+        f = _io.read(self.fileName)
+        self.X_average = f["Ehrenfest/X_Ehrenfest"]
+        self.X2_average = f["Ehrenfest/X2_Ehrenfest"]
+        self.P_average = f["Ehrenfest/P_Ehrenfest"]
+        self.P2_average = f["Ehrenfest/P2_Ehrenfest"]
+        self.XP_average = f["Ehrenfest/XP_Ehrenfest"]
+        self.dPotentialdX_average = f["Ehrenfest/dPotentialdX_Ehrenfest"]
+        self.PdPotentialdX_average = f["Ehrenfest/PdPotentialdX_average"]
+        self.XdPotentialdX_average = f["Ehrenfest/XdPotentialdX_average"]
+        self.Hamiltonian_average = f["Ehrenfest/Hamiltonian_average"]
+        self.W_init = f["W_init"] + 0j
+        self.W_end = f["W_end"] + 0j
+
+    # ---------------------------------------------------------------- device plan
+    def _make_plan(self, scheme, gp_coeff, kinetic):
+        if self.gammaDamping != 0.0 and self.dampingFunction not in ("CaldeiraLeggett",):
+            raise NotImplementedError(
+                f"dampingFunction {self.dampingFunction!r}: only 'CaldeiraLeggett' is implemented on the GPU")
+        return Wigner2D(
+            self.X_gridDIM, self.P_gridDIM, self.X_amplitude, self.P_amplitude, self.hBar, self.mass, self.dt,
+            potential=self.potentialString, dpotential=self.dPotentialString, kinetic=kinetic, scheme=scheme,
+            D_theta=self.D_Theta, D_lambda=self.D_Lambda, gamma=self.gammaDamping,
+            damping="CaldeiraLeggett" if self.gammaDamping != 0.0 else None,
+            fp_damping=self.fp_Damping_String, gp_coeff=gp_coeff, mask=self.absorbingMask,
+            truncate_constants=self.truncateConstants, precision=self.precision,
+            time_rows=self.timeSteps + 3)
+
+    def _run_loop(self, plan, post_step):
+        """Runs timeSteps+1 iterations in chunks between saved frames; returns the record."""
+        import torch
+
+        W = plan.to_device(self.W_init)
+        total = self.timeSteps + 1
+        skip = max(int(self.skipFrames), 1)
+        rows, saved = [], []
+        it = 0
+        while it < total:
+            if post_step:  # frames after iteration f (GP.py:1997-1999): chunk ends after f
+                end = min(total, (it // skip) * skip + 1 if it % skip == 0 else (it // skip + 1) * skip + 1)
+            else:  # frames before iteration f (FFT.py:1880-1882): chunk starts at f
+                if it % skip == 0:
+                    saved.append(it)
+                    self.save_Frame(it, W if it > 0 else self._normalised_init())
+                end = min(total, (it // skip + 1) * skip)
+            n = end - it
+            obs = plan.run(W, n, first_step=it, normalize_initial=(it == 0), use_graph=self.useGraph)
+            rows.append(obs[:n].cpu().numpy())
+            it = end
+            if post_step and (it - 1) % skip == 0:
+                saved.append(it - 1)
+                self.save_Frame(it - 1, W)
+        torch.cuda.synchronize()
+        self._W_device = W
+        return np.concatenate(rows, axis=0), saved
+
+    def _normalised_init(self):
+        W = np.asarray(self.W_init, dtype=np.complex128)
+        return W / (W.sum() * self.dX * self.dP)
+
+    def _store_results(self, rec, saved, extra=None):
+        n = rec.shape[0]
+        self.timeRange = np.arange(n) * self.dt
+        self.X_average, self.X2_average = rec[:, 0], rec[:, 1]
+        self.P_average, self.P2_average = rec[:, 2], rec[:, 3]
+        self.XP_average = rec[:, 4]
+        self.dPotentialdX_average = rec[:, 5]
+        self.PdPotentialdX_average = rec[:, 6]
+        self.XdPotentialdX_average = rec[:, 7]
+        self.Hamiltonian_average = rec[:, 8]
+        self.negativeArea = rec[:, 9]
+        f = self.file
+        f["timeRange"] = self.timeRange
+        f["timeRangeIndexSaved"] = np.array(saved)
+        f["/Ehrenfest/X_Ehrenfest"] = self.X_average
+        f["/Ehrenfest/X2_Ehrenfest"] = self.X2_average
+        f["/Ehrenfest/P_Ehrenfest"] = self.P_average
+        f["/Ehrenfest/P2_Ehrenfest"] = self.P2_average
+        f["/Ehrenfest/XP_Ehrenfest"] = self.XP_average
+        f["/Ehrenfest/dPotentialdX_Ehrenfest"] = self.dPotentialdX_average
+        f["/Ehrenfest/PdPotentialdX_average"] = self.PdPotentialdX_average
+        f["/Ehrenfest/XdPotentialdX_average"] = self.XdPotentialdX_average
+        f["/Ehrenfest/Hamiltonian_average"] = self.Hamiltonian_average
+        for k, v in (extra or {}).items():
+            f[k] = v
+        f["W_init"] = np.asarray(self.W_init).real
+        self.W_end = self._W_device.cpu().numpy().astype(np.complex128).real
+        f["W_end"] = self.W_end
+        f["negativeArea"] = self.negativeArea
+        f.close()
+
+
+class GPU_Wigner2D_FFT(Propagator_Base):
+    """Wigner propagator in 2D phase space with diffusion and amplitude damping
+    (GPU_Wigner2D_FFT.py:1713-1964): Lie step K(dt); V(dt) [+ GP]; CL friction; renormalisation."""
+
+    def __init__(self, X_gridDIM, P_gridDIM, X_amplitude, P_amplitude, hBar, mass, D_Theta, D_Lambda,
+                 gammaDamping, potentialString, dPotentialString, kinematicString, normalization="Wigner"):
+        self.normalization = normalization
+        self.D_Theta = D_Theta
+        self.D_Lambda = D_Lambda
+        self.gammaDamping = gammaDamping
+        self.potentialString = potentialString
+        self.dPotentialString = dPotentialString
+        self.kinematicString = kinematicString
+        self.SetPhaseSpaceBox2D(X_gridDIM, P_gridDIM, X_amplitude, P_amplitude)
+        self.hBar = hBar
+        self.mass = mass
+        self.plan = self._make_plan("lie", self.grossPitaevskiiCoefficient, None)
+        self.Hamiltonian = self.P ** 2 / (2.0 * self.mass) + self.Potential(0, self.X)
+        self.Hamiltonian = self.fft_shift2D(self.Hamiltonian)
+
+    def Run(self):
+        """GPU_Wigner2D_FFT.py:1754-1964; returns 0."""
+        self.file = _io.RunFile(self.fileName or "Wigner2D_run")
+        self.WriteHDF5_variables()
+        self.file.create_dataset("Hamiltonian", data=self.Hamiltonian.real)
+        rec, saved = self._run_loop(self.plan, post_step=False)
+        self._store_results(rec, saved)
+        return 0
+
+
+class GPU_Wigner2D_ODM2(Propagator_Base):
+    """GPU_Wigner2D_ODM2 (GPU_Wigner2D_FFT.py:1973-2230): two ODM Lindblad channels -- not built."""
+
+    def __init__(self, *args, **kwargs):
+        raise NotImplementedError("the ODM Lindblad dissipators are not implemented on the GPU path")

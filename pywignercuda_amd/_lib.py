@@ -111,6 +111,31 @@ class Wigner2DDesc(ctypes.Structure):
     ]
 
 
+class EnsembleDesc(ctypes.Structure):
+    """wh_ensemble_desc of include/wignerhip.h."""
+
+    _fields_ = [
+        ("struct_size", c_int), ("precision", c_int), ("scheme", c_int),
+        ("dt", c_double), ("hbar", c_double), ("mass", c_double),
+        ("kinetic_poly", c_int), ("kinetic_deg", c_int), ("kin_coef", c_double * (WH_MAX_DEG + 1)),
+        ("damping", c_int), ("gamma_dt", c_double), ("renormalize", c_int), ("gp_coeff", c_double),
+    ]
+
+
+class MemberDesc(ctypes.Structure):
+    """wh_member_desc of include/wignerhip.h."""
+
+    _fields_ = [
+        ("struct_size", c_int), ("x_dim", c_int), ("p_dim", c_int),
+        ("dx", c_double), ("dp", c_double), ("dtheta", c_double), ("dlambda", c_double),
+        ("pot_deg", c_int), ("pot_rows", c_int), ("pot_coef", _DP),
+        ("theta_decay", _DP), ("lambda_decay", _DP), ("x_mask", _DP), ("sigma_p", _DP),
+        ("theta_host", _DP), ("lambda_host", _DP),
+        ("x_host", _DP), ("p_host", _DP), ("dv_host", _DP), ("v0_host", _DP),
+        ("dxdp", c_double), ("inv_2m", c_double), ("dx_host", c_double), ("gp_rho_factor", c_double),
+    ]
+
+
 _lib = None
 
 
@@ -130,6 +155,14 @@ def _declare(lib):
         "wh_wigner2d_kinetic": ([vp, vp, vp], ip),
         "wh_wigner2d_potential": ([vp, vp, ip, ip, vp], ip),
         "wh_wigner2d_observe": ([vp, vp, vp, vp], ip),
+        "wh_wigner2d_ensemble": ([vp], vp),
+        "wh_wigner2d_adjoint_step": ([vp, vp, vp, ip, vp, vp], ip),
+        "wh_ensemble_create": ([POINTER(EnsembleDesc), POINTER(MemberDesc), ip, POINTER(vp)], ip),
+        "wh_ensemble_destroy": ([vp], ip),
+        "wh_ensemble_run": ([vp, POINTER(vp), ip, ip, vp, ip, ip, vp], ip),
+        "wh_ensemble_set_variants": ([vp, ip, ip, ip], ip),
+        "wh_ensemble_set_potential": ([vp, ip, vp, ip, ip, vp], ip),
+        "wh_variant_info": ([ip, ip, POINTER(ip), POINTER(ip)], ip),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

@@ -4,41 +4,43 @@
 // Replaces the reference's FFI layer cufft_wrapper.py (plans over libcufft.so)
 // and the per-step host orchestration of GPU_Wigner2D_FFT.Run /
 // GPU_Wigner2D_GPitaevskii.Run (GPU_Wigner2D_FFT.py:1754-1964, GP.py:1809-2055).
+// Everything propagates an ensemble; a single grid is an ensemble of one.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <tuple>
 #include <vector>
 
 #include "../../include/wignerhip.h"
 #include "wh_kernels.hpp"
+#include "wh_variants.hpp"
 
 namespace wh {
-bool size_supported(int n);
-int xsweep_rows(int n);
-int psweep_cols(int prec, int n, int cl);
-hipError_t launch_xsweep(int prec, const XSweepArgs& a, hipStream_t s);
-hipError_t launch_psweep(int prec, const PSweepArgs& a, hipStream_t s);
-hipError_t launch_colsum(int prec, const void* W, int X, int P, const double* p_host,
-                         double* colsums, hipStream_t s);
-hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
-hipError_t launch_scale(int prec, void* W, size_t n, const double* f, hipStream_t s);
-hipError_t launch_fft_rows(int prec, int N, int dir, void* d, int rows, int norm, const void* tw,
-                           hipStream_t s);
-hipError_t launch_fft_cols(int prec, int N, int dir, void* d, int X, int norm, const void* tw,
-                           hipStream_t s);
-hipError_t launch_set_int(int* p, int v, hipStream_t s);
-hipError_t launch_set_scale(double* p, double re, double im, hipStream_t s);
-hipError_t launch_gp_shift(int X, int P, const double* rho, const double* lambda_host,
-                           const double* theta_host, double* G, const void* twX64, hipStream_t s);
-hipError_t launch_colpart_reduce(const double* colpart, int nblk, int X, double factor, double* rho,
-                                 double* obs, int obs_base, const int* step, double dx,
+hipError_t launch_xsweep_f64(int variant, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_xsweep_f32(int variant, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_psweep_f64(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_psweep_f32(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_gpshift(const Member* members, const unsigned* work, int nwork, hipStream_t s);
+hipError_t launch_colsum(int prec, const Member* members, const unsigned* work, int nwork, hipStream_t s);
+hipError_t launch_reduce(const ReduceArgs& a, int nmembers, int* done, hipStream_t s);
+hipError_t launch_colpart_reduce(const Member* members, int nmembers, int xradix, int xblock, const int* step,
                                  hipStream_t s);
+hipError_t launch_scale(int prec, const Member* members, const unsigned* work, int nwork, hipStream_t s);
+hipError_t launch_set_int(int* p, int v, hipStream_t s);
+hipError_t launch_set_scale(const Member* members, int nmembers, double re, double im, hipStream_t s);
+hipError_t launch_fft_axis(int prec, int columns, int N, int dir, void* d, int batch, int norm,
+                           const void* tw_hi, const void* tw_lo, hipStream_t s);
+int fft_cols_batch_multiple(int prec, int N);
+hipError_t launch_adj_psweep(int prec, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_grad_reduce(const Member* members, int nmembers, double* grad, double scale, hipStream_t s);
+int adj_cols(int P);
 }  // namespace wh
 
 namespace {
@@ -63,20 +65,33 @@ int hip_fail(hipError_t e, const char* what) {
   return fail(code, "%s: %s", what, hipGetErrorString(e));
 }
 
-#define WH_HIP(call, what)                      \
-  do {                                          \
-    hipError_t e_ = (call);                     \
+#define WH_HIP(call, what)                           \
+  do {                                               \
+    hipError_t e_ = (call);                          \
     if (e_ != hipSuccess) return hip_fail(e_, what); \
   } while (0)
+#define WH_TRY(expr)     \
+  do {                   \
+    int rc_ = (expr);    \
+    if (rc_) return rc_; \
+  } while (0)
+#define WH_LAUNCH(expr, what) WH_HIP(expr, what)
+
+bool size_supported(int n) { return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048; }
+bool sweep_size_supported(int n) { return n >= 128 && n <= 2048 && size_supported(n); }
 
 // device allocation owned by a plan
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
   int alloc(size_t n) {
+    if (p) (void)hipFree(p);
     bytes = n;
     hipError_t e = hipMalloc(&p, n ? n : 16);
     if (e != hipSuccess) {
@@ -86,35 +101,47 @@ struct DevBuf {
     return WH_SUCCESS;
   }
   int upload(const void* host, size_t n) {
-    int rc = alloc(n);
-    if (rc) return rc;
-    hipError_t e = hipMemcpy(p, host, n, hipMemcpyHostToDevice);
-    if (e != hipSuccess) return hip_fail(e, "hipMemcpy H2D");
+    WH_TRY(alloc(n));
+    WH_HIP(hipMemcpy(p, host, n, hipMemcpyHostToDevice), "hipMemcpy H2D");
     return WH_SUCCESS;
   }
   template <typename T>
   T* as() const { return static_cast<T*>(p); }
 };
 
-// exp(-2 pi i n / N), n = 0..N-1, evaluated in long double and rounded
-int make_twiddles(DevBuf& buf, int N, int prec) {
+// exp(-2 pi i n / N), n = 0..N-1, evaluated in long double; FP32 tables carry
+// the rounding error of the hi part in a second (lo) table
+struct TwSet {
+  DevBuf hi, lo, hi64;
+};
+int make_twiddles(TwSet& tw, int N, int prec, bool need64) {
   const long double two_pi = 6.283185307179586476925286766559L;
-  if (prec == WH_C128) {
-    std::vector<double> h(2 * size_t(N));
-    for (int n = 0; n < N; ++n) {
-      const long double a = two_pi * (long double)n / (long double)N;
-      h[2 * n] = (double)cosl(a);
-      h[2 * n + 1] = (double)(-sinl(a));
-    }
-    return buf.upload(h.data(), h.size() * sizeof(double));
-  }
-  std::vector<float> h(2 * size_t(N));
+  std::vector<long double> c(N), s(N);
   for (int n = 0; n < N; ++n) {
     const long double a = two_pi * (long double)n / (long double)N;
-    h[2 * n] = (float)cosl(a);
-    h[2 * n + 1] = (float)(-sinl(a));
+    c[n] = cosl(a);
+    s[n] = -sinl(a);
   }
-  return buf.upload(h.data(), h.size() * sizeof(float));
+  if ((prec == WH_C128 || need64) && !tw.hi64.p) {
+    std::vector<double> h(2 * size_t(N));
+    for (int n = 0; n < N; ++n) {
+      h[2 * n] = (double)c[n];
+      h[2 * n + 1] = (double)s[n];
+    }
+    WH_TRY(tw.hi64.upload(h.data(), h.size() * sizeof(double)));
+  }
+  if (prec == WH_C64 && !tw.hi.p) {
+    std::vector<float> hi(2 * size_t(N)), lo(2 * size_t(N));
+    for (int n = 0; n < N; ++n) {
+      hi[2 * n] = (float)c[n];
+      hi[2 * n + 1] = (float)s[n];
+      lo[2 * n] = (float)(c[n] - (long double)hi[2 * n]);
+      lo[2 * n + 1] = (float)(s[n] - (long double)hi[2 * n + 1]);
+    }
+    WH_TRY(tw.hi.upload(hi.data(), hi.size() * sizeof(float)));
+    WH_TRY(tw.lo.upload(lo.data(), lo.size() * sizeof(float)));
+  }
+  return WH_SUCCESS;
 }
 
 int upload_opt(DevBuf& buf, const double* host, size_t n) {
@@ -146,22 +173,37 @@ extern "C" const char* wh_status_string(int s) {
   }
 }
 
-extern "C" int wh_version(void) { return (0 << 16) | 3; }
+extern "C" int wh_version(void) { return (0 << 16) | 4; }
+
+extern "C" int wh_variant_info(int table, int i, int* radix, int* lanes) {
+  const wh::SweepVariant* t;
+  int n;
+  switch (table) {
+    case 0: t = wh::kXVariants; n = wh::kNumX; break;
+    case 1: t = wh::kPVariants; n = wh::kNumP; break;
+    case 2: t = wh::kPCLVariants; n = wh::kNumPCL; break;
+    default: return -fail(WH_INVALID_VALUE, "wh_variant_info: table %d (0 x, 1 p, 2 p+CL)", table);
+  }
+  if (i < 0) return n;
+  if (i >= n) return -fail(WH_INVALID_VALUE, "wh_variant_info: index %d >= %d", i, n);
+  if (radix) *radix = t[i].R;
+  if (lanes) *lanes = t[i].block;
+  return WH_SUCCESS;
+}
 
 // ============================================================================ FFT plans
 struct wh_fft_plan {
   int n = 0, type = 0, batch = 0;
   bool columns = false;  // interleaved columns of an [n][batch] array
-  DevBuf tw;
+  TwSet tw;
 };
 
-extern "C" int wh_fft_plan_many(wh_fft_plan** plan, int n, int istride, int idist, int type,
-                                int batch) {
+extern "C" int wh_fft_plan_many(wh_fft_plan** plan, int n, int istride, int idist, int type, int batch) {
   if (!plan) return fail(WH_INVALID_VALUE, "wh_fft_plan_many: plan is NULL");
   *plan = nullptr;
   if (type != WH_C128 && type != WH_C64)
     return fail(WH_INVALID_TYPE, "wh_fft_plan_many: type %d is neither WH_C128 nor WH_C64", type);
-  if (!wh::size_supported(n))
+  if (!size_supported(n))
     return fail(WH_INVALID_SIZE, "wh_fft_plan_many: length %d is not a power of two in [64, 2048]", n);
   if (batch < 1) return fail(WH_INVALID_VALUE, "wh_fft_plan_many: batch %d < 1", batch);
   bool columns;
@@ -169,7 +211,7 @@ extern "C" int wh_fft_plan_many(wh_fft_plan** plan, int n, int istride, int idis
     columns = false;
   } else if (istride == batch && idist == 1) {
     columns = true;
-    const int c = wh::psweep_cols(type, n, 0);
+    const int c = wh::fft_cols_batch_multiple(type, n);
     if (batch % c != 0)
       return fail(WH_INVALID_SIZE, "wh_fft_plan_many: column batch %d is not a multiple of %d", batch, c);
   } else {
@@ -177,17 +219,13 @@ extern "C" int wh_fft_plan_many(wh_fft_plan** plan, int n, int istride, int idis
                 "wh_fft_plan_many: layout istride=%d idist=%d is neither rows (1, n) nor columns (batch, 1)",
                 istride, idist);
   }
-  auto* p = new wh_fft_plan;
+  auto p = std::make_unique<wh_fft_plan>();
   p->n = n;
   p->type = type;
   p->batch = batch;
   p->columns = columns;
-  int rc = make_twiddles(p->tw, n, type);
-  if (rc) {
-    delete p;
-    return rc;
-  }
-  *plan = p;
+  WH_TRY(make_twiddles(p->tw, n, type, false));
+  *plan = p.release();
   return WH_SUCCESS;
 }
 
@@ -195,24 +233,21 @@ static int fft_exec(wh_fft_plan* plan, void* idata, void* odata, int dir, int no
   if (!plan) return fail(WH_INVALID_PLAN, "wh_fft_exec: plan is NULL");
   if (!idata || idata != odata)
     return fail(WH_INVALID_VALUE, "wh_fft_exec: only in-place transforms (idata == odata != NULL)");
-  if (dir != WH_FORWARD && dir != WH_INVERSE)
-    return fail(WH_INVALID_VALUE, "wh_fft_exec: direction %d", dir);
+  if (dir != WH_FORWARD && dir != WH_INVERSE) return fail(WH_INVALID_VALUE, "wh_fft_exec: direction %d", dir);
   const size_t align = plan->type == WH_C128 ? 16 : 8;
   if (reinterpret_cast<uintptr_t>(idata) % align)
     return fail(WH_UNALIGNED_DATA, "wh_fft_exec: data not %zu-byte aligned", align);
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  hipError_t e = plan->columns
-                     ? wh::launch_fft_cols(plan->type, plan->n, dir, idata, plan->batch, norm, plan->tw.p, s)
-                     : wh::launch_fft_rows(plan->type, plan->n, dir, idata, plan->batch, norm, plan->tw.p, s);
-  if (e != hipSuccess) return hip_fail(e, "wh_fft_exec launch");
+  const void* hi = plan->type == WH_C128 ? plan->tw.hi64.p : plan->tw.hi.p;
+  WH_HIP(wh::launch_fft_axis(plan->type, plan->columns, plan->n, dir, idata, plan->batch, norm, hi, plan->tw.lo.p,
+                             static_cast<hipStream_t>(stream)),
+         "wh_fft_exec launch");
   return WH_SUCCESS;
 }
 
 extern "C" int wh_fft_exec(wh_fft_plan* plan, void* idata, void* odata, int direction, void* stream) {
   return fft_exec(plan, idata, odata, direction, 0, stream);
 }
-extern "C" int wh_fft_exec_normalized(wh_fft_plan* plan, void* idata, void* odata, int direction,
-                                      void* stream) {
+extern "C" int wh_fft_exec_normalized(wh_fft_plan* plan, void* idata, void* odata, int direction, void* stream) {
   return fft_exec(plan, idata, odata, direction, 1, stream);
 }
 extern "C" int wh_fft_destroy(wh_fft_plan* plan) {
@@ -221,120 +256,543 @@ extern "C" int wh_fft_destroy(wh_fft_plan* plan) {
   return WH_SUCCESS;
 }
 
-// ============================================================================ Wigner2D plan
-struct GraphKey {
-  void* W;
-  double* obs;
-  int first_step;
-  hipStream_t stream;
-  bool operator<(const GraphKey& o) const {
-    return std::tie(W, obs, first_step, stream) < std::tie(o.W, o.obs, o.first_step, o.stream);
+// ============================================================================ ensembles
+namespace {
+
+struct MemberBufs {
+  DevBuf pot, theta_decay, lambda_decay, x_mask, sigma_p, theta_host, lambda_host;
+  DevBuf x_host, p_host, dv_host, v0_host, colsums, scale, rho, G, colpart;
+};
+
+struct WorkList {
+  DevBuf buf;
+  int n = 0;
+  int upload(const std::vector<unsigned>& items) {
+    n = int(items.size());
+    return buf.upload(items.data(), items.size() * sizeof(unsigned));
   }
 };
+
 struct GraphEntry {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int steps = 0;
 };
 
-struct wh_wigner2d {
-  wh_wigner2d_desc d{};
-  DevBuf twX, twP, twX64, pot, theta_decay, lambda_decay, x_mask, sigma_p, theta_host, lambda_host;
-  DevBuf x_host, p_host, dv_host, v0_host;
-  DevBuf colsums, colpart, scale, step, rho, G;
+}  // namespace
+
+struct wh_ensemble {
+  wh_ensemble_desc d{};
+  int n = 0;
+  std::vector<wh::Member> mem;  // host mirror of the device table
+  std::vector<std::unique_ptr<MemberBufs>> bufs;
+  DevBuf mem_dev;
+  std::map<int, std::unique_ptr<TwSet>> tw;  // by size
+  int xvar = 0, pvar = 0, pclvar = 0;
+  WorkList work_x, work_p, work_colsum, work_scale, work_gp, work_adj;
+  DevBuf step, done, gcols;
   hipStream_t capture_stream = nullptr;
-  std::map<GraphKey, GraphEntry> graphs;
+  std::map<std::tuple<hipStream_t, int>, GraphEntry> graphs;  // (stream, chunk)
   int graph_chunk = 16;
-  ~wh_wigner2d() {
+  bool table_dirty = true;
+  ~wh_ensemble() { drop_graphs(); if (capture_stream) (void)hipStreamDestroy(capture_stream); }
+  void drop_graphs() {
     for (auto& kv : graphs) {
       if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
       if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
     }
-    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+    graphs.clear();
   }
+  bool gp() const { return d.gp_coeff != 0.0; }
+  bool cl() const { return d.damping == 1; }
 };
 
-static int validate_desc(const wh_wigner2d_desc* d) {
-  if (!d) return fail(WH_INVALID_VALUE, "descriptor is NULL");
-  if (d->struct_size != (int)sizeof(wh_wigner2d_desc))
-    return fail(WH_INVALID_DESCRIPTOR, "descriptor struct_size %d != %d", d->struct_size,
-                (int)sizeof(wh_wigner2d_desc));
-  if (d->precision != WH_C128 && d->precision != WH_C64)
-    return fail(WH_INVALID_TYPE, "precision %d is neither WH_C128 nor WH_C64", d->precision);
-  if (!wh::size_supported(d->x_dim) || !wh::size_supported(d->p_dim))
-    return fail(WH_INVALID_SIZE, "grid %d x %d: both sides must be powers of two in [64, 2048]",
-                d->p_dim, d->x_dim);
-  if (d->p_dim % wh::xsweep_rows(d->x_dim) != 0)
-    return fail(WH_INVALID_SIZE, "P=%d must be a multiple of %d for X=%d", d->p_dim,
-                wh::xsweep_rows(d->x_dim), d->x_dim);
-  if (d->x_dim % wh::psweep_cols(d->precision, d->p_dim, d->damping) != 0)
-    return fail(WH_INVALID_SIZE, "X=%d must be a multiple of %d for P=%d", d->x_dim,
-                wh::psweep_cols(d->precision, d->p_dim, d->damping), d->p_dim);
-  if (d->scheme != WH_SCHEME_LIE && d->scheme != WH_SCHEME_STRANG)
-    return fail(WH_INVALID_DESCRIPTOR, "scheme %d", d->scheme);
-  if (d->pot_deg < 0 || d->pot_deg > WH_MAX_DEG || d->pot_rows < 1 || !d->pot_coef)
-    return fail(WH_INVALID_DESCRIPTOR, "potential: deg %d (max %d), rows %d, coef %p", d->pot_deg,
-                WH_MAX_DEG, d->pot_rows, (const void*)d->pot_coef);
-  if (d->kinetic_poly && (d->kinetic_deg < 0 || d->kinetic_deg > WH_MAX_DEG))
-    return fail(WH_INVALID_DESCRIPTOR, "kinetic degree %d", d->kinetic_deg);
-  if (!(d->dt == d->dt) || d->hbar == 0.0 || d->mass == 0.0)
-    return fail(WH_INVALID_DESCRIPTOR, "dt/hbar/mass: %g %g %g", d->dt, d->hbar, d->mass);
-  if (d->damping != 0 && d->damping != 1)
-    return fail(WH_INVALID_DESCRIPTOR, "damping %d (0 none, 1 Caldeira-Leggett)", d->damping);
-  if (d->damping == 1 && (!d->sigma_p || !d->theta_host))
-    return fail(WH_INVALID_DESCRIPTOR, "Caldeira-Leggett needs sigma_p and theta_host");
-  if (d->gp_coeff != 0.0 && (!d->lambda_host || !d->theta_host))
-    return fail(WH_INVALID_DESCRIPTOR, "Gross-Pitaevskii needs lambda_host and theta_host");
-  if (!d->x_host || !d->p_host)
-    return fail(WH_INVALID_DESCRIPTOR, "observables need x_host and p_host");
+struct wh_wigner2d {
+  wh_ensemble* e = nullptr;
+  ~wh_wigner2d() { delete e; }
+};
+
+namespace {
+
+int validate_member(const wh_ensemble_desc& d, const wh_member_desc& m, int i) {
+  if (m.struct_size != (int)sizeof(wh_member_desc))
+    return fail(WH_INVALID_DESCRIPTOR, "member %d: struct_size %d != %d", i, m.struct_size, (int)sizeof(wh_member_desc));
+  if (!sweep_size_supported(m.x_dim) || !sweep_size_supported(m.p_dim))
+    return fail(WH_INVALID_SIZE, "member %d: grid %d x %d: both sides must be powers of two in [128, 2048]", i,
+                m.p_dim, m.x_dim);
+  if (m.pot_deg < 0 || m.pot_deg > WH_MAX_DEG || m.pot_rows < 1 || !m.pot_coef)
+    return fail(WH_INVALID_DESCRIPTOR, "member %d: potential deg %d (max %d), rows %d, coef %p", i, m.pot_deg,
+                WH_MAX_DEG, m.pot_rows, (const void*)m.pot_coef);
+  if (d.damping == 1 && (!m.sigma_p || !m.theta_host))
+    return fail(WH_INVALID_DESCRIPTOR, "member %d: Caldeira-Leggett needs sigma_p and theta_host", i);
+  if (d.gp_coeff != 0.0 && (!m.lambda_host || !m.theta_host))
+    return fail(WH_INVALID_DESCRIPTOR, "member %d: Gross-Pitaevskii needs lambda_host and theta_host", i);
+  if (!m.x_host || !m.p_host) return fail(WH_INVALID_DESCRIPTOR, "member %d: observables need x_host and p_host", i);
   return WH_SUCCESS;
 }
 
-extern "C" int wh_wigner2d_create(const wh_wigner2d_desc* desc, wh_wigner2d** out) {
+int validate_common(const wh_ensemble_desc& d) {
+  if (d.struct_size != (int)sizeof(wh_ensemble_desc))
+    return fail(WH_INVALID_DESCRIPTOR, "ensemble struct_size %d != %d", d.struct_size, (int)sizeof(wh_ensemble_desc));
+  if (d.precision != WH_C128 && d.precision != WH_C64)
+    return fail(WH_INVALID_TYPE, "precision %d is neither WH_C128 nor WH_C64", d.precision);
+  if (d.scheme != WH_SCHEME_LIE && d.scheme != WH_SCHEME_STRANG)
+    return fail(WH_INVALID_DESCRIPTOR, "scheme %d", d.scheme);
+  if (d.kinetic_poly && (d.kinetic_deg < 0 || d.kinetic_deg > WH_MAX_DEG))
+    return fail(WH_INVALID_DESCRIPTOR, "kinetic degree %d", d.kinetic_deg);
+  if (!(d.dt == d.dt) || d.hbar == 0.0 || d.mass == 0.0)
+    return fail(WH_INVALID_DESCRIPTOR, "dt/hbar/mass: %g %g %g", d.dt, d.hbar, d.mass);
+  if (d.damping != 0 && d.damping != 1)
+    return fail(WH_INVALID_DESCRIPTOR, "damping %d (0 none, 1 Caldeira-Leggett)", d.damping);
+  return WH_SUCCESS;
+}
+
+// variant v fits every member on the given axis (x sweep: rows of X; p sweep: columns of P)
+bool fits_all(const wh_ensemble* e, const wh::SweepVariant& v, bool x_axis) {
+  for (const auto& m : e->mem) {
+    const int N = x_axis ? m.X : m.P;
+    const int other = x_axis ? m.P : m.X;
+    if (!wh::variant_fits(N, v)) return false;
+    if (other % wh::tile_lines(N, v) != 0) return false;
+  }
+  return true;
+}
+
+int build_work_lists(wh_ensemble* e) {
+  const auto& xv = wh::kXVariants[e->xvar];
+  const auto& pv = e->cl() ? wh::kPCLVariants[e->pclvar] : wh::kPVariants[e->pvar];
+  std::vector<unsigned> wx, wp, wc, ws, wg;
+  for (int i = 0; i < e->n; ++i) {
+    const auto& m = e->mem[i];
+    const unsigned hi = unsigned(i) << 16;
+    const int rows = wh::tile_lines(m.X, xv);
+    for (int t = 0; t < m.P / rows; ++t) wx.push_back(hi | unsigned(t));
+    // p tiles: consecutive tiles on one XCD (blocks are dealt round-robin to the 8 XCDs)
+    const int cols = wh::tile_lines(m.P, pv);
+    const int nt = m.X / cols;
+    for (int q = 0; q < nt; ++q) {
+      const int t = (nt % 8 == 0) ? (q % 8) * (nt / 8) + q / 8 : q;
+      wp.push_back(hi | unsigned(t));
+    }
+    for (int t = 0; t < (m.X + 31) / 32; ++t) wc.push_back(hi | unsigned(t));
+    const size_t cells = size_t(m.X) * size_t(m.P);
+    for (size_t t = 0; t < (cells + 8191) / 8192; ++t) ws.push_back(hi | unsigned(t));
+    if (e->gp()) {
+      const int grows = wh::tile_lines(m.X, wh::SweepVariant{16, 256});
+      for (int t = 0; t < m.P / grows; ++t) wg.push_back(hi | unsigned(t));
+    }
+  }
+  if (wx.size() > 0x7fffffff || ws.size() > 0x7fffffff) return fail(WH_INVALID_SIZE, "work list too long");
+  for (const auto& v : {&wx, &wp, &wc, &ws, &wg})
+    for (unsigned w : *v)
+      if ((w & 0xffffu) == 0xffffu) return fail(WH_INVALID_SIZE, "a member has more than 65535 tiles");
+  WH_TRY(e->work_x.upload(wx));
+  WH_TRY(e->work_p.upload(wp));
+  WH_TRY(e->work_colsum.upload(wc));
+  WH_TRY(e->work_scale.upload(ws));
+  if (e->gp()) WH_TRY(e->work_gp.upload(wg));
+  e->drop_graphs();
+  return WH_SUCCESS;
+}
+
+int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members, int n, wh_ensemble** out) {
+  if (!out) return fail(WH_INVALID_VALUE, "out is NULL");
+  *out = nullptr;
+  if (!desc || !members || n < 1 || n > 65535)
+    return fail(WH_INVALID_VALUE, "desc %p members %p n %d", (const void*)desc, (const void*)members, n);
+  WH_TRY(validate_common(*desc));
+  for (int i = 0; i < n; ++i) WH_TRY(validate_member(*desc, members[i], i));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WH_NO_DEVICE, "no HIP device");
+  auto e = std::make_unique<wh_ensemble>();
+  e->d = *desc;
+  e->n = n;
+  e->mem.resize(n);
+  const int prec = desc->precision;
+  const bool gp = desc->gp_coeff != 0.0;
+  for (int i = 0; i < n; ++i) {
+    const auto& md = members[i];
+    auto b = std::make_unique<MemberBufs>();
+    const size_t X = size_t(md.x_dim), P = size_t(md.p_dim);
+    std::vector<double> zeros(std::max(X, P), 0.0);
+    for (int N : {md.x_dim, md.p_dim})
+      if (!e->tw.count(N)) e->tw[N] = std::make_unique<TwSet>();
+    WH_TRY(make_twiddles(*e->tw[md.x_dim], md.x_dim, prec, gp));
+    WH_TRY(make_twiddles(*e->tw[md.p_dim], md.p_dim, prec, false));
+    WH_TRY(b->pot.upload(md.pot_coef, size_t(md.pot_rows) * (WH_MAX_DEG + 1) * sizeof(double)));
+    WH_TRY(upload_opt(b->theta_decay, md.theta_decay, P));
+    WH_TRY(upload_opt(b->lambda_decay, md.lambda_decay, X));
+    WH_TRY(upload_opt(b->x_mask, md.x_mask, X));
+    WH_TRY(upload_opt(b->sigma_p, md.sigma_p, P));
+    WH_TRY(upload_opt(b->theta_host, md.theta_host, P));
+    WH_TRY(upload_opt(b->lambda_host, md.lambda_host, X));
+    WH_TRY(b->x_host.upload(md.x_host, X * sizeof(double)));
+    WH_TRY(b->p_host.upload(md.p_host, P * sizeof(double)));
+    WH_TRY(b->dv_host.upload(md.dv_host ? md.dv_host : zeros.data(), X * sizeof(double)));
+    WH_TRY(b->v0_host.upload(md.v0_host ? md.v0_host : zeros.data(), X * sizeof(double)));
+    WH_TRY(b->colsums.alloc(X * wh::kNCol * sizeof(double)));
+    const double one[2] = {1.0, 0.0};
+    WH_TRY(b->scale.upload(one, sizeof(one)));
+    if (gp) {
+      WH_TRY(b->rho.alloc(X * sizeof(double)));
+      WH_TRY(b->G.alloc(X * P * sizeof(double)));
+    }
+    wh::Member& m = e->mem[i];
+    std::memset(&m, 0, sizeof(m));
+    m.colsums = b->colsums.as<double>();
+    m.scale = b->scale.as<double>();
+    m.rho = b->rho.as<double>();
+    m.G = b->G.as<double>();
+    m.pot = b->pot.as<double>();
+    m.theta_decay = b->theta_decay.as<double>();
+    m.lambda_decay = b->lambda_decay.as<double>();
+    m.xmask = b->x_mask.as<double>();
+    m.sigma_p = b->sigma_p.as<double>();
+    m.theta_host = b->theta_host.as<double>();
+    m.lambda_host = b->lambda_host.as<double>();
+    m.x_host = b->x_host.as<double>();
+    m.p_host = b->p_host.as<double>();
+    m.dv_host = b->dv_host.as<double>();
+    m.v0_host = b->v0_host.as<double>();
+    const TwSet& tx = *e->tw[md.x_dim];
+    const TwSet& tp = *e->tw[md.p_dim];
+    m.twX_hi = prec == WH_C128 ? tx.hi64.p : tx.hi.p;
+    m.twX_lo = tx.lo.p;
+    m.twP_hi = prec == WH_C128 ? tp.hi64.p : tp.hi.p;
+    m.twP_lo = tp.lo.p;
+    m.twX64 = tx.hi64.p;
+    m.X = md.x_dim;
+    m.P = md.p_dim;
+    m.pot_rows = md.pot_rows;
+    m.pot_deg = md.pot_deg;
+    m.dx = md.dx;
+    m.dp = md.dp;
+    m.dtheta = md.dtheta;
+    m.dlambda = md.dlambda;
+    m.dxdp = md.dxdp;
+    m.inv_2m = md.inv_2m;
+    m.dx_host = md.dx_host;
+    m.rho_factor = md.gp_rho_factor;
+    e->bufs.push_back(std::move(b));
+  }
+  // default variants: the first that fits every member
+  auto pick = [&](const wh::SweepVariant* t, int nt, bool x_axis, int& out_v) -> int {
+    for (int v = 0; v < nt; ++v)
+      if (fits_all(e.get(), t[v], x_axis)) {
+        out_v = v;
+        return WH_SUCCESS;
+      }
+    return fail(WH_INVALID_SIZE, "no kernel variant fits every member's grid");
+  };
+  WH_TRY(pick(wh::kXVariants, wh::kNumX, true, e->xvar));
+  WH_TRY(pick(wh::kPVariants, wh::kNumP, false, e->pvar));
+  WH_TRY(pick(wh::kPCLVariants, wh::kNumPCL, false, e->pclvar));
+  if (gp) {
+    const wh::SweepVariant g{16, 256};
+    for (int i = 0; i < n; ++i) {
+      const auto& m = e->mem[i];
+      if (!wh::variant_fits(m.X, g) || m.P % wh::tile_lines(m.X, g) != 0)
+        return fail(WH_INVALID_SIZE, "member %d: grid does not fit the Gross-Pitaevskii shift kernel", i);
+      const size_t parts = size_t(m.P / wh::tile_lines(m.X, wh::kXVariants[e->xvar])) * size_t(m.X);
+      WH_TRY(e->bufs[i]->colpart.alloc(parts * sizeof(double)));
+      e->mem[i].colpart = e->bufs[i]->colpart.as<double>();
+    }
+  }
+  WH_TRY(e->mem_dev.alloc(sizeof(wh::Member) * size_t(n)));
+  WH_TRY(e->step.alloc(16));
+  const int zero[4] = {0, 0, 0, 0};
+  WH_TRY(e->done.upload(zero, sizeof(zero)));
+  WH_TRY(build_work_lists(e.get()));
+  *out = e.release();
+  return WH_SUCCESS;
+}
+
+// --------------------------------------------------------------- launch helpers
+wh::SweepArgs sweep_args(const wh_ensemble* e, int flags, double frac, int step_add) {
+  const auto& d = e->d;
+  wh::SweepArgs a{};
+  a.members = e->mem_dev.as<wh::Member>();
+  a.step = e->step.as<int>();
+  a.step_add = step_add;
+  a.flags = flags;
+  a.dt = d.dt;
+  a.hbar = d.hbar;
+  a.vphase = frac * d.dt / d.hbar;
+  a.hbar_half = 0.5 * d.hbar;
+  a.gp_phase = -frac * d.dt * d.gp_coeff / d.hbar;
+  a.cl_a = d.gamma_dt;
+  a.inv_hbar = 1.0 / d.hbar;
+  a.kdeg = d.kinetic_deg;
+  for (int k = 0; k <= wh::kMaxDeg; ++k) a.kcoef[k] = d.kin_coef[k];
+  if (d.scheme == WH_SCHEME_LIE) {
+    // expPLambda: phase = dt*lambda*p/mass (GPU_Wigner2D_FFT.py:153)
+    a.kin_linear = 1;
+    a.k_c0 = 0.0;
+    a.k_c1 = 1.0 / d.mass;
+  } else if (d.kinetic_poly && d.kinetic_deg <= 2) {
+    // (K(p + h lambda/2) - K(p - h lambda/2))/h = lambda (k1 + 2 k2 p) for K = k0 + k1 p + k2 p^2
+    a.kin_linear = 1;
+    a.k_c0 = d.kinetic_deg >= 1 ? d.kin_coef[1] : 0.0;
+    a.k_c1 = d.kinetic_deg >= 2 ? 2.0 * d.kin_coef[2] : 0.0;
+  } else {
+    a.kin_linear = 0;
+  }
+  return a;
+}
+
+int xsweep(wh_ensemble* e, int flags, hipStream_t s) {
+  wh::SweepArgs a = sweep_args(e, flags, 1.0, 0);
+  a.work = e->work_x.buf.as<unsigned>();
+  WH_LAUNCH(e->d.precision == WH_C128 ? wh::launch_xsweep_f64(e->xvar, a, e->work_x.n, s)
+                                      : wh::launch_xsweep_f32(e->xvar, a, e->work_x.n, s),
+            "xsweep");
+  return WH_SUCCESS;
+}
+
+int psweep(wh_ensemble* e, int flags, double frac, int step_add, hipStream_t s) {
+  wh::SweepArgs a = sweep_args(e, flags, frac, step_add);
+  a.work = e->work_p.buf.as<unsigned>();
+  const bool cl = (flags & wh::kCL) != 0;
+  const int v = cl ? e->pclvar : e->pvar;
+  WH_LAUNCH(e->d.precision == WH_C128 ? wh::launch_psweep_f64(v, cl, a, e->work_p.n, s)
+                                      : wh::launch_psweep_f32(v, cl, a, e->work_p.n, s),
+            "psweep");
+  return WH_SUCCESS;
+}
+
+int reduce(wh_ensemble* e, int renorm, int record, int rho, int update_scale, bool inc, hipStream_t s) {
+  wh::ReduceArgs r{};
+  r.members = e->mem_dev.as<wh::Member>();
+  r.renorm = renorm;
+  r.record = record;
+  r.rho = rho;
+  r.update_scale = update_scale;
+  r.step = e->step.as<int>();
+  r.step_inc = inc ? e->step.as<int>() : nullptr;
+  WH_LAUNCH(wh::launch_reduce(r, e->n, e->done.as<int>(), s), "reduce");
+  return WH_SUCCESS;
+}
+
+int gpshift(wh_ensemble* e, hipStream_t s) {
+  WH_LAUNCH(wh::launch_gpshift(e->mem_dev.as<wh::Member>(), e->work_gp.buf.as<unsigned>(), e->work_gp.n, s),
+            "gpshift");
+  return WH_SUCCESS;
+}
+
+// One iteration after the first observable row (all members).
+//  Lie    (GPU_Wigner2D_FFT.py:1843-1913): [GP terms] K (pending factor folded into the
+//         load), V [+GP] [+CL] + column sums, reduce (record row, factor, step += 1).
+//  Strang (GP.py:1898-1999): without GP the second half step of iteration k and the
+//         first of k+1 share one p sweep, the observables taken between them; with GP
+//         V/2, K + partial column sums, density, V/2 + column sums.
+int iteration(wh_ensemble* e, bool merge, hipStream_t s) {
+  const auto& d = e->d;
+  const int cl = e->cl() ? wh::kCL : 0;
+  const int renorm = d.renormalize;
+  if (d.scheme == WH_SCHEME_LIE) {
+    if (e->gp()) WH_TRY(gpshift(e, s));
+    WH_TRY(xsweep(e, wh::kScaleOnLoad, s));
+    WH_TRY(psweep(e, wh::kColSums | cl | (e->gp() ? wh::kGP : 0), 1.0, 0, s));
+    WH_TRY(reduce(e, renorm, 1, e->gp(), 1, true, s));
+  } else if (!e->gp()) {
+    WH_TRY(xsweep(e, wh::kScaleOnLoad, s));
+    WH_TRY(psweep(e, wh::kColSums | cl | (merge ? wh::kMerge : 0), 0.5, 0, s));
+    WH_TRY(reduce(e, renorm, 1, 0, 1, true, s));
+  } else {
+    WH_TRY(gpshift(e, s));
+    WH_TRY(psweep(e, wh::kScaleOnLoad | wh::kGP, 0.5, 0, s));
+    WH_TRY(xsweep(e, wh::kColPart, s));
+    WH_LAUNCH(wh::launch_colpart_reduce(e->mem_dev.as<wh::Member>(), e->n, wh::kXVariants[e->xvar].R,
+                                        wh::kXVariants[e->xvar].block, e->step.as<int>(), s),
+              "colpart_reduce");
+    WH_TRY(gpshift(e, s));
+    WH_TRY(psweep(e, wh::kGP | wh::kColSums | cl, 0.5, 0, s));
+    WH_TRY(reduce(e, renorm, 1, 1, 1, true, s));
+  }
+  return WH_SUCCESS;
+}
+
+int iterations(wh_ensemble* e, int count, bool merge_last, hipStream_t s) {
+  for (int k = 0; k < count; ++k) WH_TRY(iteration(e, (k < count - 1) || merge_last, s));
+  return WH_SUCCESS;
+}
+
+// Replays `count` middle iterations from a captured graph of graph_chunk iterations.
+// The graph reads member pointers, the step counter and the record base from
+// device memory, so one capture serves every later run of the plan.
+int run_graph(wh_ensemble* e, int count, hipStream_t s) {
+  const int chunk = e->graph_chunk;
+  if (count >= chunk) {
+    auto key = std::make_tuple(s, chunk);
+    auto it = e->graphs.find(key);
+    if (it == e->graphs.end()) {
+      if (!e->capture_stream)
+        WH_HIP(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking), "hipStreamCreate");
+      GraphEntry g;
+      WH_HIP(hipStreamBeginCapture(e->capture_stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
+      const int rc = iterations(e, chunk, true, e->capture_stream);
+      const hipError_t err = hipStreamEndCapture(e->capture_stream, &g.graph);
+      if (rc) {
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+        return rc;
+      }
+      if (err != hipSuccess) return hip_fail(err, "EndCapture");
+      WH_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "GraphInstantiate");
+      it = e->graphs.emplace(key, g).first;
+    }
+    while (count >= chunk) {
+      WH_HIP(hipGraphLaunch(it->second.exec, s), "GraphLaunch");
+      count -= chunk;
+    }
+  }
+  return iterations(e, count, true, s);
+}
+
+int sync_table(wh_ensemble* e, bool changed, hipStream_t s) {
+  if (changed || e->table_dirty) {
+    WH_HIP(hipMemcpyAsync(e->mem_dev.p, e->mem.data(), sizeof(wh::Member) * e->mem.size(), hipMemcpyHostToDevice, s),
+           "member table upload");
+    WH_HIP(hipStreamSynchronize(s), "member table upload");
+    e->table_dirty = false;
+  }
+  return WH_SUCCESS;
+}
+
+int ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_steps, double* obs,
+                 int normalize_initial, int use_graph, hipStream_t s) {
+  if (!states) return fail(WH_INVALID_VALUE, "states is NULL");
+  if (n_steps < 0 || first_step < 0) return fail(WH_INVALID_VALUE, "n_steps %d first_step %d", n_steps, first_step);
+  if (!obs) return fail(WH_INVALID_VALUE, "obs is NULL");
+  const auto& d = e->d;
+  const size_t align = d.precision == WH_C128 ? 16 : 8;
+  bool changed = false;
+  for (int i = 0; i < e->n; ++i) {
+    if (!states[i]) return fail(WH_INVALID_VALUE, "state %d is NULL", i);
+    if (reinterpret_cast<uintptr_t>(states[i]) % align) return fail(WH_UNALIGNED_DATA, "state %d: %p", i, states[i]);
+    wh::Member& m = e->mem[i];
+    double* o = obs + size_t(i) * size_t(n_steps + 1) * wh::kNObs;
+    // Lie rows are pre-step states: row k = state before iteration first_step + k,
+    // written by the reduce that follows iteration first_step + k - 1
+    const int base = d.scheme == WH_SCHEME_LIE ? first_step - 1 : first_step;
+    if (m.W != states[i] || m.obs != o || m.obs_base != base) changed = true;
+    m.W = states[i];
+    m.obs = o;
+    m.obs_base = base;
+  }
+  WH_TRY(sync_table(e, changed, s));
+  const int prec = d.precision;
+  const wh::Member* md = e->mem_dev.as<wh::Member>();
+  // the state before the first iteration: norm, renormalisation factor, (Lie) row 0
+  WH_LAUNCH(wh::launch_colsum(prec, md, e->work_colsum.buf.as<unsigned>(), e->work_colsum.n, s), "colsum");
+  WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), d.scheme == WH_SCHEME_LIE ? first_step - 1 : first_step, s),
+            "set_step");
+  WH_TRY(reduce(e, normalize_initial, d.scheme == WH_SCHEME_LIE, e->gp(), 1, false, s));
+  WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), first_step, s), "set_step");
+  const bool merged_strang = d.scheme == WH_SCHEME_STRANG && !e->gp();
+  if (n_steps > 0 && merged_strang) {
+    // first half step of the first iteration, with the pending factor folded in
+    WH_TRY(psweep(e, wh::kScaleOnLoad, 0.5, 0, s));
+    WH_LAUNCH(wh::launch_set_scale(md, e->n, 1.0, 0.0, s), "set_scale");
+  }
+  if (n_steps > 0) {
+    if (merged_strang) {
+      WH_TRY(use_graph ? run_graph(e, n_steps - 1, s) : iterations(e, n_steps - 1, true, s));
+      WH_TRY(iterations(e, 1, false, s));  // last iteration: no next half step
+    } else {
+      WH_TRY(use_graph ? run_graph(e, n_steps, s) : iterations(e, n_steps, false, s));
+    }
+  }
+  if (d.renormalize || (n_steps == 0 && normalize_initial))
+    WH_LAUNCH(wh::launch_scale(prec, md, e->work_scale.buf.as<unsigned>(), e->work_scale.n, s), "scale");
+  return WH_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" int wh_ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members, int n_members,
+                                  wh_ensemble** out) {
+  return ensemble_create(desc, members, n_members, out);
+}
+
+extern "C" int wh_ensemble_destroy(wh_ensemble* e) {
+  if (!e) return fail(WH_INVALID_PLAN, "wh_ensemble_destroy: NULL");
+  delete e;
+  return WH_SUCCESS;
+}
+
+extern "C" int wh_ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_steps, double* obs,
+                               int normalize_initial, int use_graph, void* stream) {
+  if (!e) return fail(WH_INVALID_PLAN, "wh_ensemble_run: NULL");
+  return ensemble_run(e, states, first_step, n_steps, obs, normalize_initial, use_graph,
+                      static_cast<hipStream_t>(stream));
+}
+
+extern "C" int wh_ensemble_set_variants(wh_ensemble* e, int xvar, int pvar, int pclvar) {
+  if (!e) return fail(WH_INVALID_PLAN, "wh_ensemble_set_variants: NULL");
+  if (xvar >= wh::kNumX || pvar >= wh::kNumP || pclvar >= wh::kNumPCL)
+    return fail(WH_INVALID_VALUE, "variant index out of range (%d %d %d)", xvar, pvar, pclvar);
+  if (xvar >= 0 && !fits_all(e, wh::kXVariants[xvar], true))
+    return fail(WH_INVALID_VALUE, "x variant %d does not fit every member", xvar);
+  if (pvar >= 0 && !fits_all(e, wh::kPVariants[pvar], false))
+    return fail(WH_INVALID_VALUE, "p variant %d does not fit every member", pvar);
+  if (pclvar >= 0 && !fits_all(e, wh::kPCLVariants[pclvar], false))
+    return fail(WH_INVALID_VALUE, "p+CL variant %d does not fit every member", pclvar);
+  if (e->gp() && xvar >= 0 && xvar != e->xvar)
+    return fail(WH_INVALID_VALUE, "the x variant of a Gross-Pitaevskii plan is fixed at creation");
+  if (xvar >= 0) e->xvar = xvar;
+  if (pvar >= 0) e->pvar = pvar;
+  if (pclvar >= 0) e->pclvar = pclvar;
+  return build_work_lists(e);
+}
+
+// ============================================================================ single grid
+extern "C" int wh_wigner2d_create(const wh_wigner2d_desc* d, wh_wigner2d** out) {
   if (!out) return fail(WH_INVALID_VALUE, "wh_wigner2d_create: out is NULL");
   *out = nullptr;
-  int rc = validate_desc(desc);
-  if (rc) return rc;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-    return fail(WH_NO_DEVICE, "wh_wigner2d_create: no HIP device");
-  auto* p = new wh_wigner2d;
-  p->d = *desc;
-  const auto& d = *desc;
-  const size_t X = size_t(d.x_dim), P = size_t(d.p_dim);
-  std::vector<double> zeros(std::max(X, P), 0.0);
-#define TRY(expr)   \
-  do {              \
-    rc = (expr);    \
-    if (rc) {       \
-      delete p;     \
-      return rc;    \
-    }               \
-  } while (0)
-  TRY(make_twiddles(p->twX, d.x_dim, d.precision));
-  TRY(make_twiddles(p->twP, d.p_dim, d.precision));
-  TRY(p->pot.upload(d.pot_coef, size_t(d.pot_rows) * (WH_MAX_DEG + 1) * sizeof(double)));
-  TRY(upload_opt(p->theta_decay, d.theta_decay, P));
-  TRY(upload_opt(p->lambda_decay, d.lambda_decay, X));
-  TRY(upload_opt(p->x_mask, d.x_mask, X));
-  TRY(upload_opt(p->sigma_p, d.sigma_p, P));
-  TRY(upload_opt(p->theta_host, d.theta_host, P));
-  TRY(upload_opt(p->lambda_host, d.lambda_host, X));
-  TRY(p->x_host.upload(d.x_host, X * sizeof(double)));
-  TRY(p->p_host.upload(d.p_host, P * sizeof(double)));
-  TRY(p->dv_host.upload(d.dv_host ? d.dv_host : zeros.data(), X * sizeof(double)));
-  TRY(p->v0_host.upload(d.v0_host ? d.v0_host : zeros.data(), X * sizeof(double)));
-  TRY(p->colsums.alloc(X * wh::kNCol * sizeof(double)));
-  TRY(p->scale.alloc(4 * sizeof(double)));
-  TRY(p->step.alloc(4 * sizeof(int)));
-  if (d.gp_coeff != 0.0) {
-    TRY(make_twiddles(p->twX64, d.x_dim, WH_C128));
-    TRY(p->rho.alloc(X * sizeof(double)));
-    TRY(p->G.alloc(X * P * sizeof(double)));
-    TRY(p->colpart.alloc((P / size_t(wh::xsweep_rows(d.x_dim))) * X * sizeof(double)));
-  }
-#undef TRY
-  *out = p;
+  if (!d) return fail(WH_INVALID_VALUE, "descriptor is NULL");
+  if (d->struct_size != (int)sizeof(wh_wigner2d_desc))
+    return fail(WH_INVALID_DESCRIPTOR, "descriptor struct_size %d != %d", d->struct_size, (int)sizeof(wh_wigner2d_desc));
+  wh_ensemble_desc c{};
+  c.struct_size = sizeof(c);
+  c.precision = d->precision;
+  c.scheme = d->scheme;
+  c.dt = d->dt;
+  c.hbar = d->hbar;
+  c.mass = d->mass;
+  c.kinetic_poly = d->kinetic_poly;
+  c.kinetic_deg = d->kinetic_deg;
+  std::memcpy(c.kin_coef, d->kin_coef, sizeof(c.kin_coef));
+  c.damping = d->damping;
+  c.gamma_dt = d->gamma_dt;
+  c.renormalize = d->renormalize;
+  c.gp_coeff = d->gp_coeff;
+  wh_member_desc m{};
+  m.struct_size = sizeof(m);
+  m.x_dim = d->x_dim;
+  m.p_dim = d->p_dim;
+  m.dx = d->dx;
+  m.dp = d->dp;
+  m.dtheta = d->dtheta;
+  m.dlambda = d->dlambda;
+  m.pot_deg = d->pot_deg;
+  m.pot_rows = d->pot_rows;
+  m.pot_coef = d->pot_coef;
+  m.theta_decay = d->theta_decay;
+  m.lambda_decay = d->lambda_decay;
+  m.x_mask = d->x_mask;
+  m.sigma_p = d->sigma_p;
+  m.theta_host = d->theta_host;
+  m.lambda_host = d->lambda_host;
+  m.x_host = d->x_host;
+  m.p_host = d->p_host;
+  m.dv_host = d->dv_host;
+  m.v0_host = d->v0_host;
+  m.dxdp = d->dxdp;
+  m.inv_2m = d->inv_2m;
+  m.dx_host = d->dx_host;
+  m.gp_rho_factor = d->gp_rho_factor;
+  auto p = std::make_unique<wh_wigner2d>();
+  WH_TRY(ensemble_create(&c, &m, 1, &p->e));
+  *out = p.release();
   return WH_SUCCESS;
 }
 
@@ -344,290 +802,113 @@ extern "C" int wh_wigner2d_destroy(wh_wigner2d* plan) {
   return WH_SUCCESS;
 }
 
-namespace {
-
-wh::XSweepArgs xargs(wh_wigner2d* p, void* W, const double* scale) {
-  const auto& d = p->d;
-  wh::XSweepArgs a{};
-  a.W = W;
-  a.X = d.x_dim;
-  a.P = d.p_dim;
-  a.tw = p->twX.p;
-  a.scale = scale;
-  a.kinetic_poly = d.kinetic_poly;
-  for (int n = 0; n <= wh::kMaxDeg; ++n) a.kcoef[n] = d.kin_coef[n];
-  a.kdeg = d.kinetic_deg;
-  a.dt = d.dt;
-  a.inv_mass = 1.0 / d.mass;
-  a.hbar = d.hbar;
-  a.inv_hbar = 1.0 / d.hbar;
-  a.dp = d.dp;
-  a.dlambda = d.dlambda;
-  a.lambda_decay = p->lambda_decay.as<double>();
-  a.colpart = nullptr;
-  return a;
-}
-
-wh::PSweepArgs pargs(wh_wigner2d* p, void* W, const double* scale, double frac) {
-  const auto& d = p->d;
-  wh::PSweepArgs a{};
-  a.W = W;
-  a.X = d.x_dim;
-  a.P = d.p_dim;
-  a.tw = p->twP.p;
-  a.scale = scale;
-  a.step = p->step.as<int>();
-  a.step_add = 0;
-  a.pot.coef = p->pot.as<double>();
-  a.pot.rows = d.pot_rows;
-  a.pot.deg = d.pot_deg;
-  a.vphase = frac * d.dt / d.hbar;
-  a.hbar_half = 0.5 * d.hbar;
-  a.dx = d.dx;
-  a.dtheta = d.dtheta;
-  a.theta_decay = p->theta_decay.as<double>();
-  a.xmask = p->x_mask.as<double>();
-  a.gp_G = nullptr;
-  a.gp_phase = -frac * d.dt * d.gp_coeff / d.hbar;
-  a.cl = 0;
-  a.sigma_p = p->sigma_p.as<double>();
-  a.theta_host = p->theta_host.as<double>();
-  a.cl_a = d.gamma_dt;
-  a.colsums = nullptr;
-  a.p_host = p->p_host.as<double>();
-  a.merge = 0;
-  return a;
-}
-
-wh::ReduceArgs rargs(wh_wigner2d* p, double* obs, int obs_base, int renorm) {
-  const auto& d = p->d;
-  wh::ReduceArgs r{};
-  r.colsums = p->colsums.as<double>();
-  r.X = d.x_dim;
-  r.x_host = p->x_host.as<double>();
-  r.dv_host = p->dv_host.as<double>();
-  r.v0_host = p->v0_host.as<double>();
-  r.dxdp = d.dxdp;
-  r.inv_2m = d.inv_2m;
-  r.renorm = renorm;
-  r.obs = obs;
-  r.obs_base = obs_base;
-  r.record = obs ? 1 : 0;
-  r.step = p->step.as<int>();
-  r.step_inc = nullptr;
-  r.scale_out = p->scale.as<double>();
-  r.rho_out = nullptr;
-  r.rho_factor = d.gp_rho_factor;
-  r.dx = d.dx_host;
-  return r;
-}
-
-#define WH_LAUNCH(expr, what)                     \
-  do {                                            \
-    hipError_t e_ = (expr);                       \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
-
-// ---- Lie:  [obs, K, V(+GP), CL, renorm] per iteration (GPU_Wigner2D_FFT.py:1843-1913)
-// One iteration after the first observable row: K sweep (with the pending
-// renormalisation factor folded into its load), V sweep (+ column sums),
-// reduce (record row, next factor, step counter += 1).
-int lie_iteration(wh_wigner2d* p, void* W, double* obs, int first_step, hipStream_t s) {
-  const auto& d = p->d;
-  const int prec = d.precision;
-  const bool gp = d.gp_coeff != 0.0;
-  if (gp) {
-    // shifted densities of the pre-step state (FFT.py:1845-1848)
-    WH_LAUNCH(wh::launch_gp_shift(d.x_dim, d.p_dim, p->rho.as<double>(),
-                                  p->lambda_host.as<double>(), p->theta_host.as<double>(),
-                                  p->G.as<double>(), p->twX64.p, s),
-              "gp_shift");
-  }
-  WH_LAUNCH(wh::launch_xsweep(prec, xargs(p, W, p->scale.as<double>()), s), "xsweep");
-  wh::PSweepArgs pa = pargs(p, W, nullptr, 1.0);
-  pa.colsums = p->colsums.as<double>();
-  pa.cl = d.damping == 1;
-  if (gp) pa.gp_G = p->G.as<double>();
-  WH_LAUNCH(wh::launch_psweep(prec, pa, s), "psweep");
-  wh::ReduceArgs r = rargs(p, obs, first_step - 1, d.renormalize);
-  r.step_inc = p->step.as<int>();
-  if (gp) r.rho_out = p->rho.as<double>();
-  WH_LAUNCH(wh::launch_reduce(r, s), "reduce");
-  return WH_SUCCESS;
-}
-
-// ---- Strang: V/2, K, V/2, CL, renorm, obs (GP.py:1898-1999).  Without GP the
-// second half step of iteration k and the first of k+1 share one p sweep, the
-// observables being taken between them.
-int strang_iteration(wh_wigner2d* p, void* W, double* obs, int first_step, bool merge,
-                     hipStream_t s) {
-  const auto& d = p->d;
-  const int prec = d.precision;
-  const bool gp = d.gp_coeff != 0.0;
-  if (!gp) {
-    WH_LAUNCH(wh::launch_xsweep(prec, xargs(p, W, p->scale.as<double>()), s), "xsweep");
-    wh::PSweepArgs pa = pargs(p, W, nullptr, 0.5);
-    pa.colsums = p->colsums.as<double>();
-    pa.cl = d.damping == 1;
-    pa.merge = merge ? 1 : 0;
-    WH_LAUNCH(wh::launch_psweep(prec, pa, s), "psweep");
-  } else {
-    // V/2 + GP/2 with the density of the current state (rho, G prepared by the last reduce)
-    WH_LAUNCH(wh::launch_gp_shift(d.x_dim, d.p_dim, p->rho.as<double>(),
-                                  p->lambda_host.as<double>(), p->theta_host.as<double>(),
-                                  p->G.as<double>(), p->twX64.p, s),
-              "gp_shift");
-    wh::PSweepArgs pa = pargs(p, W, p->scale.as<double>(), 0.5);
-    pa.gp_G = p->G.as<double>();
-    WH_LAUNCH(wh::launch_psweep(prec, pa, s), "psweep V/2");
-    // K with partial column sums -> density after the kinetic step (GP.py:1933-1939)
-    wh::XSweepArgs xa = xargs(p, W, nullptr);
-    xa.colpart = p->colpart.as<double>();
-    WH_LAUNCH(wh::launch_xsweep(prec, xa, s), "xsweep");
-    const int nblk = d.p_dim / wh::xsweep_rows(d.x_dim);
-    // density after K and the ProbabilitySquare record of this iteration
-    WH_LAUNCH(wh::launch_colpart_reduce(p->colpart.as<double>(), nblk, d.x_dim, d.gp_rho_factor,
-                                        p->rho.as<double>(), obs, first_step, p->step.as<int>(),
-                                        d.dx_host, s),
-              "colpart_reduce");
-    WH_LAUNCH(wh::launch_gp_shift(d.x_dim, d.p_dim, p->rho.as<double>(),
-                                  p->lambda_host.as<double>(), p->theta_host.as<double>(),
-                                  p->G.as<double>(), p->twX64.p, s),
-              "gp_shift");
-    wh::PSweepArgs pb = pargs(p, W, nullptr, 0.5);
-    pb.gp_G = p->G.as<double>();
-    pb.colsums = p->colsums.as<double>();
-    pb.cl = d.damping == 1;
-    WH_LAUNCH(wh::launch_psweep(prec, pb, s), "psweep V/2");
-  }
-  wh::ReduceArgs r = rargs(p, obs, first_step, d.renormalize);
-  r.step_inc = p->step.as<int>();
-  if (gp) r.rho_out = p->rho.as<double>();
-  WH_LAUNCH(wh::launch_reduce(r, s), "reduce");
-  return WH_SUCCESS;
-}
-
-int run_iterations(wh_wigner2d* p, void* W, double* obs, int first_step, int count, bool merge_last,
-                   hipStream_t s) {
-  for (int k = 0; k < count; ++k) {
-    int rc;
-    if (p->d.scheme == WH_SCHEME_LIE)
-      rc = lie_iteration(p, W, obs, first_step, s);
-    else
-      rc = strang_iteration(p, W, obs, first_step, (k < count - 1) || merge_last, s);
-    if (rc) return rc;
-  }
-  return WH_SUCCESS;
-}
-
-// Replays `count` iterations (all with the merge flag of the middle of a run)
-// from a captured graph of p->graph_chunk iterations.
-int run_graph(wh_wigner2d* p, void* W, double* obs, int first_step, int count, hipStream_t s) {
-  const int chunk = p->graph_chunk;
-  if (count >= chunk) {
-    GraphKey key{W, obs, first_step, s};
-    auto it = p->graphs.find(key);
-    if (it == p->graphs.end()) {
-      if (!p->capture_stream)
-        WH_HIP(hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking), "hipStreamCreate");
-      GraphEntry g;
-      WH_HIP(hipStreamBeginCapture(p->capture_stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
-      int rc = run_iterations(p, W, obs, first_step, chunk, true, p->capture_stream);
-      hipError_t e = hipStreamEndCapture(p->capture_stream, &g.graph);
-      if (rc) return rc;
-      if (e != hipSuccess) return hip_fail(e, "EndCapture");
-      WH_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "GraphInstantiate");
-      g.steps = chunk;
-      it = p->graphs.emplace(key, g).first;
-    }
-    while (count >= chunk) {
-      WH_HIP(hipGraphLaunch(it->second.exec, s), "GraphLaunch");
-      count -= chunk;
-    }
-  }
-  return run_iterations(p, W, obs, first_step, count, true, s);
-}
-
-}  // namespace
+extern "C" wh_ensemble* wh_wigner2d_ensemble(wh_wigner2d* plan) { return plan ? plan->e : nullptr; }
 
 extern "C" int wh_wigner2d_run(wh_wigner2d* p, void* W, int first_step, int n_steps, double* obs,
                                int normalize_initial, int use_graph, void* stream) {
   if (!p) return fail(WH_INVALID_PLAN, "wh_wigner2d_run: plan is NULL");
-  if (!W) return fail(WH_INVALID_VALUE, "wh_wigner2d_run: W is NULL");
-  if (n_steps < 0 || first_step < 0) return fail(WH_INVALID_VALUE, "wh_wigner2d_run: n_steps %d first_step %d", n_steps, first_step);
-  const size_t align = p->d.precision == WH_C128 ? 16 : 8;
-  if (reinterpret_cast<uintptr_t>(W) % align) return fail(WH_UNALIGNED_DATA, "W not %zu-byte aligned", align);
-  const auto& d = p->d;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const int prec = d.precision;
-  const size_t cells = size_t(d.x_dim) * size_t(d.p_dim);
-  WH_LAUNCH(wh::launch_set_int(p->step.as<int>(), first_step, s), "set_step");
-  // state before the first iteration: norm (and Lie row 0)
-  WH_LAUNCH(wh::launch_colsum(prec, W, d.x_dim, d.p_dim, p->p_host.as<double>(), p->colsums.as<double>(), s), "colsum");
-  {
-    const bool lie = d.scheme == WH_SCHEME_LIE;
-    wh::ReduceArgs r = rargs(p, lie ? obs : nullptr, first_step, normalize_initial);
-    if (d.gp_coeff != 0.0) r.rho_out = p->rho.as<double>();
-    WH_LAUNCH(wh::launch_reduce(r, s), "reduce");
-  }
-  if (n_steps == 0) {
-    if (normalize_initial) WH_LAUNCH(wh::launch_scale(prec, W, cells, p->scale.as<double>(), s), "scale");
-    return WH_SUCCESS;
-  }
-  if (d.scheme == WH_SCHEME_STRANG && d.gp_coeff == 0.0) {
-    // first half step of the first iteration, then the merged middle
-    wh::PSweepArgs pa = pargs(p, W, p->scale.as<double>(), 0.5);
-    WH_LAUNCH(wh::launch_psweep(prec, pa, s), "psweep first half");
-    WH_LAUNCH(wh::launch_set_scale(p->scale.as<double>(), 1.0, 0.0, s), "set_scale");
-  }
-  int rc;
-  if (d.scheme == WH_SCHEME_STRANG && d.gp_coeff == 0.0) {
-    if (use_graph)
-      rc = run_graph(p, W, obs, first_step, n_steps - 1, s);
-    else
-      rc = run_iterations(p, W, obs, first_step, n_steps - 1, true, s);
-    if (rc) return rc;
-    rc = run_iterations(p, W, obs, first_step, 1, false, s);  // last iteration: no merge
-  } else {
-    rc = use_graph ? run_graph(p, W, obs, first_step, n_steps, s)
-                   : run_iterations(p, W, obs, first_step, n_steps, false, s);
-  }
-  if (rc) return rc;
-  if (d.renormalize) WH_LAUNCH(wh::launch_scale(prec, W, cells, p->scale.as<double>(), s), "scale");
-  return WH_SUCCESS;
+  void* states[1] = {W};
+  return ensemble_run(p->e, states, first_step, n_steps, obs, normalize_initial, use_graph,
+                      static_cast<hipStream_t>(stream));
 }
+
+namespace {
+int bind_single(wh_ensemble* e, void* W, double* obs, int obs_base, hipStream_t s) {
+  wh::Member& m = e->mem[0];
+  const bool changed = m.W != W || m.obs != obs || m.obs_base != obs_base;
+  m.W = W;
+  m.obs = obs;
+  m.obs_base = obs_base;
+  return sync_table(e, changed, s);
+}
+}  // namespace
 
 extern "C" int wh_wigner2d_kinetic(wh_wigner2d* p, void* W, void* stream) {
   if (!p) return fail(WH_INVALID_PLAN, "wh_wigner2d_kinetic: plan is NULL");
   if (!W) return fail(WH_INVALID_VALUE, "wh_wigner2d_kinetic: W is NULL");
-  WH_LAUNCH(wh::launch_xsweep(p->d.precision, xargs(p, W, nullptr), static_cast<hipStream_t>(stream)), "xsweep");
-  return WH_SUCCESS;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  WH_TRY(bind_single(p->e, W, p->e->mem[0].obs, p->e->mem[0].obs_base, s));
+  return xsweep(p->e, 0, s);
 }
 
 extern "C" int wh_wigner2d_potential(wh_wigner2d* p, void* W, int step, int fused, void* stream) {
   if (!p) return fail(WH_INVALID_PLAN, "wh_wigner2d_potential: plan is NULL");
   if (!W || step < 0) return fail(WH_INVALID_VALUE, "wh_wigner2d_potential: W %p step %d", W, step);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  WH_LAUNCH(wh::launch_set_int(p->step.as<int>(), step, s), "set_step");
-  const double frac = p->d.scheme == WH_SCHEME_LIE ? 1.0 : 0.5;
-  wh::PSweepArgs pa = pargs(p, W, nullptr, frac);
-  if (fused) {
-    pa.colsums = p->colsums.as<double>();
-    pa.cl = p->d.damping == 1;
+  wh_ensemble* e = p->e;
+  WH_TRY(bind_single(e, W, e->mem[0].obs, e->mem[0].obs_base, s));
+  WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), step, s), "set_step");
+  const double frac = e->d.scheme == WH_SCHEME_LIE ? 1.0 : 0.5;
+  const int flags = fused ? (wh::kColSums | (e->cl() ? wh::kCL : 0)) : 0;
+  return psweep(e, flags, frac, 0, s);
+}
+
+extern "C" int wh_ensemble_set_potential(wh_ensemble* e, int member, const double* coef, int rows, int deg,
+                                         void* stream) {
+  if (!e) return fail(WH_INVALID_PLAN, "wh_ensemble_set_potential: NULL");
+  if (member < 0 || member >= e->n || !coef || rows < 1 || deg < 0 || deg > WH_MAX_DEG)
+    return fail(WH_INVALID_VALUE, "wh_ensemble_set_potential: member %d coef %p rows %d deg %d", member,
+                (const void*)coef, rows, deg);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  wh::Member& m = e->mem[member];
+  MemberBufs& b = *e->bufs[member];
+  const size_t bytes = size_t(rows) * (WH_MAX_DEG + 1) * sizeof(double);
+  bool changed = false;
+  if (b.pot.bytes != bytes) {
+    WH_HIP(hipStreamSynchronize(s), "sync");
+    WH_TRY(b.pot.alloc(bytes));
+    m.pot = b.pot.as<double>();
+    changed = true;
   }
-  WH_LAUNCH(wh::launch_psweep(p->d.precision, pa, s), "psweep");
-  return WH_SUCCESS;
+  WH_HIP(hipMemcpyAsync(b.pot.p, coef, bytes, hipMemcpyDefault, s), "potential upload");
+  changed = changed || m.pot_rows != rows || m.pot_deg != deg;
+  m.pot_rows = rows;
+  m.pot_deg = deg;
+  return sync_table(e, changed, s);
+}
+
+extern "C" int wh_wigner2d_adjoint_step(wh_wigner2d* p, void* Lambda, const void* W_next, int step, double* grad,
+                                        void* stream) {
+  if (!p) return fail(WH_INVALID_PLAN, "wh_wigner2d_adjoint_step: plan is NULL");
+  if (!Lambda || !W_next || !grad || step < 0)
+    return fail(WH_INVALID_VALUE, "wh_wigner2d_adjoint_step: Lambda %p W_next %p grad %p step %d", Lambda, W_next,
+                (void*)grad, step);
+  wh_ensemble* e = p->e;
+  if (e->d.scheme != WH_SCHEME_LIE || e->cl() || e->gp())
+    return fail(WH_INVALID_VALUE, "the adjoint step covers the Lie scheme without damping or Gross-Pitaevskii");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  wh::Member& m = e->mem[0];
+  if (!e->work_adj.n) {
+    const int cols = wh::adj_cols(m.P);
+    if (cols < 1 || cols > 64 || m.X % cols != 0 || m.P < 128)
+      return fail(WH_INVALID_SIZE, "adjoint sweep does not fit a %d x %d grid", m.P, m.X);
+    std::vector<unsigned> w;
+    for (int t = 0; t < m.X / cols; ++t) w.push_back(unsigned(t));
+    WH_TRY(e->work_adj.upload(w));
+    WH_TRY(e->gcols.alloc(size_t(m.X) * (wh::kMaxDeg + 1) * sizeof(double)));
+  }
+  const bool changed = m.W != Lambda || m.W2 != W_next || m.gcols != e->gcols.as<double>();
+  m.W = Lambda;
+  m.W2 = W_next;
+  m.gcols = e->gcols.as<double>();
+  WH_TRY(sync_table(e, changed, s));
+  WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), step, s), "set_step");
+  wh::SweepArgs a = sweep_args(e, 0, 1.0, 0);
+  a.work = e->work_adj.buf.as<unsigned>();
+  WH_LAUNCH(wh::launch_adj_psweep(e->d.precision, a, e->work_adj.n, s), "adj_psweep");
+  WH_LAUNCH(wh::launch_grad_reduce(e->mem_dev.as<wh::Member>(), 1, grad, a.vphase / double(m.P), s), "grad_reduce");
+  return xsweep(e, wh::kConj, s);
 }
 
 extern "C" int wh_wigner2d_observe(wh_wigner2d* p, const void* W, double* obs_row, void* stream) {
   if (!p) return fail(WH_INVALID_PLAN, "wh_wigner2d_observe: plan is NULL");
   if (!W || !obs_row) return fail(WH_INVALID_VALUE, "wh_wigner2d_observe: W %p obs %p", W, (void*)obs_row);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const auto& d = p->d;
-  WH_LAUNCH(wh::launch_set_int(p->step.as<int>(), 0, s), "set_step");
-  WH_LAUNCH(wh::launch_colsum(d.precision, W, d.x_dim, d.p_dim, p->p_host.as<double>(), p->colsums.as<double>(), s), "colsum");
-  wh::ReduceArgs r = rargs(p, obs_row, 0, 0);
-  r.scale_out = nullptr;
-  WH_LAUNCH(wh::launch_reduce(r, s), "reduce");
-  return WH_SUCCESS;
+  wh_ensemble* e = p->e;
+  WH_TRY(bind_single(e, const_cast<void*>(W), obs_row, 0, s));
+  WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), 0, s), "set_step");
+  WH_LAUNCH(wh::launch_colsum(e->d.precision, e->mem_dev.as<wh::Member>(), e->work_colsum.buf.as<unsigned>(),
+                              e->work_colsum.n, s),
+            "colsum");
+  return reduce(e, 0, 1, 0, 0, false, s);
 }

@@ -3,19 +3,25 @@
 // A "team" of TT = N/R lanes computes one length-N complex transform.  Lane t
 // holds R elements in registers in the strided layout  v[j] = a[t + j*TT].
 // The transform is a Stockham autosort sequence of radix passes [R, R, ..., r]
-// (r = N / R^k).  The first pass reads the registers directly, every later pass
-// exchanges through LDS, and the last pass writes back into the same strided
-// register layout, so a forward transform, a pointwise multiply and an inverse
-// transform chain in registers without extra LDS traffic, and global loads and
-// stores of the strided layout are coalesced for both row and column teams.
+// (r = N / R^k).  The first pass works on the registers directly, every later
+// pass exchanges through LDS, and the last pass writes back into the same
+// strided register layout.  So a forward transform, a pointwise multiply and an
+// inverse transform chain in registers, and global loads and stores of the
+// strided layout are coalesced for row teams and column teams alike.
 //
-// Pass with radix r after the spans Ns = prod(previous radices):
+// Pass with radix r after the span Ns = prod(previous radices):
 //   butterfly b in [0, N/r) reads a[b + j*N/r], twiddles by w^(j*(b mod Ns)),
 //   w = exp(dir*2*pi*i/(Ns*r)), applies an r-point DFT and writes
 //   a'[(b - b mod Ns)*r + (b mod Ns) + q*Ns].
 // Twiddles come from a table tw[n] = exp(-2*pi*i*n/N) computed in FP64 on the
 // host.  Forward = exponent -i (dir = -1), inverse = +i and unnormalised, as
 // cufftExecZ2Z (cufft_wrapper.py:11-12, 81-103).
+//
+// FP32: every constant the transform multiplies by (table twiddles and the
+// in-register radix-16 roots) is carried as a hi + lo float pair, so the
+// transform's operator is exact to ~2^-48 and only value-dependent rounding
+// remains: a fixed perturbation of the step operator would accumulate linearly
+// over thousands of unitary steps, rounding noise only as a random walk.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -33,11 +39,6 @@ __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.x - b.
 template <typename T>
 __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
   return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-// a * conj(b)
-template <typename T>
-__device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {
-  return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
 }
 template <typename T>
 __device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) { return {a.x * s, a.y * s}; }
@@ -63,7 +64,31 @@ constexpr double cos16(int k) {
   return t[k & 15];
 }
 
-// a * exp(DIR * 2*pi*i * K / M), M | 16; all special angles are exact.
+// a * (c + i s) with constants known at compile time; FP32 adds the products
+// with the low parts of the constants.
+template <typename T>
+struct ConstMul;
+template <>
+struct ConstMul<double> {
+  template <int K, int DIR>
+  __device__ __forceinline__ static cplx<double> mul(cplx<double> a) {
+    constexpr double C = cos16(K), S = (DIR < 0 ? -1.0 : 1.0) * cos16(K + 12);
+    return {a.x * C - a.y * S, a.x * S + a.y * C};
+  }
+};
+template <>
+struct ConstMul<float> {
+  template <int K, int DIR>
+  __device__ __forceinline__ static cplx<float> mul(cplx<float> a) {
+    constexpr double C = cos16(K), S = (DIR < 0 ? -1.0 : 1.0) * cos16(K + 12);
+    constexpr float ch = float(C), sh = float(S);
+    constexpr float cl = float(C - double(ch)), sl = float(S - double(sh));
+    const float rx = fmaf(a.x, ch, -a.y * sh), ry = fmaf(a.x, sh, a.y * ch);
+    return {rx + fmaf(a.x, cl, -a.y * sl), ry + fmaf(a.x, sl, a.y * cl)};
+  }
+};
+
+// a * exp(DIR * 2*pi*i * K / M), M | 16; the special angles are exact.
 template <int DIR, int K, int M, typename T>
 __device__ __forceinline__ cplx<T> rot(cplx<T> a) {
   constexpr int k = ((K % M) * (16 / M)) & 15;
@@ -76,11 +101,7 @@ __device__ __forceinline__ cplx<T> rot(cplx<T> a) {
   } else if constexpr (k == 4 || k == 12) {
     return {-a.y, a.x};  // * (+i)
   } else {
-    constexpr double c = cos16(k);
-    constexpr double s0 = cos16(k + 12);  // sin(2 pi k/16) = cos(2 pi (k-4)/16)
-    constexpr double s = DIR < 0 ? -s0 : s0;
-    const T cc = T(c), ss = T(s);
-    return {a.x * cc - a.y * ss, a.x * ss + a.y * cc};
+    return ConstMul<T>::template mul<k, DIR>(a);
   }
 }
 
@@ -118,32 +139,33 @@ __device__ __forceinline__ void dft(cplx<T>* x) {
   for (int q = 0; q < r; ++q) x[q] = y[q];
 }
 
-// LDS index with one pad slot every 16 entries (spreads the radix-16 strided
-// writes over the banks).
-__device__ __forceinline__ int pad16(int i) { return i + (i >> 4); }
-template <int N>
+// LDS index with one pad slot every PR entries (PR = the team's radix): spreads
+// the radix-strided writes of a pass over the banks.
+template <int PR>
+__device__ __forceinline__ int padr(int i) { return i + i / PR; }
+template <int N, int PR>
 struct Padded {
-  static constexpr int size = N + (N >> 4) + 1;
+  static constexpr int size = N + N / PR + 1;
 };
 
 // LDS views of one team's working array.  ptr(pb) is the address of padded
 // index pb; S is the element stride between consecutive padded indices.
 template <typename T>
-struct LdsRow {  // contiguous, padded: a[i] at base[pad16(i)]
+struct LdsRow {  // contiguous: a[i] at base[padr(i)]
   static constexpr int S = 1;
   cplx<T>* base;
   __device__ __forceinline__ cplx<T>* ptr(int pb) const { return base + pb; }
 };
 template <typename T, int LS>
-struct LdsStrided {  // interleaved columns: a[i] at base[pad16(i) * LS]
+struct LdsStrided {  // interleaved columns: a[i] at base[padr(i) * LS]
   static constexpr int S = LS;
   cplx<T>* base;
   __device__ __forceinline__ cplx<T>* ptr(int pb) const { return base + pb * LS; }
 };
 
-// Hides the value of an expression from the optimiser, so that the per-lane
-// addresses of one transform are not merged with (and kept live until) those of
-// the next transform in the same kernel.
+// Hides the value of an expression from the optimiser, so that per-lane
+// addresses and indices of one transform are not merged with (and kept live
+// until) those of the next transform in the same kernel.
 template <typename P>
 __device__ __forceinline__ P opaque_s(P p) {
   asm volatile("" : "+s"(p));
@@ -154,25 +176,41 @@ __device__ __forceinline__ int opaque_v(int v) {
   return v;
 }
 
+// Twiddle tables: FP64 hi only; FP32 hi and lo (the rounding error of hi).
 template <typename T>
-__device__ __forceinline__ cplx<T> twiddle(const cplx<T>* __restrict__ tw, int idx, int dir) {
-  cplx<T> w = tw[idx];
-  if (dir > 0) w.y = -w.y;
-  return w;
+struct Twiddles {
+  const cplx<T>* hi;
+  const cplx<T>* lo;
+};
+
+template <int DIR>
+__device__ __forceinline__ cplx<double> tw_mul(cplx<double> x, const Twiddles<double>& tw, int idx) {
+  cplx<double> w = tw.hi[idx];
+  if (DIR > 0) w.y = -w.y;
+  return cmul(x, w);
+}
+template <int DIR>
+__device__ __forceinline__ cplx<float> tw_mul(cplx<float> x, const Twiddles<float>& tw, int idx) {
+  cplx<float> w = tw.hi[idx], l = tw.lo[idx];
+  if (DIR > 0) {
+    w.y = -w.y;
+    l.y = -l.y;
+  }
+  const float rx = fmaf(x.x, w.x, -x.y * w.y), ry = fmaf(x.x, w.y, x.y * w.x);
+  return {rx + fmaf(x.x, l.x, -x.y * l.y), ry + fmaf(x.x, l.y, x.y * l.x)};
 }
 
 // One Stockham pass P >= 1 (LDS-sourced) through the team's LDS view.
 //
-// Every LDS address is a per-lane base plus a compile-time offset: all spans
-// after the first pass are multiples of 16, so pad16(i + 16m) = pad16(i) + 17m,
-// and the first pass (radix 16, span 1) writes 16 consecutive indices of one
-// aligned group of 16, so pad16(16b + q) = pad16(16b) + q.
+// Every LDS address is a per-lane base plus a compile-time offset: spans after
+// the first pass are multiples of R, so padr(i + R m) = padr(i) + (R+1) m, and
+// the first pass (radix R, span 1) writes the R consecutive indices of one
+// aligned group of R, so padr(R b + q) = padr(R b) + q.
 template <typename T, int N, int R, int DIR, int P, class Lds>
 struct TeamPass {
   static constexpr int NP = num_passes(N, R);
   static constexpr int TT = N / R;
-  __device__ __forceinline__ static void run(cplx<T> (&v)[R], int t, Lds lds,
-                                             const cplx<T>* __restrict__ tw) {
+  __device__ __forceinline__ static void run(cplx<T> (&v)[R], int t, Lds lds, const Twiddles<T>& tw) {
     if constexpr (P < NP) {
       constexpr int rp = pass_radix(N, R, P - 1);  // radix of the previous pass
       constexpr int Nsp = ipow(R, P - 1);          // span before the previous pass
@@ -180,19 +218,19 @@ struct TeamPass {
       constexpr int Ns = ipow(R, P);
       constexpr int nbp = R / rp, nb = R / r;
       constexpr int S = Lds::S;
-      static_assert(Nsp % 16 == 0 || (Nsp == 1 && rp == 16), "separable LDS offsets");
-      static_assert((N / r) % 16 == 0, "separable LDS offsets");
+      static_assert(Nsp % R == 0 || (Nsp == 1 && rp == R), "separable LDS offsets");
+      static_assert((N / r) % R == 0, "separable LDS offsets");
       const int tl = opaque_v(t);
-      // write outputs of the previous pass
+      // write the outputs of the previous pass
       __syncthreads();
 #pragma unroll
       for (int mm = 0; mm < nbp; ++mm) {
         const int b = tl + mm * TT;
         const int k = b % Nsp;
-        cplx<T>* p = lds.ptr(pad16((b - k) * rp + k));
+        cplx<T>* p = lds.ptr(padr<R>((b - k) * rp + k));
 #pragma unroll
         for (int q = 0; q < rp; ++q) {
-          constexpr int unit = Nsp == 1 ? 1 : Nsp * 17 / 16;
+          constexpr int unit = Nsp == 1 ? 1 : Nsp / R * (R + 1);
           p[S * q * unit] = v[mm * rp + q];
         }
       }
@@ -202,13 +240,13 @@ struct TeamPass {
       for (int mm = 0; mm < nb; ++mm) {
         const int b = tl + mm * TT;
         const int k = b % Ns;
-        const cplx<T>* p = lds.ptr(pad16(b));
+        const cplx<T>* p = lds.ptr(padr<R>(b));
         cplx<T> x[r];
 #pragma unroll
-        for (int j = 0; j < r; ++j) x[j] = p[S * j * ((N / r) * 17 / 16)];
+        for (int j = 0; j < r; ++j) x[j] = p[S * j * ((N / r) / R * (R + 1))];
         const int tstep = k * (N / (Ns * r));
 #pragma unroll
-        for (int j = 1; j < r; ++j) x[j] = cmul(x[j], twiddle(tw, (j * tstep) & (N - 1), DIR));
+        for (int j = 1; j < r; ++j) x[j] = tw_mul<DIR>(x[j], tw, (j * tstep) & (N - 1));
         dft<DIR, r>(x);
         if constexpr (P == NP - 1) {
           // last pass: outputs land on t + TT*(mm + q*nb) -> strided register layout
@@ -227,14 +265,13 @@ struct TeamPass {
 };
 
 // Full length-N transform by a team.  In/out: v[j] = a[t + j*TT] (natural order).
-// lds: view of the team's private LDS array (Padded<N>::size entries).
+// lds: view of the team's private LDS array (Padded<N,R>::size entries).
 // Contains __syncthreads(): every lane of the block must call it.
 template <typename T, int N, int R, int DIR, class Lds>
-__device__ __forceinline__ void team_fft(cplx<T> (&v)[R], int t, Lds lds,
-                                         const cplx<T>* __restrict__ tw) {
-  static_assert(N % R == 0 && R <= 16 && (R & (R - 1)) == 0, "radix");
+__device__ __forceinline__ void team_fft(cplx<T> (&v)[R], int t, Lds lds, Twiddles<T> tw) {
+  static_assert(N % R == 0 && R <= 16 && (R & (R - 1)) == 0 && N >= R, "radix");
   dft<DIR, R>(v);  // pass 0: butterfly b = t reads v[j] = a[t + j*TT], no twiddle
-  TeamPass<T, N, R, DIR, 1, Lds>::run(v, t, lds, opaque_s(tw));
+  TeamPass<T, N, R, DIR, 1, Lds>::run(v, t, lds, tw);
 }
 
 }  // namespace wh

@@ -1,91 +1,96 @@
-// Kernel argument blocks shared by the HIP kernels and the C-ABI layer.
+// Argument blocks shared by the HIP kernels and the host runtime.
+//
+// The unit of work is an ensemble of Wigner functions (members); one plan
+// propagates all members with one launch per sweep.  A single grid is an
+// ensemble of one.  Each launch walks a work list of (member, tile) items; all
+// items of a kernel variant have the same lane count and LDS footprint whatever
+// the member's grid size, so members of uneven sizes share one launch.
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "wh_fft.hpp"
-
 namespace wh {
 
-constexpr int kMaxDeg = 8;       // polynomial degree of V(t,x) and K(p)
-constexpr int kNCol = 5;         // column sums: Re S0, Im S0, sum p W, sum p^2 W, sum min(W,0)
-constexpr int kNObs = 16;        // per-step observable record
+constexpr int kMaxDeg = 8;  // polynomial degree of V(t,x) and K(p)
+constexpr int kNCol = 5;    // column sums: Re S0, Im S0, sum p W, sum p^2 W, sum min(W,0)
+constexpr int kNObs = 16;   // per-step observable record
 
-// Observable record layout (doubles), one row per recorded step.
 enum ObsIndex {
   kObsX = 0, kObsX2, kObsP, kObsP2, kObsXP, kObsDV, kObsPDV, kObsXDV, kObsH, kObsNeg,
-  kObsNorm,        // dXdP * sum Re W of the recorded (normalised) state
-  kObsRawNormRe,   // dXdP * sum W before renormalisation (complex)
+  kObsNorm,       // dXdP * sum Re W of the recorded (normalised) state
+  kObsRawNormRe,  // dXdP * sum W before renormalisation (complex)
   kObsRawNormIm,
-  kObsProbSq,      // dX * sum rho(x)^2 (Gross-Pitaevskii record)
+  kObsProbSq,     // dX * sum rho(x)^2 (Gross-Pitaevskii record)
   kObsReserved0, kObsReserved1
 };
 
-// Potential V(t,x) = sum_n c_n(t) x^n, coefficient rows indexed by step.
-struct PotentialArgs {
-  const double* coef;   // [rows][kMaxDeg+1]
-  int rows;             // rows in the table (1 = time independent)
-  int deg;
+// Sweep feature flags (uniform per launch).
+enum SweepFlags {
+  kScaleOnLoad = 1,  // multiply by the member's pending complex factor on load
+  kColSums = 2,      // p sweep: write the member's column sums (observables, norm, rho)
+  kCL = 4,           // p sweep: Caldeira-Leggett order-3 composite after the V step
+  kMerge = 8,        // p sweep: apply the next half step after the column sums (Strang)
+  kGP = 16,          // p sweep: add the Gross-Pitaevskii phase from the member's G grid
+  kColPart = 32,     // x sweep: partial column sums (Gross-Pitaevskii density after K)
+  kConj = 64,        // x sweep: conjugated kinetic multiplier (adjoint step)
 };
 
-struct XSweepArgs {
-  void* W;                       // complex [P][X]
-  int X, P;
-  const void* tw;                // twiddles for length X (cplx<T>)
-  const double* scale;           // complex factor applied on load (2 doubles) or null
-  int kinetic_poly;              // 0: dt*lambda*p/m (expPLambda); 1: polynomial K
+// One ensemble member, resident in device memory.
+struct Member {
+  void* W;                     // complex [P][X], FFT ordered
+  double* colsums;             // [X][kNCol]
+  double* scale;               // [2] pending renormalisation factor
+  double* obs;                 // record rows [..][kNObs]; row = step - obs_base
+  double* rho;                 // [X] Gross-Pitaevskii density
+  double* G;                   // [P][X] Re(B- - B+)
+  double* colpart;             // [P/rows][X]
+  const double* pot;           // [pot_rows][kMaxDeg+1] coefficients of V(t,x)
+  const double* theta_decay;   // [P] or null
+  const double* lambda_decay;  // [X] or null
+  const double* xmask;         // [X] or null
+  const double* sigma_p;       // [P] sign(p) f(p) (Caldeira-Leggett)
+  const double* theta_host;    // [P]
+  const double* lambda_host;   // [X]
+  const double* x_host;        // [X]
+  const double* p_host;        // [P]
+  const double* dv_host;       // [X]
+  const double* v0_host;       // [X]
+  const void* twX_hi;          // twiddles for X in the state precision (+ lo for FP32)
+  const void* twX_lo;
+  const void* twP_hi;
+  const void* twP_lo;
+  const void* twX64;           // FP64 twiddles for X (Gross-Pitaevskii shift)
+  int X, P, pot_rows, pot_deg;
+  double dx, dp, dtheta, dlambda;  // kernel coordinate steps
+  double dxdp, inv_2m, dx_host, rho_factor;
+  // adjoint step (single-grid plans): forward state after the step, gradient column sums
+  const void* W2;              // [P][X] W_{n+1}
+  double* gcols;               // [X][kMaxDeg+1]
+  int obs_base, pad0, pad1, pad2;
+};
+
+struct SweepArgs {
+  const Member* members;
+  const unsigned* work;  // (member << 16) | tile, one per block
+  const int* step;       // device step counter (time index)
+  int step_add;
+  int flags;
+  // uniform physics
+  double dt, hbar, vphase, hbar_half, gp_phase, cl_a;
+  // kinetic: phase = dt*lambda*(k_c0 + k_c1*p) when kin_linear, else
+  // dt*(K(p + hbar lambda/2) - K(p - hbar lambda/2))/hbar with K = sum kcoef[n] p^n
+  int kin_linear, kdeg;
+  double k_c0, k_c1, inv_hbar;
   double kcoef[kMaxDeg + 1];
-  int kdeg;
-  double dt, inv_mass, hbar, inv_hbar;
-  double dp, dlambda;            // kernel coordinate steps
-  const double* lambda_decay;    // [X] exp(-D_lambda*lambda^2) or null
-  double* colpart;               // [P/ROWS][X] partial column sums (Re W) or null
-};
-
-struct PSweepArgs {
-  void* W;                       // complex [P][X]
-  int X, P;
-  const void* tw;                // twiddles for length P
-  const double* scale;           // complex factor applied on load or null
-  const int* step;               // device step counter (time index of this V step)
-  int step_add;                  // added to *step
-  PotentialArgs pot;
-  double vphase;                 // frac*dt/hbar
-  double hbar_half;              // hbar/2
-  double dx, dtheta;             // kernel coordinate steps
-  const double* theta_decay;     // [P] exp(-frac*D_theta*theta^2*dt) or null
-  const double* xmask;           // [X] absorbing mask or null
-  const double* gp_G;            // [P][X] Re(B- - B+) or null
-  double gp_phase;               // -frac*dt*a_GP/hbar
-  // Caldeira-Leggett order-3 composite (after the inverse transform)
-  int cl;
-  const double* sigma_p;         // [P] sign(p) f(p) at kernel p
-  const double* theta_host;      // [P] host theta grid
-  double cl_a;                   // dt*gamma (a = i*cl_a)
-  // observables: column sums of the state after the V step (and damping)
-  double* colsums;               // [X][kNCol] or null
-  const double* p_host;          // [P]
-  // Strang merge: after the column sums apply the next half step (step+1)
-  int merge;
 };
 
 struct ReduceArgs {
-  const double* colsums;         // [X][kNCol]
-  int X;
-  const double* x_host;          // [X]
-  const double* dv_host;         // [X] dV/dx(0,x)
-  const double* v0_host;         // [X] V(0,x)
-  double dxdp, inv_2m;
-  int renorm;                    // divide by the complex norm
-  double* obs;                   // [rows][kNObs] record, row = *step - obs_base
-  int obs_base;
-  int record;                    // write the record row
-  const int* step;               // device step counter
-  int* step_inc;                 // incremented by one when non-null
-  double* scale_out;             // complex factor for the next load (2 doubles) or null
-  // Gross-Pitaevskii density rho(x) = Re(S0 * scale) * rho_factor
-  double* rho_out;               // [X] or null
-  double rho_factor;
-  double dx;
+  const Member* members;
+  int renorm;     // divide by the complex norm
+  int record;     // write the record row
+  int rho;        // write rho(x) from the column sums (Gross-Pitaevskii)
+  int update_scale;
+  const int* step;
+  int* step_inc;  // incremented by one by block 0 (after all blocks read it: separate kernel)
 };
 
 }  // namespace wh

@@ -1,0 +1,134 @@
+"""Autotuner: times every launch variant of the fused sweeps on the local GPU and
+writes the fastest per (precision, regime, sweep, grid size) to tuning.json.
+
+    python -m pywignercuda_amd.tune [--sizes 128,256,512,1024,2048] [--reps 50]
+
+Regimes: "single" (one Wigner function) and "batch" (an ensemble of equal
+grids holding about 1 GiB, enough to fill the GPU).  The plans read the table
+at creation (Wigner2D / WignerEnsemble, argument variants=None).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import _lib
+from .wigner2d import TUNING_FILE, Wigner2D, WignerEnsemble
+
+
+def _time(fn, reps):
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    e.synchronize()
+    return s.elapsed_time(e) / reps * 1e3  # us
+
+
+def _variants(table):
+    L = _lib.lib()
+    return range(L.wh_variant_info(table, -1, None, None))
+
+
+def tune_single(prec, N, reps, cl=False):
+    import torch
+
+    kw = dict(potential="0.5*pow(x,2) + 0.1*pow(x,4)", dpotential="x + 0.4*pow(x,3)", mask=True, precision=prec)
+    if cl:
+        kw.update(gamma=0.1, damping="CaldeiraLeggett", fp_damping="fabs(p)", D_theta=0.1)
+    p = Wigner2D(N, N, 10.0, 10.0, 1.0, 1.0, 0.01, variants=(0, 0, 0), **kw)
+    x, pp = p.x_host[None, :], p.p_host[:, None]
+    W = p.to_device(np.exp(-((x - 1.0) ** 2) - pp ** 2))
+    out = {}
+    for table, name in ((0, "x"), (2 if cl else 1, "pcl" if cl else "p")):
+        if cl and table == 0:
+            continue
+        best = None
+        for v in _variants(table):
+            args = [-1, -1, -1]
+            args[table] = v
+            try:
+                p.set_variants(*args)
+            except _lib.WignerHipError:
+                continue
+            f = (lambda: p.kinetic(W)) if table == 0 else (lambda: p.potential(W, 0, fused=True))
+            t = _time(f, reps)
+            if best is None or t < best[1]:
+                best = (v, t)
+        out[name] = best
+    torch.cuda.synchronize()
+    p.close()
+    return out
+
+
+def tune_batch(prec, N, reps):
+    import torch
+
+    bytes_per = N * N * (16 if prec == "fp64" else 8)
+    n = max(2, min(512, (1 << 30) // bytes_per))
+    m = dict(X=N, P=N, X_amp=10.0, P_amp=10.0, potential="0.5*pow(x,2) + 0.1*pow(x,4)",
+             dpotential="x + 0.4*pow(x,3)")
+    ens = WignerEnsemble([m] * n, 1.0, 1.0, 0.01, precision=prec, variants=(0, 0, 0))
+    states = [torch.zeros((N, N), dtype=ens.dtype, device=ens.device) for _ in range(n)]
+    for s in states:
+        s[0, 0] = 1.0
+    out = {}
+    for table, name in ((0, "x"), (1, "p")):
+        best = None
+        for v in _variants(table):
+            args = [-1, -1, -1]
+            args[table] = v
+            try:
+                ens.set_variants(*args)
+            except _lib.WignerHipError:
+                continue
+            t = _time(lambda: ens.run(states, 4, normalize_initial=False, use_graph=False), max(2, reps // 10))
+            if best is None or t < best[1]:
+                best = (v, t)
+        out[name] = best
+    ens.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--sizes", default="128,256,512,1024,2048")
+    ap.add_argument("--reps", type=int, default=50)
+    ap.add_argument("--no-batch", action="store_true")
+    a = ap.parse_args()
+    sizes = [int(s) for s in a.sizes.split(",")]
+    table = json.loads(TUNING_FILE.read_text()) if TUNING_FILE.exists() else {}
+    t0 = time.time()
+    for prec in ("fp64", "fp32"):
+        single = table.setdefault(prec, {})
+        batch = table.setdefault(prec + "_batch", {})
+        for N in sizes:
+            r = tune_single(prec, N, a.reps)
+            rc = tune_single(prec, N, a.reps, cl=True)
+            for k, v in list(r.items()) + list(rc.items()):
+                if v is not None:
+                    single.setdefault(k, {})[str(N)] = v[0]
+                    single.setdefault(k + "_us", {})[str(N)] = round(v[1], 2)
+            print(f"{prec} {N}: single {r} cl {rc}", flush=True)
+            if not a.no_batch:
+                rb = tune_batch(prec, N, a.reps)
+                for k, v in rb.items():
+                    if v is not None:
+                        batch.setdefault(k, {})[str(N)] = v[0]
+                        batch.setdefault(k + "_us", {})[str(N)] = round(v[1], 2)
+                print(f"{prec} {N}: batch {rb}", flush=True)
+    table["_meta"] = {"seconds": round(time.time() - t0, 1)}
+    TUNING_FILE.write_text(json.dumps(table, indent=1, sort_keys=True))
+    print(f"wrote {TUNING_FILE}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,172 @@
+"""CPU: the C-ABI library loads, exports every declared entry point, validates its
+arguments before touching a device, and maps statuses to the reference's exception
+classes (cufft_wrapper.py:15-73).  No compute call is made here."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared_functions():
+    text = (ROOT / "include" / "wignerhip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wh_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from pywignercuda_amd import _lib
+
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    names = _declared_functions()
+    assert len(names) >= 19
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_status_strings_and_version():
+    from pywignercuda_amd import _lib
+
+    L = _lib.lib()
+    assert L.wh_version() > 0
+    for code in range(0, 13):
+        assert L.wh_status_string(code)
+    assert b"size" in L.wh_status_string(8)
+
+
+def test_fft_plan_argument_validation_maps_to_reference_exceptions():
+    from pywignercuda_amd import _lib
+    from pywignercuda_amd import cufft_wrapper as cw
+
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    assert L.wh_fft_plan_many(ctypes.byref(h), 100, 1, 100, 0, 4) == 8
+    with pytest.raises(cw.cufftInvalidSize):
+        _lib.check(8)
+    assert L.wh_fft_plan_many(ctypes.byref(h), 256, 1, 256, 7, 4) == 3  # invalid type
+    assert L.wh_fft_plan_many(ctypes.byref(h), 256, 3, 5, 0, 4) == 4  # unsupported layout
+    assert L.wh_fft_plan_many(ctypes.byref(h), 256, 1, 256, 0, 0) == 4  # batch < 1
+    assert L.wh_fft_exec(None, None, None, -1, None) == 1
+    assert L.wh_fft_destroy(None) == 1
+    assert issubclass(cw.cufftInvalidPlan, cw.cufftError)
+    assert cw.cufftExceptions[9] is cw.cufftUnalignedData
+
+
+def _valid_wigner_desc():
+    from pywignercuda_amd import _lib
+
+    X = P = 256
+    keep = []
+
+    def arr(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    d = _lib.Wigner2DDesc()
+    d.struct_size = ctypes.sizeof(d)
+    d.precision, d.x_dim, d.p_dim, d.scheme = 0, X, P, 0
+    d.dt, d.dx, d.dp, d.dtheta, d.dlambda, d.hbar, d.mass = 0.01, 0.078, 0.078, 0.31, 0.31, 1.0, 1.0
+    coef = np.zeros(9)
+    coef[2] = 0.5
+    d.pot_deg, d.pot_rows, d.pot_coef = 2, 1, arr(coef)
+    d.x_host, d.p_host = arr(np.zeros(X)), arr(np.zeros(P))
+    d.dxdp, d.inv_2m, d.dx_host = 0.006, 0.5, 0.078
+    return d, keep
+
+
+def test_descriptor_layout_matches_the_library():
+    """A descriptor the library accepts up to the device query: struct sizes agree."""
+    from pywignercuda_amd import _lib
+
+    L = _lib.lib()
+    d, keep = _valid_wigner_desc()
+    h = ctypes.c_void_p()
+    rc = L.wh_wigner2d_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc in (0, 12), L.wh_last_error()  # 12 = no device (CPU container)
+    if rc == 0:
+        L.wh_wigner2d_destroy(h)
+    d.struct_size -= 8
+    assert L.wh_wigner2d_create(ctypes.byref(d), ctypes.byref(h)) == 10
+    assert b"struct_size" in L.wh_last_error()
+
+
+@pytest.mark.parametrize("field,value,code", [
+    ("x_dim", 100, 8), ("p_dim", 4096, 8), ("precision", 5, 3), ("scheme", 9, 10), ("pot_deg", 9, 10),
+    ("pot_rows", 0, 10), ("hbar", 0.0, 10), ("damping", 1, 10), ("gp_coeff", 0.5, 10)])
+def test_descriptor_validation_errors(field, value, code):
+    from pywignercuda_amd import _lib
+
+    L = _lib.lib()
+    d, keep = _valid_wigner_desc()
+    setattr(d, field, value)
+    h = ctypes.c_void_p()
+    assert L.wh_wigner2d_create(ctypes.byref(d), ctypes.byref(h)) == code, L.wh_last_error()
+
+
+def test_ensemble_validation_before_device():
+    from pywignercuda_amd import _lib
+
+    L = _lib.lib()
+    e = _lib.EnsembleDesc()
+    e.struct_size = ctypes.sizeof(e)
+    e.dt, e.hbar, e.mass = 0.01, 1.0, 1.0
+    m = (_lib.MemberDesc * 2)()
+    h = ctypes.c_void_p()
+    for i in range(2):
+        m[i].struct_size = ctypes.sizeof(_lib.MemberDesc)
+        m[i].x_dim = m[i].p_dim = 128
+    # no potential coefficients -> invalid descriptor for member 0
+    assert L.wh_ensemble_create(ctypes.byref(e), m, 2, ctypes.byref(h)) == 10
+    assert b"member 0" in L.wh_last_error()
+    assert L.wh_ensemble_create(ctypes.byref(e), m, 0, ctypes.byref(h)) == 4
+    assert L.wh_ensemble_run(None, None, 0, 1, None, 1, 0, None) == 1
+
+
+def test_variant_tables():
+    from pywignercuda_amd import _lib
+
+    L = _lib.lib()
+    for table in range(3):
+        n = L.wh_variant_info(table, -1, None, None)
+        assert n >= 2
+        r, b = ctypes.c_int(), ctypes.c_int()
+        for i in range(n):
+            assert L.wh_variant_info(table, i, ctypes.byref(r), ctypes.byref(b)) == 0
+            assert r.value in (4, 8, 16) and b.value in (256, 512, 1024)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    from pywignercuda_amd import _lib
+
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "libwignerhip.so")
+    with pytest.raises(_lib.WignerHipLibraryError):
+        _lib.lib()
+
+
+# ----------------------------------------------------------------------------- physics strings
+def test_polynomial_parsing_of_reference_strings():
+    from pywignercuda_amd.physics import NonPolynomialPhysics, coefficient_table, polynomial_coefficients
+
+    deg, tab = coefficient_table("0.5*pow(x,2) + 0.1*pow(x,4)", "x", [0.0])
+    assert deg == 4 and np.allclose(tab[0, :5], [0, 0, 0.5, 0, 0.1])
+    deg, tab = coefficient_table("0.25*pow(x,4) - pow(x,2) - (0.3*cos(2*t))*x", "x", [0.0, 1.0])
+    assert np.allclose(tab[:, 1], [-0.3, -0.3 * np.cos(2.0)])
+    deg, tab = coefficient_table("0.5*p*p/1.0", "p", [0.0])
+    assert deg == 2 and tab[0, 2] == 0.5
+    with pytest.raises(NonPolynomialPhysics):
+        polynomial_coefficients("-2/sqrt(x*x+1)", "x")
+    with pytest.raises(NonPolynomialPhysics):
+        polynomial_coefficients("pow(x,10)", "x")
+
+
+def test_host_eval_uses_numpy_semantics():
+    from pywignercuda_amd.physics import host_eval
+
+    x = np.linspace(-1, 1, 5)
+    assert np.allclose(host_eval(" p*p/sqrt( p*p + 0.5  ) ", p=x), x * x / np.sqrt(x * x + 0.5))
+    assert np.allclose(host_eval("pow(x,3)", x=x), x ** 3)

@@ -1,0 +1,102 @@
+"""GPU: ensembles of uneven Wigner functions in one launch per sweep, and every kernel variant.
+
+Each member is checked against the oracle run on its own (FP64 bar: rel-L2 1e-10).
+"""
+import numpy as np
+import pytest
+
+from oracle import wigner2d as ow
+
+pytestmark = pytest.mark.gpu
+
+
+def _member(X, a, b, x0, s):
+    return dict(X=X, P=X, X_amp=10.0, P_amp=10.0, potential=f"{a!r}*pow(x,4) - {b!r}*pow(x,2)",
+                dpotential=f"4*{a!r}*pow(x,3) - 2*{b!r}*x", x0=x0, s=s, a=a, b=b)
+
+
+def _oracle(m, dt, steps, scheme="lie"):
+    a, b = m["a"], m["b"]
+    orc = ow.Wigner2DOracle(m["X"], m["P"], m["X_amp"], m["P_amp"], 1.0, 1.0, dt, scheme=scheme,
+                            potential=lambda t, x: a * x ** 4 - b * x ** 2,
+                            dpotential=lambda t, x: 4 * a * x ** 3 - 2 * b * x,
+                            kinetic=lambda p: 0.5 * p * p, mask=False)
+    W0 = ow.gaussian_wigner(orc.g, x0=m["x0"], s=m["s"])
+    W, obs, _ = orc.run(W0, steps)
+    return W0, W, obs
+
+
+@pytest.mark.parametrize("scheme", ["lie", "strang"])
+def test_uneven_ensemble_matches_oracle(cuda, scheme):
+    import torch
+
+    from pywignercuda_amd.wigner2d import WignerEnsemble
+
+    members = [_member(128, 0.1, 1.0, 1.0, 1.0), _member(256, 0.05, 0.5, -2.0, 0.8),
+               _member(512, 0.2, 2.0, 0.5, 1.2), _member(128, 0.15, 1.5, -1.0, 0.9)]
+    dt, steps = 0.01, 25
+    ens = WignerEnsemble(members, 1.0, 1.0, dt, scheme=scheme, kinetic="0.5*p*p", device=cuda)
+    refs = [_oracle(m, dt, steps, scheme) for m in members]
+    states = ens.states_to_device([r[0] for r in refs])
+    obs = ens.run(states, steps + 1, use_graph=True).cpu().numpy()
+    torch.cuda.synchronize()
+    for i, (W0, W_ref, obs_ref) in enumerate(refs):
+        W = states[i].cpu().numpy()
+        err = np.linalg.norm(W - W_ref) / np.linalg.norm(W_ref)
+        assert err < 1e-10, (i, err)
+        np.testing.assert_allclose(obs[i, : steps + 1, 0], obs_ref["X"], rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(obs[i, : steps + 1, 8], obs_ref["H"], rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_every_variant_matches_oracle(cuda, prec):
+    import ctypes
+
+    import torch
+
+    from pywignercuda_amd import _lib
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    m = _member(512, 0.1, 1.0, 1.5, 1.0)
+    W0, W_ref, obs_ref = _oracle(m, 0.01, 10)
+    L = _lib.lib()
+    nx, npv = L.wh_variant_info(0, -1, None, None), L.wh_variant_info(1, -1, None, None)
+    tol = 1e-10 if prec == "fp64" else 2e-6
+    hip = Wigner2D(512, 512, 10.0, 10.0, 1.0, 1.0, 0.01, potential=m["potential"], dpotential=m["dpotential"],
+                   mask=False, precision=prec, device=cuda)
+    for xv in range(nx):
+        for pv in range(npv):
+            try:
+                hip.set_variants(xv, pv, -1)
+            except _lib.WignerHipInvalidValue:
+                continue
+            W = hip.to_device(W0)
+            hip.run(W, 11, use_graph=False)
+            torch.cuda.synchronize()
+            err = np.linalg.norm(W.cpu().numpy() - W_ref) / np.linalg.norm(W_ref)
+            assert err < tol, (xv, pv, err)
+
+
+def test_fp32_error_1000_steps(cuda):
+    """FP32-bulk form vs the FP64 form after 1000 unitary steps: the bound of the FP32 path."""
+    import torch
+
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    kw = dict(potential="0.5*pow(x,2)+0.1*pow(x,4)", dpotential="x+0.4*pow(x,3)", mask=False, device=cuda)
+    p64 = Wigner2D(512, 512, 10.0, 10.0, 1.0, 1.0, 0.01, precision="fp64", **kw)
+    p32 = Wigner2D(512, 512, 10.0, 10.0, 1.0, 1.0, 0.01, precision="fp32", **kw)
+    x, p = p64.x_host[None, :], p64.p_host[:, None]
+    W0 = np.exp(-((x - 2.0) ** 2) - p ** 2) / np.pi
+    A, B = p64.to_device(W0), p32.to_device(W0)
+    oa = p64.run(A, 1000).cpu().numpy()
+    ob = p32.run(B, 1000).cpu().numpy()
+    torch.cuda.synchronize()
+    a, b = A.cpu().numpy(), B.cpu().numpy().astype(np.complex128)
+    err = np.linalg.norm(a - b) / np.linalg.norm(a)
+    print(f"fp32 rel-L2 after 1000 steps: {err:.3e}")
+    assert err < 1e-5, err
+    for col, name in [(0, "X"), (2, "P"), (8, "H"), (10, "norm")]:
+        d = np.abs(oa[:, col] - ob[:, col]).max()
+        print(f"  max |<{name}>_fp32 - <{name}>_fp64| = {d:.3e}")
+        assert d < 1e-5, (name, d)

@@ -128,7 +128,7 @@ def test_strang_gross_pitaevskii_fp64(cuda):
 def test_fp32_bulk_error(cuda, scheme):
     hip, orc = make_pair(cuda, scheme=scheme, precision="fp32", mask=False)
     W0 = ow.gaussian_wigner(orc.g, x0=2.0, p0=0.0, s=1.0)
-    W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 100)
+    W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 300)
     err = rel_l2(W, W_ref)
     assert err < 1e-5, err
     for k in ["X", "P", "X2", "P2", "H", "norm"]:
