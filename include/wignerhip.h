@@ -50,6 +50,7 @@ extern "C" {
 #define WH_SCHEME_STRANG 1 /* V(dt/2); K; V(dt/2) GPU_Wigner2D_GPitaevskii.Run, GP.py:1911-1956 */
 
 #define WH_MAX_DEG 8 /* degree bound of the polynomial V(t,x) and K(p) */
+#define WH_POT_TABLE (-1) /* pot_deg value: V given as a [P][X] Bopp-difference table */
 #define WH_NOBS 16   /* doubles per observable record */
 
 /* observable record columns (one row per recorded step) */
@@ -122,7 +123,9 @@ typedef struct {
   int kinetic_deg;
   double kin_coef[WH_MAX_DEG + 1];
   /* potential V(t,x) = sum_n pot_coef[row*(WH_MAX_DEG+1)+n] x^n, row = time index
-     (clamped to pot_rows-1); pot_rows = 1 for a time-independent V */
+     (clamped to pot_rows-1); pot_rows = 1 for a time-independent V.
+     pot_deg = WH_POT_TABLE: V is not a polynomial; the p sweep reads the Bopp
+     difference from pot_dv instead (pot_coef is then unused) */
   int pot_deg;
   int pot_rows;
   const double* pot_coef;
@@ -148,6 +151,10 @@ typedef struct {
   double dxdp;                /* dX*dP */
   double inv_2m;              /* 1/(2 mass) (host mass) */
   double dx_host;             /* dX (ProbabilitySquare record) */
+  /* [P][X] V(x - hbar*theta/2) - V(x + hbar*theta/2) at kernel coordinates, FFT ordered,
+     for pot_deg = WH_POT_TABLE (time-independent V given as an arbitrary expression,
+     Potential(t,x) spliced into expPotential, FFT.py:88-94, 869-873) */
+  const double* pot_dv;
 } wh_wigner2d_desc;
 
 typedef struct wh_wigner2d wh_wigner2d;
@@ -218,6 +225,7 @@ typedef struct {
   const double *theta_decay, *lambda_decay, *x_mask, *sigma_p, *theta_host, *lambda_host;
   const double *x_host, *p_host, *dv_host, *v0_host;
   double dxdp, inv_2m, dx_host, gp_rho_factor;
+  const double* pot_dv; /* [P][X] Bopp difference of V for pot_deg = WH_POT_TABLE */
 } wh_member_desc;
 
 typedef struct wh_ensemble wh_ensemble;

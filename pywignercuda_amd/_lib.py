@@ -17,6 +17,7 @@ WH_C128, WH_C64 = 0, 1
 WH_FORWARD, WH_INVERSE = -1, 1
 WH_SCHEME_LIE, WH_SCHEME_STRANG = 0, 1
 WH_MAX_DEG = 8
+WH_POT_TABLE = -1
 WH_NOBS = 16
 OBS_COLUMNS = ["X", "X2", "P", "P2", "XP", "dV", "PdV", "XdV", "H", "neg", "norm",
                "rawnorm_re", "rawnorm_im", "probsq"]
@@ -108,6 +109,7 @@ class Wigner2DDesc(ctypes.Structure):
         ("gp_coeff", c_double), ("gp_rho_factor", c_double), ("lambda_host", _DP),
         ("x_host", _DP), ("p_host", _DP), ("dv_host", _DP), ("v0_host", _DP),
         ("dxdp", c_double), ("inv_2m", c_double), ("dx_host", c_double),
+        ("pot_dv", _DP),
     ]
 
 
@@ -133,6 +135,7 @@ class MemberDesc(ctypes.Structure):
         ("theta_host", _DP), ("lambda_host", _DP),
         ("x_host", _DP), ("p_host", _DP), ("dv_host", _DP), ("v0_host", _DP),
         ("dxdp", c_double), ("inv_2m", c_double), ("dx_host", c_double), ("gp_rho_factor", c_double),
+        ("pot_dv", _DP),
     ]
 
 

@@ -261,7 +261,7 @@ namespace {
 
 struct MemberBufs {
   DevBuf pot, theta_decay, lambda_decay, x_mask, sigma_p, theta_host, lambda_host;
-  DevBuf x_host, p_host, dv_host, v0_host, colsums, scale, rho, G, colpart;
+  DevBuf x_host, p_host, dv_host, v0_host, colsums, scale, rho, G, colpart, pot_dv;
 };
 
 struct WorkList {
@@ -319,9 +319,12 @@ int validate_member(const wh_ensemble_desc& d, const wh_member_desc& m, int i) {
   if (!sweep_size_supported(m.x_dim) || !sweep_size_supported(m.p_dim))
     return fail(WH_INVALID_SIZE, "member %d: grid %d x %d: both sides must be powers of two in [128, 2048]", i,
                 m.p_dim, m.x_dim);
-  if (m.pot_deg < 0 || m.pot_deg > WH_MAX_DEG || m.pot_rows < 1 || !m.pot_coef)
+  if (m.pot_deg == WH_POT_TABLE) {
+    if (!m.pot_dv) return fail(WH_INVALID_DESCRIPTOR, "member %d: pot_deg WH_POT_TABLE needs pot_dv", i);
+  } else if (m.pot_deg < 0 || m.pot_deg > WH_MAX_DEG || m.pot_rows < 1 || !m.pot_coef) {
     return fail(WH_INVALID_DESCRIPTOR, "member %d: potential deg %d (max %d), rows %d, coef %p", i, m.pot_deg,
                 WH_MAX_DEG, m.pot_rows, (const void*)m.pot_coef);
+  }
   if (d.damping == 1 && (!m.sigma_p || !m.theta_host))
     return fail(WH_INVALID_DESCRIPTOR, "member %d: Caldeira-Leggett needs sigma_p and theta_host", i);
   if (d.gp_coeff != 0.0 && (!m.lambda_host || !m.theta_host))
@@ -418,7 +421,13 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
       if (!e->tw.count(N)) e->tw[N] = std::make_unique<TwSet>();
     WH_TRY(make_twiddles(*e->tw[md.x_dim], md.x_dim, prec, gp));
     WH_TRY(make_twiddles(*e->tw[md.p_dim], md.p_dim, prec, false));
-    WH_TRY(b->pot.upload(md.pot_coef, size_t(md.pot_rows) * (WH_MAX_DEG + 1) * sizeof(double)));
+    if (md.pot_deg == WH_POT_TABLE) {
+      const double zero_row[WH_MAX_DEG + 1] = {};
+      WH_TRY(b->pot.upload(zero_row, sizeof(zero_row)));
+      WH_TRY(b->pot_dv.upload(md.pot_dv, X * P * sizeof(double)));
+    } else {
+      WH_TRY(b->pot.upload(md.pot_coef, size_t(md.pot_rows) * (WH_MAX_DEG + 1) * sizeof(double)));
+    }
     WH_TRY(upload_opt(b->theta_decay, md.theta_decay, P));
     WH_TRY(upload_opt(b->lambda_decay, md.lambda_decay, X));
     WH_TRY(upload_opt(b->x_mask, md.x_mask, X));
@@ -443,6 +452,7 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
     m.rho = b->rho.as<double>();
     m.G = b->G.as<double>();
     m.pot = b->pot.as<double>();
+    m.pot_dv = b->pot_dv.as<double>();
     m.theta_decay = b->theta_decay.as<double>();
     m.lambda_decay = b->lambda_decay.as<double>();
     m.xmask = b->x_mask.as<double>();
@@ -462,7 +472,7 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
     m.twX64 = tx.hi64.p;
     m.X = md.x_dim;
     m.P = md.p_dim;
-    m.pot_rows = md.pot_rows;
+    m.pot_rows = md.pot_deg == WH_POT_TABLE ? 1 : md.pot_rows;
     m.pot_deg = md.pot_deg;
     m.dx = md.dx;
     m.dp = md.dp;
@@ -776,6 +786,7 @@ extern "C" int wh_wigner2d_create(const wh_wigner2d_desc* d, wh_wigner2d** out) 
   m.pot_deg = d->pot_deg;
   m.pot_rows = d->pot_rows;
   m.pot_coef = d->pot_coef;
+  m.pot_dv = d->pot_dv;
   m.theta_decay = d->theta_decay;
   m.lambda_decay = d->lambda_decay;
   m.x_mask = d->x_mask;
@@ -852,6 +863,8 @@ extern "C" int wh_ensemble_set_potential(wh_ensemble* e, int member, const doubl
   hipStream_t s = static_cast<hipStream_t>(stream);
   wh::Member& m = e->mem[member];
   MemberBufs& b = *e->bufs[member];
+  if (m.pot_deg == WH_POT_TABLE)
+    return fail(WH_INVALID_VALUE, "wh_ensemble_set_potential: member %d has a tabulated (non-polynomial) V", member);
   const size_t bytes = size_t(rows) * (WH_MAX_DEG + 1) * sizeof(double);
   bool changed = false;
   if (b.pot.bytes != bytes) {
@@ -876,6 +889,8 @@ extern "C" int wh_wigner2d_adjoint_step(wh_wigner2d* p, void* Lambda, const void
   wh_ensemble* e = p->e;
   if (e->d.scheme != WH_SCHEME_LIE || e->cl() || e->gp())
     return fail(WH_INVALID_VALUE, "the adjoint step covers the Lie scheme without damping or Gross-Pitaevskii");
+  if (e->mem[0].pot_deg == WH_POT_TABLE)
+    return fail(WH_INVALID_VALUE, "the adjoint step needs V as polynomial coefficients (gradients are per coefficient)");
   hipStream_t s = static_cast<hipStream_t>(stream);
   wh::Member& m = e->mem[0];
   if (!e->work_adj.n) {

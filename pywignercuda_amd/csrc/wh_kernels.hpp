@@ -44,6 +44,7 @@ struct Member {
   double* G;                   // [P][X] Re(B- - B+)
   double* colpart;             // [P/rows][X]
   const double* pot;           // [pot_rows][kMaxDeg+1] coefficients of V(t,x)
+  const double* pot_dv;        // [P][X] V(x - hbar theta/2) - V(x + hbar theta/2) when pot_deg < 0
   const double* theta_decay;   // [P] or null
   const double* lambda_decay;  // [X] or null
   const double* xmask;         // [X] or null

@@ -186,7 +186,8 @@ struct PSweep {
       const int i = t + j * TT;
       const double th = m.dtheta * double(fidx(i, N));
       const double u = a.hbar_half * th;
-      double phi = a.vphase * (poly(cf, m.pot_deg, x - u) - poly(cf, m.pot_deg, x + u));
+      double phi = m.pot_deg >= 0 ? a.vphase * (poly(cf, m.pot_deg, x - u) - poly(cf, m.pot_deg, x + u))
+                                  : a.vphase * m.pot_dv[size_t(i) * m.X + col];
       if (gp) phi -= a.gp_phase * m.G[size_t(i) * m.X + col];
       const double r = (m.theta_decay ? m.theta_decay[i] : 1.0) * mask;
       v[j] = apply(v[j], expmi(r, phi));

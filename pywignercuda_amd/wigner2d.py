@@ -19,8 +19,9 @@ from pathlib import Path
 import numpy as np
 
 from . import _lib
-from ._lib import WH_C128, WH_C64, WH_MAX_DEG, WH_NOBS, WH_SCHEME_LIE, WH_SCHEME_STRANG
-from .physics import coefficient_table, depends_on_time, host_eval, polynomial_coefficients
+from ._lib import WH_C128, WH_C64, WH_MAX_DEG, WH_NOBS, WH_POT_TABLE, WH_SCHEME_LIE, WH_SCHEME_STRANG
+from .physics import (NonPolynomialPhysics, coefficient_table, depends_on_time, host_eval,
+                      polynomial_coefficients)
 
 TUNING_FILE = Path(__file__).resolve().parent / "tuning.json"
 
@@ -100,6 +101,7 @@ class _Tables:
             w2 = (w * self.k_dx) ** 2
             return (1.0 - np.exp(-((kx - xm) ** 2) / w2)) * (1.0 - np.exp(-((kx + xm) ** 2) / w2))
 
+        self.pot_dv = None
         self.x_mask = None
         if mask:
             self.x_mask = edge(10.0)  # expPotential, GPU_Wigner2D_FFT.py:121-123
@@ -115,7 +117,20 @@ class _Tables:
         else:
             self.time_dependent = depends_on_time(potential, "x")
             times = np.arange(int(time_rows) if time_rows else 4096) * dt if self.time_dependent else [0.0]
-            deg, tab = coefficient_table(potential, "x", times)
+            try:
+                deg, tab = coefficient_table(potential, "x", times)
+            except NonPolynomialPhysics:
+                if self.time_dependent:
+                    raise NonPolynomialPhysics(
+                        f"{potential!r}: a time-dependent V must be a polynomial in x (degree <= {WH_MAX_DEG})")
+                # arbitrary time-independent V: the Bopp difference V(x - hbar theta/2) - V(x + hbar theta/2)
+                # at kernel coordinates, as expPotential evaluates it per cell (FFT.py:88-94)
+                u = 0.5 * self.k_hbar * ktheta[:, None]
+                vm = host_eval(potential, t=0.0, x=kx[None, :] - u)
+                vp = host_eval(potential, t=0.0, x=kx[None, :] + u)
+                self.pot_dv = np.ascontiguousarray(
+                    np.broadcast_to(np.asarray(vm - vp, dtype=np.float64), (self.P, self.X)))
+                deg, tab = WH_POT_TABLE, np.zeros((1, WH_MAX_DEG + 1))
         self.pot_deg, self.pot_table = deg, np.ascontiguousarray(tab)
         # Caldeira-Leggett friction sigma(p) = sign(p) f(p) at kernel p (theta_fp, FFT.py:424-429)
         self.sigma = None
@@ -148,6 +163,7 @@ class _Tables:
         d.dx, d.dp, d.dtheta, d.dlambda = self.k_dx, self.k_dp, self.k_dtheta, self.k_dlambda
         d.pot_deg, d.pot_rows = self.pot_deg, self.pot_table.shape[0]
         d.pot_coef = self.ptr(self.pot_table)
+        d.pot_dv = self.ptr(self.pot_dv)
         d.theta_decay = self.ptr(self.theta_decay)
         d.lambda_decay = self.ptr(self.lambda_decay)
         d.x_mask = self.ptr(self.x_mask)

@@ -170,3 +170,28 @@ def test_host_eval_uses_numpy_semantics():
     x = np.linspace(-1, 1, 5)
     assert np.allclose(host_eval(" p*p/sqrt( p*p + 0.5  ) ", p=x), x * x / np.sqrt(x * x + 0.5))
     assert np.allclose(host_eval("pow(x,3)", x=x), x ** 3)
+
+
+def test_non_polynomial_potential_tabulated_like_oracle():
+    """A non-polynomial V is tabulated as the Bopp difference at kernel coordinates; times
+    dt/hbar it must give the oracle's potential-multiplier phase (FFT.py:88-94)."""
+    from oracle import wigner2d as ow
+    from pywignercuda_amd.wigner2d import _Tables
+
+    vs = "1.5*0.5*( 1. + tanh((x-5.)/0.3)   ) - 2/sqrt(x*x+1)"
+    vf = lambda t, x: 1.5 * 0.5 * (1.0 + np.tanh((x - 5.0) / 0.3)) - 2.0 / np.sqrt(x * x + 1.0)  # noqa: E731
+    X, P, hbar, dt = 128, 64, 0.7, 0.01
+    t = _Tables(X, P, 12.0, 9.0, hbar, 1.0, dt, potential=vs, dpotential=None, scheme="lie", D_theta=0,
+                D_lambda=0, gamma=0, damping=None, fp_damping=None, gp_coeff=0, gp_rho_factor=None, mask=False,
+                truncate_constants=True, time_rows=None, pot_table=None)
+    assert t.pot_deg == -1 and t.pot_dv.shape == (P, X)
+    orc = ow.Wigner2DOracle(X, P, 12.0, 9.0, hbar, 1.0, dt, potential=vf, mask=False, truncate_constants=True)
+    m = orc.potential_multiplier(0.0)
+    phase = t.k_dt / t.k_hbar * t.pot_dv
+    np.testing.assert_allclose(np.cos(phase) - 1j * np.sin(phase), m, rtol=0, atol=1e-14)
+    # time-dependent non-polynomial V is refused with a clear error
+    from pywignercuda_amd.physics import NonPolynomialPhysics
+    with pytest.raises(NonPolynomialPhysics):
+        _Tables(X, P, 12.0, 9.0, hbar, 1.0, dt, potential="tanh(x)*cos(t)", dpotential=None, scheme="lie",
+                D_theta=0, D_lambda=0, gamma=0, damping=None, fp_damping=None, gp_coeff=0, gp_rho_factor=None,
+                mask=False, truncate_constants=True, time_rows=None, pot_table=None)

@@ -159,3 +159,25 @@ def test_norm_conservation_1000_steps(cuda):
     obs = hip.run(W, 1000).cpu().numpy()
     drift = np.abs(obs[:, OBS_COL["norm"]] - obs[0, OBS_COL["norm"]]).max()
     assert drift < 1e-12, drift
+
+
+@pytest.mark.parametrize("scheme", ["lie", "strang"])
+def test_non_polynomial_potential_table(cuda, scheme):
+    """V given as an arbitrary expression (the Klein-step form of the Dirac notebooks):
+    the p sweep reads the tabulated Bopp difference instead of Horner coefficients."""
+    vs = "1.5*0.5*( 1. + tanh((x-5.)/0.3)   ) - 2/sqrt(x*x+1)"
+    dvs = "1.5*0.5/(0.3*pow(cosh((x-5.)/0.3),2)) + 2*x/pow(x*x+1,1.5)"
+
+    def vf(t, x):
+        return 1.5 * 0.5 * (1.0 + np.tanh((x - 5.0) / 0.3)) - 2.0 / np.sqrt(x * x + 1.0)
+
+    def dvf(t, x):
+        return 1.5 * 0.5 / (0.3 * np.cosh((x - 5.0) / 0.3) ** 2) + 2.0 * x / (x * x + 1.0) ** 1.5
+
+    hip, orc = make_pair(cuda, scheme=scheme, amp=12.0, dt=0.005, D_theta=0.05, truncate=True,
+                         pot=(vs, dvs, vf, dvf))
+    assert hip.tables.pot_deg == -1
+    W0 = ow.gaussian_wigner(orc.g, x0=-2.0, p0=2.0, s=1.0)
+    W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 40)
+    assert rel_l2(W, W_ref) < 1e-10
+    check_obs(obs, obs_ref)
