@@ -254,6 +254,77 @@ int wh_ensemble_set_potential(wh_ensemble* e, int member, const double* coef, in
 /* The single-grid plan's underlying ensemble (one member). */
 wh_ensemble* wh_wigner2d_ensemble(wh_wigner2d* plan);
 
+/* ---- Wigner-Dirac 4x4 (GPU_WignerDirac2D_4x4, GPU_WignerDirac2D_4x4.py:1880-3720) ---- */
+#define WH_DNOBS 32 /* doubles per Dirac record row */
+/* record columns (complex quantities as re, im pairs): the per-step lists of Run
+   (:3412-3500): X_Average, P_Average, XP_Average, XX_Average, PP_Average,
+   Alpha_1_Average, D_1_Potential_0_Average, X1_D_1_Potential_0_Average,
+   P1_D_1_Potential_0_Average, P1_Alpha_1_Average, X1_Alpha_1_Average, Dirac_energy;
+   then negativity, transmission, antiParticle_population, norm, raw norm */
+#define WH_DOBS_X 0
+#define WH_DOBS_P 2
+#define WH_DOBS_XP 4
+#define WH_DOBS_XX 6
+#define WH_DOBS_PP 8
+#define WH_DOBS_ALPHA1 10
+#define WH_DOBS_DV0 12
+#define WH_DOBS_XDV0 14
+#define WH_DOBS_PDV0 16
+#define WH_DOBS_PALPHA1 18
+#define WH_DOBS_XALPHA1 20
+#define WH_DOBS_ENERGY 22
+#define WH_DOBS_NEG 24
+#define WH_DOBS_TRANS 25
+#define WH_DOBS_ANTI 26
+#define WH_DOBS_NORM 27
+#define WH_DOBS_RAWNORM 28
+
+/*
+ * Replaces the per-instance set-up of GPU_WignerDirac2D_4x4 (constants of
+ * :1929-1942, Compiling_CUDA_functions :2019-2138, Allocate_GPUVariables
+ * :1992-2017) for vector potentials A1 = A2 = A3 = 0 (the potential factors are
+ * then diagonal in the spinor indices).  Host arrays, copied at creation.
+ */
+typedef struct {
+  int struct_size; /* sizeof(wh_dirac_desc) */
+  int precision;   /* WH_C128 or WH_C64 (state); factors are FP64 in both */
+  int x_dim, p_dim; /* powers of two, 128..2048 */
+  /* kernel constants (the caller applies the reference's '%f' splicing) */
+  double dt, dx, dp, dtheta, dlambda, mass, mass_half, c;
+  /* Potential_0(t,x): polynomial rows as in wh_wigner2d_desc, or pot_deg =
+     WH_POT_TABLE with pot_dv[P][X] = V0(x - theta/2) - V0(x + theta/2) */
+  int pot_deg, pot_rows;
+  const double* pot_coef;
+  const double* pot_dv;
+  const double* theta_decay;  /* [P] exp(-dt*D_Theta*theta^2) (both potential halves), NULL = 1 */
+  const double* x_mask;       /* [X] edge absorber 1-exp(-((j-X/2)/24)^2) (:111-155), NULL = off */
+  const double* x_host;       /* [X] host grids */
+  const double* p_host;       /* [P] */
+  const double* theta_host;   /* [P] */
+  const double* dv0;          /* [X] dV0/dx at kernel x (table mode) */
+  const double* vsum;         /* [X] sum_k Potential_k at kernel x (table mode, DiracEnergy) */
+  double gamma_dt;            /* CaldeiraDissipatorOrder2_4x4 (:3019-3058): dt*gammaDamping; 0 = off */
+  double dxdp;                /* host dX*dP */
+  double c_host, mc2_host;    /* host c and mass*c^2 (DiracEnergy, :2777-2805) */
+  double transmission_x;      /* x threshold of the transmission record (:72-105: 10) */
+  int antiparticle;           /* record AntiParticlePopulation (:2436-2466) */
+  int energy;                 /* record DiracEnergy */
+} wh_dirac_desc;
+
+typedef struct wh_dirac wh_dirac;
+
+int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out);
+int wh_dirac_destroy(wh_dirac* plan);
+/*
+ * Replaces the time loop of GPU_WignerDirac2D_4x4.Run (:3409-3690): advances the
+ * 16 planes W[16][P][X] (device, plane 4i+j = W_(i+1)(j+1)) by n_steps iterations
+ * with time indices first_step .. first_step+n_steps-1 (t = dt*index) and writes
+ * (n_steps+1) rows of WH_DNOBS doubles to obs (device): row k = the state before
+ * iteration first_step+k (row n_steps = the final state).  Each iteration ends with
+ * the trace normalisation; W holds the normalised final state on return.
+ */
+int wh_dirac_run(wh_dirac* plan, void* W, int first_step, int n_steps, double* obs, int use_graph, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

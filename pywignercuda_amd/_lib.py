@@ -139,6 +139,26 @@ class MemberDesc(ctypes.Structure):
     ]
 
 
+class DiracDesc(ctypes.Structure):
+    """wh_dirac_desc of include/wignerhip.h."""
+
+    _fields_ = [
+        ("struct_size", c_int), ("precision", c_int), ("x_dim", c_int), ("p_dim", c_int),
+        ("dt", c_double), ("dx", c_double), ("dp", c_double), ("dtheta", c_double), ("dlambda", c_double),
+        ("mass", c_double), ("mass_half", c_double), ("c", c_double),
+        ("pot_deg", c_int), ("pot_rows", c_int), ("pot_coef", _DP), ("pot_dv", _DP),
+        ("theta_decay", _DP), ("x_mask", _DP), ("x_host", _DP), ("p_host", _DP), ("theta_host", _DP),
+        ("dv0", _DP), ("vsum", _DP),
+        ("gamma_dt", c_double), ("dxdp", c_double), ("c_host", c_double), ("mc2_host", c_double),
+        ("transmission_x", c_double), ("antiparticle", c_int), ("energy", c_int),
+    ]
+
+
+WH_DNOBS = 32
+DIRAC_OBS = {"X": 0, "P": 2, "XP": 4, "XX": 6, "PP": 8, "Alpha1": 10, "D1V0": 12, "XD1V0": 14, "PD1V0": 16,
+             "PAlpha1": 18, "XAlpha1": 20, "energy": 22}
+DIRAC_OBS_REAL = {"negativity": 24, "transmission": 25, "antiparticle": 26, "norm": 27, "rawnorm": 28}
+
 _lib = None
 
 
@@ -166,6 +186,9 @@ def _declare(lib):
         "wh_ensemble_set_variants": ([vp, ip, ip, ip], ip),
         "wh_ensemble_set_potential": ([vp, ip, vp, ip, ip, vp], ip),
         "wh_variant_info": ([ip, ip, POINTER(ip), POINTER(ip)], ip),
+        "wh_dirac_create": ([POINTER(DiracDesc), POINTER(vp)], ip),
+        "wh_dirac_destroy": ([vp], ip),
+        "wh_dirac_run": ([vp, vp, ip, ip, vp, ip, vp], ip),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

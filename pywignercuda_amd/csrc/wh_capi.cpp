@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/wignerhip.h"
+#include "wh_dirac.hpp"
 #include "wh_kernels.hpp"
 #include "wh_variants.hpp"
 
@@ -41,6 +42,13 @@ int fft_cols_batch_multiple(int prec, int N);
 hipError_t launch_adj_psweep(int prec, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_grad_reduce(const Member* members, int nmembers, double* grad, double scale, hipStream_t s);
 int adj_cols(int P);
+hipError_t launch_dirac_xsweep(int prec, const DiracPlan* d, int nblocks, hipStream_t s);
+hipError_t launch_dirac_psweep(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s);
+hipError_t launch_dirac_colsum(int prec, const DiracPlan* d, int X, hipStream_t s);
+hipError_t launch_dirac_reduce(const DiracPlan* d, int state_add, int renorm, int anti, int inc, hipStream_t s);
+hipError_t launch_dirac_finish(int prec, const DiracPlan* d, int P, hipStream_t s);
+int dirac_x_blocks(int X, int P);
+int dirac_p_blocks(int X, int P);
 }  // namespace wh
 
 namespace {
@@ -926,4 +934,205 @@ extern "C" int wh_wigner2d_observe(wh_wigner2d* p, const void* W, double* obs_ro
                               e->work_colsum.n, s),
             "colsum");
   return reduce(e, 0, 1, 0, 0, false, s);
+}
+
+// ============================================================================ Wigner-Dirac 4x4
+struct wh_dirac {
+  wh_dirac_desc d{};
+  wh::DiracPlan host{};  // host mirror of the device plan
+  DevBuf plan_dev, pot, pot_dv, theta_decay, xmask, x_host, p_host, theta_host, dv0, vsum;
+  DevBuf colsums, rowpart, scale, step;
+  TwSet twx, twp;
+  int xblocks = 0, pblocks = 0;
+  hipStream_t capture_stream = nullptr;
+  std::map<std::tuple<hipStream_t, int>, GraphEntry> graphs;
+  int graph_chunk = 16;
+  ~wh_dirac() {
+    for (auto& kv : graphs) {
+      if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+      if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+  }
+};
+
+namespace {
+
+int dirac_validate(const wh_dirac_desc* d) {
+  if (!d) return fail(WH_INVALID_VALUE, "wh_dirac_create: desc is NULL");
+  if (d->struct_size != (int)sizeof(wh_dirac_desc))
+    return fail(WH_INVALID_DESCRIPTOR, "struct_size %d != %d", d->struct_size, (int)sizeof(wh_dirac_desc));
+  if (d->precision != WH_C128 && d->precision != WH_C64)
+    return fail(WH_INVALID_TYPE, "precision %d is neither WH_C128 nor WH_C64", d->precision);
+  if (!sweep_size_supported(d->x_dim) || !sweep_size_supported(d->p_dim))
+    return fail(WH_INVALID_SIZE, "grid %d x %d: both sides must be powers of two in [128, 2048]", d->p_dim, d->x_dim);
+  if (d->pot_deg == WH_POT_TABLE) {
+    if (!d->pot_dv || !d->dv0 || !d->vsum)
+      return fail(WH_INVALID_DESCRIPTOR, "pot_deg WH_POT_TABLE needs pot_dv, dv0 and vsum");
+  } else if (d->pot_deg < 0 || d->pot_deg > WH_MAX_DEG || d->pot_rows < 1 || !d->pot_coef) {
+    return fail(WH_INVALID_DESCRIPTOR, "potential deg %d rows %d coef %p", d->pot_deg, d->pot_rows,
+                (const void*)d->pot_coef);
+  }
+  if (!d->x_host || !d->p_host || !d->theta_host) return fail(WH_INVALID_DESCRIPTOR, "host grids missing");
+  if (!(d->dt == d->dt) || d->c == 0.0) return fail(WH_INVALID_DESCRIPTOR, "dt %g c %g", d->dt, d->c);
+  return WH_SUCCESS;
+}
+
+// One iteration: kinetic sweep, potential sweep, observables of the new state + renormalisation.
+int dirac_iteration(wh_dirac* e, hipStream_t s) {
+  const auto* dp = e->plan_dev.as<wh::DiracPlan>();
+  const int prec = e->d.precision;
+  WH_LAUNCH(wh::launch_dirac_xsweep(prec, dp, e->xblocks, s), "dirac_xsweep");
+  WH_LAUNCH(wh::launch_dirac_psweep(prec, e->d.gamma_dt != 0.0, dp, e->pblocks, s), "dirac_psweep");
+  WH_LAUNCH(wh::launch_dirac_reduce(dp, 1, 1, 1, 1, s), "dirac_reduce");
+  return WH_SUCCESS;
+}
+
+int dirac_iterations(wh_dirac* e, int count, hipStream_t s) {
+  for (int k = 0; k < count; ++k) WH_TRY(dirac_iteration(e, s));
+  return WH_SUCCESS;
+}
+
+int dirac_run_graph(wh_dirac* e, int count, hipStream_t s) {
+  const int chunk = e->graph_chunk;
+  if (count >= chunk) {
+    auto key = std::make_tuple(s, chunk);
+    auto it = e->graphs.find(key);
+    if (it == e->graphs.end()) {
+      if (!e->capture_stream)
+        WH_HIP(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking), "hipStreamCreate");
+      GraphEntry g;
+      WH_HIP(hipStreamBeginCapture(e->capture_stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
+      const int rc = dirac_iterations(e, chunk, e->capture_stream);
+      const hipError_t err = hipStreamEndCapture(e->capture_stream, &g.graph);
+      if (rc) {
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+        return rc;
+      }
+      if (err != hipSuccess) return hip_fail(err, "EndCapture");
+      WH_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "GraphInstantiate");
+      it = e->graphs.emplace(key, g).first;
+    }
+    while (count >= chunk) {
+      WH_HIP(hipGraphLaunch(it->second.exec, s), "GraphLaunch");
+      count -= chunk;
+    }
+  }
+  return dirac_iterations(e, count, s);
+}
+
+}  // namespace
+
+extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
+  if (!out) return fail(WH_INVALID_VALUE, "wh_dirac_create: out is NULL");
+  *out = nullptr;
+  WH_TRY(dirac_validate(desc));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WH_NO_DEVICE, "no HIP device");
+  auto e = std::make_unique<wh_dirac>();
+  e->d = *desc;
+  const auto& d = *desc;
+  const size_t X = size_t(d.x_dim), P = size_t(d.p_dim);
+  e->xblocks = wh::dirac_x_blocks(d.x_dim, d.p_dim);
+  e->pblocks = wh::dirac_p_blocks(d.x_dim, d.p_dim);
+  if (e->xblocks < 1 || e->pblocks < 1) return fail(WH_INVALID_SIZE, "grid %zu x %zu", P, X);
+  WH_TRY(make_twiddles(e->twx, d.x_dim, d.precision, false));
+  WH_TRY(make_twiddles(e->twp, d.p_dim, d.precision, false));
+  if (d.pot_deg == WH_POT_TABLE) {
+    const double zero_row[WH_MAX_DEG + 1] = {};
+    WH_TRY(e->pot.upload(zero_row, sizeof(zero_row)));
+    WH_TRY(e->pot_dv.upload(d.pot_dv, X * P * sizeof(double)));
+    WH_TRY(e->dv0.upload(d.dv0, X * sizeof(double)));
+    WH_TRY(e->vsum.upload(d.vsum, X * sizeof(double)));
+  } else {
+    WH_TRY(e->pot.upload(d.pot_coef, size_t(d.pot_rows) * (WH_MAX_DEG + 1) * sizeof(double)));
+  }
+  WH_TRY(upload_opt(e->theta_decay, d.theta_decay, P));
+  WH_TRY(upload_opt(e->xmask, d.x_mask, X));
+  WH_TRY(e->x_host.upload(d.x_host, X * sizeof(double)));
+  WH_TRY(e->p_host.upload(d.p_host, P * sizeof(double)));
+  WH_TRY(e->theta_host.upload(d.theta_host, P * sizeof(double)));
+  WH_TRY(e->colsums.alloc(X * wh::kDNCol * sizeof(double)));
+  WH_TRY(e->rowpart.alloc(2 * P * sizeof(double)));
+  const double one = 1.0;
+  WH_TRY(e->scale.upload(&one, sizeof(one)));
+  const int zero = 0;
+  WH_TRY(e->step.upload(&zero, sizeof(zero)));
+  WH_TRY(e->plan_dev.alloc(sizeof(wh::DiracPlan)));
+
+  wh::DiracPlan& h = e->host;
+  std::memset(&h, 0, sizeof(h));
+  h.colsums = e->colsums.as<double>();
+  h.rowpart = e->rowpart.as<double>();
+  h.scale = e->scale.as<double>();
+  h.step = e->step.as<int>();
+  h.pot = e->pot.as<double>();
+  h.pot_dv = e->pot_dv.as<double>();
+  h.theta_decay = e->theta_decay.as<double>();
+  h.xmask = e->xmask.as<double>();
+  h.x_host = e->x_host.as<double>();
+  h.p_host = e->p_host.as<double>();
+  h.theta_host = e->theta_host.as<double>();
+  h.dv0 = e->dv0.as<double>();
+  h.vsum = e->vsum.as<double>();
+  h.twX_hi = d.precision == WH_C128 ? e->twx.hi64.p : e->twx.hi.p;
+  h.twX_lo = e->twx.lo.p;
+  h.twP_hi = d.precision == WH_C128 ? e->twp.hi64.p : e->twp.hi.p;
+  h.twP_lo = e->twp.lo.p;
+  h.X = d.x_dim;
+  h.P = d.p_dim;
+  h.pot_rows = d.pot_deg == WH_POT_TABLE ? 1 : d.pot_rows;
+  h.pot_deg = d.pot_deg;
+  h.energy = d.energy;
+  h.antiparticle = d.antiparticle;
+  h.dx = d.dx;
+  h.dp = d.dp;
+  h.dtheta = d.dtheta;
+  h.dlambda = d.dlambda;
+  h.mh_c = d.mass_half * d.c;
+  h.c_dt = d.c * d.dt;
+  h.mass_phase = d.mass_half * d.c * d.c * d.dt;
+  h.dt = d.dt;
+  h.proj_mc2 = d.mass * d.c * d.c;
+  h.proj_c = d.c;
+  h.cl_a = d.gamma_dt;
+  h.dxdp = d.dxdp;
+  h.c_host = d.c_host;
+  h.mc2_host = d.mc2_host;
+  h.transmission_x = d.transmission_x;
+  *out = e.release();
+  return WH_SUCCESS;
+}
+
+extern "C" int wh_dirac_destroy(wh_dirac* plan) {
+  if (!plan) return fail(WH_INVALID_PLAN, "wh_dirac_destroy: NULL");
+  delete plan;
+  return WH_SUCCESS;
+}
+
+extern "C" int wh_dirac_run(wh_dirac* e, void* W, int first_step, int n_steps, double* obs, int use_graph,
+                            void* stream) {
+  if (!e) return fail(WH_INVALID_PLAN, "wh_dirac_run: plan is NULL");
+  if (!W || !obs || n_steps < 0 || first_step < 0)
+    return fail(WH_INVALID_VALUE, "wh_dirac_run: W %p obs %p n_steps %d first_step %d", W, (void*)obs, n_steps,
+                first_step);
+  const size_t align = e->d.precision == WH_C128 ? 16 : 8;
+  if (reinterpret_cast<uintptr_t>(W) % align) return fail(WH_UNALIGNED_DATA, "W %p", W);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // the device plan is read by every kernel (and by captured graphs): update it in stream order
+  WH_HIP(hipStreamSynchronize(s), "sync");
+  e->host.W = W;
+  e->host.obs = obs;
+  e->host.obs_base = first_step;
+  WH_HIP(hipMemcpy(e->plan_dev.p, &e->host, sizeof(wh::DiracPlan), hipMemcpyHostToDevice), "plan upload");
+  const auto* dp = e->plan_dev.as<wh::DiracPlan>();
+  WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), first_step, s), "set_step");
+  WH_LAUNCH(wh::launch_dirac_colsum(e->d.precision, dp, e->d.x_dim, s), "dirac_colsum");
+  WH_LAUNCH(wh::launch_dirac_reduce(dp, 0, 0, 0, 0, s), "dirac_reduce");
+  if (use_graph)
+    WH_TRY(dirac_run_graph(e, n_steps, s));
+  else
+    WH_TRY(dirac_iterations(e, n_steps, s));
+  WH_LAUNCH(wh::launch_dirac_finish(e->d.precision, dp, e->d.p_dim, s), "dirac_finish");
+  return WH_SUCCESS;
 }

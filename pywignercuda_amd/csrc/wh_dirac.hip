@@ -1,0 +1,597 @@
+// Wigner-Dirac 4x4 split step (GPU_WignerDirac2D_4x4.Run, GPU_WignerDirac2D_4x4.py:3409-3690),
+// two fused sweeps per step:
+//
+//   dirac_xsweep  one team per row p and plane group:  FFT_x of the group's 4 planes ->
+//                 W <- U(p + lambda/2) W U(p - lambda/2)^dagger -> iFFT_x / X
+//                 (Fourier_4X4_X_To_Lambda_GPU, DiracPropagator_P_plus_Lambda /
+//                  P_minus_Lambda, Fourier_4X4_Lambda_To_X_GPU; :159-351, 2622-2676, 3514-3558).
+//                 The kinetic factor only couples spinor rows {1,4}, {2,3} and columns {1,4},
+//                 {2,3}, so the 16 planes fall into 4 closed groups of 4 (e.g. W11 W14 W41 W44)
+//                 and a team keeps one group in registers.  The lambda = 0 bin of groups
+//                 {11,14,41,44} and {22,23,32,33} gives the antiparticle population of the
+//                 loaded state (AntiParticlePopulation, :2436-2466: the filtered trace summed
+//                 over x is Tr[Lambda_-(p) sum_x W(x, p)]).
+//   dirac_psweep  one team per column x, all 16 planes in turn:  FFT_p -> e^{-dt D theta^2}
+//                 e^{-i dt (V0(x - theta/2) - V0(x + theta/2))} times the constant mass phase of
+//                 the plane -> iFFT_p / P -> [Caldeira-Leggett order 2] -> edge absorber ->
+//                 column sums for the observables and the trace norm
+//                 (Fourier_4X4_P_To_Theta_GPU, DiracPropagator_X_minus/plus_Theta for
+//                  A1 = A2 = A3 = 0, Fourier_4X4_Theta_To_P_GPU, CaldeiraDissipatorOrder2_4x4,
+//                  AbsorbBoundary_x; :357-670, 3011-3058, 111-155, 3560-3683).
+//                 With A = 0 the potential factors are diagonal: P-(y) = e^{-iF} on spinor rows
+//                 1, 2 and e^{+iF} on rows 3, 4 (F = mass_half c^2 dt), so plane W_ij picks up
+//                 e^{-2iF} (i in {1,2}, j in {3,4}), e^{+2iF} (i in {3,4}, j in {1,2}) or 1.
+//   dirac_reduce  one block: observables of the normalised state, trace norm -> pending
+//                 factor 1/norm applied by the next x sweep's load (Wigner_4X4_Normalize,
+//                 :2740-2775 and the reductions of :3429-3492).
+#include <hip/hip_runtime.h>
+
+#include "wh_dirac.hpp"
+#include "wh_fft.hpp"
+#include "wh_kernels.hpp"
+#include "wh_math.hpp"
+#include "wh_sweeps.hpp"
+
+namespace wh {
+
+namespace {
+
+// plane groups closed under the kinetic factor, ordered [r1 c1, r1 c2, r2 c1, r2 c2]
+__constant__ int kDiracGroups[4][4] = {{0, 3, 12, 15}, {1, 2, 13, 14}, {4, 7, 8, 11}, {5, 6, 9, 10}};
+
+__device__ __forceinline__ cplx<double> widen(cplx<double> v) { return v; }
+__device__ __forceinline__ cplx<double> widen(cplx<float> v) { return {double(v.x), double(v.y)}; }
+template <typename T>
+__device__ __forceinline__ cplx<T> narrow(cplx<double> v) { return {T(v.x), T(v.y)}; }
+
+// U(q) = cos(c dt s) - i sin(c dt s) (mh_c beta + q alpha_1) / s on one {r1, r2} block:
+// [[a, b], [b, d]] with a = C - i mh_c S/s, d = C + i mh_c S/s, b = -i q S/s; sgn = -1 gives
+// the dt -> -dt factor of P_minus_Lambda (= U(q)^dagger).
+struct Blk {
+  cplx<double> a, b, d;
+};
+__device__ __forceinline__ Blk kinetic_block(double q, double mh_c, double c_dt, double sgn) {
+  const double s = sqrt(mh_c * mh_c + q * q);
+  double sn, cs;
+  sincos_fp64(c_dt * s, &sn, &cs);
+  sn *= sgn;
+  const double f = sn / s;
+  return {{cs, -mh_c * f}, {0.0, -q * f}, {cs, mh_c * f}};
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ x sweep
+template <typename T, int N, int R, int BLOCK>
+__device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group, int tile, cplx<T>* lds) {
+  constexpr int TT = N / R, ROWS = BLOCK / TT;
+  constexpr int LSZ = Padded<N, R>::size;
+  const int tid = threadIdx.x, team = tid / TT, t = tid % TT;
+  const int row = tile * ROWS + team;
+  const size_t plane = size_t(d.P) * size_t(d.X);
+  cplx<T>* __restrict__ W = reinterpret_cast<cplx<T>*>(d.W) + size_t(row) * N;
+  const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twX_hi), reinterpret_cast<const cplx<T>*>(d.twX_lo)};
+  LdsRow<T> view{lds + team * LSZ};
+  const double s = *d.scale;
+
+  cplx<T> v[4][R];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const cplx<T>* src = W + size_t(kDiracGroups[group][q]) * plane;
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[q][j] = src[t + j * TT];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[q][j] = cscale(v[q][j], T(s));
+    team_fft<T, N, R, -1>(v[q], t, view, tw);
+  }
+
+  const double p = d.dp * double(fidx(row, d.P));
+  if (d.antiparticle && (group == 0 || group == 3) && t == 0) {
+    // lambda = 0 bin (element j = 0 of lane 0): Tr[Lambda_-(p) W^(p, 0)] over this group's planes,
+    // Lambda_- = [[aa, bb], [bb, dd]] on each block (DiracElectronProjector, sign -1)
+    const double e2 = 2.0 * sqrt(d.proj_mc2 * d.proj_mc2 + d.proj_c * d.proj_c * p * p);
+    const double aa = 0.5 - d.proj_mc2 / e2, bb = -d.proj_c * p / e2, dd = 0.5 + d.proj_mc2 / e2;
+    const double c = aa * double(v[0][0].x) + bb * (double(v[1][0].x) + double(v[2][0].x)) + dd * double(v[3][0].x);
+    d.rowpart[(group == 0 ? 0 : 1) * d.P + row] = c;
+  }
+
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const double lam = d.dlambda * double(fidx(t + j * TT, N));
+    const Blk K = kinetic_block(p + 0.5 * lam, d.mh_c, d.c_dt, 1.0);
+    const Blk Rt = kinetic_block(p - 0.5 * lam, d.mh_c, d.c_dt, -1.0);
+    const cplx<double> w0 = widen(v[0][j]), w1 = widen(v[1][j]), w2 = widen(v[2][j]), w3 = widen(v[3][j]);
+    // K B
+    const cplx<double> x0 = cadd(cmul(K.a, w0), cmul(K.b, w2));
+    const cplx<double> x1 = cadd(cmul(K.a, w1), cmul(K.b, w3));
+    const cplx<double> x2 = cadd(cmul(K.b, w0), cmul(K.d, w2));
+    const cplx<double> x3 = cadd(cmul(K.b, w1), cmul(K.d, w3));
+    // (K B) Rt
+    v[0][j] = narrow<T>(cadd(cmul(x0, Rt.a), cmul(x1, Rt.b)));
+    v[1][j] = narrow<T>(cadd(cmul(x0, Rt.b), cmul(x1, Rt.d)));
+    v[2][j] = narrow<T>(cadd(cmul(x2, Rt.a), cmul(x3, Rt.b)));
+    v[3][j] = narrow<T>(cadd(cmul(x2, Rt.b), cmul(x3, Rt.d)));
+  }
+
+  const T inv = T(1.0 / N);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    team_fft<T, N, R, +1>(v[q], t, view, tw);
+    cplx<T>* dst = W + size_t(kDiracGroups[group][q]) * plane;
+#pragma unroll
+    for (int j = 0; j < R; ++j) dst[t + j * TT] = cscale(v[q][j], inv);
+  }
+}
+
+// ------------------------------------------------------------------ p sweep
+// Per-lane accumulation of the column sums of one plane.
+template <typename T, int R, int TT>
+__device__ __forceinline__ void dirac_accumulate(const DiracPlan& d, int k, const cplx<T> (&v)[R], int t,
+                                                 double (&acc)[kDNCol], double (&trre)[R]) {
+  const bool diag = (k % 5) == 0;
+  const bool alpha = k == 3 || k == 6 || k == 9 || k == 12;
+  if (!diag && !alpha) return;
+  const double sb = k < 8 ? 1.0 : -1.0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int i = t + j * TT;
+    const double p = d.dp * double(fidx(i, d.P));
+    const double wr = double(v[j].x), wi = double(v[j].y);
+    if (diag) {
+      acc[kDcTrRe] += wr;
+      acc[kDcTrIm] += wi;
+      acc[kDcPTrRe] += p * wr;
+      acc[kDcPTrIm] += p * wi;
+      acc[kDcP2TrRe] += p * p * wr;
+      acc[kDcP2TrIm] += p * p * wi;
+      acc[kDcBetaRe] += sb * wr;
+      acc[kDcBetaIm] += sb * wi;
+      trre[j] += wr;
+    } else {
+      const double ph = d.p_host[i];
+      acc[kDcA1Re] += wr;
+      acc[kDcA1Im] += wi;
+      acc[kDcPA1Re] += p * wr;
+      acc[kDcPA1Im] += p * wi;
+      acc[kDcHA1Re] += ph * wr;
+      acc[kDcHA1Im] += ph * wi;
+    }
+  }
+}
+
+template <typename T, int N, int R, int BLOCK, bool CL>
+__device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, cplx<T>* lds) {
+  constexpr int TT = N / R;
+  constexpr int C = BLOCK / TT;  // columns per tile
+  const int tid = threadIdx.x;
+  const int c = tid % C, t = tid / C;
+  const int X = d.X;
+  const int col = tile * C + c;
+  const size_t plane = size_t(d.P) * size_t(X);
+  cplx<T>* __restrict__ W0 = reinterpret_cast<cplx<T>*>(d.W);
+  const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twP_hi), reinterpret_cast<const cplx<T>*>(d.twP_lo)};
+  LdsStrided<T, C> view{lds + c};
+  const int step = *d.step;
+
+  // potential multiplier of this lane's elements (shared by the 16 planes)
+  cplx<double> M[R];
+  {
+    const double x = d.dx * double(fidx(col, X));
+    const int prow = step < d.pot_rows ? step : d.pot_rows - 1;
+    const double* __restrict__ cf = d.pot + size_t(prow) * (kMaxDeg + 1);
+    const double mask = d.xmask ? d.xmask[col] : 1.0;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int i = t + j * TT;
+      const double th = d.dtheta * double(fidx(i, N));
+      const double dv = d.pot_deg >= 0 ? poly(cf, d.pot_deg, x - 0.5 * th) - poly(cf, d.pot_deg, x + 0.5 * th)
+                                       : d.pot_dv[size_t(i) * X + col];
+      const double r = (d.theta_decay ? d.theta_decay[i] : 1.0) * mask;
+      M[j] = expmi(r, d.dt * dv);
+    }
+  }
+  double e2s, e2c;
+  sincos_fp64(2.0 * d.mass_phase, &e2s, &e2c);
+
+  double acc[kDNCol];
+#pragma unroll
+  for (int q = 0; q < kDNCol; ++q) acc[q] = 0.0;
+  double trre[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) trre[j] = 0.0;
+
+  const T inv = T(1.0 / N);
+  const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<T>));
+  for (int k = 0; k < 16; ++k) {
+    cplx<T>* __restrict__ W = W0 + size_t(k) * plane;
+    cplx<T> v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = at_off(W + size_t(j * TT) * X, boff);
+    team_fft<T, N, R, -1>(v, t, view, tw);
+    const int si = k >> 2, sj = k & 3;
+    const bool lo_hi = si < 2 && sj >= 2, hi_lo = si >= 2 && sj < 2;
+    if (lo_hi || hi_lo) {
+      const cplx<double> e{e2c, lo_hi ? -e2s : e2s};  // e^{-2iF} or e^{+2iF}
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[j] = apply(v[j], cmul(M[j], e));
+    } else {
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[j] = apply(v[j], M[j]);
+    }
+    team_fft<T, N, R, +1>(v, t, view, tw);
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = cscale(v[j], inv);
+
+    if constexpr (CL) {
+      // D(Y) = F_p^-1[ theta F_p[ p Y ] ] on host grids; LW2 = W + a D W; W += 2 a D LW2, a = i dt gamma
+      cplx<T> y[R], lw[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) y[j] = cscale(v[j], T(d.p_host[t + j * TT]));
+      team_fft<T, N, R, -1>(y, t, view, tw);
+#pragma unroll
+      for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(d.theta_host[t + j * TT]));
+      team_fft<T, N, R, +1>(y, t, view, tw);
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        lw[j] = cadd(v[j], mul_i(cscale(y[j], inv), T(d.cl_a)));
+        y[j] = cscale(lw[j], T(d.p_host[t + j * TT]));
+      }
+      team_fft<T, N, R, -1>(y, t, view, tw);
+#pragma unroll
+      for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(d.theta_host[t + j * TT]));
+      team_fft<T, N, R, +1>(y, t, view, tw);
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[j] = cadd(v[j], mul_i(cscale(y[j], inv), T(2.0 * d.cl_a)));
+    }
+
+    dirac_accumulate<T, R, TT>(d, k, v, t, acc, trre);
+#pragma unroll
+    for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = v[j];
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j) acc[kDcNeg] += trre[j] < 0.0 ? trre[j] : 0.0;
+
+  // lanes of one column inside a wave: c + C*t; fold t with xor shuffles, then across waves
+#pragma unroll
+  for (int off = C; off < 64; off <<= 1) {
+#pragma unroll
+    for (int q = 0; q < kDNCol; ++q) acc[q] += __shfl_xor(acc[q], off, 64);
+  }
+  constexpr int kWaves = (BLOCK + 63) / 64;
+  constexpr int kLanes = C < 64 ? C : 64;
+  __syncthreads();
+  double* red = reinterpret_cast<double*>(lds);
+  const int wave = tid >> 6, lane = tid & 63;
+  if (lane < kLanes) {
+#pragma unroll
+    for (int q = 0; q < kDNCol; ++q) red[(wave * kLanes + lane) * kDNCol + q] = acc[q];
+  }
+  __syncthreads();
+  if (tid < C) {
+    double s[kDNCol];
+#pragma unroll
+    for (int q = 0; q < kDNCol; ++q) s[q] = 0.0;
+    for (int w = 0; w < kWaves; ++w) {
+#pragma unroll
+      for (int q = 0; q < kDNCol; ++q) s[q] += red[(w * kLanes + tid) * kDNCol + q];
+    }
+#pragma unroll
+    for (int q = 0; q < kDNCol; ++q) d.colsums[size_t(tile * C + tid) * kDNCol + q] = s[q];
+  }
+}
+
+// ------------------------------------------------------------------ kernels
+constexpr int kDiracXBlock = 256;
+constexpr int kDiracPBlock = 256;
+
+// radix of the x sweep (4 planes live per lane) and of the p sweep (1 plane live)
+template <int N>
+constexpr int dirac_rx() {
+  return N >= 2048 ? 8 : 4;
+}
+template <int N>
+constexpr int dirac_rp() {
+  return N >= 512 ? 8 : 4;
+}
+
+// LDS elements of a block over the supported sizes: teams * padded length.
+template <int BLOCK>
+constexpr int dirac_lds_elems() {
+  int m = 0;
+  for (int N = 128; N <= 2048; N *= 2) {
+    for (int R : {N >= 2048 ? 8 : 4, N >= 512 ? 8 : 4}) {
+      const int teams = BLOCK / (N / R);
+      const int e = teams * (N + N / R + 1);
+      m = e > m ? e : m;
+    }
+  }
+  return m;
+}
+
+#define WH_DIRAC_DISPATCH(NVAR, BODY) \
+  switch (NVAR) {                     \
+    case 128: BODY(128); break;       \
+    case 256: BODY(256); break;       \
+    case 512: BODY(512); break;       \
+    case 1024: BODY(1024); break;     \
+    case 2048: BODY(2048); break;     \
+    default: break;                   \
+  }
+
+template <typename T>
+__global__ void __launch_bounds__(kDiracXBlock) dirac_xsweep_kernel(const DiracPlan* __restrict__ dp) {
+  __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
+  const DiracPlan& d = *dp;
+  const int group = int(blockIdx.x & 3u), tile = int(blockIdx.x >> 2);
+#define BODY(n) dirac_xsweep_body<T, n, dirac_rx<n>(), kDiracXBlock>(d, group, tile, lds)
+  WH_DIRAC_DISPATCH(d.X, BODY)
+#undef BODY
+}
+
+template <typename T, bool CL>
+__global__ void __launch_bounds__(kDiracPBlock) dirac_psweep_kernel(const DiracPlan* __restrict__ dp) {
+  __shared__ cplx<T> lds[dirac_lds_elems<kDiracPBlock>()];
+  const DiracPlan& d = *dp;
+#define BODY(n) dirac_psweep_body<T, n, dirac_rp<n>(), kDiracPBlock, CL>(d, int(blockIdx.x), lds)
+  WH_DIRAC_DISPATCH(d.P, BODY)
+#undef BODY
+}
+
+// Column sums of a state without propagating it (the record of the first state).
+template <typename T>
+__global__ void __launch_bounds__(256) dirac_colsum_kernel(const DiracPlan* __restrict__ dp) {
+  const DiracPlan& d = *dp;
+  const int col = blockIdx.x * 256 + threadIdx.x;
+  if (col >= d.X) return;
+  const size_t plane = size_t(d.P) * size_t(d.X);
+  const cplx<T>* W = reinterpret_cast<const cplx<T>*>(d.W);
+  double acc[kDNCol];
+  for (int q = 0; q < kDNCol; ++q) acc[q] = 0.0;
+  for (int i = 0; i < d.P; ++i) {
+    const double p = d.dp * double(fidx(i, d.P)), ph = d.p_host[i];
+    double trr = 0.0;
+    for (int k = 0; k < 16; ++k) {
+      const bool diag = (k % 5) == 0, alpha = k == 3 || k == 6 || k == 9 || k == 12;
+      if (!diag && !alpha) continue;
+      const cplx<T> w = W[size_t(k) * plane + size_t(i) * d.X + col];
+      const double wr = double(w.x), wi = double(w.y);
+      if (diag) {
+        const double sb = k < 8 ? 1.0 : -1.0;
+        acc[kDcTrRe] += wr;
+        acc[kDcTrIm] += wi;
+        acc[kDcPTrRe] += p * wr;
+        acc[kDcPTrIm] += p * wi;
+        acc[kDcP2TrRe] += p * p * wr;
+        acc[kDcP2TrIm] += p * p * wi;
+        acc[kDcBetaRe] += sb * wr;
+        acc[kDcBetaIm] += sb * wi;
+        trr += wr;
+      } else {
+        acc[kDcA1Re] += wr;
+        acc[kDcA1Im] += wi;
+        acc[kDcPA1Re] += p * wr;
+        acc[kDcPA1Im] += p * wi;
+        acc[kDcHA1Re] += ph * wr;
+        acc[kDcHA1Im] += ph * wi;
+      }
+    }
+    acc[kDcNeg] += trr < 0.0 ? trr : 0.0;
+  }
+  for (int q = 0; q < kDNCol; ++q) d.colsums[size_t(col) * kDNCol + q] = acc[q];
+}
+
+// Observables of the state whose column sums are in d.colsums (state time index = step +
+// state_add), normalised by its trace norm when renorm; the pending load factor becomes
+// 1/norm (renorm) or 1.  With anti, the antiparticle entry of the state at time index
+// `step` is filled from the x sweep's row contributions.  inc: advance the step counter.
+constexpr int kRedThreads = 256;
+constexpr int kRedQ = kDoNorm + 1;  // reduced quantities
+__global__ void __launch_bounds__(kRedThreads) dirac_reduce_kernel(const DiracPlan* __restrict__ dp, int state_add,
+                                                                   int renorm, int anti, int inc) {
+  const DiracPlan& d = *dp;
+  __shared__ double red[kRedThreads][kRedQ + 1];
+  const int step = *d.step;
+  const int tstate = step + state_add;
+  const int prow = tstate < d.pot_rows ? tstate : d.pot_rows - 1;
+  const double* __restrict__ cf = d.pot + size_t(prow) * (kMaxDeg + 1);
+  double s[kRedQ];
+  for (int q = 0; q < kRedQ; ++q) s[q] = 0.0;
+  for (int col = threadIdx.x; col < d.X; col += kRedThreads) {
+    const double* cs = d.colsums + size_t(col) * kDNCol;
+    const double x = d.dx * double(fidx(col, d.X));
+    double dv, vs;
+    if (d.pot_deg >= 0) {  // dV0/dx and V0 of this step's coefficient row
+      dv = 0.0;
+      for (int n = d.pot_deg; n >= 1; --n) dv = fma(dv, x, double(n) * cf[n]);
+      vs = poly(cf, d.pot_deg, x);
+    } else {
+      dv = d.dv0[col];
+      vs = d.vsum[col];
+    }
+    const double tr[2] = {cs[kDcTrRe], cs[kDcTrIm]}, ptr[2] = {cs[kDcPTrRe], cs[kDcPTrIm]};
+    const double a1[2] = {cs[kDcA1Re], cs[kDcA1Im]};
+    for (int c = 0; c < 2; ++c) {
+      s[kDoX + c] += x * tr[c];
+      s[kDoP + c] += ptr[c];
+      s[kDoXP + c] += x * ptr[c];
+      s[kDoXX + c] += x * x * tr[c];
+      s[kDoPP + c] += cs[kDcP2TrRe + c];
+      s[kDoA1 + c] += a1[c];
+      s[kDoDV + c] += dv * tr[c];
+      s[kDoXDV + c] += x * dv * tr[c];
+      s[kDoPDV + c] += dv * ptr[c];
+      s[kDoPA1 + c] += cs[kDcPA1Re + c];
+      s[kDoXA1 + c] += x * a1[c];
+      s[kDoEnergy + c] += d.c_host * cs[kDcHA1Re + c] + d.mc2_host * cs[kDcBetaRe + c] + vs * tr[c];
+    }
+    s[kDoNeg] += cs[kDcNeg];
+    s[kDoTrans] += x > d.transmission_x ? tr[0] : 0.0;
+    s[kDoNorm] += tr[0];
+  }
+  for (int q = 0; q < kRedQ; ++q) red[threadIdx.x][q] = s[q];
+  __syncthreads();
+  for (int h = kRedThreads / 2; h > 0; h >>= 1) {
+    if (int(threadIdx.x) < h)
+      for (int q = 0; q < kRedQ; ++q) red[threadIdx.x][q] += red[threadIdx.x + h][q];
+    __syncthreads();
+  }
+  // antiparticle population from the row contributions of the loaded state (normalised)
+  double anti_sum = 0.0;
+  if (anti && d.antiparticle) {
+    double a = 0.0;
+    for (int r = threadIdx.x; r < d.P; r += kRedThreads) a += d.rowpart[r] + d.rowpart[d.P + r];
+    red[threadIdx.x][kRedQ] = a;
+    __syncthreads();
+    for (int h = kRedThreads / 2; h > 0; h >>= 1) {
+      if (int(threadIdx.x) < h) red[threadIdx.x][kRedQ] += red[threadIdx.x + h][kRedQ];
+      __syncthreads();
+    }
+    anti_sum = red[0][kRedQ];
+  }
+  if (threadIdx.x == 0) {
+    const double raw = d.dxdp * red[0][kDoNorm];
+    const double f = renorm ? 1.0 / raw : 1.0;
+    double* o = d.obs + size_t(tstate - d.obs_base) * kDNObs;
+    for (int q = 0; q < kDoNeg; ++q) o[q] = f * d.dxdp * red[0][q];
+    if (!d.energy) o[kDoEnergy] = o[kDoEnergy + 1] = 0.0;
+    o[kDoNeg] = f * d.dxdp * red[0][kDoNeg];
+    o[kDoTrans] = f * d.dxdp * red[0][kDoTrans];
+    o[kDoNorm] = f * raw;
+    o[kDoRawNorm] = raw;
+    if (anti && d.antiparticle) d.obs[size_t(step - d.obs_base) * kDNObs + kDoAnti] = d.dxdp * anti_sum;
+    if (!anti) o[kDoAnti] = 0.0;
+    *d.scale = f;
+    if (inc) *const_cast<int*>(d.step) = step + 1;
+  }
+}
+
+// Closing pass: W *= pending factor (the returned state is normalised) and the
+// antiparticle contribution of each row (its lambda = 0 bin is the sum over x).
+template <typename T>
+__global__ void __launch_bounds__(256) dirac_finish_kernel(const DiracPlan* __restrict__ dp) {
+  const DiracPlan& d = *dp;
+  const int row = blockIdx.x;
+  const size_t plane = size_t(d.P) * size_t(d.X);
+  cplx<T>* W = reinterpret_cast<cplx<T>*>(d.W) + size_t(row) * d.X;
+  const double s = *d.scale;
+  __shared__ double red[256][4];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};  // sum_x Re of diag-a (11, 22), diag-d (33, 44), off-diagonal b
+  for (int k = 0; k < 16; ++k) {
+    const int slot = (k == 0 || k == 5) ? 0 : (k == 10 || k == 15) ? 1 : (k == 3 || k == 12 || k == 6 || k == 9) ? 2 : -1;
+    for (int x = threadIdx.x; x < d.X; x += 256) {
+      cplx<T>& w = W[size_t(k) * plane + x];
+      w = cscale(w, T(s));
+      if (slot >= 0) acc[slot] += double(w.x);
+    }
+  }
+  for (int q = 0; q < 3; ++q) red[threadIdx.x][q] = acc[q];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (int(threadIdx.x) < h)
+      for (int q = 0; q < 3; ++q) red[threadIdx.x][q] += red[threadIdx.x + h][q];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double p = d.dp * double(fidx(row, d.P));
+    const double e2 = 2.0 * sqrt(d.proj_mc2 * d.proj_mc2 + d.proj_c * d.proj_c * p * p);
+    const double aa = 0.5 - d.proj_mc2 / e2, bb = -d.proj_c * p / e2, dd = 0.5 + d.proj_mc2 / e2;
+    d.rowpart[row] = aa * red[0][0] + dd * red[0][1] + bb * red[0][2];
+    d.rowpart[d.P + row] = 0.0;
+  }
+}
+
+// Antiparticle entry of the record row of the current step (after dirac_finish_kernel).
+__global__ void __launch_bounds__(kRedThreads) dirac_anti_kernel(const DiracPlan* __restrict__ dp) {
+  const DiracPlan& d = *dp;
+  __shared__ double red[kRedThreads];
+  double a = 0.0;
+  for (int r = threadIdx.x; r < d.P; r += kRedThreads) a += d.rowpart[r] + d.rowpart[d.P + r];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int h = kRedThreads / 2; h > 0; h >>= 1) {
+    if (int(threadIdx.x) < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (d.antiparticle) d.obs[size_t(*d.step - d.obs_base) * kDNObs + kDoAnti] = d.dxdp * red[0];
+    *d.scale = 1.0;
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+template <int N>
+constexpr int xsweep_rows() {
+  return kDiracXBlock / (N / dirac_rx<N>());
+}
+template <int N>
+constexpr int psweep_cols() {
+  return kDiracPBlock / (N / dirac_rp<N>());
+}
+
+int dirac_x_blocks(int X, int P) {
+  int rows = 0;
+#define BODY(n) rows = xsweep_rows<n>()
+  WH_DIRAC_DISPATCH(X, BODY)
+#undef BODY
+  return rows > 0 && P % rows == 0 ? 4 * (P / rows) : -1;
+}
+
+int dirac_p_blocks(int X, int P) {
+  int cols = 0;
+#define BODY(n) cols = psweep_cols<n>()
+  WH_DIRAC_DISPATCH(P, BODY)
+#undef BODY
+  return cols > 0 && X % cols == 0 ? X / cols : -1;
+}
+
+hipError_t launch_dirac_xsweep(int prec, const DiracPlan* d, int nblocks, hipStream_t s) {
+  if (prec == 0)
+    hipLaunchKernelGGL(dirac_xsweep_kernel<double>, dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
+  else
+    hipLaunchKernelGGL(dirac_xsweep_kernel<float>, dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_dirac_psweep(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s) {
+  if (prec == 0) {
+    if (cl)
+      hipLaunchKernelGGL((dirac_psweep_kernel<double, true>), dim3(nblocks), dim3(kDiracPBlock), 0, s, d);
+    else
+      hipLaunchKernelGGL((dirac_psweep_kernel<double, false>), dim3(nblocks), dim3(kDiracPBlock), 0, s, d);
+  } else {
+    if (cl)
+      hipLaunchKernelGGL((dirac_psweep_kernel<float, true>), dim3(nblocks), dim3(kDiracPBlock), 0, s, d);
+    else
+      hipLaunchKernelGGL((dirac_psweep_kernel<float, false>), dim3(nblocks), dim3(kDiracPBlock), 0, s, d);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_dirac_colsum(int prec, const DiracPlan* d, int X, hipStream_t s) {
+  const int nb = (X + 255) / 256;
+  if (prec == 0)
+    hipLaunchKernelGGL(dirac_colsum_kernel<double>, dim3(nb), dim3(256), 0, s, d);
+  else
+    hipLaunchKernelGGL(dirac_colsum_kernel<float>, dim3(nb), dim3(256), 0, s, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_dirac_reduce(const DiracPlan* d, int state_add, int renorm, int anti, int inc, hipStream_t s) {
+  hipLaunchKernelGGL(dirac_reduce_kernel, dim3(1), dim3(kRedThreads), 0, s, d, state_add, renorm, anti, inc);
+  return hipGetLastError();
+}
+
+hipError_t launch_dirac_finish(int prec, const DiracPlan* d, int P, hipStream_t s) {
+  if (prec == 0)
+    hipLaunchKernelGGL(dirac_finish_kernel<double>, dim3(P), dim3(256), 0, s, d);
+  else
+    hipLaunchKernelGGL(dirac_finish_kernel<float>, dim3(P), dim3(256), 0, s, d);
+  if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+  hipLaunchKernelGGL(dirac_anti_kernel, dim3(1), dim3(kRedThreads), 0, s, d);
+  return hipGetLastError();
+}
+
+}  // namespace wh

@@ -1,0 +1,193 @@
+"""Fused Wigner-Dirac 4x4 propagator (libwignerhip wh_dirac_*).
+
+    plan = WignerDirac4x4(512, 512, 18.0, 16.0, mass=1.0, c=1.0, dt=0.002,
+                          potentials=("-2/sqrt(x*x+1)", "0.", "0.", "0."))
+    W = plan.to_device(W_init)            # [4, 4, P, X] -> device [16, P, X]
+    obs = plan.run(W, 1000)               # [1001, 32] record rows (device)
+
+Follows GPU_WignerDirac2D_4x4 (GPU_WignerDirac2D_4x4.py:1880-3720): the kinetic
+factor U(p + lambda/2) W U(p - lambda/2)^dagger with half the mass, the potential
+factors with the other half, D_Theta decay, the edge absorber (24 cells) and the
+trace normalisation after every step; per-step observables as in Run.  Vector
+potentials Potential_1..3 must vanish (the potential factors are then diagonal
+in the spinor indices); a non-zero one raises NotImplementedError.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import DIRAC_OBS, DIRAC_OBS_REAL, WH_C128, WH_C64, WH_DNOBS, WH_MAX_DEG, WH_POT_TABLE
+from .physics import NonPolynomialPhysics, coefficient_table, depends_on_time, host_eval
+from .wigner2d import fft_order_axis, host_axis, percent_f
+
+
+def _sympy_derivative(expr: str) -> str:
+    """d/dx of a physics string, as the reference builds D_1_Potential_0_String
+    (sympy.diff + ccode, GPU_WignerDirac2D_4x4.py:1964-1966)."""
+    import sympy as sp
+
+    from .physics import _sympy_expr
+
+    e, v, _ = _sympy_expr(expr, "x")
+    return str(sp.diff(e, v))
+
+
+def _is_zero(expr: str) -> bool:
+    import sympy as sp
+
+    from .physics import _sympy_expr
+
+    e, _, _ = _sympy_expr(expr, "x")
+    return sp.simplify(e) == 0
+
+
+class WignerDirac4x4:
+    """One 16-plane Wigner-Dirac state on a P x X grid.
+
+    potentials : four reference-style C expression strings Potential_0..3(t, x).
+    D_theta    : D_Theta of the potential kernels (exp(-dt D theta^2 / 2) on each side).
+    D_lambda   : accepted and unused, as in the reference (spliced, read by no kernel).
+    gamma, damping : gammaDamping and dampingModel; 'CaldeiraLegget' = order-2 dissipator.
+    absorb     : the 24-cell edge absorber applied every step (always on in the reference).
+    truncate_constants : emulate the '%f' splicing of the kernel constants.
+    antiparticle, energy : record AntiParticlePopulation / DiracEnergy.
+    precision  : 'fp64' (complex128 planes) or 'fp32' (complex64 planes, FP64 factors and sums).
+    time_rows  : rows of the coefficient table for a time-dependent polynomial V0.
+    """
+
+    def __init__(self, X, P, X_amp, P_amp, mass, c, dt, *, potentials=("0.", "0.", "0.", "0."),
+                 D_theta=0.0, D_lambda=0.0, gamma=0.0, damping=None, gp_coeff=0.0, absorb=True,
+                 truncate_constants=True, antiparticle=True, energy=False, precision="fp64",
+                 time_rows=None, device=None, transmission_x=10.0):
+        import torch
+
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        for k in (1, 2, 3):
+            if not _is_zero(potentials[k]):
+                raise NotImplementedError(
+                    f"Potential_{k}_String = {potentials[k]!r}: non-zero vector potentials are not supported")
+        if gp_coeff != 0.0:
+            raise NotImplementedError("grossPitaevskiiCoefficient != 0 is not supported for the Dirac propagator")
+        if gamma != 0.0 and damping not in (None, "CaldeiraLegget"):
+            raise NotImplementedError(f"dampingModel {damping!r} is not supported (only 'CaldeiraLegget')")
+        self.precision = precision
+        self.dtype = torch.complex128 if precision == "fp64" else torch.complex64
+        self.device = torch.device(device if device is not None else "cuda:0")
+        self.X, self.P = int(X), int(P)
+        self.X_amp, self.P_amp = float(X_amp), float(P_amp)
+        self.mass, self.c, self.dt = float(mass), float(c), float(dt)
+        self.dX = 2.0 * self.X_amp / self.X
+        self.dP = 2.0 * self.P_amp / self.P
+        self.dTheta = 2.0 * np.pi / (2.0 * self.P_amp)
+        self.dLambda = 2.0 * np.pi / (2.0 * self.X_amp)
+        self.x_host = host_axis(self.X_amp, self.X, self.dX)
+        self.p_host = host_axis(self.P_amp, self.P, self.dP)
+        self.theta_host = host_axis(self.dTheta * self.P / 2.0, self.P, self.dTheta)
+        tr = percent_f if truncate_constants else float
+        k_dt, k_dx, k_dp = tr(dt), tr(self.dX), tr(self.dP)
+        k_dtheta, k_dlambda = tr(self.dTheta), tr(self.dLambda)
+        kx = fft_order_axis(self.X, k_dx)
+        ktheta = fft_order_axis(self.P, k_dtheta)
+        keep = []
+
+        def ptr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+        d = _lib.DiracDesc()
+        d.struct_size = ctypes.sizeof(_lib.DiracDesc)
+        d.precision = WH_C128 if precision == "fp64" else WH_C64
+        d.x_dim, d.p_dim = self.X, self.P
+        d.dt, d.dx, d.dp, d.dtheta, d.dlambda = k_dt, k_dx, k_dp, k_dtheta, k_dlambda
+        d.mass, d.mass_half, d.c = tr(mass), tr(mass / 2.0), tr(c)
+        V0 = potentials[0]
+        self.time_dependent = depends_on_time(V0, "x")
+        try:
+            times = np.arange(int(time_rows) if time_rows else 4096) * self.dt if self.time_dependent else [0.0]
+            deg, tab = coefficient_table(V0, "x", times)
+            self.pot_table = np.ascontiguousarray(tab)
+            d.pot_deg, d.pot_rows = deg, tab.shape[0]
+            d.pot_coef = ptr(self.pot_table)
+        except NonPolynomialPhysics:
+            if self.time_dependent:
+                raise NonPolynomialPhysics(f"{V0!r}: a time-dependent V0 must be a polynomial in x")
+            self.pot_table = None
+            u = 0.5 * ktheta[:, None]  # x -/+ theta/2 (no hbar: GPU_WignerDirac2D_4x4.py:408, 568)
+            vm = host_eval(V0, t=0.0, x=kx[None, :] - u)
+            vp = host_eval(V0, t=0.0, x=kx[None, :] + u)
+            d.pot_deg, d.pot_rows = WH_POT_TABLE, 1
+            d.pot_dv = ptr(np.broadcast_to(np.asarray(vm - vp, dtype=np.float64), (self.P, self.X)))
+            dv0 = host_eval(_sympy_derivative(V0) + " + 0.*x", t=0.0, x=kx)
+            d.dv0 = ptr(np.broadcast_to(np.asarray(dv0, dtype=np.float64), kx.shape))
+            vs = sum(np.broadcast_to(np.asarray(host_eval(v, t=0.0, x=kx), dtype=np.float64), kx.shape)
+                     for v in potentials)
+            d.vsum = ptr(vs)
+        k_D = tr(D_theta)
+        d.theta_decay = ptr(np.exp(-k_dt * k_D * ktheta * ktheta)) if k_D != 0.0 else None
+        if absorb:
+            j = np.arange(self.X)
+            d.x_mask = ptr(1.0 - np.exp(-(((j - self.X // 2) / 24.0) ** 2)))
+        d.x_host, d.p_host, d.theta_host = ptr(self.x_host), ptr(self.p_host), ptr(self.theta_host)
+        d.gamma_dt = self.dt * float(gamma) if (gamma > 0.0 and damping == "CaldeiraLegget") else 0.0
+        d.dxdp = self.dX * self.dP
+        d.c_host, d.mc2_host = self.c, self.mass * self.c * self.c
+        d.transmission_x = float(transmission_x)
+        d.antiparticle, d.energy = int(bool(antiparticle)), int(bool(energy))
+        self._plan = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().wh_dirac_create(ctypes.byref(d), ctypes.byref(self._plan)), "wh_dirac_create")
+        del keep
+
+    def close(self):
+        if getattr(self, "_plan", None) is not None and self._plan.value:
+            _lib.lib().wh_dirac_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def to_device(self, W):
+        """[4, 4, P, X] (host or device) -> contiguous device [16, P, X] in the state precision."""
+        import torch
+
+        t = torch.as_tensor(np.ascontiguousarray(W) if isinstance(W, np.ndarray) else W)
+        if tuple(t.shape) != (4, 4, self.P, self.X):
+            raise ValueError(f"W must have shape (4, 4, {self.P}, {self.X}), got {tuple(t.shape)}")
+        return t.to(self.device).to(self.dtype).reshape(16, self.P, self.X).contiguous()
+
+    def _check_state(self, W):
+        if W.dtype != self.dtype or not W.is_cuda or not W.is_contiguous() or tuple(W.shape) != (16, self.P, self.X):
+            raise ValueError(f"state must be a contiguous {self.dtype} CUDA tensor of shape {(16, self.P, self.X)}")
+
+    def run(self, W, n_steps: int, first_step: int = 1, use_graph: bool = True, obs=None):
+        """Advances W (in place) by n_steps iterations with time indices first_step.. (t = dt * index);
+        returns the [n_steps + 1, 32] record (device; row k = state before iteration first_step + k)."""
+        import torch
+
+        self._check_state(W)
+        if self.time_dependent and first_step + n_steps + 1 >= self.pot_table.shape[0]:
+            raise ValueError(f"time-dependent potential table has {self.pot_table.shape[0]} rows; "
+                             f"need {first_step + n_steps + 2} (pass time_rows)")
+        if obs is None:
+            obs = torch.zeros((n_steps + 1, WH_DNOBS), dtype=torch.float64, device=W.device)
+        _lib.check(_lib.lib().wh_dirac_run(self._plan, _lib.dptr(W), int(first_step), int(n_steps), _lib.dptr(obs),
+                                           int(bool(use_graph)), _lib.current_stream_handle()), "wh_dirac_run")
+        return obs
+
+    @staticmethod
+    def observables(obs):
+        """Record rows (host array [n, 32]) -> dict of complex / real series."""
+        o = np.asarray(obs)
+        out = {k: o[:, i] + 1j * o[:, i + 1] for k, i in DIRAC_OBS.items()}
+        out.update({k: o[:, i] for k, i in DIRAC_OBS_REAL.items()})
+        return out

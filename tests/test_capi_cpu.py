@@ -195,3 +195,44 @@ def test_non_polynomial_potential_tabulated_like_oracle():
         _Tables(X, P, 12.0, 9.0, hbar, 1.0, dt, potential="tanh(x)*cos(t)", dpotential=None, scheme="lie",
                 D_theta=0, D_lambda=0, gamma=0, damping=None, fp_damping=None, gp_coeff=0, gp_rho_factor=None,
                 mask=False, truncate_constants=True, time_rows=None, pot_table=None)
+
+
+def test_dirac_descriptor_validation_before_device():
+    """wh_dirac_create validates its descriptor before the device query (struct sizes agree)."""
+    from pywignercuda_amd import _lib
+
+    L = _lib.lib()
+    X = P = 128
+    keep = []
+
+    def arr(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    d = _lib.DiracDesc()
+    d.struct_size = ctypes.sizeof(d)
+    d.x_dim, d.p_dim = X, P
+    d.dt, d.dx, d.dp, d.dtheta, d.dlambda, d.mass, d.mass_half, d.c = 0.01, 0.28, 0.25, 0.19, 0.17, 1.0, 0.5, 1.0
+    d.pot_deg, d.pot_rows, d.pot_coef = 0, 1, arr(np.zeros(9))
+    d.x_host, d.p_host, d.theta_host = arr(np.zeros(X)), arr(np.zeros(P)), arr(np.zeros(P))
+    h = ctypes.c_void_p()
+    assert L.wh_dirac_create(ctypes.byref(d), ctypes.byref(h)) in (0, 12), L.wh_last_error()
+    if h.value:
+        L.wh_dirac_destroy(h)
+    for field, value, code in [("x_dim", 96, 8), ("precision", 4, 3), ("pot_deg", -1, 10), ("c", 0.0, 10),
+                               ("struct_size", 8, 10)]:
+        old = getattr(d, field)
+        setattr(d, field, value)
+        assert L.wh_dirac_create(ctypes.byref(d), ctypes.byref(h)) == code, (field, L.wh_last_error())
+        setattr(d, field, old)
+    assert L.wh_dirac_run(None, None, 0, 1, None, 0, None) == 1
+
+
+def test_dirac_plan_rejects_unsupported_physics():
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    with pytest.raises(NotImplementedError):
+        WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=("0.", "0.", " -x ", "0."))
+    with pytest.raises(NotImplementedError):
+        WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, gp_coeff=0.02)

@@ -1,0 +1,159 @@
+"""GPU parity of the fused Wigner-Dirac 4x4 step (libwignerhip wh_dirac_*, through its C ABI)
+against the oracle (oracle/dirac4x4.py).
+
+Bar: FP64 relative L2 <= 1e-10 in the 16 planes; observables within 1e-10 relative
+(1e-12 absolute near 0).  FP32 planes (FP64 factors and sums): relative L2 <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+from oracle.dirac4x4 import Dirac4x4Oracle, gaussian_spinor_particle_up, wigner_from_spinor
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ["X", "P", "XP", "XX", "PP", "Alpha1", "D1V0", "XD1V0", "PD1V0", "PAlpha1", "XAlpha1"]
+
+
+def zero(t, x):
+    return 0.0 * x
+
+
+CASES = {
+    # Coulomb-like V0 of the hydrogen-like notebooks: tabulated (non-polynomial)
+    "coulomb": ("-2/sqrt(x*x+1)", lambda t, x: -2.0 / np.sqrt(x * x + 1.0),
+                lambda t, x: 2.0 * x / (x * x + 1.0) ** 1.5),
+    # Klein step (instances/WignerDirac2D_4x4/Klein.ipynb): tabulated
+    "klein": ("1.5*0.5*( 1. + tanh((x-5.)/0.25)   )", lambda t, x: 0.75 * (1.0 + np.tanh((x - 5.0) / 0.25)),
+              lambda t, x: 0.75 / (0.25 * np.cosh((x - 5.0) / 0.25) ** 2)),
+    # polynomial V0 (Horner path), linear ramp of Free_Caldeira-type runs
+    "ramp": ("1.5*x + 0.01*pow(x,2)", lambda t, x: 1.5 * x + 0.01 * x * x, lambda t, x: 1.5 + 0.02 * x),
+}
+
+
+def make_pair(cuda, case, X=128, P=128, dt=0.01, precision="fp64", D_theta=0.0, gamma=0.0, absorb=True,
+              energy=True):
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    vs, vf, dvf = CASES[case]
+    damping = "CaldeiraLegget" if gamma else None
+    hip = WignerDirac4x4(X, P, 18.0, 16.0, 1.0, 1.0, dt, potentials=(vs, "0.", " 0.", "0.*x"), D_theta=D_theta,
+                         gamma=gamma, damping=damping, absorb=absorb, energy=energy, precision=precision,
+                         device=cuda)
+    orc = Dirac4x4Oracle(X, P, 18.0, 16.0, 1.0, 1.0, dt, [vf, zero, zero, zero], dpotential0=dvf, D_theta=D_theta,
+                         gamma=gamma, damping=damping, absorb=absorb, compute_energy=energy)
+    W0 = wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 3.0, 1.0, Xg, 1.0, 1.0), orc)
+    return hip, orc, W0
+
+
+def run_both(hip, orc, W0, steps, use_graph=False):
+    import torch
+
+    W_ref, rec = orc.run(W0, steps)
+    W = hip.to_device(W0)
+    obs = hip.run(W, steps, first_step=1, use_graph=use_graph)
+    torch.cuda.synchronize()
+    return W.cpu().numpy().reshape(4, 4, hip.P, hip.X), obs.cpu().numpy(), W_ref, rec
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def check_obs(obs, rec, energy=True, rtol=1e-10, atol=1e-12):
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    o = WignerDirac4x4.observables(obs)
+    n = len(rec)
+    for k in KEYS + (["energy"] if energy else []):
+        ref = np.array([r[k] for r in rec])
+        np.testing.assert_allclose(o[k][:n], ref, rtol=rtol, atol=atol, err_msg=k)
+    for k in ["negativity", "transmission", "antiparticle"]:
+        ref = np.array([r[k] for r in rec])
+        np.testing.assert_allclose(o[k][:n], ref, rtol=rtol, atol=atol, err_msg=k)
+
+
+@pytest.mark.parametrize("case", ["coulomb", "klein", "ramp"])
+def test_dirac_fp64_matches_oracle(cuda, case):
+    hip, orc, W0 = make_pair(cuda, case)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 25)
+    err = rel_l2(W, W_ref)
+    print(f"{case}: rel L2 {err:.3e}")
+    assert err < 1e-10
+    check_obs(obs, rec)
+    # the final row is the normalised final state
+    assert abs(obs[-1, 27] - 1.0) < 1e-12
+
+
+def test_dirac_rectangular_damped(cuda):
+    """256 x 128 grid, D_Theta decay and the Caldeira-Leggett order-2 dissipator."""
+    hip, orc, W0 = make_pair(cuda, "klein", X=256, P=128, D_theta=0.02, gamma=0.05)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 15)
+    assert rel_l2(W, W_ref) < 1e-10
+    check_obs(obs, rec)
+
+
+def test_dirac_graph_replay_matches_eager(cuda):
+    import torch
+
+    hip, orc, W0 = make_pair(cuda, "coulomb")
+    Wa = hip.to_device(W0)
+    oa = hip.run(Wa, 37, use_graph=False)
+    for _ in range(3):
+        Wb = hip.to_device(W0)
+        ob = hip.run(Wb, 37, use_graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(Wa, Wb)
+        assert torch.equal(oa, ob)
+
+
+def test_dirac_fp32_error(cuda):
+    hip, orc, W0 = make_pair(cuda, "coulomb", precision="fp32", X=256, P=256)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 100)
+    err = rel_l2(W, W_ref)
+    print(f"fp32 dirac: rel L2 {err:.3e}")
+    assert err < 1e-5
+
+
+def test_dropin_dirac_class_free_notebook_setup(cuda, tmp_path):
+    """instances/WignerDirac2D_4x4/Free.ipynb cell 2 shape (smaller grid and fewer steps):
+    subclass sets the strings, builds W_init from GaussianSpinor_ParticleUp at x -/+ theta/2,
+    Run() fills the reference's result attributes; checked against the oracle."""
+    from pywignercuda_amd.GPU_WignerDirac2D_4x4 import GPU_WignerDirac2D_4x4
+
+    class Klein(GPU_WignerDirac2D_4x4):
+        def __init__(self):
+            self.gammaDamping = 0.0
+            self.D_Theta = 0.0
+            self.D_Lambda = 0.0
+            self.grossPitaevskiiCoefficient = 0.0
+            self.Potential_0_String = "{0}*0.5*( 1. + tanh((x-5.)/{1})   )".format(1.5, 0.25)
+            self.Potential_1_String = " 0."
+            self.Potential_2_String = " 0."
+            self.Potential_3_String = " 0."
+            GPU_WignerDirac2D_4x4.__init__(self, 128, 128, 18, 16, 1.0, 1.0, 0.01, 20, 10,
+                                           frameSaveMode="Density", antiParticleNorm=True, computeEnergy=True)
+
+    inst = Klein()
+    inst.fileName = str(tmp_path / "Free.hdf5")
+    W = np.empty([4, 4, 128, 128], dtype=np.complex128)
+    psiL = inst.GaussianSpinor_ParticleUp(-2, 6.0, 1, inst.X - 0.5 * inst.Theta)
+    psiR = inst.GaussianSpinor_ParticleUp(-2, 6.0, 1, inst.X + 0.5 * inst.Theta)
+    for i in range(4):
+        for j in range(4):
+            W[i, j][:, :] = psiL[i] * psiR[j].conj()
+    inst.Fourier_4X4_Theta_To_P(W)
+    W *= 1.0 / inst.Wigner_4x4_Norm(W)
+    inst.W_init = W
+    assert inst.Run() == 0
+    orc = Dirac4x4Oracle(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01,
+                         [lambda t, x: 0.75 * (1.0 + np.tanh((x - 5.0) / 0.25)), zero, zero, zero],
+                         dpotential0=lambda t, x: 0.75 / (0.25 * np.cosh((x - 5.0) / 0.25) ** 2),
+                         compute_energy=True)
+    W_ref, rec = orc.run(W, 20)
+    assert rel_l2(inst.W_end, W_ref) < 1e-10
+    np.testing.assert_allclose(inst.X_Average, [np.real(r["X"]) for r in rec], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(inst.Dirac_energy, [np.real(r["energy"]) for r in rec], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(inst.antiParticle_population, [r["antiparticle"] for r in rec], rtol=1e-9,
+                               atol=1e-13)
+    assert list(inst.timeSavedIndexRange) == [0, 10, 20]
+    assert inst.TakabayashiAngle_end.shape == (128, 128)
