@@ -8,8 +8,11 @@ renormalisation and the Ehrenfest observables every step).  A step is one full
 dt of the whole grid; value = cell-steps/s (cells x steps / wall time).
 
 Also reported (extra keys): the same workload in the FP32-bulk form and its
-relative L2 error against the FP64 run, and the undamped quartic-oscillator
-step at 1024^2 (FP64) -- the plain step without the friction transforms.
+relative L2 error against the FP64 run; the undamped quartic-oscillator step at
+1024^2 (FP64), alone and as HBM-resident batches (64 x 1024^2 FP64 against
+128 x 1024^2 FP32); the seeded 256-member uneven ensemble (sides 128..1024, FP32,
+1000 steps) with its error against the FP64 run of the same ensemble; and the
+Wigner-Dirac 4x4 step at 512^2 x 16 planes in FP64 and FP32.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -88,6 +91,129 @@ def kernel_time(fn, reps):
     e.record()
     e.synchronize()
     return s.elapsed_time(e) / reps * 1e-3
+
+
+def _gauss(prop_x, prop_p, dX, dP, x0=2.0, p0=0.0, s=1.0):
+    import numpy as np
+
+    x, p = prop_x[None, :], prop_p[:, None]
+    W = np.exp(-((x - x0) ** 2) / s ** 2 - s ** 2 * (p - p0) ** 2) / np.pi
+    return W / (W.sum() * dX * dP)
+
+
+def timed_ensemble(ens, states, steps, warmup):
+    import torch
+
+    ens.run(states, warmup, use_graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run(states, steps, first_step=warmup, normalize_initial=False, use_graph=True)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def hbm_batch(precision, n, steps, warmup, dev):
+    """n copies of the plain 1024^2 quartic-oscillator step in one ensemble: the state
+    (n x 16 MiB FP64 / 8 MiB FP32) is far beyond the 256 MB last-level cache."""
+    from pywignercuda_amd.wigner2d import WignerEnsemble
+
+    c = plain(precision)
+    m = dict(X=c["X"], P=c["P"], X_amp=c["X_amp"], P_amp=c["P_amp"], potential=c["potential"],
+             dpotential=c["dpotential"], mask=False)
+    ens = WignerEnsemble([m] * n, 1.0, 1.0, c["dt"], scheme="lie", precision=precision, device=dev)
+    t = ens.tables[0]
+    W0 = _gauss(t.x_host, t.p_host, t.dX, t.dP)
+    states = ens.states_to_device([W0] * n)
+    wall = timed_ensemble(ens, states, steps, warmup)
+    cells = n * c["X"] * c["P"]
+    return {"members": n, "cell_steps_per_s": cells * steps / wall, "steps": steps,
+            "hbm_frac_algorithmic": cells * steps * (64 if precision == "fp64" else 32) / wall / 1e9 / HBM_PEAK_GBS}
+
+
+def uneven_ensemble_members(seed=20261202, n=256):
+    """256 Wigner functions, sides U{128,256,512,1024}, V = a x^4 - b x^2 with a~U[0.05,0.2],
+    b~U[0.5,2], Gaussian W0 with x0~U[-3,3], p0~U[-1,1], s~U[0.7,1.3]; box [-10,10)^2."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([128, 256, 512, 1024], size=n)
+    a, b = rng.uniform(0.05, 0.2, n), rng.uniform(0.5, 2.0, n)
+    x0, p0, s = rng.uniform(-3, 3, n), rng.uniform(-1, 1, n), rng.uniform(0.7, 1.3, n)
+    members, init = [], []
+    for i in range(n):
+        tab = np.zeros((1, 9))
+        tab[0, 4], tab[0, 2] = a[i], -b[i]
+        members.append(dict(X=int(sides[i]), P=int(sides[i]), X_amp=10.0, P_amp=10.0,
+                            potential=f"{a[i]!r}*pow(x,4) - {b[i]!r}*pow(x,2)",
+                            dpotential=f"{4 * a[i]!r}*pow(x,3) - {2 * b[i]!r}*x", pot_table=tab, mask=False))
+        init.append((x0[i], p0[i], s[i]))
+    return members, init
+
+
+def uneven_ensemble(steps, warmup, dev):
+    """FP32 run of the seeded uneven ensemble (one launch per sweep for all 256 members) and its
+    relative L2 error against the FP64 run of the same ensemble."""
+    import numpy as np
+    import torch
+
+    from pywignercuda_amd.wigner2d import WignerEnsemble
+
+    members, init = uneven_ensemble_members()
+    out = {}
+    finals = {}
+    for prec in ("fp32", "fp64"):
+        ens = WignerEnsemble(members, 1.0, 1.0, 0.01, scheme="lie", precision=prec, device=dev)
+        W0 = [_gauss(t.x_host, t.p_host, t.dX, t.dP, *ini) for t, ini in zip(ens.tables, init)]
+        states = ens.states_to_device(W0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ens.run(states, steps, use_graph=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        cells = sum(t.X * t.P for t in ens.tables)
+        out[prec] = {"cell_steps_per_s": cells * steps / wall, "wall_s": wall}
+        finals[prec] = torch.cat([W.reshape(-1).to(torch.complex128) for W in states]).cpu().numpy()
+        del ens, states
+    d = finals["fp32"] - finals["fp64"]
+    out["fp32_rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(finals["fp64"]))
+    out["cells"] = int(sum(m["X"] * m["P"] for m in members))
+    out["members"] = len(members)
+    out["steps"] = steps
+    return out
+
+
+def dirac_workload(precision, steps, dev):
+    """Wigner-Dirac 4x4, 512^2 x 16 planes, c = m = 1, V0 = -2/sqrt(x^2+1), dt = 0.002, positive-energy
+    Gaussian spinor at x0 = -2, px = 6 (X_amp = 18, P_amp = 16)."""
+    import numpy as np
+    import torch
+
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    plan = WignerDirac4x4(512, 512, 18.0, 16.0, 1.0, 1.0, 0.002,
+                          potentials=("-2/sqrt(x*x+1)", "0.", "0.", "0."), precision=precision, device=dev)
+    x = plan.x_host[None, :]
+    th = plan.theta_host[:, None]
+
+    def spinor(X):
+        rho = np.exp(1j * X * 6.0) * np.exp(-0.5 * (X + 2.0) ** 2)
+        p0 = np.sqrt(36.0 + 1.0)
+        return np.array([rho * (p0 + 1.0), 0 * X, 0 * X, rho * 6.0])
+
+    L, R = spinor(x - 0.5 * th), spinor(x + 0.5 * th)
+    W = np.fft.ifft(L[:, None] * np.conj(R[None, :]), axis=2)
+    W = W / (np.real(W[0, 0] + W[1, 1] + W[2, 2] + W[3, 3]).sum() * plan.dX * plan.dP)
+    Wd = plan.to_device(W)
+    plan.run(Wd, 10)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.run(Wd, steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = 512 * 512
+    return {"cell_steps_per_s": cells * steps / wall, "steps": steps,
+            "hbm_frac_algorithmic": cells * steps * (1024 if precision == "fp64" else 512) / wall / 1e9
+            / HBM_PEAK_GBS}
 
 
 def cpu_baseline(target_seconds=15.0):
@@ -186,6 +312,20 @@ def main():
             "plain_fp64_cell_steps_per_s": cells * a.steps / wallp,
             "plain_fp64_hbm_frac": cells * 64 * a.steps / wallp / 1e9 / HBM_PEAK_GBS,
         }
+        del p32, W32, pp, Wp
+        torch.cuda.empty_cache()
+        bsteps = max(a.steps // 10, 50)
+        out["extra"]["hbm_batch_fp64_64x1024^2"] = hbm_batch("fp64", 64, bsteps, 5, dev)
+        torch.cuda.empty_cache()
+        out["extra"]["hbm_batch_fp32_128x1024^2"] = hbm_batch("fp32", 128, bsteps, 5, dev)
+        torch.cuda.empty_cache()
+        b64 = out["extra"]["hbm_batch_fp64_64x1024^2"]["cell_steps_per_s"]
+        b32 = out["extra"]["hbm_batch_fp32_128x1024^2"]["cell_steps_per_s"]
+        out["extra"]["batched_fp32_over_fp64"] = b32 / b64
+        out["extra"]["uneven_ensemble_256"] = uneven_ensemble(1000, 0, dev)
+        torch.cuda.empty_cache()
+        out["extra"]["dirac_512^2_fp64"] = dirac_workload("fp64", 1000, dev)
+        out["extra"]["dirac_512^2_fp32"] = dirac_workload("fp32", 1000, dev)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

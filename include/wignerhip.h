@@ -309,6 +309,12 @@ typedef struct {
   double transmission_x;      /* x threshold of the transmission record (:72-105: 10) */
   int antiparticle;           /* record AntiParticlePopulation (:2436-2466) */
   int energy;                 /* record DiracEnergy */
+  /* position-dependent mass of the potential factors (GPU_WignerDirac2D_MassPotential_4x4,
+     Mass(t,x) in place of mass_half, GPU_WignerDirac2D_MassPotential_4x4.py:387-391, 422-427):
+     [P][X] F(x - theta/2) - F(x + theta/2) and F(x - theta/2) + F(x + theta/2) with
+     F = Mass(x) c^2 dt; NULL = the constant mass_half */
+  const double* mass_dm;
+  const double* mass_sm;
 } wh_dirac_desc;
 
 typedef struct wh_dirac wh_dirac;

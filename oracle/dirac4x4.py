@@ -48,7 +48,11 @@ class Dirac4x4Oracle:
 
     def __init__(self, X, P, X_amp, P_amp, mass, c, dt, potentials, dpotential0=None, D_theta=0.0,
                  gamma=0.0, damping=None, gp_coeff=0.0, absorb=True, truncate_constants=True,
-                 antiparticle=True, compute_energy=False):
+                 antiparticle=True, compute_energy=False, mass_potential=None, kinetic_mass_half=None):
+        """mass_potential(t, x), kinetic_mass_half: the MassPotential variant
+        (GPU_WignerDirac2D_MassPotential_4x4.py:387-391, 422-427, 1668-1669): Mass(t, x) in the
+        potential factors, a fixed mass_half (1e-4 there) in the kinetic factors, the bare mass
+        in the projectors."""
         self.X, self.P = int(X), int(P)
         self.X_amp, self.P_amp = float(X_amp), float(P_amp)
         self.mass, self.c, self.dt = float(mass), float(c), float(dt)
@@ -62,7 +66,9 @@ class Dirac4x4Oracle:
         self.theta_host = host_axis(self.dTheta * self.P / 2.0, self.P, self.dTheta)[:, None]
         self.lambda_host = host_axis(self.dLambda * self.X / 2.0, self.X, self.dLambda)[None, :]
         tr = percent_f if truncate_constants else float
-        self.k_mass, self.k_mh, self.k_c, self.k_dt = tr(mass), tr(mass / 2.0), tr(c), tr(dt)
+        self.k_mass, self.k_c, self.k_dt = tr(mass), tr(c), tr(dt)
+        self.k_mh = tr(mass / 2.0 if kinetic_mass_half is None else kinetic_mass_half)
+        self.mass_potential = mass_potential
         self.k_dx, self.k_dp = tr(self.dX), tr(self.dP)
         self.k_dtheta, self.k_dlambda = tr(self.dTheta), tr(self.dLambda)
         self.k_D = tr(D_theta)
@@ -126,7 +132,8 @@ class Dirac4x4Oracle:
     def potential_factor(self, t, sign, B=None):
         """P- (sign -1, x - theta/2) and P+ (sign +1, x + theta/2) with their scalar e^{...}, :406-441, :566-596."""
         x = self.kx + sign * 0.5 * self.ktheta
-        mh, c, dt = self.k_mh, self.k_c, self.k_dt
+        c, dt = self.k_c, self.k_dt
+        mh = self.k_mh if self.mass_potential is None else np.asarray(self.mass_potential(t, x), dtype=np.float64)
         A1, A2, A3 = (np.broadcast_to(np.asarray(self.V[k](t, x), dtype=np.float64), x.shape) for k in (1, 2, 3))
         F = np.sqrt((mh * c * c * dt) ** 2 + (A1 * A1 + A2 * A2 + A3 * A3) * dt * dt)
         sF = np.sin(F) / F

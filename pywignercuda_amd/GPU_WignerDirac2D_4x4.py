@@ -69,13 +69,17 @@ class GPU_WignerDirac2D_4x4:
         self.precision = getattr(self, "precision", "fp64")
         self.truncateConstants = getattr(self, "truncateConstants", True)
         self.useGraph = getattr(self, "useGraph", True)
-        self._plan = WignerDirac4x4(
-            X_gridDIM, P_gridDIM, X_amplitude, P_amplitude, mass, c, dt,
+        self._plan = self._make_plan()
+
+    def _make_plan(self, **variant):
+        return WignerDirac4x4(
+            self.X_gridDIM, self.P_gridDIM, self.X_amplitude, self.P_amplitude, self.mass, self.c, self.dt,
             potentials=tuple(getattr(self, f"Potential_{k}_String") for k in range(4)),
             D_theta=self.D_Theta, D_lambda=self.D_Lambda, gamma=self.gammaDamping,
             damping=self.dampingModel if self.gammaDamping > 0.0 else None,
             gp_coeff=self.grossPitaevskiiCoefficient, absorb=True, truncate_constants=self.truncateConstants,
-            antiparticle=bool(antiParticleNorm), energy=(computeEnergy is True), precision=self.precision)
+            antiparticle=bool(self.antiParticleNorm), energy=(self.computeEnergy is True),
+            precision=self.precision, **variant)
 
     # ------------------------------------------------------------------ potentials (host, :2170-2228)
     def PotentialFunction_0(self, t, x):

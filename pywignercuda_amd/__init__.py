@@ -5,6 +5,7 @@ Drop-in for the propagator classes of cabrer7/PyWignerCUDA:
     from pywignercuda_amd.GPU_Wigner2D_FFT import *           # GPU_Wigner2D_FFT
     from pywignercuda_amd.GPU_Wigner2D_GPitaevskii import *   # GPU_Wigner2D_GPitaevskii
     from pywignercuda_amd.GPU_WignerDirac2D_4x4 import *      # GPU_WignerDirac2D_4x4
+    from pywignercuda_amd.GPU_WignerDirac2D_MassPotential_4x4 import *   # ..._MassPotential_4x4
     from pywignercuda_amd import cufft_wrapper                 # plan / exec / destroy
 
 The compute path is libwignerhip.so (hand-written HIP for gfx950, C ABI in

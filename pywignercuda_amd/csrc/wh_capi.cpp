@@ -940,7 +940,7 @@ extern "C" int wh_wigner2d_observe(wh_wigner2d* p, const void* W, double* obs_ro
 struct wh_dirac {
   wh_dirac_desc d{};
   wh::DiracPlan host{};  // host mirror of the device plan
-  DevBuf plan_dev, pot, pot_dv, theta_decay, xmask, x_host, p_host, theta_host, dv0, vsum;
+  DevBuf plan_dev, pot, pot_dv, theta_decay, xmask, x_host, p_host, theta_host, dv0, vsum, mass_dm, mass_sm;
   DevBuf colsums, rowpart, scale, step;
   TwSet twx, twp;
   int xblocks = 0, pblocks = 0;
@@ -974,6 +974,8 @@ int dirac_validate(const wh_dirac_desc* d) {
                 (const void*)d->pot_coef);
   }
   if (!d->x_host || !d->p_host || !d->theta_host) return fail(WH_INVALID_DESCRIPTOR, "host grids missing");
+  if ((d->mass_dm == nullptr) != (d->mass_sm == nullptr))
+    return fail(WH_INVALID_DESCRIPTOR, "mass_dm and mass_sm go together");
   if (!(d->dt == d->dt) || d->c == 0.0) return fail(WH_INVALID_DESCRIPTOR, "dt %g c %g", d->dt, d->c);
   return WH_SUCCESS;
 }
@@ -1049,6 +1051,10 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   }
   WH_TRY(upload_opt(e->theta_decay, d.theta_decay, P));
   WH_TRY(upload_opt(e->xmask, d.x_mask, X));
+  if (d.mass_dm) {
+    WH_TRY(e->mass_dm.upload(d.mass_dm, X * P * sizeof(double)));
+    WH_TRY(e->mass_sm.upload(d.mass_sm, X * P * sizeof(double)));
+  }
   WH_TRY(e->x_host.upload(d.x_host, X * sizeof(double)));
   WH_TRY(e->p_host.upload(d.p_host, P * sizeof(double)));
   WH_TRY(e->theta_host.upload(d.theta_host, P * sizeof(double)));
@@ -1075,6 +1081,8 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   h.theta_host = e->theta_host.as<double>();
   h.dv0 = e->dv0.as<double>();
   h.vsum = e->vsum.as<double>();
+  h.mass_dm = e->mass_dm.as<double>();
+  h.mass_sm = e->mass_sm.as<double>();
   h.twX_hi = d.precision == WH_C128 ? e->twx.hi64.p : e->twx.hi.p;
   h.twX_lo = e->twx.lo.p;
   h.twP_hi = d.precision == WH_C128 ? e->twp.hi64.p : e->twp.hi.p;

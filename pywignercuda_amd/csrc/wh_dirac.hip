@@ -176,25 +176,22 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
   LdsStrided<T, C> view{lds + c};
   const int step = *d.step;
 
-  // potential multiplier of this lane's elements (shared by the 16 planes)
-  cplx<double> M[R];
+  // Bopp phase dt (V0(x - theta/2) - V0(x + theta/2)) of this lane's elements (shared by the 16 planes)
+  const double x = d.dx * double(fidx(col, X));
+  const double mask = d.xmask ? d.xmask[col] : 1.0;
+  double ph[R];
   {
-    const double x = d.dx * double(fidx(col, X));
     const int prow = step < d.pot_rows ? step : d.pot_rows - 1;
     const double* __restrict__ cf = d.pot + size_t(prow) * (kMaxDeg + 1);
-    const double mask = d.xmask ? d.xmask[col] : 1.0;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int i = t + j * TT;
       const double th = d.dtheta * double(fidx(i, N));
       const double dv = d.pot_deg >= 0 ? poly(cf, d.pot_deg, x - 0.5 * th) - poly(cf, d.pot_deg, x + 0.5 * th)
                                        : d.pot_dv[size_t(i) * X + col];
-      const double r = (d.theta_decay ? d.theta_decay[i] : 1.0) * mask;
-      M[j] = expmi(r, d.dt * dv);
+      ph[j] = d.dt * dv;
     }
   }
-  double e2s, e2c;
-  sincos_fp64(2.0 * d.mass_phase, &e2s, &e2c);
 
   double acc[kDNCol];
 #pragma unroll
@@ -205,51 +202,63 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
 
   const T inv = T(1.0 / N);
   const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<T>));
-  for (int k = 0; k < 16; ++k) {
-    cplx<T>* __restrict__ W = W0 + size_t(k) * plane;
-    cplx<T> v[R];
+  // planes by mass-phase class: spinor rows {1,2} or {3,4} x columns {1,2} or {3,4}.  Class
+  // multiplier r e^{-i (ph + m)}: m = F- - F+ (lo, lo), F- + F+ (lo, hi), -(F- + F+) (hi, lo),
+  // -(F- - F+) (hi, hi), F -/+ = Mass(x -/+ theta/2) c^2 dt; a constant mass gives 0, 2F, -2F, 0.
+  for (int cls = 0; cls < 4; ++cls) {
+    cplx<double> M[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = at_off(W + size_t(j * TT) * X, boff);
-    team_fft<T, N, R, -1>(v, t, view, tw);
-    const int si = k >> 2, sj = k & 3;
-    const bool lo_hi = si < 2 && sj >= 2, hi_lo = si >= 2 && sj < 2;
-    if (lo_hi || hi_lo) {
-      const cplx<double> e{e2c, lo_hi ? -e2s : e2s};  // e^{-2iF} or e^{+2iF}
+    for (int j = 0; j < R; ++j) {
+      const int i = t + j * TT;
+      double m;
+      if (d.mass_dm) {
+        const size_t e = size_t(i) * X + col;
+        m = cls == 0 ? d.mass_dm[e] : cls == 1 ? d.mass_sm[e] : cls == 2 ? -d.mass_sm[e] : -d.mass_dm[e];
+      } else {
+        m = cls == 1 ? 2.0 * d.mass_phase : cls == 2 ? -2.0 * d.mass_phase : 0.0;
+      }
+      const double r = (d.theta_decay ? d.theta_decay[i] : 1.0) * mask;
+      M[j] = expmi(r, ph[j] + m);
+    }
+    for (int q = 0; q < 4; ++q) {
+      const int k = ((cls >> 1) << 3) | ((cls & 1) << 1) | ((q >> 1) << 2) | (q & 1);
+      cplx<T>* __restrict__ W = W0 + size_t(k) * plane;
+      cplx<T> v[R];
 #pragma unroll
-      for (int j = 0; j < R; ++j) v[j] = apply(v[j], cmul(M[j], e));
-    } else {
+      for (int j = 0; j < R; ++j) v[j] = at_off(W + size_t(j * TT) * X, boff);
+      team_fft<T, N, R, -1>(v, t, view, tw);
 #pragma unroll
       for (int j = 0; j < R; ++j) v[j] = apply(v[j], M[j]);
-    }
-    team_fft<T, N, R, +1>(v, t, view, tw);
+      team_fft<T, N, R, +1>(v, t, view, tw);
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = cscale(v[j], inv);
+      for (int j = 0; j < R; ++j) v[j] = cscale(v[j], inv);
 
-    if constexpr (CL) {
-      // D(Y) = F_p^-1[ theta F_p[ p Y ] ] on host grids; LW2 = W + a D W; W += 2 a D LW2, a = i dt gamma
-      cplx<T> y[R], lw[R];
+      if constexpr (CL) {
+        // D(Y) = F_p^-1[ theta F_p[ p Y ] ] on host grids; LW2 = W + a D W; W += 2 a D LW2, a = i dt gamma
+        cplx<T> y[R], lw[R];
 #pragma unroll
-      for (int j = 0; j < R; ++j) y[j] = cscale(v[j], T(d.p_host[t + j * TT]));
-      team_fft<T, N, R, -1>(y, t, view, tw);
+        for (int j = 0; j < R; ++j) y[j] = cscale(v[j], T(d.p_host[t + j * TT]));
+        team_fft<T, N, R, -1>(y, t, view, tw);
 #pragma unroll
-      for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(d.theta_host[t + j * TT]));
-      team_fft<T, N, R, +1>(y, t, view, tw);
+        for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(d.theta_host[t + j * TT]));
+        team_fft<T, N, R, +1>(y, t, view, tw);
 #pragma unroll
-      for (int j = 0; j < R; ++j) {
-        lw[j] = cadd(v[j], mul_i(cscale(y[j], inv), T(d.cl_a)));
-        y[j] = cscale(lw[j], T(d.p_host[t + j * TT]));
+        for (int j = 0; j < R; ++j) {
+          lw[j] = cadd(v[j], mul_i(cscale(y[j], inv), T(d.cl_a)));
+          y[j] = cscale(lw[j], T(d.p_host[t + j * TT]));
+        }
+        team_fft<T, N, R, -1>(y, t, view, tw);
+#pragma unroll
+        for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(d.theta_host[t + j * TT]));
+        team_fft<T, N, R, +1>(y, t, view, tw);
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = cadd(v[j], mul_i(cscale(y[j], inv), T(2.0 * d.cl_a)));
       }
-      team_fft<T, N, R, -1>(y, t, view, tw);
-#pragma unroll
-      for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(d.theta_host[t + j * TT]));
-      team_fft<T, N, R, +1>(y, t, view, tw);
-#pragma unroll
-      for (int j = 0; j < R; ++j) v[j] = cadd(v[j], mul_i(cscale(y[j], inv), T(2.0 * d.cl_a)));
-    }
 
-    dirac_accumulate<T, R, TT>(d, k, v, t, acc, trre);
+      dirac_accumulate<T, R, TT>(d, k, v, t, acc, trre);
 #pragma unroll
-    for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = v[j];
+      for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = v[j];
+    }
   }
 #pragma unroll
   for (int j = 0; j < R; ++j) acc[kDcNeg] += trre[j] < 0.0 ? trre[j] : 0.0;

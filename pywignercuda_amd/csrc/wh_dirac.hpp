@@ -50,6 +50,8 @@ struct DiracPlan {
   const double* theta_host;    // [P] host grid (Caldeira-Leggett)
   const double* dv0;           // [X] dV0/dx at kernel x (time-independent V0)
   const double* vsum;          // [X] sum_k V_k at kernel x (DiracEnergy, time-independent)
+  const double* mass_dm;       // [P][X] F(x - theta/2) - F(x + theta/2), F = Mass(x) c^2 dt, or null
+  const double* mass_sm;       // [P][X] F(x - theta/2) + F(x + theta/2)
   const void* twX_hi;
   const void* twX_lo;
   const void* twP_hi;
@@ -59,7 +61,7 @@ struct DiracPlan {
   double dx, dp, dtheta, dlambda;   // kernel steps ('%f')
   double mh_c;                      // mass_half * c          (kinetic shell)
   double c_dt;                      // c * dt
-  double mass_phase;                // F = mass_half c^2 dt   (potential mass term, A = 0)
+  double mass_phase;                // F = mass_half c^2 dt   (constant mass of the potential factors, A = 0)
   double dt;                        // kernel dt
   double proj_mc2, proj_c;          // mass c^2, c            (projector, kernel constants)
   double cl_a;                      // dt * gamma of Caldeira-Leggett order 2 (host values); 0 = off

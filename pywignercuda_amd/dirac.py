@@ -56,12 +56,15 @@ class WignerDirac4x4:
     antiparticle, energy : record AntiParticlePopulation / DiracEnergy.
     precision  : 'fp64' (complex128 planes) or 'fp32' (complex64 planes, FP64 factors and sums).
     time_rows  : rows of the coefficient table for a time-dependent polynomial V0.
+    mass_potential, kinetic_mass_half : the MassPotential variant -- Mass(t, x) string of the
+                 potential factors and the fixed mass_half of the kinetic factors.
     """
 
     def __init__(self, X, P, X_amp, P_amp, mass, c, dt, *, potentials=("0.", "0.", "0.", "0."),
                  D_theta=0.0, D_lambda=0.0, gamma=0.0, damping=None, gp_coeff=0.0, absorb=True,
                  truncate_constants=True, antiparticle=True, energy=False, precision="fp64",
-                 time_rows=None, device=None, transmission_x=10.0):
+                 time_rows=None, device=None, transmission_x=10.0, mass_potential=None,
+                 kinetic_mass_half=None):
         import torch
 
         if precision not in ("fp64", "fp32"):
@@ -106,7 +109,20 @@ class WignerDirac4x4:
         d.precision = WH_C128 if precision == "fp64" else WH_C64
         d.x_dim, d.p_dim = self.X, self.P
         d.dt, d.dx, d.dp, d.dtheta, d.dlambda = k_dt, k_dx, k_dp, k_dtheta, k_dlambda
-        d.mass, d.mass_half, d.c = tr(mass), tr(mass / 2.0), tr(c)
+        k_c = tr(c)
+        d.mass, d.c = tr(mass), k_c
+        d.mass_half = tr(mass / 2.0 if kinetic_mass_half is None else kinetic_mass_half)
+        if mass_potential is not None:
+            # Mass(t, x) of the potential factors (GPU_WignerDirac2D_MassPotential_4x4.py:422-427, 585-591)
+            if depends_on_time(mass_potential, "x"):
+                raise NotImplementedError(f"Mass_String {mass_potential!r}: a time-dependent mass is not supported")
+            u = 0.5 * ktheta[:, None]
+            shape = (self.P, self.X)
+            Fm = np.broadcast_to(np.asarray(host_eval(mass_potential, t=0.0, x=kx[None, :] - u), dtype=np.float64),
+                                 shape) * k_c * k_c * k_dt
+            Fp = np.broadcast_to(np.asarray(host_eval(mass_potential, t=0.0, x=kx[None, :] + u), dtype=np.float64),
+                                 shape) * k_c * k_c * k_dt
+            d.mass_dm, d.mass_sm = ptr(Fm - Fp), ptr(Fm + Fp)
         V0 = potentials[0]
         self.time_dependent = depends_on_time(V0, "x")
         try:

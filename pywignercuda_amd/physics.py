@@ -11,6 +11,8 @@ coefficient table evaluated in-kernel by Horner's rule.
 """
 from __future__ import annotations
 
+import functools
+
 import numpy as np
 
 from ._lib import WH_MAX_DEG
@@ -40,10 +42,12 @@ def _sympy_expr(expr: str, var: str):
     return sp.sympify(expr, locals=loc), v, t
 
 
+@functools.lru_cache(maxsize=256)
 def polynomial_coefficients(expr: str, var: str = "x"):
-    """Coefficients c_n(t) of expr = sum_n c_n(t) var^n.
+    """Coefficients c_n(t) of expr = sum_n c_n(t) var^n (cached per string: ensembles
+    parse the same expression for every member).
 
-    Returns (degree, [callable(t) -> float, ...]).  Raises NonPolynomialPhysics.
+    Returns (degree, (callable(t) -> float, ...)).  Raises NonPolynomialPhysics.
     """
     import sympy as sp
 
@@ -63,7 +67,7 @@ def polynomial_coefficients(expr: str, var: str = "x"):
         if c.free_symbols - {t}:
             raise NonPolynomialPhysics(f"{expr!r}: coefficient {c} depends on {c.free_symbols}")
         coeffs.append(sp.lambdify(t, c, "numpy"))
-    return deg, coeffs
+    return deg, tuple(coeffs)
 
 
 def coefficient_table(expr: str, var: str, times) -> tuple[int, np.ndarray]:
@@ -76,6 +80,7 @@ def coefficient_table(expr: str, var: str, times) -> tuple[int, np.ndarray]:
     return deg, tab
 
 
+@functools.lru_cache(maxsize=256)
 def depends_on_time(expr: str, var: str = "x") -> bool:
     import sympy as sp
 

@@ -20,8 +20,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import WH_C128, WH_C64, WH_MAX_DEG, WH_NOBS, WH_POT_TABLE, WH_SCHEME_LIE, WH_SCHEME_STRANG
-from .physics import (NonPolynomialPhysics, coefficient_table, depends_on_time, host_eval,
-                      polynomial_coefficients)
+from .physics import NonPolynomialPhysics, coefficient_table, depends_on_time, host_eval
 
 TUNING_FILE = Path(__file__).resolve().parent / "tuning.json"
 
@@ -109,10 +108,10 @@ class _Tables:
                 self.x_mask = self.x_mask * edge(20.0)  # expPotential_GrossPitaevskii, :514-516
         # potential coefficient rows
         if pot_table is not None:
-            deg, _ = polynomial_coefficients(potential, "x")
+            # explicit device coefficients: the degree is the highest non-zero column
             tab = np.ascontiguousarray(pot_table, dtype=np.float64)
             nz = np.nonzero(np.any(tab != 0, axis=0))[0]
-            deg = max(deg, int(nz.max()) if nz.size else 0)
+            deg = int(nz.max()) if nz.size else 0
             self.time_dependent = tab.shape[0] > 1
         else:
             self.time_dependent = depends_on_time(potential, "x")

@@ -157,3 +157,21 @@ def test_dropin_dirac_class_free_notebook_setup(cuda, tmp_path):
                                atol=1e-13)
     assert list(inst.timeSavedIndexRange) == [0, 10, 20]
     assert inst.TakabayashiAngle_end.shape == (128, 128)
+
+
+def test_dirac_mass_potential_variant(cuda):
+    """GPU_WignerDirac2D_MassPotential_4x4: Mass(x) in the potential factors (tabulated phases),
+    mass_half = 1e-4 in the kinetic factors, bare mass in the projectors."""
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    mass_s = "0.5 + 0.4*tanh(x/2.)"
+    mass_f = lambda t, x: 0.5 + 0.4 * np.tanh(x / 2.0)  # noqa: E731
+    vs, vf, dvf = CASES["klein"]
+    hip = WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=(vs, "0.", "0.", "0."), energy=True,
+                         mass_potential=mass_s, kinetic_mass_half=0.0001, device=cuda)
+    orc = Dirac4x4Oracle(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, [vf, zero, zero, zero], dpotential0=dvf,
+                         compute_energy=True, mass_potential=mass_f, kinetic_mass_half=0.0001)
+    W0 = wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 3.0, 1.0, Xg, 1.0, 1.0), orc)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 20)
+    assert rel_l2(W, W_ref) < 1e-10
+    check_obs(obs, rec)

@@ -97,9 +97,10 @@ def test_kinetic_step_exact_on_pure_state_form():
     np.testing.assert_allclose(out, np.einsum("i...,j...->ij...", Ua, np.conj(Ub)), atol=1e-14)
 
 
-def test_potential_step_exact_on_pure_state_form():
+@pytest.mark.parametrize("mass_potential", [None, lambda t, x: 0.5 + 0.4 * np.tanh(x / 2.0)])
+def test_potential_step_exact_on_pure_state_form(mass_potential):
     pots = [coulomb, lambda t, x: 0.2 * np.sin(x), lambda t, x: -0.1 * x, lambda t, x: 0.05 + 0 * x]
-    orc = make(potentials=pots)
+    orc = make(potentials=pots, mass_potential=mass_potential)
     x, th = orc.kx, orc.ktheta
 
     def Phi(y):
@@ -111,7 +112,7 @@ def test_potential_step_exact_on_pure_state_form():
     out = orc.potential_step(W, 0.0)
     Um, em = orc.potential_factor(0.0, -1)  # at x - theta/2
     # the + factor at x + theta/2 is the adjoint of the - factor there (D_theta = 0)
-    orc2 = make(potentials=pots)
+    orc2 = make(potentials=pots, mass_potential=mass_potential)
     orc2.kx = orc.kx + orc.ktheta  # evaluate the - factor at x + theta/2
     Ub, eb = orc2.potential_factor(0.0, -1)
     A = em * np.einsum("ik...,k...->i...", Um, a)
