@@ -253,6 +253,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run 8 plain device copies of 256 MiB (scale for the HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -326,6 +328,14 @@ def main():
         torch.cuda.empty_cache()
         out["extra"]["dirac_512^2_fp64"] = dirac_workload("fp64", 1000, dev)
         out["extra"]["dirac_512^2_fp32"] = dirac_workload("fp32", 1000, dev)
+    if a.calibrate_copy:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(8):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration"] = {"bytes_read_per_copy": 256 << 20, "bytes_written_per_copy": 256 << 20,
+                                   "copies": 8}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -315,6 +315,10 @@ typedef struct {
      F = Mass(x) c^2 dt; NULL = the constant mass_half */
   const double* mass_dm;
   const double* mass_sm;
+  /* Gross-Pitaevskii term (MakeGrossPitaevskiiTerms :2695-2715, gpu_sum_axis0 :1791-1817):
+     phase dt*a*Re(B- - B+), B-/+ = rho(x -/+ theta/2), rho = Re sum_p trace of the state */
+  double gp_coeff;            /* grossPitaevskiiCoefficient (host value); 0 = off */
+  const double* lambda_host;  /* [X] host Lambda grid (exp(i lambda theta/2)) */
 } wh_dirac_desc;
 
 typedef struct wh_dirac wh_dirac;

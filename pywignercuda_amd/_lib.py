@@ -151,7 +151,7 @@ class DiracDesc(ctypes.Structure):
         ("dv0", _DP), ("vsum", _DP),
         ("gamma_dt", c_double), ("dxdp", c_double), ("c_host", c_double), ("mc2_host", c_double),
         ("transmission_x", c_double), ("antiparticle", c_int), ("energy", c_int),
-        ("mass_dm", _DP), ("mass_sm", _DP),
+        ("mass_dm", _DP), ("mass_sm", _DP), ("gp_coeff", c_double), ("lambda_host", _DP),
     ]
 
 

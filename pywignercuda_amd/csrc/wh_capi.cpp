@@ -48,6 +48,8 @@ hipError_t launch_dirac_colsum(int prec, const DiracPlan* d, int X, hipStream_t 
 hipError_t launch_dirac_reduce(const DiracPlan* d, int state_add, int renorm, int anti, int inc, hipStream_t s);
 hipError_t launch_dirac_finish(int prec, const DiracPlan* d, int P, hipStream_t s);
 int dirac_x_blocks(int X, int P);
+int dirac_gp_blocks(int X, int P);
+hipError_t launch_dirac_gpshift(const DiracPlan* d, int nblocks, hipStream_t s);
 int dirac_p_blocks(int X, int P);
 }  // namespace wh
 
@@ -941,9 +943,9 @@ struct wh_dirac {
   wh_dirac_desc d{};
   wh::DiracPlan host{};  // host mirror of the device plan
   DevBuf plan_dev, pot, pot_dv, theta_decay, xmask, x_host, p_host, theta_host, dv0, vsum, mass_dm, mass_sm;
-  DevBuf colsums, rowpart, scale, step;
+  DevBuf colsums, rowpart, scale, step, rho, G, lambda_host;
   TwSet twx, twp;
-  int xblocks = 0, pblocks = 0;
+  int xblocks = 0, pblocks = 0, gpblocks = 0;
   hipStream_t capture_stream = nullptr;
   std::map<std::tuple<hipStream_t, int>, GraphEntry> graphs;
   int graph_chunk = 16;
@@ -984,6 +986,7 @@ int dirac_validate(const wh_dirac_desc* d) {
 int dirac_iteration(wh_dirac* e, hipStream_t s) {
   const auto* dp = e->plan_dev.as<wh::DiracPlan>();
   const int prec = e->d.precision;
+  if (e->d.gp_coeff != 0.0) WH_LAUNCH(wh::launch_dirac_gpshift(dp, e->gpblocks, s), "dirac_gpshift");
   WH_LAUNCH(wh::launch_dirac_xsweep(prec, dp, e->xblocks, s), "dirac_xsweep");
   WH_LAUNCH(wh::launch_dirac_psweep(prec, e->d.gamma_dt != 0.0, dp, e->pblocks, s), "dirac_psweep");
   WH_LAUNCH(wh::launch_dirac_reduce(dp, 1, 1, 1, 1, s), "dirac_reduce");
@@ -1038,7 +1041,7 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   e->xblocks = wh::dirac_x_blocks(d.x_dim, d.p_dim);
   e->pblocks = wh::dirac_p_blocks(d.x_dim, d.p_dim);
   if (e->xblocks < 1 || e->pblocks < 1) return fail(WH_INVALID_SIZE, "grid %zu x %zu", P, X);
-  WH_TRY(make_twiddles(e->twx, d.x_dim, d.precision, false));
+  WH_TRY(make_twiddles(e->twx, d.x_dim, d.precision, d.gp_coeff != 0.0));
   WH_TRY(make_twiddles(e->twp, d.p_dim, d.precision, false));
   if (d.pot_deg == WH_POT_TABLE) {
     const double zero_row[WH_MAX_DEG + 1] = {};
@@ -1065,6 +1068,14 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   const int zero = 0;
   WH_TRY(e->step.upload(&zero, sizeof(zero)));
   WH_TRY(e->plan_dev.alloc(sizeof(wh::DiracPlan)));
+  if (d.gp_coeff != 0.0) {
+    if (!d.lambda_host) return fail(WH_INVALID_DESCRIPTOR, "Gross-Pitaevskii needs lambda_host");
+    e->gpblocks = wh::dirac_gp_blocks(d.x_dim, d.p_dim);
+    if (e->gpblocks < 1) return fail(WH_INVALID_SIZE, "grid %zu x %zu", P, X);
+    WH_TRY(e->rho.alloc(X * sizeof(double)));
+    WH_TRY(e->G.alloc(X * P * sizeof(double)));
+    WH_TRY(e->lambda_host.upload(d.lambda_host, X * sizeof(double)));
+  }
 
   wh::DiracPlan& h = e->host;
   std::memset(&h, 0, sizeof(h));
@@ -1083,6 +1094,11 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   h.vsum = e->vsum.as<double>();
   h.mass_dm = e->mass_dm.as<double>();
   h.mass_sm = e->mass_sm.as<double>();
+  h.rho = e->rho.as<double>();
+  h.G = e->G.as<double>();
+  h.lambda_host = e->lambda_host.as<double>();
+  h.twX64 = e->twx.hi64.p;
+  h.gp_coeff = d.gp_coeff;
   h.twX_hi = d.precision == WH_C128 ? e->twx.hi64.p : e->twx.hi.p;
   h.twX_lo = e->twx.lo.p;
   h.twP_hi = d.precision == WH_C128 ? e->twp.hi64.p : e->twp.hi.p;

@@ -190,6 +190,7 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
       const double dv = d.pot_deg >= 0 ? poly(cf, d.pot_deg, x - 0.5 * th) - poly(cf, d.pot_deg, x + 0.5 * th)
                                        : d.pot_dv[size_t(i) * X + col];
       ph[j] = d.dt * dv;
+      if (d.gp_coeff != 0.0) ph[j] += d.dt * d.gp_coeff * d.G[size_t(i) * X + col];
     }
   }
 
@@ -460,9 +461,16 @@ __global__ void __launch_bounds__(kRedThreads) dirac_reduce_kernel(const DiracPl
     }
     anti_sum = red[0][kRedQ];
   }
+  __shared__ double fshare;
+  if (threadIdx.x == 0) fshare = renorm ? 1.0 / (d.dxdp * red[0][kDoNorm]) : 1.0;
+  __syncthreads();
+  if (d.gp_coeff != 0.0) {  // density of the (normalised) state for the next Gross-Pitaevskii phase
+    for (int col = threadIdx.x; col < d.X; col += kRedThreads)
+      d.rho[col] = fshare * d.colsums[size_t(col) * kDNCol + kDcTrRe];
+  }
   if (threadIdx.x == 0) {
     const double raw = d.dxdp * red[0][kDoNorm];
-    const double f = renorm ? 1.0 / raw : 1.0;
+    const double f = fshare;
     double* o = d.obs + size_t(tstate - d.obs_base) * kDNObs;
     for (int q = 0; q < kDoNeg; ++q) o[q] = f * d.dxdp * red[0][q];
     if (!d.energy) o[kDoEnergy] = o[kDoEnergy + 1] = 0.0;
@@ -530,6 +538,26 @@ __global__ void __launch_bounds__(kRedThreads) dirac_anti_kernel(const DiracPlan
   }
 }
 
+// G(theta, x) = Re(B- - B+) of the Gross-Pitaevskii density (MakeGrossPitaevskiiTerms,
+// GPU_WignerDirac2D_4x4.py:2695-2715), one theta row per team.
+constexpr int kDiracGpR = 16;
+__global__ void __launch_bounds__(256) dirac_gpshift_kernel(const DiracPlan* __restrict__ dp) {
+  __shared__ cplx<double> lds[lds_elems<256, kDiracGpR>()];
+  const DiracPlan& d = *dp;
+  Member m{};
+  m.rho = d.rho;
+  m.G = d.G;
+  m.theta_host = d.theta_host;
+  m.lambda_host = d.lambda_host;
+  m.twX64 = d.twX64;
+  m.X = d.X;
+  m.P = d.P;
+  constexpr int R = kDiracGpR, BLOCK = 256;
+#define BODY(n) gpshift_body<n, R, BLOCK>(m, int(blockIdx.x), lds)
+  WH_DISPATCH(d.X, BODY)
+#undef BODY
+}
+
 // ------------------------------------------------------------------ launchers
 template <int N>
 constexpr int xsweep_rows() {
@@ -585,6 +613,16 @@ hipError_t launch_dirac_colsum(int prec, const DiracPlan* d, int X, hipStream_t 
     hipLaunchKernelGGL(dirac_colsum_kernel<double>, dim3(nb), dim3(256), 0, s, d);
   else
     hipLaunchKernelGGL(dirac_colsum_kernel<float>, dim3(nb), dim3(256), 0, s, d);
+  return hipGetLastError();
+}
+
+int dirac_gp_blocks(int X, int P) {
+  const int rows = 256 / (X / kDiracGpR);
+  return rows > 0 && P % rows == 0 ? P / rows : -1;
+}
+
+hipError_t launch_dirac_gpshift(const DiracPlan* d, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(dirac_gpshift_kernel, dim3(nblocks), dim3(256), 0, s, d);
   return hipGetLastError();
 }
 

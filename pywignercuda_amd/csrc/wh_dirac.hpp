@@ -52,6 +52,10 @@ struct DiracPlan {
   const double* vsum;          // [X] sum_k V_k at kernel x (DiracEnergy, time-independent)
   const double* mass_dm;       // [P][X] F(x - theta/2) - F(x + theta/2), F = Mass(x) c^2 dt, or null
   const double* mass_sm;       // [P][X] F(x - theta/2) + F(x + theta/2)
+  double* rho;                 // [X] Gross-Pitaevskii density Re sum_p tr W (normalised state)
+  double* G;                   // [P][X] Re(B- - B+), B-/+ = rho(x -/+ theta/2)
+  const double* lambda_host;   // [X] host grid (shift theorem)
+  const void* twX64;           // FP64 twiddles for X (density shift)
   const void* twX_hi;
   const void* twX_lo;
   const void* twP_hi;
@@ -67,6 +71,7 @@ struct DiracPlan {
   double cl_a;                      // dt * gamma of Caldeira-Leggett order 2 (host values); 0 = off
   double dxdp, c_host, mc2_host;    // observables (host constants)
   double transmission_x;            // x threshold of the transmission record
+  double gp_coeff;                  // aGPitaevskii (host value); 0 = off
 };
 
 }  // namespace wh

@@ -10,7 +10,9 @@ factor U(p + lambda/2) W U(p - lambda/2)^dagger with half the mass, the potentia
 factors with the other half, D_Theta decay, the edge absorber (24 cells) and the
 trace normalisation after every step; per-step observables as in Run.  Vector
 potentials Potential_1..3 must vanish (the potential factors are then diagonal
-in the spinor indices); a non-zero one raises NotImplementedError.
+in the spinor indices); a non-zero one raises NotImplementedError.  The
+Gross-Pitaevskii term (density Re sum_p trace W, shifted by -/+ theta/2) is
+supported.
 """
 from __future__ import annotations
 
@@ -73,8 +75,6 @@ class WignerDirac4x4:
             if not _is_zero(potentials[k]):
                 raise NotImplementedError(
                     f"Potential_{k}_String = {potentials[k]!r}: non-zero vector potentials are not supported")
-        if gp_coeff != 0.0:
-            raise NotImplementedError("grossPitaevskiiCoefficient != 0 is not supported for the Dirac propagator")
         if gamma != 0.0 and damping not in (None, "CaldeiraLegget"):
             raise NotImplementedError(f"dampingModel {damping!r} is not supported (only 'CaldeiraLegget')")
         self.precision = precision
@@ -156,6 +156,8 @@ class WignerDirac4x4:
         d.c_host, d.mc2_host = self.c, self.mass * self.c * self.c
         d.transmission_x = float(transmission_x)
         d.antiparticle, d.energy = int(bool(antiparticle)), int(bool(energy))
+        d.gp_coeff = float(gp_coeff)  # passed as a kernel argument, not spliced (:3401, 3583)
+        d.lambda_host = ptr(host_axis(self.dLambda * self.X / 2.0, self.X, self.dLambda))
         self._plan = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(_lib.lib().wh_dirac_create(ctypes.byref(d), ctypes.byref(self._plan)), "wh_dirac_create")

@@ -235,4 +235,4 @@ def test_dirac_plan_rejects_unsupported_physics():
     with pytest.raises(NotImplementedError):
         WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=("0.", "0.", " -x ", "0."))
     with pytest.raises(NotImplementedError):
-        WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, gp_coeff=0.02)
+        WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, gamma=0.1, damping="ODM")

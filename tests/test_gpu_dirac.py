@@ -175,3 +175,18 @@ def test_dirac_mass_potential_variant(cuda):
     W, obs, W_ref, rec = run_both(hip, orc, W0, 20)
     assert rel_l2(W, W_ref) < 1e-10
     check_obs(obs, rec)
+
+
+def test_dirac_gross_pitaevskii(cuda):
+    """ConstantB_GP0.02-style nonlinearity (A = 0 here): phase dt a Re(B- - B+) from the trace density."""
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    vs, vf, dvf = CASES["klein"]
+    hip = WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=(vs, "0.", "0.", "0."), gp_coeff=0.5,
+                         device=cuda)
+    orc = Dirac4x4Oracle(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, [vf, zero, zero, zero], dpotential0=dvf,
+                         gp_coeff=0.5)
+    W0 = wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 3.0, 1.0, Xg, 1.0, 1.0), orc)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 20)
+    assert rel_l2(W, W_ref) < 1e-10
+    check_obs(obs, rec, energy=False)
