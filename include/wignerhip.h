@@ -319,6 +319,11 @@ typedef struct {
      phase dt*a*Re(B- - B+), B-/+ = rho(x -/+ theta/2), rho = Re sum_p trace of the state */
   double gp_coeff;            /* grossPitaevskiiCoefficient (host value); 0 = off */
   const double* lambda_host;  /* [X] host Lambda grid (exp(i lambda theta/2)) */
+  /* vector potentials Potential_1..3 as time-independent polynomials in x (a_deg[k] = -1:
+     identically zero).  Any non-zero one switches the potential step to two sweeps (the
+     factors then couple all 16 planes); incompatible with mass_dm/mass_sm */
+  int a_deg[3];
+  double a_coef[3][WH_MAX_DEG + 1];
 } wh_dirac_desc;
 
 typedef struct wh_dirac wh_dirac;

@@ -11,8 +11,9 @@ step, observables reduced on the device, copied once per saved frame).
 
 Extra attributes read if present: precision ('fp64' default or 'fp32'),
 truncateConstants (default True, the reference's '%f' splicing), useGraph
-(default True).  Not implemented (NotImplementedError): non-zero vector potentials
-Potential_1..3, the ODM damping model and antiParticleStepFiltering.
+(default True).  Not implemented (NotImplementedError): vector potentials that
+are not time-independent polynomials in x, the ODM damping model and
+antiParticleStepFiltering.
 """
 from __future__ import annotations
 

@@ -152,6 +152,7 @@ class DiracDesc(ctypes.Structure):
         ("gamma_dt", c_double), ("dxdp", c_double), ("c_host", c_double), ("mc2_host", c_double),
         ("transmission_x", c_double), ("antiparticle", c_int), ("energy", c_int),
         ("mass_dm", _DP), ("mass_sm", _DP), ("gp_coeff", c_double), ("lambda_host", _DP),
+        ("a_deg", c_int * 3), ("a_coef", (c_double * (WH_MAX_DEG + 1)) * 3),
     ]
 
 

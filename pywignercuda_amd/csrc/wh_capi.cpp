@@ -49,6 +49,8 @@ hipError_t launch_dirac_reduce(const DiracPlan* d, int state_add, int renorm, in
 hipError_t launch_dirac_finish(int prec, const DiracPlan* d, int P, hipStream_t s);
 int dirac_x_blocks(int X, int P);
 int dirac_gp_blocks(int X, int P);
+int dirac_pg_blocks(int X, int P);
+hipError_t launch_dirac_pgeneral(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_gpshift(const DiracPlan* d, int nblocks, hipStream_t s);
 int dirac_p_blocks(int X, int P);
 }  // namespace wh
@@ -945,7 +947,7 @@ struct wh_dirac {
   DevBuf plan_dev, pot, pot_dv, theta_decay, xmask, x_host, p_host, theta_host, dv0, vsum, mass_dm, mass_sm;
   DevBuf colsums, rowpart, scale, step, rho, G, lambda_host;
   TwSet twx, twp;
-  int xblocks = 0, pblocks = 0, gpblocks = 0;
+  int xblocks = 0, pblocks = 0, gpblocks = 0, pgblocks = 0;
   hipStream_t capture_stream = nullptr;
   std::map<std::tuple<hipStream_t, int>, GraphEntry> graphs;
   int graph_chunk = 16;
@@ -978,6 +980,11 @@ int dirac_validate(const wh_dirac_desc* d) {
   if (!d->x_host || !d->p_host || !d->theta_host) return fail(WH_INVALID_DESCRIPTOR, "host grids missing");
   if ((d->mass_dm == nullptr) != (d->mass_sm == nullptr))
     return fail(WH_INVALID_DESCRIPTOR, "mass_dm and mass_sm go together");
+  for (int k = 0; k < 3; ++k)
+    if (d->a_deg[k] < -1 || d->a_deg[k] > WH_MAX_DEG)
+      return fail(WH_INVALID_DESCRIPTOR, "vector potential %d: degree %d", k + 1, d->a_deg[k]);
+  if (d->mass_dm && (d->a_deg[0] >= 0 || d->a_deg[1] >= 0 || d->a_deg[2] >= 0))
+    return fail(WH_INVALID_DESCRIPTOR, "a position-dependent mass needs A1 = A2 = A3 = 0");
   if (!(d->dt == d->dt) || d->c == 0.0) return fail(WH_INVALID_DESCRIPTOR, "dt %g c %g", d->dt, d->c);
   return WH_SUCCESS;
 }
@@ -988,7 +995,10 @@ int dirac_iteration(wh_dirac* e, hipStream_t s) {
   const int prec = e->d.precision;
   if (e->d.gp_coeff != 0.0) WH_LAUNCH(wh::launch_dirac_gpshift(dp, e->gpblocks, s), "dirac_gpshift");
   WH_LAUNCH(wh::launch_dirac_xsweep(prec, dp, e->xblocks, s), "dirac_xsweep");
-  WH_LAUNCH(wh::launch_dirac_psweep(prec, e->d.gamma_dt != 0.0, dp, e->pblocks, s), "dirac_psweep");
+  if (e->host.general)
+    WH_LAUNCH(wh::launch_dirac_pgeneral(prec, e->d.gamma_dt != 0.0, dp, e->pgblocks, s), "dirac_pgeneral");
+  else
+    WH_LAUNCH(wh::launch_dirac_psweep(prec, e->d.gamma_dt != 0.0, dp, e->pblocks, s), "dirac_psweep");
   WH_LAUNCH(wh::launch_dirac_reduce(dp, 1, 1, 1, 1, s), "dirac_reduce");
   return WH_SUCCESS;
 }
@@ -1099,6 +1109,15 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   h.lambda_host = e->lambda_host.as<double>();
   h.twX64 = e->twx.hi64.p;
   h.gp_coeff = d.gp_coeff;
+  h.adeg0 = d.a_deg[0];
+  h.adeg1 = d.a_deg[1];
+  h.adeg2 = d.a_deg[2];
+  h.general = (d.a_deg[0] >= 0 || d.a_deg[1] >= 0 || d.a_deg[2] >= 0) ? 1 : 0;
+  std::memcpy(h.acoef, d.a_coef, sizeof(h.acoef));
+  if (h.general) {
+    e->pgblocks = wh::dirac_pg_blocks(d.x_dim, d.p_dim);
+    if (e->pgblocks < 1) return fail(WH_INVALID_SIZE, "grid %zu x %zu", P, X);
+  }
   h.twX_hi = d.precision == WH_C128 ? e->twx.hi64.p : e->twx.hi.p;
   h.twX_lo = e->twx.lo.p;
   h.twP_hi = d.precision == WH_C128 ? e->twp.hi64.p : e->twp.hi.p;

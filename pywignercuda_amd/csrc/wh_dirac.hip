@@ -162,6 +162,66 @@ __device__ __forceinline__ void dirac_accumulate(const DiracPlan& d, int k, cons
   }
 }
 
+// CaldeiraDissipatorOrder2 on one plane (GPU_WignerDirac2D_4x4.py:3011-3030), host grids:
+// D(Y) = F_p^-1[ theta F_p[ p Y ] ]; LW2 = W + a D W; W += 2 a D LW2, a = i dt gamma.
+template <typename T, int N, int R, int C>
+__device__ __forceinline__ void dirac_cl2(cplx<T> (&v)[R], const DiracPlan& d, int t, LdsStrided<T, C> view,
+                                          const Twiddles<T>& tw) {
+  constexpr int TT = N / R;
+  const T inv = T(1.0 / N);
+  cplx<T> y[R], lw[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) y[j] = cscale(v[j], T(d.p_host[t + j * TT]));
+  team_fft<T, N, R, -1>(y, t, view, tw);
+#pragma unroll
+  for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(d.theta_host[t + j * TT]));
+  team_fft<T, N, R, +1>(y, t, view, tw);
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    lw[j] = cadd(v[j], mul_i(cscale(y[j], inv), T(d.cl_a)));
+    y[j] = cscale(lw[j], T(d.p_host[t + j * TT]));
+  }
+  team_fft<T, N, R, -1>(y, t, view, tw);
+#pragma unroll
+  for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(d.theta_host[t + j * TT]));
+  team_fft<T, N, R, +1>(y, t, view, tw);
+#pragma unroll
+  for (int j = 0; j < R; ++j) v[j] = cadd(v[j], mul_i(cscale(y[j], inv), T(2.0 * d.cl_a)));
+}
+
+// Column sums of a tile: lanes of one column inside a wave are c + C*t; fold t with xor
+// shuffles, then across waves through LDS.
+template <int BLOCK, int C, typename L>
+__device__ __forceinline__ void dirac_store_colsums(const DiracPlan& d, int tile, double (&acc)[kDNCol], L* lds) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int off = C; off < 64; off <<= 1) {
+#pragma unroll
+    for (int q = 0; q < kDNCol; ++q) acc[q] += __shfl_xor(acc[q], off, 64);
+  }
+  constexpr int kWaves = (BLOCK + 63) / 64;
+  constexpr int kLanes = C < 64 ? C : 64;
+  __syncthreads();
+  double* red = reinterpret_cast<double*>(lds);
+  const int wave = tid >> 6, lane = tid & 63;
+  if (lane < kLanes) {
+#pragma unroll
+    for (int q = 0; q < kDNCol; ++q) red[(wave * kLanes + lane) * kDNCol + q] = acc[q];
+  }
+  __syncthreads();
+  if (tid < C) {
+    double sm[kDNCol];
+#pragma unroll
+    for (int q = 0; q < kDNCol; ++q) sm[q] = 0.0;
+    for (int w = 0; w < kWaves; ++w) {
+#pragma unroll
+      for (int q = 0; q < kDNCol; ++q) sm[q] += red[(w * kLanes + tid) * kDNCol + q];
+    }
+#pragma unroll
+    for (int q = 0; q < kDNCol; ++q) d.colsums[size_t(tile * C + tid) * kDNCol + q] = sm[q];
+  }
+}
+
 template <typename T, int N, int R, int BLOCK, bool CL>
 __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, cplx<T>* lds) {
   constexpr int TT = N / R;
@@ -234,27 +294,7 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
 #pragma unroll
       for (int j = 0; j < R; ++j) v[j] = cscale(v[j], inv);
 
-      if constexpr (CL) {
-        // D(Y) = F_p^-1[ theta F_p[ p Y ] ] on host grids; LW2 = W + a D W; W += 2 a D LW2, a = i dt gamma
-        cplx<T> y[R], lw[R];
-#pragma unroll
-        for (int j = 0; j < R; ++j) y[j] = cscale(v[j], T(d.p_host[t + j * TT]));
-        team_fft<T, N, R, -1>(y, t, view, tw);
-#pragma unroll
-        for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(d.theta_host[t + j * TT]));
-        team_fft<T, N, R, +1>(y, t, view, tw);
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-          lw[j] = cadd(v[j], mul_i(cscale(y[j], inv), T(d.cl_a)));
-          y[j] = cscale(lw[j], T(d.p_host[t + j * TT]));
-        }
-        team_fft<T, N, R, -1>(y, t, view, tw);
-#pragma unroll
-        for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(d.theta_host[t + j * TT]));
-        team_fft<T, N, R, +1>(y, t, view, tw);
-#pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = cadd(v[j], mul_i(cscale(y[j], inv), T(2.0 * d.cl_a)));
-      }
+      if constexpr (CL) dirac_cl2<T, N, R>(v, d, t, view, tw);
 
       dirac_accumulate<T, R, TT>(d, k, v, t, acc, trre);
 #pragma unroll
@@ -264,33 +304,160 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
 #pragma unroll
   for (int j = 0; j < R; ++j) acc[kDcNeg] += trre[j] < 0.0 ? trre[j] : 0.0;
 
-  // lanes of one column inside a wave: c + C*t; fold t with xor shuffles, then across waves
+  dirac_store_colsums<BLOCK, C>(d, tile, acc, lds);
+}
+
+
+// ------------------------------------------------------------------ p sweeps with vector potentials
+// With A = (A1, A2, A3) != 0 the potential factors P-(x - theta/2) (left) and P+(x + theta/2)
+// (right) couple all four spinor rows (columns) at each (theta, x) cell
+// (DiracPropagator_X_minus/plus_Theta, GPU_WignerDirac2D_4x4.py:357-670).  The left factor
+// acts on the 4 planes of one spinor column, the right factor on the 4 planes of one row, so
+// the step uses two sweeps: dirac_pleft (FFT_p, scalar phase, left factor; the state stays in
+// the (theta, x) picture) and dirac_pright (right factor, iFFT_p, damping, column sums).
+//
+// Factor at y with sign s (-1: left at x - theta/2, +1: right at x + theta/2), F = dt
+// sqrt((mh c^2)^2 + |A|^2): U11 = U22 = cos F + s i mh c^2 dt sF, U33 = U44 = conj,
+// U13 = U31 = -s i dt A3 sF, U14 = U32 = -s dt (i A1 + A2) sF, U23 = U41 = -s dt (i A1 - A2) sF,
+// U24 = U42 = -U13, zero elsewhere; sF = sin F / F.
+struct DiracPot {
+  cplx<double> u11, u33, u13, u14, u23;
+};
+__device__ __forceinline__ DiracPot dirac_pot_factor(const DiracPlan& d, double y, double s) {
+  const double a1 = d.adeg0 >= 0 ? poly(d.acoef[0], d.adeg0, y) : 0.0;
+  const double a2 = d.adeg1 >= 0 ? poly(d.acoef[1], d.adeg1, y) : 0.0;
+  const double a3 = d.adeg2 >= 0 ? poly(d.acoef[2], d.adeg2, y) : 0.0;
+  const double mdt = d.mass_phase;  // mass_half c^2 dt
+  const double F = sqrt(mdt * mdt + (a1 * a1 + a2 * a2 + a3 * a3) * d.dt * d.dt);
+  double sn, cs;
+  sincos_fp64(F, &sn, &cs);
+  const double sf = F > 0.0 ? sn / F : 1.0;
+  DiracPot u;
+  u.u11 = {cs, s * mdt * sf};
+  u.u33 = {cs, -s * mdt * sf};
+  u.u13 = {0.0, -s * d.dt * a3 * sf};
+  u.u14 = {-s * d.dt * a2 * sf, -s * d.dt * a1 * sf};
+  u.u23 = {s * d.dt * a2 * sf, -s * d.dt * a1 * sf};
+  return u;
+}
+
+template <typename T, int N, int R, int BLOCK>
+__device__ __forceinline__ void dirac_pleft_body(const DiracPlan& d, int tile, cplx<T>* lds) {
+  constexpr int TT = N / R;
+  constexpr int C = BLOCK / TT;
+  const int tid = threadIdx.x;
+  const int c = tid % C, t = tid / C;
+  const int X = d.X;
+  const int col = tile * C + c;
+  const size_t plane = size_t(d.P) * size_t(X);
+  cplx<T>* __restrict__ W0 = reinterpret_cast<cplx<T>*>(d.W);
+  const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twP_hi), reinterpret_cast<const cplx<T>*>(d.twP_lo)};
+  LdsStrided<T, C> view{lds + c};
+  const int step = *d.step;
+  const double x = d.dx * double(fidx(col, X));
+  const double mask = d.xmask ? d.xmask[col] : 1.0;
+  const int prow = step < d.pot_rows ? step : d.pot_rows - 1;
+  const double* __restrict__ cf = d.pot + size_t(prow) * (kMaxDeg + 1);
+  const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<T>));
+  for (int jc = 0; jc < 4; ++jc) {  // spinor column jc: planes 4 i + jc
+    cplx<T> v[4][R];
 #pragma unroll
-  for (int off = C; off < 64; off <<= 1) {
+    for (int q = 0; q < 4; ++q) {
+      cplx<T>* W = W0 + size_t(4 * q + jc) * plane;
 #pragma unroll
-    for (int q = 0; q < kDNCol; ++q) acc[q] += __shfl_xor(acc[q], off, 64);
-  }
-  constexpr int kWaves = (BLOCK + 63) / 64;
-  constexpr int kLanes = C < 64 ? C : 64;
-  __syncthreads();
-  double* red = reinterpret_cast<double*>(lds);
-  const int wave = tid >> 6, lane = tid & 63;
-  if (lane < kLanes) {
-#pragma unroll
-    for (int q = 0; q < kDNCol; ++q) red[(wave * kLanes + lane) * kDNCol + q] = acc[q];
-  }
-  __syncthreads();
-  if (tid < C) {
-    double s[kDNCol];
-#pragma unroll
-    for (int q = 0; q < kDNCol; ++q) s[q] = 0.0;
-    for (int w = 0; w < kWaves; ++w) {
-#pragma unroll
-      for (int q = 0; q < kDNCol; ++q) s[q] += red[(w * kLanes + tid) * kDNCol + q];
+      for (int j = 0; j < R; ++j) v[q][j] = at_off(W + size_t(j * TT) * X, boff);
+      team_fft<T, N, R, -1>(v[q], t, view, tw);
     }
 #pragma unroll
-    for (int q = 0; q < kDNCol; ++q) d.colsums[size_t(tile * C + tid) * kDNCol + q] = s[q];
+    for (int j = 0; j < R; ++j) {
+      const int i = t + j * TT;
+      const double th = d.dtheta * double(fidx(i, N));
+      // both scalar halves e- e+ = r e^{-i dt (V0(x - th/2) - V0(x + th/2) + a G)} applied here
+      double ph = d.pot_deg >= 0 ? poly(cf, d.pot_deg, x - 0.5 * th) - poly(cf, d.pot_deg, x + 0.5 * th)
+                                 : d.pot_dv[size_t(i) * X + col];
+      ph *= d.dt;
+      if (d.gp_coeff != 0.0) ph += d.dt * d.gp_coeff * d.G[size_t(i) * X + col];
+      const double r = (d.theta_decay ? d.theta_decay[i] : 1.0) * mask;
+      const cplx<double> e = expmi(r, ph);
+      const DiracPot u = dirac_pot_factor(d, x - 0.5 * th, -1.0);
+      const cplx<double> w0 = widen(v[0][j]), w1 = widen(v[1][j]), w2 = widen(v[2][j]), w3 = widen(v[3][j]);
+      const cplx<double> mu13{-u.u13.x, -u.u13.y};
+      const cplx<double> o0 = cadd(cmul(u.u11, w0), cadd(cmul(u.u13, w2), cmul(u.u14, w3)));
+      const cplx<double> o1 = cadd(cmul(u.u11, w1), cadd(cmul(u.u23, w2), cmul(mu13, w3)));
+      const cplx<double> o2 = cadd(cmul(u.u13, w0), cadd(cmul(u.u14, w1), cmul(u.u33, w2)));
+      const cplx<double> o3 = cadd(cmul(u.u23, w0), cadd(cmul(mu13, w1), cmul(u.u33, w3)));
+      v[0][j] = narrow<T>(cmul(e, o0));
+      v[1][j] = narrow<T>(cmul(e, o1));
+      v[2][j] = narrow<T>(cmul(e, o2));
+      v[3][j] = narrow<T>(cmul(e, o3));
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      cplx<T>* W = W0 + size_t(4 * q + jc) * plane;
+#pragma unroll
+      for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = v[q][j];
+    }
   }
+}
+
+template <typename T, int N, int R, int BLOCK, bool CL>
+__device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, cplx<T>* lds) {
+  constexpr int TT = N / R;
+  constexpr int C = BLOCK / TT;
+  const int tid = threadIdx.x;
+  const int c = tid % C, t = tid / C;
+  const int X = d.X;
+  const int col = tile * C + c;
+  const size_t plane = size_t(d.P) * size_t(X);
+  cplx<T>* __restrict__ W0 = reinterpret_cast<cplx<T>*>(d.W);
+  const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twP_hi), reinterpret_cast<const cplx<T>*>(d.twP_lo)};
+  LdsStrided<T, C> view{lds + c};
+  const double x = d.dx * double(fidx(col, X));
+  const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<T>));
+  const T inv = T(1.0 / N);
+  double acc[kDNCol];
+#pragma unroll
+  for (int q = 0; q < kDNCol; ++q) acc[q] = 0.0;
+  double trre[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) trre[j] = 0.0;
+  for (int ir = 0; ir < 4; ++ir) {  // spinor row ir: planes 4 ir + j
+    cplx<T> v[4][R];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      cplx<T>* W = W0 + size_t(4 * ir + q) * plane;
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[q][j] = at_off(W + size_t(j * TT) * X, boff);
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int i = t + j * TT;
+      const double th = d.dtheta * double(fidx(i, N));
+      const DiracPot u = dirac_pot_factor(d, x + 0.5 * th, 1.0);
+      const cplx<double> w0 = widen(v[0][j]), w1 = widen(v[1][j]), w2 = widen(v[2][j]), w3 = widen(v[3][j]);
+      const cplx<double> mu13{-u.u13.x, -u.u13.y};
+      // (W U)_{ij} = sum_k W_ik U_kj with U31 = U13, U32 = U14, U41 = U23, U42 = U24 = -U13
+      v[0][j] = narrow<T>(cadd(cmul(w0, u.u11), cadd(cmul(w2, u.u13), cmul(w3, u.u23))));
+      v[1][j] = narrow<T>(cadd(cmul(w1, u.u11), cadd(cmul(w2, u.u14), cmul(w3, mu13))));
+      v[2][j] = narrow<T>(cadd(cmul(w0, u.u13), cadd(cmul(w1, u.u23), cmul(w2, u.u33))));
+      v[3][j] = narrow<T>(cadd(cmul(w0, u.u14), cadd(cmul(w1, mu13), cmul(w3, u.u33))));
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      team_fft<T, N, R, +1>(v[q], t, view, tw);
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[q][j] = cscale(v[q][j], inv);
+      if constexpr (CL) dirac_cl2<T, N, R>(v[q], d, t, view, tw);
+      const int k = 4 * ir + q;
+      dirac_accumulate<T, R, TT>(d, k, v[q], t, acc, trre);
+      cplx<T>* W = W0 + size_t(k) * plane;
+#pragma unroll
+      for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = v[q][j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j) acc[kDcNeg] += trre[j] < 0.0 ? trre[j] : 0.0;
+  dirac_store_colsums<BLOCK, C>(d, tile, acc, lds);
 }
 
 // ------------------------------------------------------------------ kernels
@@ -346,6 +513,24 @@ __global__ void __launch_bounds__(kDiracPBlock) dirac_psweep_kernel(const DiracP
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracPBlock>()];
   const DiracPlan& d = *dp;
 #define BODY(n) dirac_psweep_body<T, n, dirac_rp<n>(), kDiracPBlock, CL>(d, int(blockIdx.x), lds)
+  WH_DIRAC_DISPATCH(d.P, BODY)
+#undef BODY
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kDiracXBlock) dirac_pleft_kernel(const DiracPlan* __restrict__ dp) {
+  __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
+  const DiracPlan& d = *dp;
+#define BODY(n) dirac_pleft_body<T, n, dirac_rx<n>(), kDiracXBlock>(d, int(blockIdx.x), lds)
+  WH_DIRAC_DISPATCH(d.P, BODY)
+#undef BODY
+}
+
+template <typename T, bool CL>
+__global__ void __launch_bounds__(kDiracXBlock) dirac_pright_kernel(const DiracPlan* __restrict__ dp) {
+  __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
+  const DiracPlan& d = *dp;
+#define BODY(n) dirac_pright_body<T, n, dirac_rx<n>(), kDiracXBlock, CL>(d, int(blockIdx.x), lds)
   WH_DIRAC_DISPATCH(d.P, BODY)
 #undef BODY
 }
@@ -420,6 +605,11 @@ __global__ void __launch_bounds__(kRedThreads) dirac_reduce_kernel(const DiracPl
     } else {
       dv = d.dv0[col];
       vs = d.vsum[col];
+    }
+    if (d.pot_deg >= 0 && d.general) {  // DiracEnergy adds Potential_1..3 like Potential_0 (:2801-2804)
+      if (d.adeg0 >= 0) vs += poly(d.acoef[0], d.adeg0, x);
+      if (d.adeg1 >= 0) vs += poly(d.acoef[1], d.adeg1, x);
+      if (d.adeg2 >= 0) vs += poly(d.acoef[2], d.adeg2, x);
     }
     const double tr[2] = {cs[kDcTrRe], cs[kDcTrIm]}, ptr[2] = {cs[kDcPTrRe], cs[kDcPTrIm]};
     const double a1[2] = {cs[kDcA1Re], cs[kDcA1Im]};
@@ -603,6 +793,32 @@ hipError_t launch_dirac_psweep(int prec, bool cl, const DiracPlan* d, int nblock
       hipLaunchKernelGGL((dirac_psweep_kernel<float, true>), dim3(nblocks), dim3(kDiracPBlock), 0, s, d);
     else
       hipLaunchKernelGGL((dirac_psweep_kernel<float, false>), dim3(nblocks), dim3(kDiracPBlock), 0, s, d);
+  }
+  return hipGetLastError();
+}
+
+int dirac_pg_blocks(int X, int P) {
+  int cols = 0;
+#define BODY(n) cols = kDiracXBlock / (n / dirac_rx<n>())
+  WH_DIRAC_DISPATCH(P, BODY)
+#undef BODY
+  return cols > 0 && X % cols == 0 ? X / cols : -1;
+}
+
+hipError_t launch_dirac_pgeneral(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s) {
+  const dim3 g(nblocks), b(kDiracXBlock);
+  if (prec == 0) {
+    hipLaunchKernelGGL(dirac_pleft_kernel<double>, g, b, 0, s, d);
+    if (cl)
+      hipLaunchKernelGGL((dirac_pright_kernel<double, true>), g, b, 0, s, d);
+    else
+      hipLaunchKernelGGL((dirac_pright_kernel<double, false>), g, b, 0, s, d);
+  } else {
+    hipLaunchKernelGGL(dirac_pleft_kernel<float>, g, b, 0, s, d);
+    if (cl)
+      hipLaunchKernelGGL((dirac_pright_kernel<float, true>), g, b, 0, s, d);
+    else
+      hipLaunchKernelGGL((dirac_pright_kernel<float, false>), g, b, 0, s, d);
   }
   return hipGetLastError();
 }

@@ -7,6 +7,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "wh_kernels.hpp"
+
 namespace wh {
 
 constexpr int kDNCol = 16;  // per-column sums of one p sweep (see DiracCol)
@@ -72,6 +74,9 @@ struct DiracPlan {
   double dxdp, c_host, mc2_host;    // observables (host constants)
   double transmission_x;            // x threshold of the transmission record
   double gp_coeff;                  // aGPitaevskii (host value); 0 = off
+  // vector potentials A1..A3 (Potential_1..3) as polynomials in x; general = any non-zero
+  int general, adeg0, adeg1, adeg2;
+  double acoef[3][kMaxDeg + 1];
 };
 
 }  // namespace wh

@@ -8,11 +8,12 @@
 Follows GPU_WignerDirac2D_4x4 (GPU_WignerDirac2D_4x4.py:1880-3720): the kinetic
 factor U(p + lambda/2) W U(p - lambda/2)^dagger with half the mass, the potential
 factors with the other half, D_Theta decay, the edge absorber (24 cells) and the
-trace normalisation after every step; per-step observables as in Run.  Vector
-potentials Potential_1..3 must vanish (the potential factors are then diagonal
-in the spinor indices); a non-zero one raises NotImplementedError.  The
-Gross-Pitaevskii term (density Re sum_p trace W, shifted by -/+ theta/2) is
-supported.
+trace normalisation after every step; per-step observables as in Run.  With
+Potential_1..3 = 0 the potential factors are diagonal in the spinor indices and
+the step is two fused sweeps; vector potentials (time-independent polynomials in
+x, e.g. the constant magnetic field of the ConstantB notebooks, A2 = -x) couple
+all 16 planes and take one more sweep.  The Gross-Pitaevskii term (density
+Re sum_p trace W, shifted by -/+ theta/2) is supported.
 """
 from __future__ import annotations
 
@@ -71,10 +72,20 @@ class WignerDirac4x4:
 
         if precision not in ("fp64", "fp32"):
             raise ValueError("precision must be 'fp64' or 'fp32'")
+        a_deg, a_coef = [-1, -1, -1], np.zeros((3, WH_MAX_DEG + 1))
         for k in (1, 2, 3):
-            if not _is_zero(potentials[k]):
+            if _is_zero(potentials[k]):
+                continue
+            if depends_on_time(potentials[k], "x"):
+                raise NotImplementedError(f"Potential_{k}_String = {potentials[k]!r}: time-dependent vector potential")
+            try:
+                deg, tab = coefficient_table(potentials[k], "x", [0.0])
+            except NonPolynomialPhysics as exc:
                 raise NotImplementedError(
-                    f"Potential_{k}_String = {potentials[k]!r}: non-zero vector potentials are not supported")
+                    f"Potential_{k}_String = {potentials[k]!r}: vector potentials must be polynomials in x") from exc
+            a_deg[k - 1], a_coef[k - 1] = deg, tab[0]
+        if max(a_deg) >= 0 and mass_potential is not None:
+            raise NotImplementedError("a position-dependent mass together with a vector potential")
         if gamma != 0.0 and damping not in (None, "CaldeiraLegget"):
             raise NotImplementedError(f"dampingModel {damping!r} is not supported (only 'CaldeiraLegget')")
         self.precision = precision
@@ -156,6 +167,11 @@ class WignerDirac4x4:
         d.c_host, d.mc2_host = self.c, self.mass * self.c * self.c
         d.transmission_x = float(transmission_x)
         d.antiparticle, d.energy = int(bool(antiparticle)), int(bool(energy))
+        for k in range(3):
+            d.a_deg[k] = a_deg[k]
+            for n in range(WH_MAX_DEG + 1):
+                d.a_coef[k][n] = float(a_coef[k][n])
+        self.vector_potential = max(a_deg) >= 0
         d.gp_coeff = float(gp_coeff)  # passed as a kernel argument, not spliced (:3401, 3583)
         d.lambda_host = ptr(host_axis(self.dLambda * self.X / 2.0, self.X, self.dLambda))
         self._plan = ctypes.c_void_p()

@@ -233,6 +233,6 @@ def test_dirac_plan_rejects_unsupported_physics():
     from pywignercuda_amd.dirac import WignerDirac4x4
 
     with pytest.raises(NotImplementedError):
-        WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=("0.", "0.", " -x ", "0."))
+        WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=("0.", "0.", " tanh(x) ", "0."))
     with pytest.raises(NotImplementedError):
         WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, gamma=0.1, damping="ODM")

@@ -190,3 +190,28 @@ def test_dirac_gross_pitaevskii(cuda):
     W, obs, W_ref, rec = run_both(hip, orc, W0, 20)
     assert rel_l2(W, W_ref) < 1e-10
     check_obs(obs, rec, energy=False)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_dirac_vector_potential_constant_field(cuda, precision):
+    """instances/WignerDirac2D_4x4/ConstantB.ipynb form: Potential_2 = -x (a constant field) plus
+    a small A1, A3 so that every coupling of the 4x4 factors is exercised; two p sweeps."""
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    vs, vf, dvf = CASES["klein"]
+    pots = (vs, "0.05*x", " -x ", "0.1 + 0.02*pow(x,2)")
+    hip = WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=pots, energy=True, D_theta=0.01,
+                         precision=precision, device=cuda)
+    assert hip.vector_potential
+    orc = Dirac4x4Oracle(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01,
+                         [vf, lambda t, x: 0.05 * x, lambda t, x: -x, lambda t, x: 0.1 + 0.02 * x * x],
+                         dpotential0=dvf, compute_energy=True, D_theta=0.01)
+    W0 = wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 3.0, 1.0, Xg, 1.0, 1.0), orc)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 20)
+    err = rel_l2(W, W_ref)
+    print(f"vector potential {precision}: rel L2 {err:.3e}")
+    if precision == "fp64":
+        assert err < 1e-10
+        check_obs(obs, rec)
+    else:
+        assert err < 1e-5
