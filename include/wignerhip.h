@@ -133,8 +133,8 @@ typedef struct {
   const double* theta_decay;  /* [P] exp(-frac*D_theta*theta^2*dt) (FFT.py:117; GP.py:80) */
   const double* lambda_decay; /* [X] exp(-D_lambda*lambda^2) (FFT.py:154) */
   const double* x_mask;       /* [X] absorbing mask (FFT.py:121-123, times :514-516 with GP) */
-  /* Caldeira-Leggett order-3 dissipator (FFT.py:1145-1157), applied after the V step */
-  int damping;                /* 0 none, 1 Caldeira-Leggett order 3 */
+  /* dissipator applied after the V step */
+  int damping;                /* 0 none, 1 Caldeira-Leggett order 3 (FFT.py:1145-1157), 2 ODM (below) */
   double gamma_dt;            /* dt*gammaDamping */
   const double* sigma_p;      /* [P] sign(p) f(p) at kernel p (theta_fp, FFT.py:399-431) */
   const double* theta_host;   /* [P] host Theta grid (FFT.py:578) */
@@ -155,6 +155,14 @@ typedef struct {
      for pot_deg = WH_POT_TABLE (time-independent V given as an arbitrary expression,
      Potential(t,x) spliced into expPotential, FFT.py:88-94, 869-873) */
   const double* pot_dv;
+  /* ODM Lindblad dissipator (damping = 2; Lindbladian_ODM_Order1, FFT.py:1516-1634, 1907-1910):
+     odm_g [4][P][X] = g(p + hbar lambda/2), g(p - hbar lambda/2) and both times sign(p -/+ ...),
+     g(q) = sqrt(L_material f(|q| + hbar/(2 L_material))) on host grids (SetA_ODM, :1196-1244);
+     odm_x [2][X] = cos 2x, sin 2x; odm_th [2][P] = cos(hbar theta), sin(hbar theta) */
+  const double* odm_g;
+  const double* odm_x;
+  const double* odm_th;
+  double odm_sign;            /* 1 damping, -1 pumping */
 } wh_wigner2d_desc;
 
 typedef struct wh_wigner2d wh_wigner2d;
@@ -210,7 +218,7 @@ typedef struct {
   double dt, hbar, mass;
   int kinetic_poly, kinetic_deg;
   double kin_coef[WH_MAX_DEG + 1];
-  int damping;     /* 0 none, 1 Caldeira-Leggett order 3 */
+  int damping;     /* 0 none, 1 Caldeira-Leggett order 3, 2 ODM (Lie scheme) */
   double gamma_dt;
   int renormalize;
   double gp_coeff;
@@ -226,6 +234,8 @@ typedef struct {
   const double *x_host, *p_host, *dv_host, *v0_host;
   double dxdp, inv_2m, dx_host, gp_rho_factor;
   const double* pot_dv; /* [P][X] Bopp difference of V for pot_deg = WH_POT_TABLE */
+  const double *odm_g, *odm_x, *odm_th; /* ODM tables, as in wh_wigner2d_desc */
+  double odm_sign;
 } wh_member_desc;
 
 typedef struct wh_ensemble wh_ensemble;

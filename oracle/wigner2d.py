@@ -17,6 +17,7 @@ Follows, line by line in behaviour (not in code):
 * Gross-Pitaevskii .............. GPU_Wigner2D_FFT.py:435-519, 1687-1705, 1845-1848;
                                   GPU_Wigner2D_GPitaevskii.py:466-496, 1743-1748, 1904-1956
 * Caldeira-Leggett order 3 ...... GPU_Wigner2D_FFT.py:399-431, 1135-1157, 1903-1913
+* ODM Lindblad (order 1) ........ GPU_Wigner2D_FFT.py:1196-1244, 1516-1634, 1907-1910
 * Lie driver / observables ...... GPU_Wigner2D_FFT.py:1754-1964
 * Strang driver / observables ... GPU_Wigner2D_GPitaevskii.py:1809-2055
 * '%f' constant splicing ........ GPU_Wigner2D_FFT.py:833-842 (optional emulation)
@@ -97,7 +98,7 @@ class Wigner2DOracle:
     def __init__(self, X, P, X_amp, P_amp, hbar, mass, dt, D_theta=0.0, D_lambda=0.0,
                  gamma=0.0, potential=None, dpotential=None, kinetic=None,
                  fp_damping=None, damping=None, gp_coeff=0.0, scheme="lie",
-                 mask=True, truncate_constants=False, time_dependent=False):
+                 mask=True, truncate_constants=False, time_dependent=False, L_material=None):
         self.g = Grid(X, P, X_amp, P_amp)
         self.hbar, self.mass, self.dt = float(hbar), float(mass), float(dt)
         self.D_theta, self.D_lambda, self.gamma = float(D_theta), float(D_lambda), float(gamma)
@@ -107,6 +108,7 @@ class Wigner2DOracle:
         self.fp_damping = fp_damping
         self.damping = damping
         self.gp_coeff = float(gp_coeff)
+        self.L_material = L_material  # ODM (FFT.py:1196-1197)
         self.scheme = scheme
         self.mask = mask
         self.time_dependent = bool(time_dependent)
@@ -203,6 +205,41 @@ class Wigner2DOracle:
         LW = LW + (2.0 / 2.0) * a * self.cl_operator(LW)
         return W + 2.0 * a * self.cl_operator(LW)
 
+    # ------------------------------------------------------------------ ODM Lindblad dissipator
+    def odm_tables(self):
+        """SetA_ODM (FFT.py:1206-1244) on host grids: g(p) = sqrt(L f(|p| + hbar/(2L))) (g_ODM,
+        :1196-1200) at p -/+ hbar lambda/2, its signed form, and the cos/sin of x +/- hbar theta/2."""
+        g = self.g
+        h, L = self.hbar, self.L_material
+
+        def gfun(q):
+            return np.sqrt(L * self.fp_damping(np.abs(q) + h / (2.0 * L)))
+
+        qp = g.p_host + 0.5 * h * g.lambda_host
+        qm = g.p_host - 0.5 * h * g.lambda_host
+        gp_, gm_ = gfun(qp), gfun(qm)
+        xp = g.x_host + 0.5 * h * g.theta_host
+        xm = g.x_host - 0.5 * h * g.theta_host
+        return dict(gp=gp_, gm=gm_, gsp=gp_ * np.sign(qp), gsm=gm_ * np.sign(qm),
+                    cp=np.cos(xp), cm=np.cos(xm), sp=np.sin(xp), sm=np.sin(xm))
+
+    def odm_operator(self, W, weight=1.0, sign=1.0, n=1.0):
+        """_Lindbladian_ODM (FFT.py:1516-1634): sum over six terms of
+        h_k(theta, x) F_p F_x^-1 [ g_k(p, lambda) F_x W ], back to p, times dt 2 gamma weight / n."""
+        T = self._cached("odm", self.odm_tables)
+
+        def term(gk):
+            Y = np.fft.ifft(np.fft.fft(W, axis=1) * gk, axis=1)
+            return np.fft.fft(Y, axis=0)
+
+        LW = term(T["gp"] * T["gm"]) * (T["cp"] * T["cm"])
+        LW = LW + term(T["gp"] * T["gsm"]) * T["cm"] * T["sp"] * (1j * sign)
+        LW = LW + term(T["gsp"] * T["gm"]) * T["sm"] * T["cp"] * (-1j * sign)
+        LW = LW + term(T["gsp"] * T["gsm"]) * (T["sp"] * T["sm"])
+        LW = LW + term(-0.5 * T["gp"] ** 2)
+        LW = LW + term(-0.5 * T["gm"] ** 2)
+        return np.fft.ifft(LW, axis=0) * (self.dt * 2.0 * self.gamma * weight / n)
+
     # ------------------------------------------------------------------ sweeps
     def _cached(self, key, fn):
         if key not in self._cache:
@@ -280,6 +317,8 @@ class Wigner2DOracle:
             if self.gamma != 0.0:
                 if self.damping == "CaldeiraLeggett":
                     W = self.caldeira_order3(W)
+                elif self.damping == "ODM":  # Lindbladian_ODM_Order1, weight 1, sign 1 (FFT.py:1907-1910)
+                    W = W + self.odm_operator(W)
                 W = W / (W.sum() * dxdp)
             if self.scheme == "strang" and record:
                 for k, v in self.observables(W).items():

@@ -250,14 +250,16 @@ class Propagator_Base:
 
     # ---------------------------------------------------------------- device plan
     def _make_plan(self, scheme, gp_coeff, kinetic):
-        if self.gammaDamping != 0.0 and self.dampingFunction not in ("CaldeiraLeggett",):
-            raise NotImplementedError(
-                f"dampingFunction {self.dampingFunction!r}: only 'CaldeiraLeggett' is implemented on the GPU")
+        if self.gammaDamping != 0.0 and self.dampingFunction not in ("CaldeiraLeggett", "ODM"):
+            raise NotImplementedError(f"dampingFunction {self.dampingFunction!r} is not implemented")
+        if self.gammaDamping != 0.0 and self.dampingFunction == "ODM" and scheme != "lie":
+            raise NotImplementedError("ODM damping is part of GPU_Wigner2D_FFT (Lie scheme) only")
         return Wigner2D(
             self.X_gridDIM, self.P_gridDIM, self.X_amplitude, self.P_amplitude, self.hBar, self.mass, self.dt,
             potential=self.potentialString, dpotential=self.dPotentialString, kinetic=kinetic, scheme=scheme,
             D_theta=self.D_Theta, D_lambda=self.D_Lambda, gamma=self.gammaDamping,
-            damping="CaldeiraLeggett" if self.gammaDamping != 0.0 else None,
+            damping=self.dampingFunction if self.gammaDamping != 0.0 else None,
+            L_material=getattr(self, "L_material", None),
             fp_damping=self.fp_Damping_String, gp_coeff=gp_coeff, mask=self.absorbingMask,
             truncate_constants=self.truncateConstants, precision=self.precision,
             time_rows=self.timeSteps + 3)

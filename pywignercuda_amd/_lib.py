@@ -109,7 +109,7 @@ class Wigner2DDesc(ctypes.Structure):
         ("gp_coeff", c_double), ("gp_rho_factor", c_double), ("lambda_host", _DP),
         ("x_host", _DP), ("p_host", _DP), ("dv_host", _DP), ("v0_host", _DP),
         ("dxdp", c_double), ("inv_2m", c_double), ("dx_host", c_double),
-        ("pot_dv", _DP),
+        ("pot_dv", _DP), ("odm_g", _DP), ("odm_x", _DP), ("odm_th", _DP), ("odm_sign", c_double),
     ]
 
 
@@ -135,7 +135,7 @@ class MemberDesc(ctypes.Structure):
         ("theta_host", _DP), ("lambda_host", _DP),
         ("x_host", _DP), ("p_host", _DP), ("dv_host", _DP), ("v0_host", _DP),
         ("dxdp", c_double), ("inv_2m", c_double), ("dx_host", c_double), ("gp_rho_factor", c_double),
-        ("pot_dv", _DP),
+        ("pot_dv", _DP), ("odm_g", _DP), ("odm_x", _DP), ("odm_th", _DP), ("odm_sign", c_double),
     ]
 
 

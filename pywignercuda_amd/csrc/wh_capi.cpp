@@ -42,6 +42,10 @@ int fft_cols_batch_multiple(int prec, int N);
 hipError_t launch_adj_psweep(int prec, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_grad_reduce(const Member* members, int nmembers, double* grad, double scale, hipStream_t s);
 int adj_cols(int P);
+int odm_rows_per_tile(int X);
+int odm_cols_per_tile(int P);
+hipError_t launch_odm(int prec, const Member* members, const unsigned* work_rows, int nrows,
+                      const unsigned* work_cols, int ncols, hipStream_t s);
 hipError_t launch_dirac_xsweep(int prec, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_psweep(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_colsum(int prec, const DiracPlan* d, int X, hipStream_t s);
@@ -274,6 +278,7 @@ namespace {
 struct MemberBufs {
   DevBuf pot, theta_decay, lambda_decay, x_mask, sigma_p, theta_host, lambda_host;
   DevBuf x_host, p_host, dv_host, v0_host, colsums, scale, rho, G, colpart, pot_dv;
+  DevBuf odm_g, odm_x, odm_th, odm_s;
 };
 
 struct WorkList {
@@ -300,7 +305,7 @@ struct wh_ensemble {
   DevBuf mem_dev;
   std::map<int, std::unique_ptr<TwSet>> tw;  // by size
   int xvar = 0, pvar = 0, pclvar = 0;
-  WorkList work_x, work_p, work_colsum, work_scale, work_gp, work_adj;
+  WorkList work_x, work_p, work_colsum, work_scale, work_gp, work_adj, work_odm_x, work_odm_p;
   DevBuf step, done, gcols;
   hipStream_t capture_stream = nullptr;
   std::map<std::tuple<hipStream_t, int>, GraphEntry> graphs;  // (stream, chunk)
@@ -316,6 +321,7 @@ struct wh_ensemble {
   }
   bool gp() const { return d.gp_coeff != 0.0; }
   bool cl() const { return d.damping == 1; }
+  bool odm() const { return d.damping == 2; }
 };
 
 struct wh_wigner2d {
@@ -339,6 +345,8 @@ int validate_member(const wh_ensemble_desc& d, const wh_member_desc& m, int i) {
   }
   if (d.damping == 1 && (!m.sigma_p || !m.theta_host))
     return fail(WH_INVALID_DESCRIPTOR, "member %d: Caldeira-Leggett needs sigma_p and theta_host", i);
+  if (d.damping == 2 && (!m.odm_g || !m.odm_x || !m.odm_th))
+    return fail(WH_INVALID_DESCRIPTOR, "member %d: ODM needs odm_g, odm_x and odm_th", i);
   if (d.gp_coeff != 0.0 && (!m.lambda_host || !m.theta_host))
     return fail(WH_INVALID_DESCRIPTOR, "member %d: Gross-Pitaevskii needs lambda_host and theta_host", i);
   if (!m.x_host || !m.p_host) return fail(WH_INVALID_DESCRIPTOR, "member %d: observables need x_host and p_host", i);
@@ -356,8 +364,10 @@ int validate_common(const wh_ensemble_desc& d) {
     return fail(WH_INVALID_DESCRIPTOR, "kinetic degree %d", d.kinetic_deg);
   if (!(d.dt == d.dt) || d.hbar == 0.0 || d.mass == 0.0)
     return fail(WH_INVALID_DESCRIPTOR, "dt/hbar/mass: %g %g %g", d.dt, d.hbar, d.mass);
-  if (d.damping != 0 && d.damping != 1)
-    return fail(WH_INVALID_DESCRIPTOR, "damping %d (0 none, 1 Caldeira-Leggett)", d.damping);
+  if (d.damping < 0 || d.damping > 2)
+    return fail(WH_INVALID_DESCRIPTOR, "damping %d (0 none, 1 Caldeira-Leggett, 2 ODM)", d.damping);
+  if (d.damping == 2 && d.scheme != WH_SCHEME_LIE)
+    return fail(WH_INVALID_DESCRIPTOR, "the ODM dissipator is part of the Lie scheme (GPU_Wigner2D_FFT)");
   return WH_SUCCESS;
 }
 
@@ -375,7 +385,7 @@ bool fits_all(const wh_ensemble* e, const wh::SweepVariant& v, bool x_axis) {
 int build_work_lists(wh_ensemble* e) {
   const auto& xv = wh::kXVariants[e->xvar];
   const auto& pv = e->cl() ? wh::kPCLVariants[e->pclvar] : wh::kPVariants[e->pvar];
-  std::vector<unsigned> wx, wp, wc, ws, wg;
+  std::vector<unsigned> wx, wp, wc, ws, wg, wox, wop;
   for (int i = 0; i < e->n; ++i) {
     const auto& m = e->mem[i];
     const unsigned hi = unsigned(i) << 16;
@@ -395,6 +405,12 @@ int build_work_lists(wh_ensemble* e) {
       const int grows = wh::tile_lines(m.X, wh::SweepVariant{16, 256});
       for (int t = 0; t < m.P / grows; ++t) wg.push_back(hi | unsigned(t));
     }
+    if (e->odm()) {
+      const int orows = wh::odm_rows_per_tile(m.X), ocols = wh::odm_cols_per_tile(m.P);
+      if (orows < 1 || ocols < 1 || m.P % orows || m.X % ocols) return fail(WH_INVALID_SIZE, "ODM tiles");
+      for (int t = 0; t < m.P / orows; ++t) wox.push_back(hi | unsigned(t));
+      for (int t = 0; t < m.X / ocols; ++t) wop.push_back(hi | unsigned(t));
+    }
   }
   if (wx.size() > 0x7fffffff || ws.size() > 0x7fffffff) return fail(WH_INVALID_SIZE, "work list too long");
   for (const auto& v : {&wx, &wp, &wc, &ws, &wg})
@@ -405,6 +421,10 @@ int build_work_lists(wh_ensemble* e) {
   WH_TRY(e->work_colsum.upload(wc));
   WH_TRY(e->work_scale.upload(ws));
   if (e->gp()) WH_TRY(e->work_gp.upload(wg));
+  if (e->odm()) {
+    WH_TRY(e->work_odm_x.upload(wox));
+    WH_TRY(e->work_odm_p.upload(wop));
+  }
   e->drop_graphs();
   return WH_SUCCESS;
 }
@@ -457,6 +477,12 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
       WH_TRY(b->rho.alloc(X * sizeof(double)));
       WH_TRY(b->G.alloc(X * P * sizeof(double)));
     }
+    if (desc->damping == 2) {
+      WH_TRY(b->odm_g.upload(md.odm_g, 4 * X * P * sizeof(double)));
+      WH_TRY(b->odm_x.upload(md.odm_x, 2 * X * sizeof(double)));
+      WH_TRY(b->odm_th.upload(md.odm_th, 2 * P * sizeof(double)));
+      WH_TRY(b->odm_s.alloc(2 * X * P * (prec == WH_C128 ? 16 : 8)));
+    }
     wh::Member& m = e->mem[i];
     std::memset(&m, 0, sizeof(m));
     m.colsums = b->colsums.as<double>();
@@ -494,6 +520,12 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
     m.inv_2m = md.inv_2m;
     m.dx_host = md.dx_host;
     m.rho_factor = md.gp_rho_factor;
+    m.odm_g = b->odm_g.as<double>();
+    m.odm_x = b->odm_x.as<double>();
+    m.odm_th = b->odm_th.as<double>();
+    m.odm_s = b->odm_s.p;
+    m.odm_c = desc->gamma_dt * 2.0;
+    m.odm_sign = md.odm_sign;
     e->bufs.push_back(std::move(b));
   }
   // default variants: the first that fits every member
@@ -610,7 +642,19 @@ int iteration(wh_ensemble* e, bool merge, hipStream_t s) {
   const auto& d = e->d;
   const int cl = e->cl() ? wh::kCL : 0;
   const int renorm = d.renormalize;
-  if (d.scheme == WH_SCHEME_LIE) {
+  if (d.scheme == WH_SCHEME_LIE && e->odm()) {
+    // K; V; W += L_ODM(W) (three sweeps); column sums of the result; renormalise
+    if (e->gp()) WH_TRY(gpshift(e, s));
+    WH_TRY(xsweep(e, wh::kScaleOnLoad, s));
+    WH_TRY(psweep(e, e->gp() ? wh::kGP : 0, 1.0, 0, s));
+    WH_LAUNCH(wh::launch_odm(d.precision, e->mem_dev.as<wh::Member>(), e->work_odm_x.buf.as<unsigned>(),
+                             e->work_odm_x.n, e->work_odm_p.buf.as<unsigned>(), e->work_odm_p.n, s),
+              "odm");
+    WH_LAUNCH(wh::launch_colsum(d.precision, e->mem_dev.as<wh::Member>(), e->work_colsum.buf.as<unsigned>(),
+                                e->work_colsum.n, s),
+              "colsum");
+    WH_TRY(reduce(e, renorm, 1, e->gp(), 1, true, s));
+  } else if (d.scheme == WH_SCHEME_LIE) {
     if (e->gp()) WH_TRY(gpshift(e, s));
     WH_TRY(xsweep(e, wh::kScaleOnLoad, s));
     WH_TRY(psweep(e, wh::kColSums | cl | (e->gp() ? wh::kGP : 0), 1.0, 0, s));
@@ -799,6 +843,10 @@ extern "C" int wh_wigner2d_create(const wh_wigner2d_desc* d, wh_wigner2d** out) 
   m.pot_rows = d->pot_rows;
   m.pot_coef = d->pot_coef;
   m.pot_dv = d->pot_dv;
+  m.odm_g = d->odm_g;
+  m.odm_x = d->odm_x;
+  m.odm_th = d->odm_th;
+  m.odm_sign = d->odm_sign;
   m.theta_decay = d->theta_decay;
   m.lambda_decay = d->lambda_decay;
   m.x_mask = d->x_mask;
@@ -899,7 +947,7 @@ extern "C" int wh_wigner2d_adjoint_step(wh_wigner2d* p, void* Lambda, const void
     return fail(WH_INVALID_VALUE, "wh_wigner2d_adjoint_step: Lambda %p W_next %p grad %p step %d", Lambda, W_next,
                 (void*)grad, step);
   wh_ensemble* e = p->e;
-  if (e->d.scheme != WH_SCHEME_LIE || e->cl() || e->gp())
+  if (e->d.scheme != WH_SCHEME_LIE || e->cl() || e->odm() || e->gp())
     return fail(WH_INVALID_VALUE, "the adjoint step covers the Lie scheme without damping or Gross-Pitaevskii");
   if (e->mem[0].pot_deg == WH_POT_TABLE)
     return fail(WH_INVALID_VALUE, "the adjoint step needs V as polynomial coefficients (gradients are per coefficient)");

@@ -63,6 +63,12 @@ struct Member {
   int X, P, pot_rows, pot_deg;
   double dx, dp, dtheta, dlambda;  // kernel coordinate steps
   double dxdp, inv_2m, dx_host, rho_factor;
+  // ODM Lindblad dissipator (dampingFunction 'ODM'), order 1
+  const double* odm_g;         // [4][P][X] g(p + hbar l/2), g(p - hbar l/2) and their signed forms
+  const double* odm_x;         // [2][X] cos 2x, sin 2x (host x)
+  const double* odm_th;        // [2][P] cos(hbar theta), sin(hbar theta) (host theta)
+  void* odm_s;                 // [2][P][X] scratch in the state precision (lambda picture)
+  double odm_c, odm_sign;      // 2 gamma dt; +1 damping, -1 pumping
   // adjoint step (single-grid plans): forward state after the step, gradient column sums
   const void* W2;              // [P][X] W_{n+1}
   double* gcols;               // [X][kMaxDeg+1]

@@ -68,7 +68,7 @@ class _Tables:
 
     def __init__(self, X, P, X_amp, P_amp, hbar, mass, dt, *, potential, dpotential, scheme,
                  D_theta, D_lambda, gamma, damping, fp_damping, gp_coeff, gp_rho_factor, mask,
-                 truncate_constants, time_rows, pot_table):
+                 truncate_constants, time_rows, pot_table, L_material=None, odm_sign=1.0):
         self.X, self.P = int(X), int(P)
         self.X_amp, self.P_amp = float(X_amp), float(P_amp)
         tr = percent_f if truncate_constants else float
@@ -136,6 +136,29 @@ class _Tables:
         if damping == "CaldeiraLeggett" and gamma != 0.0:
             f = np.broadcast_to(np.asarray(host_eval(fp_damping, p=kp), dtype=np.float64), kp.shape)
             self.sigma = np.where(kp >= 0.0, f, -f)
+        # ODM tables (SetA_ODM, GPU_Wigner2D_FFT.py:1196-1244) on host grids, full precision
+        self.odm_g = self.odm_x = self.odm_th = None
+        self.odm_sign = float(odm_sign)
+        if damping == "ODM" and gamma != 0.0:
+            if scheme != "lie":
+                raise ValueError("the ODM dissipator belongs to the Lie scheme (GPU_Wigner2D_FFT)")
+            if L_material is None or fp_damping is None:
+                raise ValueError("ODM damping needs L_material and fp_damping")
+            h, L = float(hbar), float(L_material)
+
+            def gfun(q):
+                f = np.broadcast_to(np.asarray(host_eval(fp_damping, p=np.abs(q) + h / (2.0 * L)),
+                                               dtype=np.float64), q.shape)
+                return np.sqrt(L * f)
+
+            lam = self.lambda_host[None, :]
+            qp = self.p_host[:, None] + 0.5 * h * lam
+            qm = self.p_host[:, None] - 0.5 * h * lam
+            gp_, gm_ = gfun(qp), gfun(qm)
+            self.odm_g = np.ascontiguousarray(np.stack([gp_, gm_, gp_ * np.sign(qp), gm_ * np.sign(qm)]))
+            self.odm_x = np.ascontiguousarray(np.stack([np.cos(2.0 * self.x_host), np.sin(2.0 * self.x_host)]))
+            ht = h * self.theta_host
+            self.odm_th = np.ascontiguousarray(np.stack([np.cos(ht), np.sin(ht)]))
         # observables (FFT.py:1747, 1785-1792): host grids, dV/dx and V at t = 0
         if dpotential:
             dv = host_eval(dpotential, t=0.0, x=self.x_host)
@@ -163,6 +186,8 @@ class _Tables:
         d.pot_deg, d.pot_rows = self.pot_deg, self.pot_table.shape[0]
         d.pot_coef = self.ptr(self.pot_table)
         d.pot_dv = self.ptr(self.pot_dv)
+        d.odm_g, d.odm_x, d.odm_th = self.ptr(self.odm_g), self.ptr(self.odm_x), self.ptr(self.odm_th)
+        d.odm_sign = self.odm_sign
         d.theta_decay = self.ptr(self.theta_decay)
         d.lambda_decay = self.ptr(self.lambda_decay)
         d.x_mask = self.ptr(self.x_mask)
@@ -191,7 +216,7 @@ def _common(d, *, precision, scheme, t: _Tables, dt, gamma, damping, kinetic, gp
     for n in range(WH_MAX_DEG + 1):
         d.kin_coef[n] = float(kcoef[n])
     cl = damping == "CaldeiraLeggett" and gamma != 0.0
-    d.damping = 1 if cl else 0
+    d.damping = 1 if cl else (2 if (damping == "ODM" and gamma != 0.0) else 0)
     d.gamma_dt = float(dt) * float(gamma)  # host constants (FFT.py:1150-1156)
     d.renormalize = 1 if gamma != 0.0 else 0
     d.gp_coeff = t.k_agp
@@ -229,7 +254,8 @@ class Wigner2D(_PlanBase):
     scheme      : 'lie' (K; V -- GPU_Wigner2D_FFT) or 'strang' (V/2; K; V/2 --
                   GPU_Wigner2D_GPitaevskii).
     kinetic     : ignored by the Lie scheme (the reference hard-wires dt*lambda*p/m).
-    damping     : None or 'CaldeiraLeggett' (order-3 composite).
+    damping     : None, 'CaldeiraLeggett' (order-3 composite) or 'ODM' (Lindblad dissipator of
+                  order 1, Lie scheme; needs L_material and fp_damping; odm_sign -1 pumps).
     gp_coeff    : Gross-Pitaevskii coefficient a_GP (0 = off).
     mask        : absorbing edge mask of the potential kernel (on in the reference).
     truncate_constants : emulate the '%f' splicing of kernel constants.
@@ -245,7 +271,7 @@ class Wigner2D(_PlanBase):
                  kinetic=None, scheme="lie", D_theta=0.0, D_lambda=0.0, gamma=0.0,
                  damping=None, fp_damping=None, gp_coeff=0.0, gp_rho_factor=None, mask=True,
                  truncate_constants=False, precision="fp64", time_rows=None, pot_table=None,
-                 device=None, variants=None):
+                 device=None, variants=None, L_material=None, odm_sign=1.0):
         import torch
 
         if precision not in ("fp64", "fp32"):
@@ -258,7 +284,8 @@ class Wigner2D(_PlanBase):
         t = _Tables(X, P, X_amp, P_amp, hbar, mass, dt, potential=potential, dpotential=dpotential,
                     scheme=scheme, D_theta=D_theta, D_lambda=D_lambda, gamma=gamma, damping=damping,
                     fp_damping=fp_damping, gp_coeff=gp_coeff, gp_rho_factor=gp_rho_factor, mask=mask,
-                    truncate_constants=truncate_constants, time_rows=time_rows, pot_table=pot_table)
+                    truncate_constants=truncate_constants, time_rows=time_rows, pot_table=pot_table,
+                    L_material=L_material, odm_sign=odm_sign)
         self.tables = t
         self.X, self.P = t.X, t.P
         self.dX, self.dP = t.dX, t.dP

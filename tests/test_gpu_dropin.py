@@ -11,6 +11,7 @@ Sources (raw notebook JSON lines):
   Caldeira free particle instances/Wigner2D/Wigner2D_FFT_Caldeira_Free.ipynb (cell 10 output)
   Caldeira + GP          instances/Wigner2D/Wigner2D_FFT_Caldeira_GPitaevskii_X2.ipynb:999
   GP Strang              instances/Wigner2D_GPitaevskii/Wigner2D_GPitaevskii_X2.ipynb:405
+  ODM X2                 instances/Wigner2D/Wigner2D_FFT_ODM_X2_D0.ipynb:453
 """
 import numpy as np
 import pytest
@@ -34,10 +35,12 @@ def _check(inst, wmin, wmax):
     assert abs(norm - 1.0) < 1e-9
 
 
-def _fft_frame(base, X_amp, P_amp, dt, steps, D_Theta, gamma, eps, gp, omega, tmp_path):
+def _fft_frame(base, X_amp, P_amp, dt, steps, D_Theta, gamma, eps, gp, omega, tmp_path, damping="CaldeiraLeggett"):
     class frame(base):
         def __init__(self):
-            self.dampingFunction = "CaldeiraLeggett"
+            self.dampingFunction = damping
+            self.L_material = 3.0
+            self.epsilon = eps
             self.grossPitaevskiiCoefficient = gp
             self.omega = omega
             X2_constant = 0.5 * 1.0 * self.omega ** 2
@@ -118,3 +121,14 @@ def test_dropin_output_file_and_attributes(cuda, tmp_path):
     assert f["200"].shape == (512, 512)
     inst.LoadEhrenfestFromFile()
     assert inst.Hamiltonian_average.shape == (451,)
+
+
+def test_odm_x2_notebook(cuda, tmp_path):
+    """Wigner2D_FFT_ODM_X2_D0.ipynb cell 4: gammaDamping 0.07/4, L_material 3, epsilon 0.5."""
+    from pywignercuda_amd.GPU_Wigner2D_FFT import GPU_Wigner2D_FFT
+
+    inst = _fft_frame(GPU_Wigner2D_FFT, 12, 12, 0.005, 10000, 0.0, 0.07 / 4.0, 0.5, 0.0, 1.0, tmp_path,
+                      damping="ODM")
+    inst.W_init = inst.Wigner_HarmonicOscillator(0, inst.omega, 0.0, 3.0)
+    assert inst.Run() == 0
+    _check(inst, -1.28963238389e-05, 0.313738969442)

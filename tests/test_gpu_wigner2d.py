@@ -29,19 +29,21 @@ def rel_l2(a, b):
 
 def make_pair(cuda, X=256, P=256, amp=10.0, dt=0.01, scheme="lie", mask=True, truncate=False,
               D_theta=0.0, D_lambda=0.0, gamma=0.0, damping=None, fp_str=None, fp_fn=None, gp=0.0,
-              precision="fp64", pot=("0.5*pow(x,2)+0.1*pow(x,4)", "x+0.4*pow(x,3)", V_quartic, dV_quartic)):
+              precision="fp64", pot=("0.5*pow(x,2)+0.1*pow(x,4)", "x+0.4*pow(x,3)", V_quartic, dV_quartic),
+              L_material=None, hbar=1.0):
     from pywignercuda_amd.wigner2d import Wigner2D
 
     vs, dvs, vf, dvf = pot
-    hip = Wigner2D(X, P, amp, amp, 1.0, 1.0, dt, potential=vs, dpotential=dvs,
+    hip = Wigner2D(X, P, amp, amp, hbar, 1.0, dt, potential=vs, dpotential=dvs,
                    kinetic="0.5*p*p" if scheme == "strang" else None, scheme=scheme,
                    D_theta=D_theta, D_lambda=D_lambda, gamma=gamma, damping=damping,
                    fp_damping=fp_str, gp_coeff=gp, mask=mask, truncate_constants=truncate,
-                   precision=precision, device=cuda)
-    orc = ow.Wigner2DOracle(X, P, amp, amp, 1.0, 1.0, dt, D_theta=D_theta, D_lambda=D_lambda,
+                   precision=precision, device=cuda, L_material=L_material)
+    orc = ow.Wigner2DOracle(X, P, amp, amp, hbar, 1.0, dt, D_theta=D_theta, D_lambda=D_lambda,
                             gamma=gamma, potential=vf, dpotential=dvf,
                             kinetic=(lambda p: 0.5 * p * p), fp_damping=fp_fn, damping=damping,
-                            gp_coeff=gp, scheme=scheme, mask=mask, truncate_constants=truncate)
+                            gp_coeff=gp, scheme=scheme, mask=mask, truncate_constants=truncate,
+                            L_material=L_material)
     return hip, orc
 
 
@@ -181,3 +183,29 @@ def test_non_polynomial_potential_table(cuda, scheme):
     W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 40)
     assert rel_l2(W, W_ref) < 1e-10
     check_obs(obs, obs_ref)
+
+
+@pytest.mark.parametrize("gp,hbar", [(0.0, 1.0), (0.5, 0.8)])
+def test_lie_odm_fp64(cuda, gp, hbar):
+    """ODM Lindblad dissipator (order 1) after the V step, with and without the GP term."""
+    fp_str = " p*p/sqrt( p*p + 0.5  ) "
+    fp_fn = lambda p: p * p / np.sqrt(p * p + 0.5)  # noqa: E731
+    hip, orc = make_pair(cuda, amp=12.0, dt=0.005, gamma=0.07 / 4.0, damping="ODM", fp_str=fp_str, fp_fn=fp_fn,
+                         truncate=True, gp=gp, L_material=3.0, hbar=hbar,
+                         pot=("0.5*pow(x,2)", "x", lambda t, x: 0.5 * x * x, lambda t, x: x))
+    W0 = ow.harmonic_oscillator_wigner(orc.g, 0, 1.0, 0.0, 3.0)
+    W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 30)
+    assert rel_l2(W, W_ref) < 1e-10
+    check_obs(obs, obs_ref)
+
+
+def test_lie_odm_fp32(cuda):
+    fp_fn = lambda p: p * p / np.sqrt(p * p + 0.5)  # noqa: E731
+    hip, orc = make_pair(cuda, amp=12.0, dt=0.005, gamma=0.07 / 4.0, damping="ODM",
+                         fp_str=" p*p/sqrt( p*p + 0.5  ) ", fp_fn=fp_fn, L_material=3.0, precision="fp32",
+                         pot=("0.5*pow(x,2)", "x", lambda t, x: 0.5 * x * x, lambda t, x: x))
+    W0 = ow.harmonic_oscillator_wigner(orc.g, 0, 1.0, 0.0, 3.0)
+    W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 100)
+    err = rel_l2(W, W_ref)
+    print(f"odm fp32 rel L2 {err:.3e}")
+    assert err < 1e-5
