@@ -167,3 +167,35 @@ def test_fft_ordered_axes_match_reference_kernel_map():
     expect = [d * (((k + P // 2) % P) - P / 2) for k in range(P)]
     assert np.allclose(ow.fft_order_axis(P, d), expect)
     assert np.allclose(ow.host_axis(2.0, P, d), np.fft.fftshift(np.linspace(-2.0, 2.0 - d, P)))
+
+
+def test_odm_six_terms_split_into_x_and_theta_factors():
+    """_Lindbladian_ODM (FFT.py:1516-1634) as restated equals its split into x-only and
+    theta-only factors (product-to-sum identities of cos/sin(x +- hbar theta/2)), the form the
+    fused ODM sweeps compute; and the host tables of the product path equal the oracle's."""
+    from pywignercuda_amd.wigner2d import _Tables
+
+    orc = ow.Wigner2DOracle(128, 64, 12.0, 10.0, 0.7, 1.0, 0.005, gamma=0.0175, potential=lambda t, x: 0.5 * x * x,
+                            fp_damping=lambda p: p * p / np.sqrt(p * p + 0.5), damping="ODM", L_material=3.0)
+    W = ow.harmonic_oscillator_wigner(orc.g, 0, 1.0, 0.5, 2.0) * (1 + 0.1j)
+    ref = orc.odm_operator(W)
+    T = orc.odm_tables()
+    g, h = orc.g, orc.hbar
+    A = T["gp"] * T["gm"] - T["gsp"] * T["gsm"]
+    B = T["gp"] * T["gsm"] - T["gsp"] * T["gm"]
+    C = -0.5 * (T["gp"] ** 2 + T["gm"] ** 2)
+    E = T["gp"] * T["gm"] + T["gsp"] * T["gsm"]
+    F = T["gp"] * T["gsm"] + T["gsp"] * T["gm"]
+    Wh = np.fft.fft(W, axis=1)
+    ix = lambda Y: np.fft.ifft(Y, axis=1)  # noqa: E731
+    Lx = 0.5 * np.cos(2 * g.x_host) * ix(A * Wh) + 0.5j * np.sin(2 * g.x_host) * ix(B * Wh) + ix(C * Wh)
+    Q = 0.5 * (np.cos(h * g.theta_host) * np.fft.fft(E * Wh, axis=0)
+               + 1j * np.sin(h * g.theta_host) * np.fft.fft(F * Wh, axis=0))
+    got = (Lx + ix(np.fft.ifft(Q, axis=0))) * (orc.dt * 2 * orc.gamma)
+    assert np.abs(got - ref).max() < 1e-14 * np.abs(ref).max() * 100
+    t = _Tables(128, 64, 12.0, 10.0, 0.7, 1.0, 0.005, potential="0.5*pow(x,2)", dpotential="x", scheme="lie",
+                D_theta=0, D_lambda=0, gamma=0.0175, damping="ODM", fp_damping=" p*p/sqrt( p*p + 0.5  ) ",
+                gp_coeff=0, gp_rho_factor=None, mask=False, truncate_constants=True, time_rows=None,
+                pot_table=None, L_material=3.0)
+    for k, name in enumerate(["gp", "gm", "gsp", "gsm"]):
+        np.testing.assert_array_equal(t.odm_g[k], np.broadcast_to(T[name], t.odm_g[k].shape))
