@@ -349,6 +349,12 @@ int wh_dirac_destroy(wh_dirac* plan);
  * the trace normalisation; W holds the normalised final state on return.
  */
 int wh_dirac_run(wh_dirac* plan, void* W, int first_step, int n_steps, double* obs, int use_graph, void* stream);
+/* Single sweeps for callers composing their own loop (no normalisation, no record):
+   the kinetic factors (DiracPropagator_P_plus/minus_Lambda between the x transforms,
+   :3514-3558) and the potential factors at time index `step` with damping and absorber
+   (DiracPropagator_X_minus/plus_Theta between the p transforms, :3560-3683). */
+int wh_dirac_kinetic(wh_dirac* plan, void* W, void* stream);
+int wh_dirac_potential(wh_dirac* plan, void* W, int step, void* stream);
 
 #ifdef __cplusplus
 }

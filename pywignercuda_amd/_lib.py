@@ -191,6 +191,8 @@ def _declare(lib):
         "wh_dirac_create": ([POINTER(DiracDesc), POINTER(vp)], ip),
         "wh_dirac_destroy": ([vp], ip),
         "wh_dirac_run": ([vp, vp, ip, ip, vp, ip, vp], ip),
+        "wh_dirac_kinetic": ([vp, vp, vp], ip),
+        "wh_dirac_potential": ([vp, vp, ip, vp], ip),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

@@ -1201,23 +1201,58 @@ extern "C" int wh_dirac_destroy(wh_dirac* plan) {
   return WH_SUCCESS;
 }
 
+namespace {
+// Points the device plan at W (and obs) in stream order and sets the step counter.
+int dirac_bind(wh_dirac* e, void* W, double* obs, int base, int step, hipStream_t s) {
+  const size_t align = e->d.precision == WH_C128 ? 16 : 8;
+  if (reinterpret_cast<uintptr_t>(W) % align) return fail(WH_UNALIGNED_DATA, "W %p", W);
+  WH_HIP(hipStreamSynchronize(s), "sync");
+  e->host.W = W;
+  e->host.obs = obs;
+  e->host.obs_base = base;
+  WH_HIP(hipMemcpy(e->plan_dev.p, &e->host, sizeof(wh::DiracPlan), hipMemcpyHostToDevice), "plan upload");
+  WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), step, s), "set_step");
+  return WH_SUCCESS;
+}
+}  // namespace
+
+extern "C" int wh_dirac_kinetic(wh_dirac* e, void* W, void* stream) {
+  if (!e) return fail(WH_INVALID_PLAN, "wh_dirac_kinetic: plan is NULL");
+  if (!W) return fail(WH_INVALID_VALUE, "wh_dirac_kinetic: W is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  WH_TRY(dirac_bind(e, W, e->host.obs, e->host.obs_base, 0, s));
+  const double one = 1.0;
+  WH_HIP(hipMemcpyAsync(e->scale.p, &one, sizeof(one), hipMemcpyHostToDevice, s), "scale");
+  WH_HIP(hipStreamSynchronize(s), "sync");
+  WH_LAUNCH(wh::launch_dirac_xsweep(e->d.precision, e->plan_dev.as<wh::DiracPlan>(), e->xblocks, s), "dirac_xsweep");
+  return WH_SUCCESS;
+}
+
+extern "C" int wh_dirac_potential(wh_dirac* e, void* W, int step, void* stream) {
+  if (!e) return fail(WH_INVALID_PLAN, "wh_dirac_potential: plan is NULL");
+  if (!W || step < 0) return fail(WH_INVALID_VALUE, "wh_dirac_potential: W %p step %d", W, step);
+  if (e->d.gp_coeff != 0.0)
+    return fail(WH_INVALID_VALUE, "wh_dirac_potential: the Gross-Pitaevskii density comes from wh_dirac_run");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  WH_TRY(dirac_bind(e, W, e->host.obs, e->host.obs_base, step, s));
+  const auto* dp = e->plan_dev.as<wh::DiracPlan>();
+  if (e->host.general)
+    WH_LAUNCH(wh::launch_dirac_pgeneral(e->d.precision, e->d.gamma_dt != 0.0, dp, e->pgblocks, s), "dirac_pgeneral");
+  else
+    WH_LAUNCH(wh::launch_dirac_psweep(e->d.precision, e->d.gamma_dt != 0.0, dp, e->pblocks, s), "dirac_psweep");
+  return WH_SUCCESS;
+}
+
 extern "C" int wh_dirac_run(wh_dirac* e, void* W, int first_step, int n_steps, double* obs, int use_graph,
                             void* stream) {
   if (!e) return fail(WH_INVALID_PLAN, "wh_dirac_run: plan is NULL");
   if (!W || !obs || n_steps < 0 || first_step < 0)
     return fail(WH_INVALID_VALUE, "wh_dirac_run: W %p obs %p n_steps %d first_step %d", W, (void*)obs, n_steps,
                 first_step);
-  const size_t align = e->d.precision == WH_C128 ? 16 : 8;
-  if (reinterpret_cast<uintptr_t>(W) % align) return fail(WH_UNALIGNED_DATA, "W %p", W);
   hipStream_t s = static_cast<hipStream_t>(stream);
   // the device plan is read by every kernel (and by captured graphs): update it in stream order
-  WH_HIP(hipStreamSynchronize(s), "sync");
-  e->host.W = W;
-  e->host.obs = obs;
-  e->host.obs_base = first_step;
-  WH_HIP(hipMemcpy(e->plan_dev.p, &e->host, sizeof(wh::DiracPlan), hipMemcpyHostToDevice), "plan upload");
+  WH_TRY(dirac_bind(e, W, obs, first_step, first_step, s));
   const auto* dp = e->plan_dev.as<wh::DiracPlan>();
-  WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), first_step, s), "set_step");
   WH_LAUNCH(wh::launch_dirac_colsum(e->d.precision, dp, e->d.x_dim, s), "dirac_colsum");
   WH_LAUNCH(wh::launch_dirac_reduce(dp, 0, 0, 0, 0, s), "dirac_reduce");
   if (use_graph)

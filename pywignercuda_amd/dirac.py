@@ -218,6 +218,18 @@ class WignerDirac4x4:
                                            int(bool(use_graph)), _lib.current_stream_handle()), "wh_dirac_run")
         return obs
 
+    def kinetic(self, W):
+        """W <- U(p + lambda/2) W U(p - lambda/2)^dagger between the x transforms (one sweep)."""
+        self._check_state(W)
+        _lib.check(_lib.lib().wh_dirac_kinetic(self._plan, _lib.dptr(W), _lib.current_stream_handle()),
+                   "wh_dirac_kinetic")
+
+    def potential(self, W, step: int = 1):
+        """The potential factors at time index `step` (with D_Theta decay, absorber and damping)."""
+        self._check_state(W)
+        _lib.check(_lib.lib().wh_dirac_potential(self._plan, _lib.dptr(W), int(step), _lib.current_stream_handle()),
+                   "wh_dirac_potential")
+
     @staticmethod
     def observables(obs):
         """Record rows (host array [n, 32]) -> dict of complex / real series."""

@@ -215,3 +215,20 @@ def test_dirac_vector_potential_constant_field(cuda, precision):
         check_obs(obs, rec)
     else:
         assert err < 1e-5
+
+
+def test_dirac_single_sweeps_match_oracle_factors(cuda):
+    """wh_dirac_kinetic / wh_dirac_potential (one sweep each, no normalisation) against the
+    oracle's kinetic and potential steps between the transforms."""
+    import torch
+
+    hip, orc, W0 = make_pair(cuda, "coulomb", D_theta=0.02)
+    W = hip.to_device(W0)
+    hip.kinetic(W)
+    torch.cuda.synchronize()
+    ref = orc.lambda_to_x(orc.kinetic_step(orc.x_to_lambda(W0)))
+    assert rel_l2(W.cpu().numpy().reshape(4, 4, hip.P, hip.X), ref) < 1e-12
+    hip.potential(W, step=3)
+    torch.cuda.synchronize()
+    ref = orc.theta_to_p(orc.potential_step(orc.p_to_theta(ref), orc.dt * 3)) * orc.absorber()
+    assert rel_l2(W.cpu().numpy().reshape(4, 4, hip.P, hip.X), ref) < 1e-12
