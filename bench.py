@@ -143,9 +143,10 @@ def uneven_ensemble_members(seed=20261202, n=256):
     for i in range(n):
         tab = np.zeros((1, 9))
         tab[0, 4], tab[0, 2] = a[i], -b[i]
+        ai, bi = float(a[i]), float(b[i])
         members.append(dict(X=int(sides[i]), P=int(sides[i]), X_amp=10.0, P_amp=10.0,
-                            potential=f"{a[i]!r}*pow(x,4) - {b[i]!r}*pow(x,2)",
-                            dpotential=f"{4 * a[i]!r}*pow(x,3) - {2 * b[i]!r}*x", pot_table=tab, mask=False))
+                            potential=f"{ai!r}*pow(x,4) - {bi!r}*pow(x,2)",
+                            dpotential=f"{4 * ai!r}*pow(x,3) - {2 * bi!r}*x", pot_table=tab, mask=False))
         init.append((x0[i], p0[i], s[i]))
     return members, init
 

@@ -41,6 +41,7 @@ extern "C" {
 /* precisions: CUFFT_Z2Z / CUFFT_C2C of cufft_wrapper.py:8-9 */
 #define WH_C128 0
 #define WH_C64 1
+#define WH_C64_WIDE 2 /* complex64 state, FP64 arithmetic inside the sweeps (Wigner2D plans and ensembles) */
 /* directions: CUFFT_FORWARD / CUFFT_INVERSE of cufft_wrapper.py:11-12 */
 #define WH_FORWARD (-1)
 #define WH_INVERSE 1

@@ -13,7 +13,7 @@ from pathlib import Path
 _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libwignerhip.so"
 
-WH_C128, WH_C64 = 0, 1
+WH_C128, WH_C64, WH_C64_WIDE = 0, 1, 2
 WH_FORWARD, WH_INVERSE = -1, 1
 WH_SCHEME_LIE, WH_SCHEME_STRANG = 0, 1
 WH_MAX_DEG = 8

@@ -57,10 +57,11 @@ __device__ __forceinline__ void adj_psweep_body(const SweepArgs& a, const Member
     const double xm = x - u, xp = x + u;
     // Im(conj(v) w)
     const double im = double(v[j].x) * double(w[j].y) - double(v[j].y) * double(w[j].x);
+    // every coefficient up to kMaxDeg: dL/da_k is not zero where a_k currently is
     double pm = 1.0, pp = 1.0;
 #pragma unroll
     for (int k = 0; k <= kMaxDeg; ++k) {
-      if (k <= deg) acc[k] += (pm - pp) * im;
+      acc[k] += (pm - pp) * im;
       pm *= xm;
       pp *= xp;
     }

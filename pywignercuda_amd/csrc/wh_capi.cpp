@@ -30,6 +30,8 @@ hipError_t launch_psweep_f64(int variant, bool cl, const SweepArgs& a, int nwork
 hipError_t launch_psweep_f32(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_gpshift(const Member* members, const unsigned* work, int nwork, hipStream_t s);
 hipError_t launch_colsum(int prec, const Member* members, const unsigned* work, int nwork, hipStream_t s);
+hipError_t launch_xsweep_f32w(int variant, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_psweep_f32w(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_reduce(const ReduceArgs& a, int nmembers, int* done, hipStream_t s);
 hipError_t launch_colpart_reduce(const Member* members, int nmembers, int xradix, int xblock, const int* step,
                                  hipStream_t s);
@@ -95,6 +97,9 @@ int hip_fail(hipError_t e, const char* what) {
 
 bool size_supported(int n) { return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048; }
 bool sweep_size_supported(int n) { return n >= 128 && n <= 2048 && size_supported(n); }
+// bytes of one complex element of the state, and the kernel code of its storage type
+size_t state_bytes(int prec) { return prec == WH_C128 ? 16 : 8; }
+int storage_code(int prec) { return prec == WH_C128 ? WH_C128 : WH_C64; }
 
 // device allocation owned by a plan
 struct DevBuf {
@@ -138,7 +143,7 @@ int make_twiddles(TwSet& tw, int N, int prec, bool need64) {
     c[n] = cosl(a);
     s[n] = -sinl(a);
   }
-  if ((prec == WH_C128 || need64) && !tw.hi64.p) {
+  if ((prec != WH_C64 || need64) && !tw.hi64.p) {
     std::vector<double> h(2 * size_t(N));
     for (int n = 0; n < N; ++n) {
       h[2 * n] = (double)c[n];
@@ -356,8 +361,8 @@ int validate_member(const wh_ensemble_desc& d, const wh_member_desc& m, int i) {
 int validate_common(const wh_ensemble_desc& d) {
   if (d.struct_size != (int)sizeof(wh_ensemble_desc))
     return fail(WH_INVALID_DESCRIPTOR, "ensemble struct_size %d != %d", d.struct_size, (int)sizeof(wh_ensemble_desc));
-  if (d.precision != WH_C128 && d.precision != WH_C64)
-    return fail(WH_INVALID_TYPE, "precision %d is neither WH_C128 nor WH_C64", d.precision);
+  if (d.precision != WH_C128 && d.precision != WH_C64 && d.precision != WH_C64_WIDE)
+    return fail(WH_INVALID_TYPE, "precision %d is not WH_C128, WH_C64 or WH_C64_WIDE", d.precision);
   if (d.scheme != WH_SCHEME_LIE && d.scheme != WH_SCHEME_STRANG)
     return fail(WH_INVALID_DESCRIPTOR, "scheme %d", d.scheme);
   if (d.kinetic_poly && (d.kinetic_deg < 0 || d.kinetic_deg > WH_MAX_DEG))
@@ -481,7 +486,7 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
       WH_TRY(b->odm_g.upload(md.odm_g, 4 * X * P * sizeof(double)));
       WH_TRY(b->odm_x.upload(md.odm_x, 2 * X * sizeof(double)));
       WH_TRY(b->odm_th.upload(md.odm_th, 2 * P * sizeof(double)));
-      WH_TRY(b->odm_s.alloc(2 * X * P * (prec == WH_C128 ? 16 : 8)));
+      WH_TRY(b->odm_s.alloc(2 * X * P * (prec == WH_C64 ? 8 : 16)));  // arithmetic precision
     }
     wh::Member& m = e->mem[i];
     std::memset(&m, 0, sizeof(m));
@@ -503,9 +508,9 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
     m.v0_host = b->v0_host.as<double>();
     const TwSet& tx = *e->tw[md.x_dim];
     const TwSet& tp = *e->tw[md.p_dim];
-    m.twX_hi = prec == WH_C128 ? tx.hi64.p : tx.hi.p;
+    m.twX_hi = prec != WH_C64 ? tx.hi64.p : tx.hi.p;
     m.twX_lo = tx.lo.p;
-    m.twP_hi = prec == WH_C128 ? tp.hi64.p : tp.hi.p;
+    m.twP_hi = prec != WH_C64 ? tp.hi64.p : tp.hi.p;
     m.twP_lo = tp.lo.p;
     m.twX64 = tx.hi64.p;
     m.X = md.x_dim;
@@ -596,7 +601,8 @@ wh::SweepArgs sweep_args(const wh_ensemble* e, int flags, double frac, int step_
 int xsweep(wh_ensemble* e, int flags, hipStream_t s) {
   wh::SweepArgs a = sweep_args(e, flags, 1.0, 0);
   a.work = e->work_x.buf.as<unsigned>();
-  WH_LAUNCH(e->d.precision == WH_C128 ? wh::launch_xsweep_f64(e->xvar, a, e->work_x.n, s)
+  WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_xsweep_f64(e->xvar, a, e->work_x.n, s)
+            : e->d.precision == WH_C64_WIDE ? wh::launch_xsweep_f32w(e->xvar, a, e->work_x.n, s)
                                       : wh::launch_xsweep_f32(e->xvar, a, e->work_x.n, s),
             "xsweep");
   return WH_SUCCESS;
@@ -607,7 +613,8 @@ int psweep(wh_ensemble* e, int flags, double frac, int step_add, hipStream_t s) 
   a.work = e->work_p.buf.as<unsigned>();
   const bool cl = (flags & wh::kCL) != 0;
   const int v = cl ? e->pclvar : e->pvar;
-  WH_LAUNCH(e->d.precision == WH_C128 ? wh::launch_psweep_f64(v, cl, a, e->work_p.n, s)
+  WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_psweep_f64(v, cl, a, e->work_p.n, s)
+            : e->d.precision == WH_C64_WIDE ? wh::launch_psweep_f32w(v, cl, a, e->work_p.n, s)
                                       : wh::launch_psweep_f32(v, cl, a, e->work_p.n, s),
             "psweep");
   return WH_SUCCESS;
@@ -650,7 +657,8 @@ int iteration(wh_ensemble* e, bool merge, hipStream_t s) {
     WH_LAUNCH(wh::launch_odm(d.precision, e->mem_dev.as<wh::Member>(), e->work_odm_x.buf.as<unsigned>(),
                              e->work_odm_x.n, e->work_odm_p.buf.as<unsigned>(), e->work_odm_p.n, s),
               "odm");
-    WH_LAUNCH(wh::launch_colsum(d.precision, e->mem_dev.as<wh::Member>(), e->work_colsum.buf.as<unsigned>(),
+    WH_LAUNCH(wh::launch_colsum(storage_code(d.precision), e->mem_dev.as<wh::Member>(),
+                                e->work_colsum.buf.as<unsigned>(),
                                 e->work_colsum.n, s),
               "colsum");
     WH_TRY(reduce(e, renorm, 1, e->gp(), 1, true, s));
@@ -729,7 +737,7 @@ int ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_step
   if (n_steps < 0 || first_step < 0) return fail(WH_INVALID_VALUE, "n_steps %d first_step %d", n_steps, first_step);
   if (!obs) return fail(WH_INVALID_VALUE, "obs is NULL");
   const auto& d = e->d;
-  const size_t align = d.precision == WH_C128 ? 16 : 8;
+  const size_t align = state_bytes(d.precision);
   bool changed = false;
   for (int i = 0; i < e->n; ++i) {
     if (!states[i]) return fail(WH_INVALID_VALUE, "state %d is NULL", i);
@@ -748,7 +756,8 @@ int ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_step
   const int prec = d.precision;
   const wh::Member* md = e->mem_dev.as<wh::Member>();
   // the state before the first iteration: norm, renormalisation factor, (Lie) row 0
-  WH_LAUNCH(wh::launch_colsum(prec, md, e->work_colsum.buf.as<unsigned>(), e->work_colsum.n, s), "colsum");
+  WH_LAUNCH(wh::launch_colsum(storage_code(prec), md, e->work_colsum.buf.as<unsigned>(), e->work_colsum.n, s),
+            "colsum");
   WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), d.scheme == WH_SCHEME_LIE ? first_step - 1 : first_step, s),
             "set_step");
   WH_TRY(reduce(e, normalize_initial, d.scheme == WH_SCHEME_LIE, e->gp(), 1, false, s));
@@ -768,7 +777,8 @@ int ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_step
     }
   }
   if (d.renormalize || (n_steps == 0 && normalize_initial))
-    WH_LAUNCH(wh::launch_scale(prec, md, e->work_scale.buf.as<unsigned>(), e->work_scale.n, s), "scale");
+    WH_LAUNCH(wh::launch_scale(storage_code(prec), md, e->work_scale.buf.as<unsigned>(), e->work_scale.n, s),
+              "scale");
   return WH_SUCCESS;
 }
 
@@ -951,6 +961,8 @@ extern "C" int wh_wigner2d_adjoint_step(wh_wigner2d* p, void* Lambda, const void
     return fail(WH_INVALID_VALUE, "the adjoint step covers the Lie scheme without damping or Gross-Pitaevskii");
   if (e->mem[0].pot_deg == WH_POT_TABLE)
     return fail(WH_INVALID_VALUE, "the adjoint step needs V as polynomial coefficients (gradients are per coefficient)");
+  if (e->d.precision == WH_C64_WIDE)
+    return fail(WH_INVALID_TYPE, "the adjoint step runs on WH_C128 or WH_C64 states");
   hipStream_t s = static_cast<hipStream_t>(stream);
   wh::Member& m = e->mem[0];
   if (!e->work_adj.n) {
@@ -982,7 +994,8 @@ extern "C" int wh_wigner2d_observe(wh_wigner2d* p, const void* W, double* obs_ro
   wh_ensemble* e = p->e;
   WH_TRY(bind_single(e, const_cast<void*>(W), obs_row, 0, s));
   WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), 0, s), "set_step");
-  WH_LAUNCH(wh::launch_colsum(e->d.precision, e->mem_dev.as<wh::Member>(), e->work_colsum.buf.as<unsigned>(),
+  WH_LAUNCH(wh::launch_colsum(storage_code(e->d.precision), e->mem_dev.as<wh::Member>(),
+                              e->work_colsum.buf.as<unsigned>(),
                               e->work_colsum.n, s),
             "colsum");
   return reduce(e, 0, 1, 0, 0, false, s);

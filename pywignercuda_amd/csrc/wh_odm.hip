@@ -29,25 +29,25 @@ constexpr int kOdmBlock = 256;
 
 namespace {
 
-template <typename T>
-__device__ __forceinline__ cplx<double> wide(cplx<T> v) {
+template <typename U>
+__device__ __forceinline__ cplx<double> wide(cplx<U> v) {
   return {double(v.x), double(v.y)};
 }
-template <typename T>
-__device__ __forceinline__ cplx<T> nar(cplx<double> v) {
-  return {T(v.x), T(v.y)};
+template <typename U>
+__device__ __forceinline__ cplx<U> nar(cplx<double> v) {
+  return {U(v.x), U(v.y)};
 }
 
 }  // namespace
 
-template <typename T, int N, int R, int BLOCK>
+template <typename S, typename T, int N, int R, int BLOCK>
 __device__ __forceinline__ void odm_x_body(const Member& m, int tile, cplx<T>* lds) {
   constexpr int TT = N / R, ROWS = BLOCK / TT;
   constexpr int LSZ = Padded<N, R>::size;
   const int tid = threadIdx.x, team = tid / TT, t = tid % TT;
   const int row = tile * ROWS + team;
   const size_t PX = size_t(m.P) * size_t(N);
-  cplx<T>* __restrict__ W = reinterpret_cast<cplx<T>*>(m.W) + size_t(row) * N;
+  cplx<S>* __restrict__ W = reinterpret_cast<cplx<S>*>(m.W) + size_t(row) * N;
   cplx<T>* __restrict__ S1 = reinterpret_cast<cplx<T>*>(m.odm_s) + size_t(row) * N;
   cplx<T>* __restrict__ S2 = S1 + PX;
   const double* __restrict__ g = m.odm_g + size_t(row) * N;
@@ -56,7 +56,7 @@ __device__ __forceinline__ void odm_x_body(const Member& m, int tile, cplx<T>* l
 
   cplx<T> f[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) f[j] = W[t + j * TT];
+  for (int j = 0; j < R; ++j) f[j] = to_arith<T>(W[t + j * TT]);
   team_fft<T, N, R, -1>(f, t, view, tw);  // W^ = F_x W
   // theta terms' inputs E W^, F W^ (lambda picture)
 #pragma unroll
@@ -102,11 +102,11 @@ __device__ __forceinline__ void odm_x_body(const Member& m, int tile, cplx<T>* l
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const cplx<double> w = wide(W[t + j * TT]);
-    W[t + j * TT] = nar<T>({w.x + c * acc[j].x, w.y + c * acc[j].y});
+    W[t + j * TT] = nar<S>({w.x + c * acc[j].x, w.y + c * acc[j].y});
   }
 }
 
-template <typename T, int N, int R, int BLOCK>
+template <typename S, typename T, int N, int R, int BLOCK>
 __device__ __forceinline__ void odm_p_body(const Member& m, int tile, cplx<T>* lds) {
   constexpr int TT = N / R;
   constexpr int C = BLOCK / TT;
@@ -143,13 +143,13 @@ __device__ __forceinline__ void odm_p_body(const Member& m, int tile, cplx<T>* l
   for (int j = 0; j < R; ++j) at_off(S1 + size_t(j * TT) * X, boff) = a[j];
 }
 
-template <typename T, int N, int R, int BLOCK>
+template <typename S, typename T, int N, int R, int BLOCK>
 __device__ __forceinline__ void odm_back_body(const Member& m, int tile, cplx<T>* lds) {
   constexpr int TT = N / R, ROWS = BLOCK / TT;
   constexpr int LSZ = Padded<N, R>::size;
   const int tid = threadIdx.x, team = tid / TT, t = tid % TT;
   const int row = tile * ROWS + team;
-  cplx<T>* __restrict__ W = reinterpret_cast<cplx<T>*>(m.W) + size_t(row) * N;
+  cplx<S>* __restrict__ W = reinterpret_cast<cplx<S>*>(m.W) + size_t(row) * N;
   const cplx<T>* __restrict__ S1 = reinterpret_cast<const cplx<T>*>(m.odm_s) + size_t(row) * N;
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(m.twX_hi), reinterpret_cast<const cplx<T>*>(m.twX_lo)};
   LdsRow<T> view{lds + team * LSZ};
@@ -161,11 +161,11 @@ __device__ __forceinline__ void odm_back_body(const Member& m, int tile, cplx<T>
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const cplx<double> w = wide(W[t + j * TT]), u = wide(y[j]);
-    W[t + j * TT] = nar<T>({w.x + inv * u.x, w.y + inv * u.y});
+    W[t + j * TT] = nar<S>({w.x + inv * u.x, w.y + inv * u.y});
   }
 }
 
-template <typename T, int KIND>
+template <typename S, typename T, int KIND>
 __global__ void __launch_bounds__(kOdmBlock) odm_kernel(const Member* members, const unsigned* work) {
   __shared__ cplx<T> lds[lds_elems<kOdmBlock, kOdmR>()];
   const unsigned w = work[blockIdx.x];
@@ -173,15 +173,15 @@ __global__ void __launch_bounds__(kOdmBlock) odm_kernel(const Member* members, c
   const int tile = int(w & 0xffffu);
   constexpr int R = kOdmR, BLOCK = kOdmBlock;
   if constexpr (KIND == 0) {
-#define BODY(n) odm_x_body<T, n, R, BLOCK>(m, tile, lds)
+#define BODY(n) odm_x_body<S, T, n, R, BLOCK>(m, tile, lds)
     WH_DISPATCH(m.X, BODY)
 #undef BODY
   } else if constexpr (KIND == 1) {
-#define BODY(n) odm_p_body<T, n, R, BLOCK>(m, tile, lds)
+#define BODY(n) odm_p_body<S, T, n, R, BLOCK>(m, tile, lds)
     WH_DISPATCH(m.P, BODY)
 #undef BODY
   } else {
-#define BODY(n) odm_back_body<T, n, R, BLOCK>(m, tile, lds)
+#define BODY(n) odm_back_body<S, T, n, R, BLOCK>(m, tile, lds)
     WH_DISPATCH(m.X, BODY)
 #undef BODY
   }
@@ -195,13 +195,17 @@ hipError_t launch_odm(int prec, const Member* members, const unsigned* work_rows
                       const unsigned* work_cols, int ncols, hipStream_t s) {
   const dim3 b(kOdmBlock);
   if (prec == 0) {
-    hipLaunchKernelGGL((odm_kernel<double, 0>), dim3(nrows), b, 0, s, members, work_rows);
-    hipLaunchKernelGGL((odm_kernel<double, 1>), dim3(ncols), b, 0, s, members, work_cols);
-    hipLaunchKernelGGL((odm_kernel<double, 2>), dim3(nrows), b, 0, s, members, work_rows);
+    hipLaunchKernelGGL((odm_kernel<double, double, 0>), dim3(nrows), b, 0, s, members, work_rows);
+    hipLaunchKernelGGL((odm_kernel<double, double, 1>), dim3(ncols), b, 0, s, members, work_cols);
+    hipLaunchKernelGGL((odm_kernel<double, double, 2>), dim3(nrows), b, 0, s, members, work_rows);
+  } else if (prec == 1) {
+    hipLaunchKernelGGL((odm_kernel<float, float, 0>), dim3(nrows), b, 0, s, members, work_rows);
+    hipLaunchKernelGGL((odm_kernel<float, float, 1>), dim3(ncols), b, 0, s, members, work_cols);
+    hipLaunchKernelGGL((odm_kernel<float, float, 2>), dim3(nrows), b, 0, s, members, work_rows);
   } else {
-    hipLaunchKernelGGL((odm_kernel<float, 0>), dim3(nrows), b, 0, s, members, work_rows);
-    hipLaunchKernelGGL((odm_kernel<float, 1>), dim3(ncols), b, 0, s, members, work_cols);
-    hipLaunchKernelGGL((odm_kernel<float, 2>), dim3(nrows), b, 0, s, members, work_rows);
+    hipLaunchKernelGGL((odm_kernel<float, double, 0>), dim3(nrows), b, 0, s, members, work_rows);
+    hipLaunchKernelGGL((odm_kernel<float, double, 1>), dim3(ncols), b, 0, s, members, work_cols);
+    hipLaunchKernelGGL((odm_kernel<float, double, 2>), dim3(nrows), b, 0, s, members, work_rows);
   }
   return hipGetLastError();
 }

@@ -63,6 +63,13 @@ __device__ __forceinline__ cplx<T>& at_off(cplx<T>* base, unsigned boff) {
 template <typename T>
 __device__ __forceinline__ cplx<T> mul_i(cplx<T> a, T s) { return {-s * a.y, s * a.x}; }
 
+// state storage S <-> arithmetic T (complex64 states with FP64 arithmetic convert at the
+// sweep's load and store; every other combination is the identity)
+template <typename T, typename S>
+__device__ __forceinline__ cplx<T> to_arith(cplx<S> v) {
+  return {T(v.x), T(v.y)};
+}
+
 // Largest LDS footprint over the supported sizes for a (block, radix) variant:
 // ROWS*Padded(N) = (BLOCK/(N/R)) * (N + N/R + 1) elements.
 template <int BLOCK, int R>
@@ -87,20 +94,20 @@ constexpr bool size_ok() {
 // exp(-i a m_s), a = dt*dlambda*(k_c0 + k_c1 p), m_s = t + js*TT with
 // js = j (j < R/2) or j - R; computed as E_t * Z^js, E_t = exp(-i a t),
 // Z = exp(-i a TT): two sincos per lane instead of R.
-template <typename T, int N, int R, int BLOCK>
+template <typename S, typename T, int N, int R, int BLOCK>
 __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m, int tile,
                                             cplx<T>* lds) {
   constexpr int TT = N / R, ROWS = BLOCK / TT;
   constexpr int LSZ = Padded<N, R>::size;
   const int tid = threadIdx.x, team = tid / TT, t = tid % TT;
   const int row = tile * ROWS + team;
-  cplx<T>* __restrict__ W = reinterpret_cast<cplx<T>*>(m.W) + size_t(row) * N;
+  cplx<S>* __restrict__ W = reinterpret_cast<cplx<S>*>(m.W) + size_t(row) * N;
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(m.twX_hi), reinterpret_cast<const cplx<T>*>(m.twX_lo)};
   LdsRow<T> view{lds + team * LSZ};
 
   cplx<T> v[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = W[t + j * TT];
+  for (int j = 0; j < R; ++j) v[j] = to_arith<T>(W[t + j * TT]);
   if (a.flags & kScaleOnLoad) {
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = scale_load(v[j], m.scale);
@@ -144,7 +151,7 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     v[j] = cscale(v[j], inv);
-    W[t + j * TT] = v[j];
+    W[t + j * TT] = to_arith<S>(v[j]);
   }
 
   if (a.flags & kColPart) {
@@ -209,31 +216,31 @@ struct PSweep {
   }
 };
 
-template <typename T, int N, int R, int BLOCK, bool CL>
+template <typename S, typename T, int N, int R, int BLOCK, bool CL>
 __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m, int tile,
                                             cplx<T>* lds) {
-  using S = PSweep<T, N, R, BLOCK>;
-  constexpr int TT = S::TT, C = S::C;
+  using PS = PSweep<T, N, R, BLOCK>;
+  constexpr int TT = PS::TT, C = PS::C;
   const int tid = threadIdx.x;
   const int c = tid % C, t = tid / C;
   const int X = m.X;
   const int col = tile * C + c;
-  cplx<T>* __restrict__ W = reinterpret_cast<cplx<T>*>(m.W);
+  cplx<S>* __restrict__ W = reinterpret_cast<cplx<S>*>(m.W);
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(m.twP_hi), reinterpret_cast<const cplx<T>*>(m.twP_lo)};
-  typename S::View view{lds + c};
+  typename PS::View view{lds + c};
   const int step = *a.step + a.step_add;
 
-  const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<T>));
+  const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<S>));
   cplx<T> v[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = at_off(W + size_t(j * TT) * X, boff);
+  for (int j = 0; j < R; ++j) v[j] = to_arith<T>(at_off(W + size_t(j * TT) * X, boff));
   if (a.flags & kScaleOnLoad) {
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = scale_load(v[j], m.scale);
   }
   const T inv = T(1.0 / N);
   team_fft<T, N, R, -1>(v, t, view, tw);
-  S::potential(v, a, m, t, col, step);
+  PS::potential(v, a, m, t, col, step);
   team_fft<T, N, R, +1>(v, t, view, tw);
 #pragma unroll
   for (int j = 0; j < R; ++j) v[j] = cscale(v[j], inv);
@@ -244,18 +251,18 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
     cplx<T> lw[R], d[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) d[j] = v[j];
-    S::cl_operator(d, m, t, view, tw);
+    PS::cl_operator(d, m, t, view, tw);
     const T c1 = T((2.0 / 3.0) * a.cl_a), c2 = T(a.cl_a), c3 = T(2.0 * a.cl_a);
 #pragma unroll
     for (int j = 0; j < R; ++j) lw[j] = cadd(v[j], mul_i(d[j], c1));
 #pragma unroll
     for (int j = 0; j < R; ++j) d[j] = lw[j];
-    S::cl_operator(d, m, t, view, tw);
+    PS::cl_operator(d, m, t, view, tw);
 #pragma unroll
     for (int j = 0; j < R; ++j) lw[j] = cadd(lw[j], mul_i(d[j], c2));
 #pragma unroll
     for (int j = 0; j < R; ++j) d[j] = lw[j];
-    S::cl_operator(d, m, t, view, tw);
+    PS::cl_operator(d, m, t, view, tw);
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = cadd(v[j], mul_i(d[j], c3));
   }
@@ -302,14 +309,14 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
 
   if (a.flags & kMerge) {
     team_fft<T, N, R, -1>(v, t, view, tw);
-    S::potential(v, a, m, t, col, step + 1);
+    PS::potential(v, a, m, t, col, step + 1);
     team_fft<T, N, R, +1>(v, t, view, tw);
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = cscale(v[j], inv);
   }
 
 #pragma unroll
-  for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = v[j];
+  for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = to_arith<S>(v[j]);
 }
 
 // ------------------------------------------------------------------ Gross-Pitaevskii shift
@@ -349,24 +356,24 @@ __device__ __forceinline__ void gpshift_body(const Member& m, int tile, cplx<dou
     default: break;                                                           \
   }
 
-template <typename T, int R, int BLOCK>
+template <typename S, typename T, int R, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) xsweep_kernel(SweepArgs a) {
   __shared__ cplx<T> lds[lds_elems<BLOCK, R>()];
   const unsigned w = a.work[blockIdx.x];
   const Member& m = a.members[w >> 16];
   const int tile = int(w & 0xffffu);
-#define BODY(n) xsweep_body<T, n, R, BLOCK>(a, m, tile, lds)
+#define BODY(n) xsweep_body<S, T, n, R, BLOCK>(a, m, tile, lds)
   WH_DISPATCH(m.X, BODY)
 #undef BODY
 }
 
-template <typename T, int R, int BLOCK, bool CL>
+template <typename S, typename T, int R, int BLOCK, bool CL>
 __global__ void __launch_bounds__(BLOCK) psweep_kernel(SweepArgs a) {
   __shared__ cplx<T> lds[lds_elems<BLOCK, R>()];
   const unsigned w = a.work[blockIdx.x];
   const Member& m = a.members[w >> 16];
   const int tile = int(w & 0xffffu);
-#define BODY(n) psweep_body<T, n, R, BLOCK, CL>(a, m, tile, lds)
+#define BODY(n) psweep_body<S, T, n, R, BLOCK, CL>(a, m, tile, lds)
   WH_DISPATCH(m.P, BODY)
 #undef BODY
 }

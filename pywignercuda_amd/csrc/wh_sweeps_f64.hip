@@ -1,4 +1,5 @@
 // FP64 (complex128) instantiation of the fused sweeps.
+#define WH_S double
 #define WH_T double
 #define WH_SUFFIX f64
 #include "wh_sweeps_inst.inc"

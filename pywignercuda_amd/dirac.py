@@ -44,7 +44,8 @@ def _is_zero(expr: str) -> bool:
     from .physics import _sympy_expr
 
     e, _, _ = _sympy_expr(expr, "x")
-    return sp.simplify(e) == 0
+    e = sp.simplify(e)
+    return bool(e.is_zero) or (e.is_number and float(e) == 0.0)
 
 
 class WignerDirac4x4:

@@ -107,7 +107,7 @@ def main():
     sizes = [int(s) for s in a.sizes.split(",")]
     table = json.loads(TUNING_FILE.read_text()) if TUNING_FILE.exists() else {}
     t0 = time.time()
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "fp32fast"):
         single = table.setdefault(prec, {})
         batch = table.setdefault(prec + "_batch", {})
         for N in sizes:

@@ -19,7 +19,12 @@ from pathlib import Path
 import numpy as np
 
 from . import _lib
-from ._lib import WH_C128, WH_C64, WH_MAX_DEG, WH_NOBS, WH_POT_TABLE, WH_SCHEME_LIE, WH_SCHEME_STRANG
+from ._lib import (WH_C64, WH_C64_WIDE, WH_C128, WH_MAX_DEG, WH_NOBS, WH_POT_TABLE, WH_SCHEME_LIE,
+                   WH_SCHEME_STRANG)
+
+# state precision names: complex128; complex64 state with FP64 arithmetic inside the sweeps
+# (one FP32 rounding of the state per sweep); complex64 state with FP32 arithmetic
+PRECISIONS = {"fp64": WH_C128, "fp32": WH_C64_WIDE, "fp32fast": WH_C64}
 from .physics import NonPolynomialPhysics, coefficient_table, depends_on_time, host_eval
 
 TUNING_FILE = Path(__file__).resolve().parent / "tuning.json"
@@ -201,7 +206,7 @@ class _Tables:
 
 
 def _common(d, *, precision, scheme, t: _Tables, dt, gamma, damping, kinetic, gp_coeff):
-    d.precision = WH_C128 if precision == "fp64" else WH_C64
+    d.precision = PRECISIONS[precision]
     d.scheme = WH_SCHEME_LIE if scheme == "lie" else WH_SCHEME_STRANG
     d.dt, d.hbar, d.mass = t.k_dt, t.k_hbar, t.k_mass
     kin_poly, kdeg, kcoef = 0, 0, np.zeros(WH_MAX_DEG + 1)
@@ -259,8 +264,10 @@ class Wigner2D(_PlanBase):
     gp_coeff    : Gross-Pitaevskii coefficient a_GP (0 = off).
     mask        : absorbing edge mask of the potential kernel (on in the reference).
     truncate_constants : emulate the '%f' splicing of kernel constants.
-    precision   : 'fp64' (complex128 state) or 'fp32' (complex64 state; FP64 phases,
-                  twiddles and observable sums).
+    precision   : 'fp64' (complex128 state), 'fp32' (complex64 state, FP64 arithmetic in the
+                  sweeps: half the HBM bytes, one FP32 rounding per sweep) or 'fp32fast'
+                  (complex64 state and FP32 arithmetic with FP64 phases, compensated twiddles
+                  and FP64 observable sums).
     time_rows   : rows of the coefficient table for a time-dependent V (>= steps + 2).
     pot_table   : explicit [rows, 9] coefficient table for the device (the string
                   is still used for the host observables).
@@ -274,8 +281,8 @@ class Wigner2D(_PlanBase):
                  device=None, variants=None, L_material=None, odm_sign=1.0):
         import torch
 
-        if precision not in ("fp64", "fp32"):
-            raise ValueError("precision must be 'fp64' or 'fp32'")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         self.precision = precision
         self.dtype = torch.complex128 if precision == "fp64" else torch.complex64
         self.device = torch.device(device if device is not None else "cuda:0")
@@ -408,8 +415,8 @@ class WignerEnsemble(_PlanBase):
                  time_rows=None, device=None, variants=None):
         import torch
 
-        if precision not in ("fp64", "fp32"):
-            raise ValueError("precision must be 'fp64' or 'fp32'")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         self.precision = precision
         self.dtype = torch.complex128 if precision == "fp64" else torch.complex64
         self.device = torch.device(device if device is not None else "cuda:0")

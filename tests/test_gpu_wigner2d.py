@@ -126,15 +126,21 @@ def test_strang_gross_pitaevskii_fp64(cuda):
     check_obs(obs, obs_ref)
 
 
+# complex64 state: 'fp32' keeps FP64 arithmetic inside the sweeps (one FP32 rounding of the
+# state per sweep: ~6e-7 after 300 steps here); 'fp32fast' also does the transforms in FP32,
+# whose rounding error a unitary step accumulates (~3e-5 after 300 steps -- the same as
+# numpy's single-precision FFT gives for this workload, i.e. the FP32 arithmetic floor).
+@pytest.mark.parametrize("precision,bound", [("fp32", 1e-5), ("fp32fast", 1e-4)])
 @pytest.mark.parametrize("scheme", ["lie", "strang"])
-def test_fp32_bulk_error(cuda, scheme):
-    hip, orc = make_pair(cuda, scheme=scheme, precision="fp32", mask=False)
+def test_fp32_bulk_error(cuda, scheme, precision, bound):
+    hip, orc = make_pair(cuda, scheme=scheme, precision=precision, mask=False)
     W0 = ow.gaussian_wigner(orc.g, x0=2.0, p0=0.0, s=1.0)
     W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 300)
     err = rel_l2(W, W_ref)
-    assert err < 1e-5, err
+    print(f"{precision} {scheme}: rel L2 after 300 steps {err:.3e}")
+    assert err < bound, err
     for k in ["X", "P", "X2", "P2", "H", "norm"]:
-        np.testing.assert_allclose(obs[:, OBS_COL[k]], obs_ref[k], rtol=1e-5, atol=1e-6, err_msg=k)
+        np.testing.assert_allclose(obs[:, OBS_COL[k]], obs_ref[k], rtol=bound, atol=bound / 10, err_msg=k)
 
 
 def test_graph_replay_matches_eager(cuda):
