@@ -117,7 +117,7 @@ def hbm_batch(precision, n, steps, warmup, dev):
     (n x 16 MiB FP64 / 8 MiB FP32) is far beyond the 256 MB last-level cache."""
     from pywignercuda_amd.wigner2d import WignerEnsemble
 
-    c = plain(precision)
+    c = plain("fp64")
     m = dict(X=c["X"], P=c["P"], X_amp=c["X_amp"], P_amp=c["P_amp"], potential=c["potential"],
              dpotential=c["dpotential"], mask=False)
     ens = WignerEnsemble([m] * n, 1.0, 1.0, c["dt"], scheme="lie", precision=precision, device=dev)
@@ -127,7 +127,8 @@ def hbm_batch(precision, n, steps, warmup, dev):
     wall = timed_ensemble(ens, states, steps, warmup)
     cells = n * c["X"] * c["P"]
     return {"members": n, "cell_steps_per_s": cells * steps / wall, "steps": steps,
-            "hbm_frac_algorithmic": cells * steps * (64 if precision == "fp64" else 32) / wall / 1e9 / HBM_PEAK_GBS}
+            "hbm_frac_algorithmic": cells * steps * (64 if precision == "fp64" else 32) / wall / 1e9 / HBM_PEAK_GBS,
+            "state_bytes": cells * (16 if precision == "fp64" else 8)}
 
 
 def uneven_ensemble_members(seed=20261202, n=256):
@@ -162,7 +163,7 @@ def uneven_ensemble(steps, warmup, dev):
     members, init = uneven_ensemble_members()
     out = {}
     finals = {}
-    for prec in ("fp32", "fp64"):
+    for prec in ("fp32", "fp32fast", "fp64"):
         ens = WignerEnsemble(members, 1.0, 1.0, 0.01, scheme="lie", precision=prec, device=dev)
         W0 = [_gauss(t.x_host, t.p_host, t.dX, t.dP, *ini) for t, ini in zip(ens.tables, init)]
         states = ens.states_to_device(W0)
@@ -175,8 +176,9 @@ def uneven_ensemble(steps, warmup, dev):
         out[prec] = {"cell_steps_per_s": cells * steps / wall, "wall_s": wall}
         finals[prec] = torch.cat([W.reshape(-1).to(torch.complex128) for W in states]).cpu().numpy()
         del ens, states
-    d = finals["fp32"] - finals["fp64"]
-    out["fp32_rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(finals["fp64"]))
+    for prec in ("fp32", "fp32fast"):
+        d = finals[prec] - finals["fp64"]
+        out[prec]["rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(finals["fp64"]))
     out["cells"] = int(sum(m["X"] * m["P"] for m in members))
     out["members"] = len(members)
     out["steps"] = steps
@@ -301,17 +303,22 @@ def main():
     if not a.no_extra:
         import numpy as np
 
-        p32 = make(cfg2("fp32"), dev)
-        W32 = p32.to_device(initial_state(p32, "cat"))
-        wall32, _ = timed_run(p32, W32, a.steps, a.warmup)
         ref = W.cpu().numpy()
-        err32 = float(np.linalg.norm(W32.cpu().numpy() - ref) / np.linalg.norm(ref))
+        f32 = {}
+        for prec in ("fp32", "fp32fast"):
+            p32 = make(cfg2(prec), dev)
+            W32 = p32.to_device(initial_state(p32, "cat"))
+            wall32, _ = timed_run(p32, W32, a.steps, a.warmup)
+            f32[prec] = (cells * a.steps / wall32,
+                         float(np.linalg.norm(W32.cpu().numpy() - ref) / np.linalg.norm(ref)))
         pp = make(plain("fp64"), dev)
         Wp = pp.to_device(initial_state(pp, "gauss"))
         wallp, _ = timed_run(pp, Wp, a.steps, a.warmup)
         out["extra"] = {
-            "fp32_bulk_cell_steps_per_s": cells * a.steps / wall32,
-            "fp32_rel_l2_vs_fp64": err32,
+            "fp32_cell_steps_per_s": f32["fp32"][0],
+            "fp32_rel_l2_vs_fp64": f32["fp32"][1],
+            "fp32fast_cell_steps_per_s": f32["fp32fast"][0],
+            "fp32fast_rel_l2_vs_fp64": f32["fp32fast"][1],
             "plain_fp64_cell_steps_per_s": cells * a.steps / wallp,
             "plain_fp64_hbm_frac": cells * 64 * a.steps / wallp / 1e9 / HBM_PEAK_GBS,
         }
@@ -322,9 +329,12 @@ def main():
         torch.cuda.empty_cache()
         out["extra"]["hbm_batch_fp32_128x1024^2"] = hbm_batch("fp32", 128, bsteps, 5, dev)
         torch.cuda.empty_cache()
+        out["extra"]["hbm_batch_fp32fast_128x1024^2"] = hbm_batch("fp32fast", 128, bsteps, 5, dev)
+        torch.cuda.empty_cache()
         b64 = out["extra"]["hbm_batch_fp64_64x1024^2"]["cell_steps_per_s"]
-        b32 = out["extra"]["hbm_batch_fp32_128x1024^2"]["cell_steps_per_s"]
-        out["extra"]["batched_fp32_over_fp64"] = b32 / b64
+        out["extra"]["batched_fp32_over_fp64"] = out["extra"]["hbm_batch_fp32_128x1024^2"]["cell_steps_per_s"] / b64
+        out["extra"]["batched_fp32fast_over_fp64"] = (
+            out["extra"]["hbm_batch_fp32fast_128x1024^2"]["cell_steps_per_s"] / b64)
         out["extra"]["uneven_ensemble_256"] = uneven_ensemble(1000, 0, dev)
         torch.cuda.empty_cache()
         out["extra"]["dirac_512^2_fp64"] = dirac_workload("fp64", 1000, dev)

@@ -1027,8 +1027,8 @@ int dirac_validate(const wh_dirac_desc* d) {
   if (!d) return fail(WH_INVALID_VALUE, "wh_dirac_create: desc is NULL");
   if (d->struct_size != (int)sizeof(wh_dirac_desc))
     return fail(WH_INVALID_DESCRIPTOR, "struct_size %d != %d", d->struct_size, (int)sizeof(wh_dirac_desc));
-  if (d->precision != WH_C128 && d->precision != WH_C64)
-    return fail(WH_INVALID_TYPE, "precision %d is neither WH_C128 nor WH_C64", d->precision);
+  if (d->precision != WH_C128 && d->precision != WH_C64 && d->precision != WH_C64_WIDE)
+    return fail(WH_INVALID_TYPE, "precision %d is not WH_C128, WH_C64 or WH_C64_WIDE", d->precision);
   if (!sweep_size_supported(d->x_dim) || !sweep_size_supported(d->p_dim))
     return fail(WH_INVALID_SIZE, "grid %d x %d: both sides must be powers of two in [128, 2048]", d->p_dim, d->x_dim);
   if (d->pot_deg == WH_POT_TABLE) {
@@ -1179,9 +1179,9 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
     e->pgblocks = wh::dirac_pg_blocks(d.x_dim, d.p_dim);
     if (e->pgblocks < 1) return fail(WH_INVALID_SIZE, "grid %zu x %zu", P, X);
   }
-  h.twX_hi = d.precision == WH_C128 ? e->twx.hi64.p : e->twx.hi.p;
+  h.twX_hi = d.precision != WH_C64 ? e->twx.hi64.p : e->twx.hi.p;
   h.twX_lo = e->twx.lo.p;
-  h.twP_hi = d.precision == WH_C128 ? e->twp.hi64.p : e->twp.hi.p;
+  h.twP_hi = d.precision != WH_C64 ? e->twp.hi64.p : e->twp.hi.p;
   h.twP_lo = e->twp.lo.p;
   h.X = d.x_dim;
   h.P = d.p_dim;
@@ -1217,7 +1217,7 @@ extern "C" int wh_dirac_destroy(wh_dirac* plan) {
 namespace {
 // Points the device plan at W (and obs) in stream order and sets the step counter.
 int dirac_bind(wh_dirac* e, void* W, double* obs, int base, int step, hipStream_t s) {
-  const size_t align = e->d.precision == WH_C128 ? 16 : 8;
+  const size_t align = state_bytes(e->d.precision);
   if (reinterpret_cast<uintptr_t>(W) % align) return fail(WH_UNALIGNED_DATA, "W %p", W);
   WH_HIP(hipStreamSynchronize(s), "sync");
   e->host.W = W;
@@ -1266,12 +1266,12 @@ extern "C" int wh_dirac_run(wh_dirac* e, void* W, int first_step, int n_steps, d
   // the device plan is read by every kernel (and by captured graphs): update it in stream order
   WH_TRY(dirac_bind(e, W, obs, first_step, first_step, s));
   const auto* dp = e->plan_dev.as<wh::DiracPlan>();
-  WH_LAUNCH(wh::launch_dirac_colsum(e->d.precision, dp, e->d.x_dim, s), "dirac_colsum");
+  WH_LAUNCH(wh::launch_dirac_colsum(storage_code(e->d.precision), dp, e->d.x_dim, s), "dirac_colsum");
   WH_LAUNCH(wh::launch_dirac_reduce(dp, 0, 0, 0, 0, s), "dirac_reduce");
   if (use_graph)
     WH_TRY(dirac_run_graph(e, n_steps, s));
   else
     WH_TRY(dirac_iterations(e, n_steps, s));
-  WH_LAUNCH(wh::launch_dirac_finish(e->d.precision, dp, e->d.p_dim, s), "dirac_finish");
+  WH_LAUNCH(wh::launch_dirac_finish(storage_code(e->d.precision), dp, e->d.p_dim, s), "dirac_finish");
   return WH_SUCCESS;
 }

@@ -62,14 +62,14 @@ __device__ __forceinline__ Blk kinetic_block(double q, double mh_c, double c_dt,
 }  // namespace
 
 // ------------------------------------------------------------------ x sweep
-template <typename T, int N, int R, int BLOCK>
+template <typename S, typename T, int N, int R, int BLOCK>
 __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group, int tile, cplx<T>* lds) {
   constexpr int TT = N / R, ROWS = BLOCK / TT;
   constexpr int LSZ = Padded<N, R>::size;
   const int tid = threadIdx.x, team = tid / TT, t = tid % TT;
   const int row = tile * ROWS + team;
   const size_t plane = size_t(d.P) * size_t(d.X);
-  cplx<T>* __restrict__ W = reinterpret_cast<cplx<T>*>(d.W) + size_t(row) * N;
+  cplx<S>* __restrict__ W = reinterpret_cast<cplx<S>*>(d.W) + size_t(row) * N;
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twX_hi), reinterpret_cast<const cplx<T>*>(d.twX_lo)};
   LdsRow<T> view{lds + team * LSZ};
   const double s = *d.scale;
@@ -77,9 +77,9 @@ __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group,
   cplx<T> v[4][R];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const cplx<T>* src = W + size_t(kDiracGroups[group][q]) * plane;
+    const cplx<S>* src = W + size_t(kDiracGroups[group][q]) * plane;
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[q][j] = src[t + j * TT];
+    for (int j = 0; j < R; ++j) v[q][j] = to_arith<T>(src[t + j * TT]);
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -120,9 +120,9 @@ __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group,
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     team_fft<T, N, R, +1>(v[q], t, view, tw);
-    cplx<T>* dst = W + size_t(kDiracGroups[group][q]) * plane;
+    cplx<S>* dst = W + size_t(kDiracGroups[group][q]) * plane;
 #pragma unroll
-    for (int j = 0; j < R; ++j) dst[t + j * TT] = cscale(v[q][j], inv);
+    for (int j = 0; j < R; ++j) dst[t + j * TT] = to_arith<S>(cscale(v[q][j], inv));
   }
 }
 
@@ -222,7 +222,7 @@ __device__ __forceinline__ void dirac_store_colsums(const DiracPlan& d, int tile
   }
 }
 
-template <typename T, int N, int R, int BLOCK, bool CL>
+template <typename S, typename T, int N, int R, int BLOCK, bool CL>
 __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, cplx<T>* lds) {
   constexpr int TT = N / R;
   constexpr int C = BLOCK / TT;  // columns per tile
@@ -231,7 +231,7 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
   const int X = d.X;
   const int col = tile * C + c;
   const size_t plane = size_t(d.P) * size_t(X);
-  cplx<T>* __restrict__ W0 = reinterpret_cast<cplx<T>*>(d.W);
+  cplx<S>* __restrict__ W0 = reinterpret_cast<cplx<S>*>(d.W);
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twP_hi), reinterpret_cast<const cplx<T>*>(d.twP_lo)};
   LdsStrided<T, C> view{lds + c};
   const int step = *d.step;
@@ -262,7 +262,7 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
   for (int j = 0; j < R; ++j) trre[j] = 0.0;
 
   const T inv = T(1.0 / N);
-  const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<T>));
+  const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<S>));
   // planes by mass-phase class: spinor rows {1,2} or {3,4} x columns {1,2} or {3,4}.  Class
   // multiplier r e^{-i (ph + m)}: m = F- - F+ (lo, lo), F- + F+ (lo, hi), -(F- + F+) (hi, lo),
   // -(F- - F+) (hi, hi), F -/+ = Mass(x -/+ theta/2) c^2 dt; a constant mass gives 0, 2F, -2F, 0.
@@ -283,10 +283,10 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
     }
     for (int q = 0; q < 4; ++q) {
       const int k = ((cls >> 1) << 3) | ((cls & 1) << 1) | ((q >> 1) << 2) | (q & 1);
-      cplx<T>* __restrict__ W = W0 + size_t(k) * plane;
+      cplx<S>* __restrict__ W = W0 + size_t(k) * plane;
       cplx<T> v[R];
 #pragma unroll
-      for (int j = 0; j < R; ++j) v[j] = at_off(W + size_t(j * TT) * X, boff);
+      for (int j = 0; j < R; ++j) v[j] = to_arith<T>(at_off(W + size_t(j * TT) * X, boff));
       team_fft<T, N, R, -1>(v, t, view, tw);
 #pragma unroll
       for (int j = 0; j < R; ++j) v[j] = apply(v[j], M[j]);
@@ -298,7 +298,7 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
 
       dirac_accumulate<T, R, TT>(d, k, v, t, acc, trre);
 #pragma unroll
-      for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = v[j];
+      for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = to_arith<S>(v[j]);
     }
   }
 #pragma unroll
@@ -341,7 +341,7 @@ __device__ __forceinline__ DiracPot dirac_pot_factor(const DiracPlan& d, double 
   return u;
 }
 
-template <typename T, int N, int R, int BLOCK>
+template <typename S, typename T, int N, int R, int BLOCK>
 __device__ __forceinline__ void dirac_pleft_body(const DiracPlan& d, int tile, cplx<T>* lds) {
   constexpr int TT = N / R;
   constexpr int C = BLOCK / TT;
@@ -350,7 +350,7 @@ __device__ __forceinline__ void dirac_pleft_body(const DiracPlan& d, int tile, c
   const int X = d.X;
   const int col = tile * C + c;
   const size_t plane = size_t(d.P) * size_t(X);
-  cplx<T>* __restrict__ W0 = reinterpret_cast<cplx<T>*>(d.W);
+  cplx<S>* __restrict__ W0 = reinterpret_cast<cplx<S>*>(d.W);
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twP_hi), reinterpret_cast<const cplx<T>*>(d.twP_lo)};
   LdsStrided<T, C> view{lds + c};
   const int step = *d.step;
@@ -358,14 +358,14 @@ __device__ __forceinline__ void dirac_pleft_body(const DiracPlan& d, int tile, c
   const double mask = d.xmask ? d.xmask[col] : 1.0;
   const int prow = step < d.pot_rows ? step : d.pot_rows - 1;
   const double* __restrict__ cf = d.pot + size_t(prow) * (kMaxDeg + 1);
-  const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<T>));
+  const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<S>));
   for (int jc = 0; jc < 4; ++jc) {  // spinor column jc: planes 4 i + jc
     cplx<T> v[4][R];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      cplx<T>* W = W0 + size_t(4 * q + jc) * plane;
+      cplx<S>* W = W0 + size_t(4 * q + jc) * plane;
 #pragma unroll
-      for (int j = 0; j < R; ++j) v[q][j] = at_off(W + size_t(j * TT) * X, boff);
+      for (int j = 0; j < R; ++j) v[q][j] = to_arith<T>(at_off(W + size_t(j * TT) * X, boff));
       team_fft<T, N, R, -1>(v[q], t, view, tw);
     }
 #pragma unroll
@@ -393,14 +393,14 @@ __device__ __forceinline__ void dirac_pleft_body(const DiracPlan& d, int tile, c
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      cplx<T>* W = W0 + size_t(4 * q + jc) * plane;
+      cplx<S>* W = W0 + size_t(4 * q + jc) * plane;
 #pragma unroll
-      for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = v[q][j];
+      for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = to_arith<S>(v[q][j]);
     }
   }
 }
 
-template <typename T, int N, int R, int BLOCK, bool CL>
+template <typename S, typename T, int N, int R, int BLOCK, bool CL>
 __device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, cplx<T>* lds) {
   constexpr int TT = N / R;
   constexpr int C = BLOCK / TT;
@@ -409,11 +409,11 @@ __device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, 
   const int X = d.X;
   const int col = tile * C + c;
   const size_t plane = size_t(d.P) * size_t(X);
-  cplx<T>* __restrict__ W0 = reinterpret_cast<cplx<T>*>(d.W);
+  cplx<S>* __restrict__ W0 = reinterpret_cast<cplx<S>*>(d.W);
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twP_hi), reinterpret_cast<const cplx<T>*>(d.twP_lo)};
   LdsStrided<T, C> view{lds + c};
   const double x = d.dx * double(fidx(col, X));
-  const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<T>));
+  const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<S>));
   const T inv = T(1.0 / N);
   double acc[kDNCol];
 #pragma unroll
@@ -425,9 +425,9 @@ __device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, 
     cplx<T> v[4][R];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      cplx<T>* W = W0 + size_t(4 * ir + q) * plane;
+      cplx<S>* W = W0 + size_t(4 * ir + q) * plane;
 #pragma unroll
-      for (int j = 0; j < R; ++j) v[q][j] = at_off(W + size_t(j * TT) * X, boff);
+      for (int j = 0; j < R; ++j) v[q][j] = to_arith<T>(at_off(W + size_t(j * TT) * X, boff));
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -450,9 +450,9 @@ __device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, 
       if constexpr (CL) dirac_cl2<T, N, R>(v[q], d, t, view, tw);
       const int k = 4 * ir + q;
       dirac_accumulate<T, R, TT>(d, k, v[q], t, acc, trre);
-      cplx<T>* W = W0 + size_t(k) * plane;
+      cplx<S>* W = W0 + size_t(k) * plane;
 #pragma unroll
-      for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = v[q][j];
+      for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = to_arith<S>(v[q][j]);
     }
   }
 #pragma unroll
@@ -498,39 +498,39 @@ constexpr int dirac_lds_elems() {
     default: break;                   \
   }
 
-template <typename T>
+template <typename S, typename T>
 __global__ void __launch_bounds__(kDiracXBlock) dirac_xsweep_kernel(const DiracPlan* __restrict__ dp) {
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
   const DiracPlan& d = *dp;
   const int group = int(blockIdx.x & 3u), tile = int(blockIdx.x >> 2);
-#define BODY(n) dirac_xsweep_body<T, n, dirac_rx<n>(), kDiracXBlock>(d, group, tile, lds)
+#define BODY(n) dirac_xsweep_body<S, T, n, dirac_rx<n>(), kDiracXBlock>(d, group, tile, lds)
   WH_DIRAC_DISPATCH(d.X, BODY)
 #undef BODY
 }
 
-template <typename T, bool CL>
+template <typename S, typename T, bool CL>
 __global__ void __launch_bounds__(kDiracPBlock) dirac_psweep_kernel(const DiracPlan* __restrict__ dp) {
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracPBlock>()];
   const DiracPlan& d = *dp;
-#define BODY(n) dirac_psweep_body<T, n, dirac_rp<n>(), kDiracPBlock, CL>(d, int(blockIdx.x), lds)
+#define BODY(n) dirac_psweep_body<S, T, n, dirac_rp<n>(), kDiracPBlock, CL>(d, int(blockIdx.x), lds)
   WH_DIRAC_DISPATCH(d.P, BODY)
 #undef BODY
 }
 
-template <typename T>
+template <typename S, typename T>
 __global__ void __launch_bounds__(kDiracXBlock) dirac_pleft_kernel(const DiracPlan* __restrict__ dp) {
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
   const DiracPlan& d = *dp;
-#define BODY(n) dirac_pleft_body<T, n, dirac_rx<n>(), kDiracXBlock>(d, int(blockIdx.x), lds)
+#define BODY(n) dirac_pleft_body<S, T, n, dirac_rx<n>(), kDiracXBlock>(d, int(blockIdx.x), lds)
   WH_DIRAC_DISPATCH(d.P, BODY)
 #undef BODY
 }
 
-template <typename T, bool CL>
+template <typename S, typename T, bool CL>
 __global__ void __launch_bounds__(kDiracXBlock) dirac_pright_kernel(const DiracPlan* __restrict__ dp) {
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
   const DiracPlan& d = *dp;
-#define BODY(n) dirac_pright_body<T, n, dirac_rx<n>(), kDiracXBlock, CL>(d, int(blockIdx.x), lds)
+#define BODY(n) dirac_pright_body<S, T, n, dirac_rx<n>(), kDiracXBlock, CL>(d, int(blockIdx.x), lds)
   WH_DIRAC_DISPATCH(d.P, BODY)
 #undef BODY
 }
@@ -776,23 +776,31 @@ int dirac_p_blocks(int X, int P) {
 
 hipError_t launch_dirac_xsweep(int prec, const DiracPlan* d, int nblocks, hipStream_t s) {
   if (prec == 0)
-    hipLaunchKernelGGL(dirac_xsweep_kernel<double>, dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
+    hipLaunchKernelGGL((dirac_xsweep_kernel<double, double>), dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
+  else if (prec == 1)
+    hipLaunchKernelGGL((dirac_xsweep_kernel<float, float>), dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
   else
-    hipLaunchKernelGGL(dirac_xsweep_kernel<float>, dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
+    hipLaunchKernelGGL((dirac_xsweep_kernel<float, double>), dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
   return hipGetLastError();
 }
 
 hipError_t launch_dirac_psweep(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s) {
+  const dim3 g(nblocks), b(kDiracPBlock);
   if (prec == 0) {
     if (cl)
-      hipLaunchKernelGGL((dirac_psweep_kernel<double, true>), dim3(nblocks), dim3(kDiracPBlock), 0, s, d);
+      hipLaunchKernelGGL((dirac_psweep_kernel<double, double, true>), g, b, 0, s, d);
     else
-      hipLaunchKernelGGL((dirac_psweep_kernel<double, false>), dim3(nblocks), dim3(kDiracPBlock), 0, s, d);
+      hipLaunchKernelGGL((dirac_psweep_kernel<double, double, false>), g, b, 0, s, d);
+  } else if (prec == 1) {
+    if (cl)
+      hipLaunchKernelGGL((dirac_psweep_kernel<float, float, true>), g, b, 0, s, d);
+    else
+      hipLaunchKernelGGL((dirac_psweep_kernel<float, float, false>), g, b, 0, s, d);
   } else {
     if (cl)
-      hipLaunchKernelGGL((dirac_psweep_kernel<float, true>), dim3(nblocks), dim3(kDiracPBlock), 0, s, d);
+      hipLaunchKernelGGL((dirac_psweep_kernel<float, double, true>), g, b, 0, s, d);
     else
-      hipLaunchKernelGGL((dirac_psweep_kernel<float, false>), dim3(nblocks), dim3(kDiracPBlock), 0, s, d);
+      hipLaunchKernelGGL((dirac_psweep_kernel<float, double, false>), g, b, 0, s, d);
   }
   return hipGetLastError();
 }
@@ -808,24 +816,30 @@ int dirac_pg_blocks(int X, int P) {
 hipError_t launch_dirac_pgeneral(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s) {
   const dim3 g(nblocks), b(kDiracXBlock);
   if (prec == 0) {
-    hipLaunchKernelGGL(dirac_pleft_kernel<double>, g, b, 0, s, d);
+    hipLaunchKernelGGL((dirac_pleft_kernel<double, double>), g, b, 0, s, d);
     if (cl)
-      hipLaunchKernelGGL((dirac_pright_kernel<double, true>), g, b, 0, s, d);
+      hipLaunchKernelGGL((dirac_pright_kernel<double, double, true>), g, b, 0, s, d);
     else
-      hipLaunchKernelGGL((dirac_pright_kernel<double, false>), g, b, 0, s, d);
+      hipLaunchKernelGGL((dirac_pright_kernel<double, double, false>), g, b, 0, s, d);
+  } else if (prec == 1) {
+    hipLaunchKernelGGL((dirac_pleft_kernel<float, float>), g, b, 0, s, d);
+    if (cl)
+      hipLaunchKernelGGL((dirac_pright_kernel<float, float, true>), g, b, 0, s, d);
+    else
+      hipLaunchKernelGGL((dirac_pright_kernel<float, float, false>), g, b, 0, s, d);
   } else {
-    hipLaunchKernelGGL(dirac_pleft_kernel<float>, g, b, 0, s, d);
+    hipLaunchKernelGGL((dirac_pleft_kernel<float, double>), g, b, 0, s, d);
     if (cl)
-      hipLaunchKernelGGL((dirac_pright_kernel<float, true>), g, b, 0, s, d);
+      hipLaunchKernelGGL((dirac_pright_kernel<float, double, true>), g, b, 0, s, d);
     else
-      hipLaunchKernelGGL((dirac_pright_kernel<float, false>), g, b, 0, s, d);
+      hipLaunchKernelGGL((dirac_pright_kernel<float, double, false>), g, b, 0, s, d);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_dirac_colsum(int prec, const DiracPlan* d, int X, hipStream_t s) {
   const int nb = (X + 255) / 256;
-  if (prec == 0)
+  if (prec == 0)  // state storage only: complex128 or complex64
     hipLaunchKernelGGL(dirac_colsum_kernel<double>, dim3(nb), dim3(256), 0, s, d);
   else
     hipLaunchKernelGGL(dirac_colsum_kernel<float>, dim3(nb), dim3(256), 0, s, d);
@@ -848,7 +862,7 @@ hipError_t launch_dirac_reduce(const DiracPlan* d, int state_add, int renorm, in
 }
 
 hipError_t launch_dirac_finish(int prec, const DiracPlan* d, int P, hipStream_t s) {
-  if (prec == 0)
+  if (prec == 0)  // state storage only
     hipLaunchKernelGGL(dirac_finish_kernel<double>, dim3(P), dim3(256), 0, s, d);
   else
     hipLaunchKernelGGL(dirac_finish_kernel<float>, dim3(P), dim3(256), 0, s, d);

@@ -22,9 +22,9 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import DIRAC_OBS, DIRAC_OBS_REAL, WH_C128, WH_C64, WH_DNOBS, WH_MAX_DEG, WH_POT_TABLE
+from ._lib import DIRAC_OBS, DIRAC_OBS_REAL, WH_DNOBS, WH_MAX_DEG, WH_POT_TABLE
 from .physics import NonPolynomialPhysics, coefficient_table, depends_on_time, host_eval
-from .wigner2d import fft_order_axis, host_axis, percent_f
+from .wigner2d import PRECISIONS, fft_order_axis, host_axis, percent_f
 
 
 def _sympy_derivative(expr: str) -> str:
@@ -58,7 +58,8 @@ class WignerDirac4x4:
     absorb     : the 24-cell edge absorber applied every step (always on in the reference).
     truncate_constants : emulate the '%f' splicing of the kernel constants.
     antiparticle, energy : record AntiParticlePopulation / DiracEnergy.
-    precision  : 'fp64' (complex128 planes) or 'fp32' (complex64 planes, FP64 factors and sums).
+    precision  : 'fp64' (complex128 planes), 'fp32' (complex64 planes, FP64 arithmetic in the
+                 sweeps) or 'fp32fast' (complex64 planes, FP32 transforms, FP64 factors and sums).
     time_rows  : rows of the coefficient table for a time-dependent polynomial V0.
     mass_potential, kinetic_mass_half : the MassPotential variant -- Mass(t, x) string of the
                  potential factors and the fixed mass_half of the kinetic factors.
@@ -71,8 +72,8 @@ class WignerDirac4x4:
                  kinetic_mass_half=None):
         import torch
 
-        if precision not in ("fp64", "fp32"):
-            raise ValueError("precision must be 'fp64' or 'fp32'")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         a_deg, a_coef = [-1, -1, -1], np.zeros((3, WH_MAX_DEG + 1))
         for k in (1, 2, 3):
             if _is_zero(potentials[k]):
@@ -118,7 +119,7 @@ class WignerDirac4x4:
 
         d = _lib.DiracDesc()
         d.struct_size = ctypes.sizeof(_lib.DiracDesc)
-        d.precision = WH_C128 if precision == "fp64" else WH_C64
+        d.precision = PRECISIONS[precision]
         d.x_dim, d.p_dim = self.X, self.P
         d.dt, d.dx, d.dp, d.dtheta, d.dlambda = k_dt, k_dx, k_dp, k_dtheta, k_dlambda
         k_c = tr(c)
