@@ -140,7 +140,7 @@ def test_fp32_bulk_error(cuda, scheme, precision, bound):
     print(f"{precision} {scheme}: rel L2 after 300 steps {err:.3e}")
     assert err < bound, err
     for k in ["X", "P", "X2", "P2", "H", "norm"]:
-        np.testing.assert_allclose(obs[:, OBS_COL[k]], obs_ref[k], rtol=bound, atol=bound / 10, err_msg=k)
+        np.testing.assert_allclose(obs[:, OBS_COL[k]], obs_ref[k], rtol=bound, atol=bound, err_msg=k)
 
 
 def test_graph_replay_matches_eager(cuda):
