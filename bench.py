@@ -256,6 +256,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--workload", choices=("damped", "plain"), default="damped",
+                    help="timed workload: the damped 1024^2 step (default) or the undamped quartic step")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 8 plain device copies of 256 MiB (scale for the HBM byte counters)")
     a = ap.parse_args()
@@ -266,8 +268,12 @@ def main():
     import torch
 
     dev = torch.device("cuda:0")
-    prop = make(cfg2("fp64"), dev)
-    W = prop.to_device(initial_state(prop, "cat"))
+    if a.workload == "plain":
+        prop = make(plain("fp64"), dev)
+        W = prop.to_device(initial_state(prop, "gauss"))
+    else:
+        prop = make(cfg2("fp64"), dev)
+        W = prop.to_device(initial_state(prop, "cat"))
     wall, obs = timed_run(prop, W, a.steps, a.warmup)
     cells = prop.X * prop.P
     value = cells * a.steps / wall

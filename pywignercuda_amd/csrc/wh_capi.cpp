@@ -282,7 +282,7 @@ namespace {
 
 struct MemberBufs {
   DevBuf pot, theta_decay, lambda_decay, x_mask, sigma_p, theta_host, lambda_host;
-  DevBuf x_host, p_host, dv_host, v0_host, colsums, scale, rho, G, colpart, pot_dv;
+  DevBuf x_host, p_host, dv_host, v0_host, colsums, tilepart, scale, rho, G, colpart, pot_dv;
   DevBuf odm_g, odm_x, odm_th, odm_s;
 };
 
@@ -476,6 +476,7 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
     WH_TRY(b->dv_host.upload(md.dv_host ? md.dv_host : zeros.data(), X * sizeof(double)));
     WH_TRY(b->v0_host.upload(md.v0_host ? md.v0_host : zeros.data(), X * sizeof(double)));
     WH_TRY(b->colsums.alloc(X * wh::kNCol * sizeof(double)));
+    WH_TRY(b->tilepart.alloc(X * wh::kTileObs * sizeof(double)));  // at most one tile per column
     const double one[2] = {1.0, 0.0};
     WH_TRY(b->scale.upload(one, sizeof(one)));
     if (gp) {
@@ -491,6 +492,7 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
     wh::Member& m = e->mem[i];
     std::memset(&m, 0, sizeof(m));
     m.colsums = b->colsums.as<double>();
+    m.tilepart = b->tilepart.as<double>();
     m.scale = b->scale.as<double>();
     m.rho = b->rho.as<double>();
     m.G = b->G.as<double>();
@@ -560,6 +562,7 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
   WH_TRY(e->step.alloc(16));
   const int zero[4] = {0, 0, 0, 0};
   WH_TRY(e->done.upload(zero, sizeof(zero)));
+
   WH_TRY(build_work_lists(e.get()));
   *out = e.release();
   return WH_SUCCESS;
@@ -582,6 +585,12 @@ wh::SweepArgs sweep_args(const wh_ensemble* e, int flags, double frac, int step_
   a.inv_hbar = 1.0 / d.hbar;
   a.kdeg = d.kinetic_deg;
   for (int k = 0; k <= wh::kMaxDeg; ++k) a.kcoef[k] = d.kin_coef[k];
+  a.rec_step = e->step.as<int>() + 1;
+  const wh::SweepVariant pv = e->cl() ? wh::kPCLVariants[e->pclvar] : wh::kPVariants[e->pvar];
+  a.pt_R = pv.R;
+  a.pt_block = pv.block;
+  a.fin_renorm = d.renormalize;
+  a.fin_record = 1;
   if (d.scheme == WH_SCHEME_LIE) {
     // expPLambda: phase = dt*lambda*p/mass (GPU_Wigner2D_FFT.py:153)
     a.kin_linear = 1;
@@ -645,7 +654,14 @@ int gpshift(wh_ensemble* e, hipStream_t s) {
 //  Strang (GP.py:1898-1999): without GP the second half step of iteration k and the
 //         first of k+1 share one p sweep, the observables taken between them; with GP
 //         V/2, K + partial column sums, density, V/2 + column sums.
-int iteration(wh_ensemble* e, bool merge, hipStream_t s) {
+//  Lie and Strang without GP or ODM: no reduce launch; each p sweep writes tile partials
+//  of the observables and the next x sweep turns them into the record row and the
+//  factor (kTileScale).  `first`: the pending factor is the one of the initial reduce.
+//  After the last iteration tiles_pending(e) asks for one closing reduce.
+bool tiles_pending(const wh_ensemble* e) { return !e->gp() && !e->odm(); }
+int renorm_flag(const wh_ensemble* e) { return e->d.renormalize; }
+
+int iteration(wh_ensemble* e, bool merge, bool first, hipStream_t s) {
   const auto& d = e->d;
   const int cl = e->cl() ? wh::kCL : 0;
   const int renorm = d.renormalize;
@@ -662,15 +678,17 @@ int iteration(wh_ensemble* e, bool merge, hipStream_t s) {
                                 e->work_colsum.n, s),
               "colsum");
     WH_TRY(reduce(e, renorm, 1, e->gp(), 1, true, s));
+  } else if (d.scheme == WH_SCHEME_LIE && e->gp()) {
+    WH_TRY(gpshift(e, s));
+    WH_TRY(xsweep(e, wh::kScaleOnLoad, s));
+    WH_TRY(psweep(e, wh::kColSums | cl | wh::kGP, 1.0, 0, s));
+    WH_TRY(reduce(e, renorm, 1, 1, 1, true, s));
   } else if (d.scheme == WH_SCHEME_LIE) {
-    if (e->gp()) WH_TRY(gpshift(e, s));
-    WH_TRY(xsweep(e, wh::kScaleOnLoad, s));
-    WH_TRY(psweep(e, wh::kColSums | cl | (e->gp() ? wh::kGP : 0), 1.0, 0, s));
-    WH_TRY(reduce(e, renorm, 1, e->gp(), 1, true, s));
+    WH_TRY(xsweep(e, first ? wh::kScaleOnLoad : wh::kTileScale, s));
+    WH_TRY(psweep(e, wh::kColSums | wh::kTilePart | cl, 1.0, 0, s));
   } else if (!e->gp()) {
-    WH_TRY(xsweep(e, wh::kScaleOnLoad, s));
-    WH_TRY(psweep(e, wh::kColSums | cl | (merge ? wh::kMerge : 0), 0.5, 0, s));
-    WH_TRY(reduce(e, renorm, 1, 0, 1, true, s));
+    WH_TRY(xsweep(e, first ? wh::kScaleOnLoad : wh::kTileScale, s));
+    WH_TRY(psweep(e, wh::kColSums | wh::kTilePart | cl | (merge ? wh::kMerge : 0), 0.5, 0, s));
   } else {
     WH_TRY(gpshift(e, s));
     WH_TRY(psweep(e, wh::kScaleOnLoad | wh::kGP, 0.5, 0, s));
@@ -685,8 +703,8 @@ int iteration(wh_ensemble* e, bool merge, hipStream_t s) {
   return WH_SUCCESS;
 }
 
-int iterations(wh_ensemble* e, int count, bool merge_last, hipStream_t s) {
-  for (int k = 0; k < count; ++k) WH_TRY(iteration(e, (k < count - 1) || merge_last, s));
+int iterations(wh_ensemble* e, int count, bool merge_last, bool first, hipStream_t s) {
+  for (int k = 0; k < count; ++k) WH_TRY(iteration(e, (k < count - 1) || merge_last, first && k == 0, s));
   return WH_SUCCESS;
 }
 
@@ -703,7 +721,7 @@ int run_graph(wh_ensemble* e, int count, hipStream_t s) {
         WH_HIP(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking), "hipStreamCreate");
       GraphEntry g;
       WH_HIP(hipStreamBeginCapture(e->capture_stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
-      const int rc = iterations(e, chunk, true, e->capture_stream);
+      const int rc = iterations(e, chunk, true, false, e->capture_stream);
       const hipError_t err = hipStreamEndCapture(e->capture_stream, &g.graph);
       if (rc) {
         if (g.graph) (void)hipGraphDestroy(g.graph);
@@ -718,7 +736,7 @@ int run_graph(wh_ensemble* e, int count, hipStream_t s) {
       count -= chunk;
     }
   }
-  return iterations(e, count, true, s);
+  return iterations(e, count, true, false, s);
 }
 
 int sync_table(wh_ensemble* e, bool changed, hipStream_t s) {
@@ -769,12 +787,18 @@ int ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_step
     WH_LAUNCH(wh::launch_set_scale(md, e->n, 1.0, 0.0, s), "set_scale");
   }
   if (n_steps > 0) {
+    // the first iteration takes the factor of the initial reduce; the graph replays the
+    // uniform middle iterations
+    WH_TRY(iterations(e, 1, merged_strang && n_steps > 1, true, s));
     if (merged_strang) {
-      WH_TRY(use_graph ? run_graph(e, n_steps - 1, s) : iterations(e, n_steps - 1, true, s));
-      WH_TRY(iterations(e, 1, false, s));  // last iteration: no next half step
+      if (n_steps > 1) {
+        WH_TRY(use_graph ? run_graph(e, n_steps - 2, s) : iterations(e, n_steps - 2, true, false, s));
+        WH_TRY(iterations(e, 1, false, false, s));  // last iteration: no next half step
+      }
     } else {
-      WH_TRY(use_graph ? run_graph(e, n_steps, s) : iterations(e, n_steps, false, s));
+      WH_TRY(use_graph ? run_graph(e, n_steps - 1, s) : iterations(e, n_steps - 1, false, false, s));
     }
+    if (tiles_pending(e)) WH_TRY(reduce(e, renorm_flag(e), 1, 0, 1, true, s));
   }
   if (d.renormalize || (n_steps == 0 && normalize_initial))
     WH_LAUNCH(wh::launch_scale(storage_code(prec), md, e->work_scale.buf.as<unsigned>(), e->work_scale.n, s),

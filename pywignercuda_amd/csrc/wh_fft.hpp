@@ -25,6 +25,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace wh {
 
 template <typename T>
@@ -183,15 +185,14 @@ struct Twiddles {
   const cplx<T>* lo;
 };
 
+// x * w (DIR < 0) or x * conj(w) (DIR > 0); FP32 adds the product with the low part l.
 template <int DIR>
-__device__ __forceinline__ cplx<double> tw_mul(cplx<double> x, const Twiddles<double>& tw, int idx) {
-  cplx<double> w = tw.hi[idx];
+__device__ __forceinline__ cplx<double> tw_apply(cplx<double> x, cplx<double> w) {
   if (DIR > 0) w.y = -w.y;
   return cmul(x, w);
 }
 template <int DIR>
-__device__ __forceinline__ cplx<float> tw_mul(cplx<float> x, const Twiddles<float>& tw, int idx) {
-  cplx<float> w = tw.hi[idx], l = tw.lo[idx];
+__device__ __forceinline__ cplx<float> tw_apply(cplx<float> x, cplx<float> w, cplx<float> l) {
   if (DIR > 0) {
     w.y = -w.y;
     l.y = -l.y;
@@ -200,78 +201,213 @@ __device__ __forceinline__ cplx<float> tw_mul(cplx<float> x, const Twiddles<floa
   return {rx + fmaf(x.x, l.x, -x.y * l.y), ry + fmaf(x.x, l.y, x.y * l.x)};
 }
 
+template <int DIR>
+__device__ __forceinline__ cplx<double> tw_mul(cplx<double> x, const Twiddles<double>& tw, int idx) {
+  return tw_apply<DIR>(x, tw.hi[idx]);
+}
+template <int DIR>
+__device__ __forceinline__ cplx<float> tw_mul(cplx<float> x, const Twiddles<float>& tw, int idx) {
+  return tw_apply<DIR>(x, tw.hi[idx], tw.lo[idx]);
+}
+
+// Twiddles of every pass of one lane, held in registers for the whole kernel (a lane's
+// twiddles depend only on its team index t, so all transforms of a sweep share them and
+// the inverse uses their conjugates).  Only butterfly mm = 0 of a pass is stored: the
+// passes before the last have one butterfly per lane (r = R), and in the last pass
+// (span Ns = N / r, twiddle w^(j b), b = t + mm TT) butterfly mm differs from mm = 0 by
+// the exact constant w^(j mm TT) = exp(-2 pi i j mm / R).  Slot of (pass P, input j >= 1):
+// off(P) + j - 1.
+template <int N, int R>
+struct TwSlots {
+  static constexpr int NP = num_passes(N, R);
+  static constexpr int off(int P) {
+    int o = 0;
+    for (int q = 1; q < P; ++q) o += pass_radix(N, R, q) - 1;
+    return o;
+  }
+  static constexpr int count = off(NP) > 0 ? off(NP) : 1;
+};
+
+template <typename T, int N, int R>
+struct TwRegs;
+template <int N, int R>
+struct TwRegs<double, N, R> {
+  cplx<double> hi[TwSlots<N, R>::count];
+  template <int DIR>
+  __device__ __forceinline__ cplx<double> mul(cplx<double> x, int slot, int) const {
+    return tw_apply<DIR>(x, hi[slot]);
+  }
+  __device__ __forceinline__ void set(int slot, const Twiddles<double>& tw, int idx) { hi[slot] = tw.hi[idx]; }
+};
+template <int N, int R>
+struct TwRegs<float, N, R> {
+  cplx<float> hi[TwSlots<N, R>::count], lo[TwSlots<N, R>::count];
+  template <int DIR>
+  __device__ __forceinline__ cplx<float> mul(cplx<float> x, int slot, int) const {
+    return tw_apply<DIR>(x, hi[slot], lo[slot]);
+  }
+  __device__ __forceinline__ void set(int slot, const Twiddles<float>& tw, int idx) {
+    hi[slot] = tw.hi[idx];
+    lo[slot] = tw.lo[idx];
+  }
+};
+
+// Table-backed source with the same interface (twiddle index idx, slot unused).
+template <typename T>
+struct TwMem {
+  Twiddles<T> tw;
+  template <int DIR>
+  __device__ __forceinline__ cplx<T> mul(cplx<T> x, int, int idx) const {
+    return tw_mul<DIR>(x, tw, idx);
+  }
+};
+
+// Fills the registers of team index t from the table.
+template <typename T, int N, int R, int P = 1>
+__device__ __forceinline__ void load_twiddles(TwRegs<T, N, R>& w, int t, const Twiddles<T>& tw) {
+  if constexpr (P < num_passes(N, R)) {
+    constexpr int r = pass_radix(N, R, P), Ns = ipow(R, P);
+    const int tstep = (t % Ns) * (N / (Ns * r));
+#pragma unroll
+    for (int j = 1; j < r; ++j) w.set(TwSlots<N, R>::off(P) + j - 1, tw, (j * tstep) & (N - 1));
+    load_twiddles<T, N, R, P + 1>(w, t, tw);
+  }
+}
+
+// Calls f(std::integral_constant<int, i>) for i = B .. E-1.
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// Ordering of a pass's LDS write and read.  BlockSync: the team spans several waves.
+// WaveSync: every lane of the team is in one wave, whose LDS operations execute in
+// program order, so only the compiler has to keep the order (no s_barrier).
+struct BlockSync {
+  __device__ __forceinline__ static void sync() { __syncthreads(); }
+};
+struct WaveSync {
+  __device__ __forceinline__ static void sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
+
 // One Stockham pass P >= 1 (LDS-sourced) through the team's LDS view.
 //
 // Every LDS address is a per-lane base plus a compile-time offset: spans after
 // the first pass are multiples of R, so padr(i + R m) = padr(i) + (R+1) m, and
 // the first pass (radix R, span 1) writes the R consecutive indices of one
 // aligned group of R, so padr(R b + q) = padr(R b) + q.
-template <typename T, int N, int R, int DIR, int P, class Lds>
-struct TeamPass {
+template <typename T, int N, int R, int P>
+struct PassIO {
   static constexpr int NP = num_passes(N, R);
   static constexpr int TT = N / R;
-  __device__ __forceinline__ static void run(cplx<T> (&v)[R], int t, Lds lds, const Twiddles<T>& tw) {
-    if constexpr (P < NP) {
-      constexpr int rp = pass_radix(N, R, P - 1);  // radix of the previous pass
-      constexpr int Nsp = ipow(R, P - 1);          // span before the previous pass
-      constexpr int r = pass_radix(N, R, P);
-      constexpr int Ns = ipow(R, P);
-      constexpr int nbp = R / rp, nb = R / r;
-      constexpr int S = Lds::S;
-      static_assert(Nsp % R == 0 || (Nsp == 1 && rp == R), "separable LDS offsets");
-      static_assert((N / r) % R == 0, "separable LDS offsets");
+  // write the outputs of pass P-1 (team index t, strided register layout of that pass)
+  template <class Lds>
+  __device__ __forceinline__ static void write(const cplx<T> (&v)[R], int tl, Lds lds) {
+    constexpr int rp = pass_radix(N, R, P - 1);  // radix of the previous pass
+    constexpr int Nsp = ipow(R, P - 1);          // span before the previous pass
+    constexpr int nbp = R / rp;
+    constexpr int S = Lds::S;
+    static_assert(Nsp % R == 0 || (Nsp == 1 && rp == R), "separable LDS offsets");
+#pragma unroll
+    for (int mm = 0; mm < nbp; ++mm) {
+      const int b = tl + mm * TT;
+      const int k = b % Nsp;
+      cplx<T>* p = lds.ptr(padr<R>((b - k) * rp + k));
+#pragma unroll
+      for (int q = 0; q < rp; ++q) {
+        constexpr int unit = Nsp == 1 ? 1 : Nsp / R * (R + 1);
+        p[S * q * unit] = v[mm * rp + q];
+      }
+    }
+  }
+  // read the inputs of pass P, twiddle, r-point DFTs; outputs in v
+  template <int DIR, class Lds, class Tw>
+  __device__ __forceinline__ static void read(cplx<T> (&v)[R], int tl, Lds lds, const Tw& tw) {
+    constexpr int r = pass_radix(N, R, P);
+    constexpr int Ns = ipow(R, P);
+    constexpr int nb = R / r;
+    constexpr int S = Lds::S;
+    static_assert((N / r) % R == 0, "separable LDS offsets");
+    static_assert(nb == 1 || P == NP - 1, "several butterflies per lane only in the last pass");
+    static_assert(nb == 1 || Ns == N / r, "the last pass spans N / r");
+    cplx<T> o[R];
+    // twiddle of input j: w^(j k s), k = b mod Ns, s = N / (Ns r); for mm > 0 (last pass,
+    // k = t + mm TT, s = 1) it is w^(j t) * exp(DIR 2 pi i j mm / R)
+    const int tstep = (tl % Ns) * (N / (Ns * r));
+    static_for<0, nb>([&](auto MM) {
+      constexpr int mm = decltype(MM)::value;
+      const cplx<T>* p = lds.ptr(padr<R>(tl + mm * TT));
+      cplx<T> x[r];
+#pragma unroll
+      for (int j = 0; j < r; ++j) x[j] = p[S * j * ((N / r) / R * (R + 1))];
+      static_for<1, r>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        x[j] = tw.template mul<DIR>(x[j], TwSlots<N, R>::off(P) + j - 1, (j * tstep) & (N - 1));
+        x[j] = rot<DIR, (j * mm) % R, R>(x[j]);
+      });
+      dft<DIR, r>(x);
+      if constexpr (P == NP - 1) {
+        // last pass: outputs land on t + TT*(mm + q*nb) -> strided register layout
+#pragma unroll
+        for (int q = 0; q < r; ++q) o[mm + q * nb] = x[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < r; ++q) o[mm * r + q] = x[q];
+      }
+    });
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = o[j];
+  }
+};
+
+template <typename T, int N, int R, int DIR, int P, class Sync, class Lds, class Tw>
+struct TeamPass {
+  __device__ __forceinline__ static void run(cplx<T> (&v)[R], int t, Lds lds, const Tw& tw) {
+    if constexpr (P < num_passes(N, R)) {
       const int tl = opaque_v(t);
-      // write the outputs of the previous pass
-      __syncthreads();
-#pragma unroll
-      for (int mm = 0; mm < nbp; ++mm) {
-        const int b = tl + mm * TT;
-        const int k = b % Nsp;
-        cplx<T>* p = lds.ptr(padr<R>((b - k) * rp + k));
-#pragma unroll
-        for (int q = 0; q < rp; ++q) {
-          constexpr int unit = Nsp == 1 ? 1 : Nsp / R * (R + 1);
-          p[S * q * unit] = v[mm * rp + q];
-        }
-      }
-      __syncthreads();
-      cplx<T> o[R];
-#pragma unroll
-      for (int mm = 0; mm < nb; ++mm) {
-        const int b = tl + mm * TT;
-        const int k = b % Ns;
-        const cplx<T>* p = lds.ptr(padr<R>(b));
-        cplx<T> x[r];
-#pragma unroll
-        for (int j = 0; j < r; ++j) x[j] = p[S * j * ((N / r) / R * (R + 1))];
-        const int tstep = k * (N / (Ns * r));
-#pragma unroll
-        for (int j = 1; j < r; ++j) x[j] = tw_mul<DIR>(x[j], tw, (j * tstep) & (N - 1));
-        dft<DIR, r>(x);
-        if constexpr (P == NP - 1) {
-          // last pass: outputs land on t + TT*(mm + q*nb) -> strided register layout
-#pragma unroll
-          for (int q = 0; q < r; ++q) o[mm + q * nb] = x[q];
-        } else {
-#pragma unroll
-          for (int q = 0; q < r; ++q) o[mm * r + q] = x[q];
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < R; ++j) v[j] = o[j];
-      TeamPass<T, N, R, DIR, P + 1, Lds>::run(v, t, lds, tw);
+      Sync::sync();  // every lane has read the previous pass's inputs
+      PassIO<T, N, R, P>::write(v, tl, lds);
+      Sync::sync();
+      PassIO<T, N, R, P>::template read<DIR>(v, tl, lds, tw);
+      TeamPass<T, N, R, DIR, P + 1, Sync, Lds, Tw>::run(v, t, lds, tw);
     }
   }
 };
 
 // Full length-N transform by a team.  In/out: v[j] = a[t + j*TT] (natural order).
 // lds: view of the team's private LDS array (Padded<N,R>::size entries).
-// Contains __syncthreads(): every lane of the block must call it.
-template <typename T, int N, int R, int DIR, class Lds>
-__device__ __forceinline__ void team_fft(cplx<T> (&v)[R], int t, Lds lds, Twiddles<T> tw) {
+// BlockSync contains __syncthreads(): every lane of the block must call it.
+template <typename T, int N, int R, int DIR, class Lds, class Sync = BlockSync, class Tw = TwMem<T>>
+__device__ __forceinline__ void team_fft(cplx<T> (&v)[R], int t, Lds lds, const Tw& tw) {
   static_assert(N % R == 0 && R <= 16 && (R & (R - 1)) == 0 && N >= R, "radix");
   dft<DIR, R>(v);  // pass 0: butterfly b = t reads v[j] = a[t + j*TT], no twiddle
-  TeamPass<T, N, R, DIR, 1, Lds>::run(v, t, lds, tw);
+  TeamPass<T, N, R, DIR, 1, Sync, Lds, Tw>::run(v, t, lds, tw);
+}
+template <typename T, int N, int R, int DIR, class Lds>
+__device__ __forceinline__ void team_fft(cplx<T> (&v)[R], int t, Lds lds, Twiddles<T> tw) {
+  team_fft<T, N, R, DIR, Lds, BlockSync, TwMem<T>>(v, t, lds, TwMem<T>{tw});
+}
+
+// Transform whose lanes change teams at the first exchange: the input is held in a
+// load layout (team index tw_, LDS view lw of its team), the output in a compute layout
+// (team index tc, view lc) whose teams each sit in one wave.  The first exchange is
+// block-wide, every later one wave-local.  Every lane of the block must call it.
+template <typename T, int N, int R, int DIR, class Lds, class Tw>
+__device__ __forceinline__ void team_fft_remap(cplx<T> (&v)[R], int tl, Lds lw, int tc, Lds lc, const Tw& tw) {
+  static_assert(num_passes(N, R) >= 2, "the remap happens at the first exchange");
+  dft<DIR, R>(v);
+  PassIO<T, N, R, 1>::write(v, opaque_v(tl), lw);
+  __syncthreads();
+  const int t = opaque_v(tc);
+  PassIO<T, N, R, 1>::template read<DIR>(v, t, lc, tw);
+  TeamPass<T, N, R, DIR, 2, WaveSync, Lds, Tw>::run(v, tc, lc, tw);
 }
 
 }  // namespace wh

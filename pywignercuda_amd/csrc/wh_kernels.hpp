@@ -32,7 +32,13 @@ enum SweepFlags {
   kGP = 16,          // p sweep: add the Gross-Pitaevskii phase from the member's G grid
   kColPart = 32,     // x sweep: partial column sums (Gross-Pitaevskii density after K)
   kConj = 64,        // x sweep: conjugated kinetic multiplier (adjoint step)
+  kTilePart = 128,   // p sweep (with kColSums): each tile writes its 12 observable partial sums
+  kTileScale = 256,  // x sweep: the pending factor from the previous p sweep's tile partials;
+                     // the member's tile 0 writes the record row and the factor (replaces
+                     // the reduce launch between the sweeps)
 };
+
+constexpr int kTileObs = 16;  // doubles per tile partial (12 used; one 128-byte line)
 
 // One ensemble member, resident in device memory.
 struct Member {
@@ -43,6 +49,7 @@ struct Member {
   double* rho;                 // [X] Gross-Pitaevskii density
   double* G;                   // [P][X] Re(B- - B+)
   double* colpart;             // [P/rows][X]
+  double* tilepart;            // [X][kTileObs] per-tile observable partials (kTilePart)
   const double* pot;           // [pot_rows][kMaxDeg+1] coefficients of V(t,x)
   const double* pot_dv;        // [P][X] V(x - hbar theta/2) - V(x + hbar theta/2) when pot_deg < 0
   const double* theta_decay;   // [P] or null
@@ -78,7 +85,7 @@ struct Member {
 struct SweepArgs {
   const Member* members;
   const unsigned* work;  // (member << 16) | tile, one per block
-  const int* step;       // device step counter (time index)
+  int* step;             // device step counter (time index)
   int step_add;
   int flags;
   // uniform physics
@@ -88,6 +95,13 @@ struct SweepArgs {
   int kin_linear, kdeg;
   double k_c0, k_c1, inv_hbar;
   double kcoef[kMaxDeg + 1];
+  // Step counters.  step: the time index, read by p sweeps; rec_step: the index of the
+  // last p sweep, read by the x sweeps that record it.  A p sweep with kTilePart copies
+  // step to rec_step, an x sweep with kTileScale sets step = rec_step + 1: each kernel
+  // writes only the counter the other one reads.
+  int* rec_step;
+  int pt_R, pt_block;  // kTileScale: launch shape of the p sweep that wrote the partials
+  int fin_renorm, fin_record;
 };
 
 struct ReduceArgs {

@@ -2,11 +2,14 @@
 // Cody-Waite reduction by pi/2 with a three-part constant (W. J. Cody and
 // W. Waite, "Software Manual for the Elementary Functions", 1980), then the
 // Taylor series of sin and cos on |r| <= pi/4 truncated where the next term is
-// below half an ulp (FP64: through r^17 / r^18, FP32: r^9 / r^10).  The first
-// two parts of pi/2 carry 33 significant bits, so k*C1 and k*C2 are exact for
-// |k| < 2^20, i.e. |phi| < 1e6; larger arguments take the library sincos.
-// Compared with the library call this keeps the per-element code short, which
-// matters when a lane evaluates 16 phases back to back.
+// below half an ulp (FP64: through r^17 / r^18, FP32: r^9 / r^10).  Each part
+// k*C_i is formed exactly inside an FMA and C1 + C2 + C3 equals pi/2 to ~2^-120,
+// so the reduced argument's absolute error is ~1e-16 for |phi| < 1e6 and stays
+// below ulp(phi) (the precision of the phase itself) for every |phi| < 2^40.
+// Larger phases (a time step that
+// turns a phase by more than 1e12 rad is not resolved by any grid) give NaN, so the
+// per-element code stays a short branch-free sequence, which matters when a lane
+// evaluates 16 phases back to back.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -23,15 +26,14 @@ __host__ __device__ __forceinline__ double reduce_pio2(double phi, int& q) {
   double r = fma(-k, kPio2C1, phi);
   r = fma(-k, kPio2C2, r);
   r = fma(-k, kPio2C3, r);
-  q = int(static_cast<long long>(k) & 3);
+  q = int(k - 4.0 * floor(k * 0.25));  // k mod 4, exact for |k| < 2^53
   return r;
 }
 
+constexpr double kMaxPhase = 1099511627776.0;  // 2^40
+
 __host__ __device__ __forceinline__ void sincos_fp64(double phi, double* s, double* c) {
-  if (fabs(phi) >= 1.0e6) {
-    sincos(phi, s, c);
-    return;
-  }
+  if (!(fabs(phi) < kMaxPhase)) phi = __builtin_nan("");
   int q;
   const double r = reduce_pio2(phi, q);
   const double r2 = r * r;
@@ -56,23 +58,15 @@ __host__ __device__ __forceinline__ void sincos_fp64(double phi, double* s, doub
   pc = fma(pc, r2, 4.1666666666666664e-02);  // 1/4!
   pc = fma(pc, r2, -0.5);
   const double cs = fma(pc, r2, 1.0);
-  switch (q) {
-    case 0: *s = sn; *c = cs; break;
-    case 1: *s = cs; *c = -sn; break;
-    case 2: *s = -sn; *c = -cs; break;
-    default: *s = -cs; *c = sn; break;
-  }
+  // quadrant q: (sin, cos) = (sn, cs), (cs, -sn), (-sn, -cs), (-cs, sn), branch-free
+  const double a = (q & 1) ? cs : sn, b = (q & 1) ? sn : cs;
+  *s = (q & 2) ? -a : a;
+  *c = ((q + 1) & 2) ? -b : b;
 }
 
 // FP64 argument, FP64 reduction, FP32 polynomials.
 __host__ __device__ __forceinline__ void sincos_fp32(double phi, float* s, float* c) {
-  if (fabs(phi) >= 1.0e6) {
-    double sd, cd;
-    sincos(phi, &sd, &cd);
-    *s = float(sd);
-    *c = float(cd);
-    return;
-  }
+  if (!(fabs(phi) < kMaxPhase)) phi = __builtin_nan("");
   int q;
   const float r = float(reduce_pio2(phi, q));
   const float r2 = r * r;
@@ -87,12 +81,9 @@ __host__ __device__ __forceinline__ void sincos_fp32(double phi, float* s, float
   pc = fmaf(pc, r2, 4.1666668e-02f);   // 1/4!
   pc = fmaf(pc, r2, -0.5f);
   const float cs = fmaf(pc, r2, 1.0f);
-  switch (q) {
-    case 0: *s = sn; *c = cs; break;
-    case 1: *s = cs; *c = -sn; break;
-    case 2: *s = -sn; *c = -cs; break;
-    default: *s = -cs; *c = sn; break;
-  }
+  const float a = (q & 1) ? cs : sn, b = (q & 1) ? sn : cs;
+  *s = (q & 2) ? -a : a;
+  *c = ((q + 1) & 2) ? -b : b;
 }
 
 }  // namespace wh

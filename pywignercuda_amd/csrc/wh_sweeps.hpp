@@ -19,6 +19,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "wh_fft.hpp"
 #include "wh_kernels.hpp"
 #include "wh_math.hpp"
@@ -89,6 +91,148 @@ constexpr bool size_ok() {
   return N >= R * 4 && N / R <= BLOCK && BLOCK / (N / R) <= 64;
 }
 
+// ------------------------------------------------------------------ observables
+// Sum over the 64 lanes of a wave (fixed xor tree: deterministic).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Observable terms of one column from its sums s = (Re S0, Im S0, sum p W, sum p^2 W,
+// sum min(W, 0)): the integrands of <x>, <x^2>, <p>, <p^2>, <xp>, <dV/dx>, <p dV/dx>,
+// <x dV/dx>, energy, negative area (the per-step record of GPU_Wigner2D_FFT.py:1856-1876).
+__device__ __forceinline__ void column_terms(const Member& m, int col, const double (&s)[kNCol],
+                                             double (&o)[12]) {
+  const double x = m.x_host[col], dv = m.dv_host[col], v0 = m.v0_host[col];
+  o[0] = s[0];
+  o[1] = s[1];
+  o[2] = x * s[0];
+  o[3] = x * x * s[0];
+  o[4] = s[2];
+  o[5] = s[3];
+  o[6] = x * s[2];
+  o[7] = dv * s[0];
+  o[8] = dv * s[2];
+  o[9] = x * dv * s[0];
+  o[10] = s[3] * m.inv_2m + v0 * s[0];
+  o[11] = s[4];
+}
+
+// Record row, pending factor and step counter from the 12 observable sums of a member
+// (the normalised state is W / n, n = dXdP sum W; observables Re(dXdP sum W A / n)).
+__device__ __forceinline__ void write_record(const Member& m, const double (&r)[12], int renorm, int record,
+                                             int step) {
+  const double nr = m.dxdp * r[0], ni = m.dxdp * r[1];
+  double f = 1.0, sr = 1.0, si = 0.0;
+  if (renorm) {
+    const double m2 = nr * nr + ni * ni;
+    f = nr / m2;
+    sr = nr / m2;
+    si = -ni / m2;
+  }
+  m.scale[0] = sr;
+  m.scale[1] = si;
+  if (record) {
+    double* o = m.obs + size_t(step - m.obs_base) * kNObs;
+    const double d = m.dxdp * f;
+    o[kObsX] = d * r[2];
+    o[kObsX2] = d * r[3];
+    o[kObsP] = d * r[4];
+    o[kObsP2] = d * r[5];
+    o[kObsXP] = d * r[6];
+    o[kObsDV] = d * r[7];
+    o[kObsPDV] = d * r[8];
+    o[kObsXDV] = d * r[9];
+    o[kObsH] = d * r[10];
+    o[kObsNeg] = d * r[11];
+    o[kObsNorm] = d * r[0];
+    o[kObsRawNormRe] = nr;
+    o[kObsRawNormIm] = ni;
+  }
+}
+
+// kTilePart epilogue of a p-sweep tile: threads tid < C (wave 0, C <= 64) hold the
+// sums s of column tile*C + tid; the 12 observable partials of the tile go to
+// m.tilepart[tile].  Tile 0 of member 0 copies the step index to rec_step.
+template <int BLOCK, int C>
+__device__ __forceinline__ void write_tile_partial(const SweepArgs& a, const Member& m, int tile,
+                                                   const double (&s)[kNCol]) {
+  static_assert(C <= 64, "the columns of a tile are lanes of wave 0");
+  const int tid = threadIdx.x;
+  if (tid >= 64) return;
+  double o[12];
+  if (tid < C) {
+    column_terms(m, tile * C + tid, s, o);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) o[k] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 12; ++k) o[k] = wave_sum(o[k]);
+  if (tid == 0) {
+    double* tp = m.tilepart + size_t(tile) * kTileObs;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) tp[k] = o[k];
+    if (tile == 0 && &m == a.members) *a.rec_step = *a.step;
+  }
+}
+
+// kTileScale prologue of an x-sweep block: sums the tile partials of the member's last
+// p sweep (every block in the same order, so all blocks derive the same factor) and
+// returns the pending factor 1 / norm (or 1).  Tile 0 also forms all 12 sums, writes
+// the record row and the factor; tile 0 of member 0 advances the step index.
+// Contains __syncthreads(); uses the block's LDS before the transforms do.
+template <int BLOCK>
+__device__ __forceinline__ cplx<double> tile_scale(const SweepArgs& a, const Member& m, int tile, double* red) {
+  constexpr int kWaves = BLOCK / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int pc = a.pt_block / (m.P / a.pt_R);  // columns per p-sweep tile
+  const int ntiles = m.X / pc;
+  const bool rec = tile == 0;
+  const int nk = rec ? 12 : 2;
+  double r[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) r[k] = 0.0;
+  for (int i = tid; i < ntiles; i += BLOCK) {
+    const double* tp = m.tilepart + size_t(i) * kTileObs;
+#pragma unroll
+    for (int k = 0; k < 12; ++k)
+      if (k < nk) r[k] += tp[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 12; ++k)
+    if (k < nk) r[k] = wave_sum(r[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k)
+      if (k < nk) red[wave * 12 + k] = r[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    if (k < nk) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) t += red[w * 12 + k];
+      r[k] = t;
+    }
+  }
+  __syncthreads();  // red is transform scratch from here on
+  const double nr = m.dxdp * r[0], ni = m.dxdp * r[1];
+  cplx<double> f{1.0, 0.0};
+  if (a.fin_renorm) {
+    const double m2 = nr * nr + ni * ni;
+    f = {nr / m2, -ni / m2};
+  }
+  if (rec && tid == 0) {
+    const int step = *a.rec_step;
+    write_record(m, r, a.fin_renorm, a.fin_record, step);
+    if (&m == a.members) *a.step = step + 1;
+  }
+  return f;
+}
+
 // ------------------------------------------------------------------ x sweep
 // Kinetic multiplier of element m (lambda index) in row p, linear form:
 // exp(-i a m_s), a = dt*dlambda*(k_c0 + k_c1 p), m_s = t + js*TT with
@@ -104,15 +248,31 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
   cplx<S>* __restrict__ W = reinterpret_cast<cplx<S>*>(m.W) + size_t(row) * N;
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(m.twX_hi), reinterpret_cast<const cplx<T>*>(m.twX_lo)};
   LdsRow<T> view{lds + team * LSZ};
+  // a row's team inside one wave: wave-local exchanges, twiddles loaded once into registers
+  constexpr bool kWave = TT <= 64;
+  using Sync = std::conditional_t<kWave, WaveSync, BlockSync>;
 
   cplx<T> v[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) v[j] = to_arith<T>(W[t + j * TT]);
-  if (a.flags & kScaleOnLoad) {
+  const auto twsrc = [&]() {
+    if constexpr (kWave) {
+      TwRegs<T, N, R> r;
+      load_twiddles<T, N, R>(r, t, tw);
+      return r;
+    } else {
+      return TwMem<T>{tw};
+    }
+  }();
+  if (a.flags & kTileScale) {
+    const cplx<double> f = tile_scale<BLOCK>(a, m, tile, reinterpret_cast<double*>(lds));
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = apply(v[j], f);
+  } else if (a.flags & kScaleOnLoad) {
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = scale_load(v[j], m.scale);
   }
-  team_fft<T, N, R, -1>(v, t, view, tw);
+  team_fft<T, N, R, -1, LdsRow<T>, Sync>(v, t, view, twsrc);
 
   const double p = m.dp * double(fidx(row, m.P));
   const double sgn = (a.flags & kConj) ? -1.0 : 1.0;  // adjoint: conj(r e^{-i phi}) = r e^{+i phi}
@@ -125,16 +285,15 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
     Zp[1] = Z1;
 #pragma unroll
     for (int k = 2; k <= R / 2; ++k) Zp[k] = cmul(Zp[k / 2], Zp[k - k / 2]);
+    cplx<double> mul[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
-      cplx<double> z = j < R / 2 ? Zp[j] : cplx<double>{Zp[R - j].x, -Zp[R - j].y};
-      cplx<double> mj = cmul(Et, z);
-      if (m.lambda_decay) {
-        const double r = m.lambda_decay[t + j * TT];
-        mj = {mj.x * r, mj.y * r};
-      }
-      v[j] = apply(v[j], mj);
+    for (int j = 0; j < R; ++j) mul[j] = cmul(Et, j < R / 2 ? Zp[j] : cplx<double>{Zp[R - j].x, -Zp[R - j].y});
+    if (m.lambda_decay) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) mul[j] = cscale(mul[j], m.lambda_decay[t + j * TT]);
     }
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = apply(v[j], mul[j]);
   } else {
     const double hh = a.hbar_half;
 #pragma unroll
@@ -146,7 +305,7 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
       v[j] = apply(v[j], expmi(r, phi));
     }
   }
-  team_fft<T, N, R, +1>(v, t, view, tw);
+  team_fft<T, N, R, +1, LdsRow<T>, Sync>(v, t, view, twsrc);
   const T inv = T(1.0 / N);
 #pragma unroll
   for (int j = 0; j < R; ++j) {
@@ -175,7 +334,11 @@ struct PSweep {
   static constexpr int TT = N / R;
   static constexpr int C = BLOCK / TT;  // columns per tile
   static constexpr int LSZ = Padded<N, R>::size;
-  using View = LdsStrided<T, C>;
+  // wave teams: a column's team lies inside one wave and the transform has an exchange
+  // at which the lanes can leave the load layout (then every later exchange is wave-local)
+  static constexpr bool kWave = TT <= 64 && num_passes(N, R) >= 2;
+  using View = std::conditional_t<kWave, LdsRow<T>, LdsStrided<T, C>>;
+  using Sync = std::conditional_t<kWave, WaveSync, BlockSync>;
 
   // potential step multiplier of time index `step` (expPotential; GP phase optional)
   __device__ __forceinline__ static void potential(cplx<T> (&v)[R], const SweepArgs& a,
@@ -188,60 +351,114 @@ struct PSweep {
     const double* __restrict__ cf = m.pot + size_t(row) * (kMaxDeg + 1);
     const double mask = m.xmask ? m.xmask[col] : 1.0;
     const bool gp = (a.flags & kGP) != 0;
+    static_assert(kMaxDeg == 8, "odd Taylor terms u, u^3, u^5, u^7");
+    // V(x - u) - V(x + u) = -2 sum_{k odd} D_k(x) u^k with the Taylor coefficients
+    // D_k(x) = V^(k)(x) / k! of the column, from a Taylor shift of the coefficients
+    double d[kMaxDeg + 1];
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const int i = t + j * TT;
-      const double th = m.dtheta * double(fidx(i, N));
-      const double u = a.hbar_half * th;
-      double phi = m.pot_deg >= 0 ? a.vphase * (poly(cf, m.pot_deg, x - u) - poly(cf, m.pot_deg, x + u))
-                                  : a.vphase * m.pot_dv[size_t(i) * m.X + col];
-      if (gp) phi -= a.gp_phase * m.G[size_t(i) * m.X + col];
-      const double r = (m.theta_decay ? m.theta_decay[i] : 1.0) * mask;
-      v[j] = apply(v[j], expmi(r, phi));
+    for (int n = 0; n <= kMaxDeg; ++n) d[n] = n <= m.pot_deg ? cf[n] : 0.0;
+#pragma unroll
+    for (int i = 0; i < kMaxDeg; ++i) {
+#pragma unroll
+      for (int k = kMaxDeg - 1; k >= i; --k) d[k] = fma(x, d[k + 1], d[k]);
     }
+    const double f = -2.0 * a.vphase;
+    const double e1 = f * d[1], e3 = f * d[3], e5 = f * d[5], e7 = f * d[7];
+    // uniform conditions outside the element loops: the multiply loop is one basic block
+    double phi[R], rr[R];
+    if (m.pot_deg >= 0) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const double u = a.hbar_half * m.dtheta * double(fidx(t + j * TT, N));
+        const double u2 = u * u;
+        phi[j] = u * fma(u2, fma(u2, fma(u2, e7, e5), e3), e1);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < R; ++j) phi[j] = a.vphase * m.pot_dv[size_t(t + j * TT) * m.X + col];
+    }
+    if (gp) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) phi[j] -= a.gp_phase * m.G[size_t(t + j * TT) * m.X + col];
+    }
+    if (m.theta_decay) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) rr[j] = m.theta_decay[t + j * TT] * mask;
+    } else {
+#pragma unroll
+      for (int j = 0; j < R; ++j) rr[j] = mask;
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = apply(v[j], expmi(rr[j], phi[j]));
   }
 
   // D(Y) = F_p^{-1}[ theta * F_p[ sigma(p) Y ] ]   (Theta_fp_Damping, FFT.py:1135-1143)
+  template <class Tw>
   __device__ __forceinline__ static void cl_operator(cplx<T> (&y)[R], const Member& m, int t,
-                                                     View view, const Twiddles<T>& tw) {
+                                                     View view, const Tw& tw) {
 #pragma unroll
     for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(m.sigma_p[t + j * TT]));
-    team_fft<T, N, R, -1>(y, t, view, tw);
+    team_fft<T, N, R, -1, View, Sync>(y, t, view, tw);
 #pragma unroll
     for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(m.theta_host[t + j * TT]));
-    team_fft<T, N, R, +1>(y, t, view, tw);
+    team_fft<T, N, R, +1, View, Sync>(y, t, view, tw);
     const T inv = T(1.0 / N);
 #pragma unroll
     for (int j = 0; j < R; ++j) y[j] = cscale(y[j], inv);
   }
 };
 
+// LDS bytes past the transform arrays of a p-sweep block: per-column sums (<= 64 columns)
+constexpr int kPTailBytes = 64 * kNCol * 8;
+
+// Lanes: the load layout (column cl = tid % C, team index tl = tid / C) makes every
+// global load and store instruction cover rows of C consecutive columns.  With wave
+// teams the first exchange of the first transform moves each column to one team of
+// TT consecutive lanes (column cc = tid / TT, team index tc = tid % TT) and the store
+// moves it back through LDS; otherwise the load layout is kept throughout.
 template <typename S, typename T, int N, int R, int BLOCK, bool CL>
 __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m, int tile,
                                             cplx<T>* lds) {
   using PS = PSweep<T, N, R, BLOCK>;
-  constexpr int TT = PS::TT, C = PS::C;
+  using View = typename PS::View;
+  using Sync = typename PS::Sync;
+  constexpr int TT = PS::TT, C = PS::C, LSZ = PS::LSZ;
+  constexpr bool kWave = PS::kWave;
   const int tid = threadIdx.x;
-  const int c = tid % C, t = tid / C;
+  const int cl = tid % C, tl = tid / C;
+  const int cc = kWave ? tid / TT : cl, tc = kWave ? tid % TT : tl;
   const int X = m.X;
-  const int col = tile * C + c;
+  const int col = tile * C + cc;
   cplx<S>* __restrict__ W = reinterpret_cast<cplx<S>*>(m.W);
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(m.twP_hi), reinterpret_cast<const cplx<T>*>(m.twP_lo)};
-  typename PS::View view{lds + c};
+  const View view_c{kWave ? lds + cc * LSZ : lds + cl};
   const int step = *a.step + a.step_add;
 
-  const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<S>));
+  const unsigned boff = (unsigned(tl) * unsigned(X) + unsigned(tile * C + cl)) * unsigned(sizeof(cplx<S>));
   cplx<T> v[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) v[j] = to_arith<T>(at_off(W + size_t(j * TT) * X, boff));
+  const auto twsrc = [&]() {
+    if constexpr (kWave) {
+      TwRegs<T, N, R> r;
+      load_twiddles<T, N, R>(r, tc, tw);
+      return r;
+    } else {
+      return TwMem<T>{tw};
+    }
+  }();
   if (a.flags & kScaleOnLoad) {
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = scale_load(v[j], m.scale);
   }
   const T inv = T(1.0 / N);
-  team_fft<T, N, R, -1>(v, t, view, tw);
-  PS::potential(v, a, m, t, col, step);
-  team_fft<T, N, R, +1>(v, t, view, tw);
+  if constexpr (kWave) {
+    team_fft_remap<T, N, R, -1>(v, tl, LdsRow<T>{lds + cl * LSZ}, tc, view_c, twsrc);
+  } else {
+    team_fft<T, N, R, -1, View, Sync>(v, tc, view_c, twsrc);
+  }
+  PS::potential(v, a, m, tc, col, step);
+  team_fft<T, N, R, +1, View, Sync>(v, tc, view_c, twsrc);
 #pragma unroll
   for (int j = 0; j < R; ++j) v[j] = cscale(v[j], inv);
 
@@ -251,27 +468,28 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
     cplx<T> lw[R], d[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) d[j] = v[j];
-    PS::cl_operator(d, m, t, view, tw);
+    PS::cl_operator(d, m, tc, view_c, twsrc);
     const T c1 = T((2.0 / 3.0) * a.cl_a), c2 = T(a.cl_a), c3 = T(2.0 * a.cl_a);
 #pragma unroll
     for (int j = 0; j < R; ++j) lw[j] = cadd(v[j], mul_i(d[j], c1));
 #pragma unroll
     for (int j = 0; j < R; ++j) d[j] = lw[j];
-    PS::cl_operator(d, m, t, view, tw);
+    PS::cl_operator(d, m, tc, view_c, twsrc);
 #pragma unroll
     for (int j = 0; j < R; ++j) lw[j] = cadd(lw[j], mul_i(d[j], c2));
 #pragma unroll
     for (int j = 0; j < R; ++j) d[j] = lw[j];
-    PS::cl_operator(d, m, t, view, tw);
+    PS::cl_operator(d, m, tc, view_c, twsrc);
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = cadd(v[j], mul_i(d[j], c3));
   }
 
+  double s[kNCol] = {0.0, 0.0, 0.0, 0.0, 0.0};  // sums of column tile*C + tid (tid < C)
   if (a.flags & kColSums) {
     double acc[kNCol] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      const double p = m.p_host[t + j * TT];
+      const double p = m.p_host[tc + j * TT];
       const double w = double(v[j].x);
       acc[0] += w;
       acc[1] += double(v[j].y);
@@ -279,44 +497,76 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
       acc[3] += p * p * w;
       acc[4] += w < 0.0 ? w : 0.0;
     }
-    // lanes of one column inside a wave: c + C*t; fold t with xor shuffles
+    if constexpr (kWave) {
+      // a column's lanes are TT consecutive lanes of one wave
 #pragma unroll
-    for (int off = C; off < 64; off <<= 1) {
+      for (int off = 1; off < TT; off <<= 1) {
 #pragma unroll
-      for (int k = 0; k < kNCol; ++k) acc[k] += __shfl_xor(acc[k], off, 64);
-    }
-    constexpr int kWaves = (BLOCK + 63) / 64;
-    constexpr int kLanes = C < 64 ? C : 64;  // distinct columns per wave
-    __syncthreads();
-    double* red = reinterpret_cast<double*>(lds);
-    const int wave = tid >> 6, lane = tid & 63;
-    if (lane < kLanes) {
-#pragma unroll
-      for (int k = 0; k < kNCol; ++k) red[(wave * kLanes + lane) * kNCol + k] = acc[k];
-    }
-    __syncthreads();
-    if (tid < C) {
-      // column tid lives in waves w with (w*64) % C == tid - lane... (C <= 64: every wave holds all C)
-      double s[kNCol] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int w = 0; w < kWaves; ++w) {
-#pragma unroll
-        for (int k = 0; k < kNCol; ++k) s[k] += red[(w * kLanes + tid) * kNCol + k];
+        for (int k = 0; k < kNCol; ++k) acc[k] += __shfl_xor(acc[k], off, 64);
       }
+      double* csum = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + lds_elems<BLOCK, R>() * sizeof(cplx<T>));
+      if (tc == 0) {
 #pragma unroll
-      for (int k = 0; k < kNCol; ++k) m.colsums[size_t(tile * C + tid) * kNCol + k] = s[k];
+        for (int k = 0; k < kNCol; ++k) csum[cc * kNCol + k] = acc[k];
+      }
+      __syncthreads();
+      if (tid < C) {
+#pragma unroll
+        for (int k = 0; k < kNCol; ++k) {
+          s[k] = csum[tid * kNCol + k];
+          m.colsums[size_t(tile * C + tid) * kNCol + k] = s[k];
+        }
+      }
+    } else {
+      // lanes of one column inside a wave: c + C*t; fold t with xor shuffles
+#pragma unroll
+      for (int off = C; off < 64; off <<= 1) {
+#pragma unroll
+        for (int k = 0; k < kNCol; ++k) acc[k] += __shfl_xor(acc[k], off, 64);
+      }
+      constexpr int kWaves = (BLOCK + 63) / 64;
+      constexpr int kLanes = C < 64 ? C : 64;  // distinct columns per wave
+      __syncthreads();
+      double* red = reinterpret_cast<double*>(lds);
+      const int wave = tid >> 6, lane = tid & 63;
+      if (lane < kLanes) {
+#pragma unroll
+        for (int k = 0; k < kNCol; ++k) red[(wave * kLanes + lane) * kNCol + k] = acc[k];
+      }
+      __syncthreads();
+      if (tid < C) {
+        // C <= 64: every wave holds all C columns
+        for (int w = 0; w < kWaves; ++w) {
+#pragma unroll
+          for (int k = 0; k < kNCol; ++k) s[k] += red[(w * kLanes + tid) * kNCol + k];
+        }
+#pragma unroll
+        for (int k = 0; k < kNCol; ++k) m.colsums[size_t(tile * C + tid) * kNCol + k] = s[k];
+      }
     }
   }
 
   if (a.flags & kMerge) {
-    team_fft<T, N, R, -1>(v, t, view, tw);
-    PS::potential(v, a, m, t, col, step + 1);
-    team_fft<T, N, R, +1>(v, t, view, tw);
+    team_fft<T, N, R, -1, View, Sync>(v, tc, view_c, twsrc);
+    PS::potential(v, a, m, tc, col, step + 1);
+    team_fft<T, N, R, +1, View, Sync>(v, tc, view_c, twsrc);
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = cscale(v[j], inv);
   }
 
+  if constexpr (kWave) {
+    // back to the load layout: natural order through the team's LDS array
+    Sync::sync();
+#pragma unroll
+    for (int j = 0; j < R; ++j) *view_c.ptr(padr<R>(tc + j * TT)) = v[j];
+    __syncthreads();
+    const LdsRow<T> view_l{lds + cl * LSZ};
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = *view_l.ptr(padr<R>(tl + j * TT));
+  }
 #pragma unroll
   for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = to_arith<S>(v[j]);
+  if (a.flags & kTilePart) write_tile_partial<BLOCK, C>(a, m, tile, s);
 }
 
 // ------------------------------------------------------------------ Gross-Pitaevskii shift
@@ -369,7 +619,7 @@ __global__ void __launch_bounds__(BLOCK) xsweep_kernel(SweepArgs a) {
 
 template <typename S, typename T, int R, int BLOCK, bool CL>
 __global__ void __launch_bounds__(BLOCK) psweep_kernel(SweepArgs a) {
-  __shared__ cplx<T> lds[lds_elems<BLOCK, R>()];
+  __shared__ cplx<T> lds[lds_elems<BLOCK, R>() + kPTailBytes / sizeof(cplx<T>)];
   const unsigned w = a.work[blockIdx.x];
   const Member& m = a.members[w >> 16];
   const int tile = int(w & 0xffffu);

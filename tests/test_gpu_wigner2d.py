@@ -130,9 +130,11 @@ def test_strang_gross_pitaevskii_fp64(cuda):
 # state per sweep: ~6e-7 after 300 steps here); 'fp32fast' also does the transforms in FP32,
 # whose rounding error a unitary step accumulates (~3e-5 after 300 steps -- the same as
 # numpy's single-precision FFT gives for this workload, i.e. the FP32 arithmetic floor).
-@pytest.mark.parametrize("precision,bound", [("fp32", 1e-5), ("fp32fast", 1e-4)])
+# Observables: the moments weight the state error by up to x^2, p^2 ~ 100 at the grid
+# edges, so 'fp32fast' records <x^2> etc. to ~6e-4 relative (1e-3 bound); 'fp32' to 1e-5.
+@pytest.mark.parametrize("precision,bound,obs_tol", [("fp32", 1e-5, 1e-5), ("fp32fast", 1e-4, 1e-3)])
 @pytest.mark.parametrize("scheme", ["lie", "strang"])
-def test_fp32_bulk_error(cuda, scheme, precision, bound):
+def test_fp32_bulk_error(cuda, scheme, precision, bound, obs_tol):
     hip, orc = make_pair(cuda, scheme=scheme, precision=precision, mask=False)
     W0 = ow.gaussian_wigner(orc.g, x0=2.0, p0=0.0, s=1.0)
     W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 300)
@@ -140,7 +142,7 @@ def test_fp32_bulk_error(cuda, scheme, precision, bound):
     print(f"{precision} {scheme}: rel L2 after 300 steps {err:.3e}")
     assert err < bound, err
     for k in ["X", "P", "X2", "P2", "H", "norm"]:
-        np.testing.assert_allclose(obs[:, OBS_COL[k]], obs_ref[k], rtol=bound, atol=bound, err_msg=k)
+        np.testing.assert_allclose(obs[:, OBS_COL[k]], obs_ref[k], rtol=obs_tol, atol=obs_tol, err_msg=k)
 
 
 def test_graph_replay_matches_eager(cuda):
