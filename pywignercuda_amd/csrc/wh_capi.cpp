@@ -476,7 +476,7 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
     WH_TRY(b->dv_host.upload(md.dv_host ? md.dv_host : zeros.data(), X * sizeof(double)));
     WH_TRY(b->v0_host.upload(md.v0_host ? md.v0_host : zeros.data(), X * sizeof(double)));
     WH_TRY(b->colsums.alloc(X * wh::kNCol * sizeof(double)));
-    WH_TRY(b->tilepart.alloc(X * wh::kTileObs * sizeof(double)));  // at most one tile per column
+    WH_TRY(b->tilepart.alloc(X * wh::kTileObs * sizeof(double)));  // at most X tiles (one column each)
     const double one[2] = {1.0, 0.0};
     WH_TRY(b->scale.upload(one, sizeof(one)));
     if (gp) {

@@ -34,6 +34,36 @@ struct alignas(2 * sizeof(T)) cplx {
   T x, y;
 };
 
+// Address-space qualified pointers.  Pointers that reach device code through memory
+// (tables, member records) are generic, for which the compiler emits flat_ instructions;
+// those also count against the LDS counter, so every LDS wait of an exchange would
+// wait for outstanding memory loads as well.  gptr: global memory (global_ loads and
+// stores).  cptr: read-only for the whole kernel (uniform addresses load through the
+// scalar cache; never written through).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> gmem(T* p) {
+  return (gptr<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ cptr<T> cmem(const T* p) {
+  return (cptr<T>)p;
+}
+// complex element through a global pointer (one 2-vector load)
+template <typename T>
+__device__ __forceinline__ cplx<T> gload(gptr<const cplx<T>> p) {
+  typedef T vec2 __attribute__((ext_vector_type(2)));
+  const vec2 r = *reinterpret_cast<gptr<const vec2>>(p);
+  return {r.x, r.y};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> gload(gptr<cplx<T>> p) {
+  return gload<T>(gptr<const cplx<T>>(p));
+}
+
 template <typename T>
 __device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>
@@ -203,11 +233,11 @@ __device__ __forceinline__ cplx<float> tw_apply(cplx<float> x, cplx<float> w, cp
 
 template <int DIR>
 __device__ __forceinline__ cplx<double> tw_mul(cplx<double> x, const Twiddles<double>& tw, int idx) {
-  return tw_apply<DIR>(x, tw.hi[idx]);
+  return tw_apply<DIR>(x, gload(gmem(tw.hi) + idx));
 }
 template <int DIR>
 __device__ __forceinline__ cplx<float> tw_mul(cplx<float> x, const Twiddles<float>& tw, int idx) {
-  return tw_apply<DIR>(x, tw.hi[idx], tw.lo[idx]);
+  return tw_apply<DIR>(x, gload(gmem(tw.hi) + idx), gload(gmem(tw.lo) + idx));
 }
 
 // Twiddles of every pass of one lane, held in registers for the whole kernel (a lane's
@@ -237,7 +267,7 @@ struct TwRegs<double, N, R> {
   __device__ __forceinline__ cplx<double> mul(cplx<double> x, int slot, int) const {
     return tw_apply<DIR>(x, hi[slot]);
   }
-  __device__ __forceinline__ void set(int slot, const Twiddles<double>& tw, int idx) { hi[slot] = tw.hi[idx]; }
+  __device__ __forceinline__ void set(int slot, const Twiddles<double>& tw, int idx) { hi[slot] = gload(gmem(tw.hi) + idx); }
 };
 template <int N, int R>
 struct TwRegs<float, N, R> {
@@ -247,8 +277,8 @@ struct TwRegs<float, N, R> {
     return tw_apply<DIR>(x, hi[slot], lo[slot]);
   }
   __device__ __forceinline__ void set(int slot, const Twiddles<float>& tw, int idx) {
-    hi[slot] = tw.hi[idx];
-    lo[slot] = tw.lo[idx];
+    hi[slot] = gload(gmem(tw.hi) + idx);
+    lo[slot] = gload(gmem(tw.lo) + idx);
   }
 };
 

@@ -38,7 +38,7 @@ enum SweepFlags {
                      // the reduce launch between the sweeps)
 };
 
-constexpr int kTileObs = 16;  // doubles per tile partial (12 used; one 128-byte line)
+constexpr int kTileObs = 12;  // observable partials per tile
 
 // One ensemble member, resident in device memory.
 struct Member {
@@ -49,7 +49,7 @@ struct Member {
   double* rho;                 // [X] Gross-Pitaevskii density
   double* G;                   // [P][X] Re(B- - B+)
   double* colpart;             // [P/rows][X]
-  double* tilepart;            // [X][kTileObs] per-tile observable partials (kTilePart)
+  double* tilepart;            // [12][X] per-tile observable partials (kTilePart): k * X + tile
   const double* pot;           // [pot_rows][kMaxDeg+1] coefficients of V(t,x)
   const double* pot_dv;        // [P][X] V(x - hbar theta/2) - V(x + hbar theta/2) when pot_deg < 0
   const double* theta_decay;   // [P] or null

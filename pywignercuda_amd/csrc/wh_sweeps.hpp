@@ -52,7 +52,7 @@ __device__ __forceinline__ cplx<double> expmi(double r, double phi) {
 
 template <typename T>
 __device__ __forceinline__ cplx<T> scale_load(cplx<T> v, const double* s) {
-  return apply(v, cplx<double>{s[0], s[1]});
+  return apply(v, cplx<double>{cmem(s)[0], cmem(s)[1]});
 }
 
 // element of a uniform row base at a per-lane 32-bit byte offset (SGPR base +
@@ -61,9 +61,22 @@ template <typename T>
 __device__ __forceinline__ cplx<T>& at_off(cplx<T>* base, unsigned boff) {
   return *reinterpret_cast<cplx<T>*>(reinterpret_cast<char*>(base) + boff);
 }
+template <typename T>
+__device__ __forceinline__ gptr<cplx<T>> at_off(gptr<cplx<T>> base, unsigned boff) {
+  return reinterpret_cast<gptr<cplx<T>>>(reinterpret_cast<gptr<char>>(base) + boff);
+}
 
 template <typename T>
 __device__ __forceinline__ cplx<T> mul_i(cplx<T> a, T s) { return {-s * a.y, s * a.x}; }
+
+// Streaming store of a state element: the sweep writes each element once and the next
+// sweep reads it from another CU, so the line is not kept dirty in this CU's L2 (less
+// write-back at the end of the kernel).
+template <typename S>
+__device__ __forceinline__ void store_stream(gptr<cplx<S>> p, cplx<S> v) {
+  typedef S vec2 __attribute__((ext_vector_type(2)));
+  __builtin_nontemporal_store(vec2{v.x, v.y}, reinterpret_cast<gptr<vec2>>(p));
+}
 
 // state storage S <-> arithmetic T (complex64 states with FP64 arithmetic convert at the
 // sweep's load and store; every other combination is the identity)
@@ -104,7 +117,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // <x dV/dx>, energy, negative area (the per-step record of GPU_Wigner2D_FFT.py:1856-1876).
 __device__ __forceinline__ void column_terms(const Member& m, int col, const double (&s)[kNCol],
                                              double (&o)[12]) {
-  const double x = m.x_host[col], dv = m.dv_host[col], v0 = m.v0_host[col];
+  const double x = gmem(m.x_host)[col], dv = gmem(m.dv_host)[col], v0 = gmem(m.v0_host)[col];
   o[0] = s[0];
   o[1] = s[1];
   o[2] = x * s[0];
@@ -131,10 +144,10 @@ __device__ __forceinline__ void write_record(const Member& m, const double (&r)[
     sr = nr / m2;
     si = -ni / m2;
   }
-  m.scale[0] = sr;
-  m.scale[1] = si;
+  gmem(m.scale)[0] = sr;
+  gmem(m.scale)[1] = si;
   if (record) {
-    double* o = m.obs + size_t(step - m.obs_base) * kNObs;
+    const gptr<double> o = gmem(m.obs) + size_t(step - m.obs_base) * kNObs;
     const double d = m.dxdp * f;
     o[kObsX] = d * r[2];
     o[kObsX2] = d * r[3];
@@ -171,10 +184,9 @@ __device__ __forceinline__ void write_tile_partial(const SweepArgs& a, const Mem
 #pragma unroll
   for (int k = 0; k < 12; ++k) o[k] = wave_sum(o[k]);
   if (tid == 0) {
-    double* tp = m.tilepart + size_t(tile) * kTileObs;
 #pragma unroll
-    for (int k = 0; k < 12; ++k) tp[k] = o[k];
-    if (tile == 0 && &m == a.members) *a.rec_step = *a.step;
+    for (int k = 0; k < 12; ++k) gmem(m.tilepart)[size_t(k) * m.X + tile] = o[k];
+    if (tile == 0 && &m == a.members) *gmem(a.rec_step) = *gmem(a.step);
   }
 }
 
@@ -182,7 +194,8 @@ __device__ __forceinline__ void write_tile_partial(const SweepArgs& a, const Mem
 // p sweep (every block in the same order, so all blocks derive the same factor) and
 // returns the pending factor 1 / norm (or 1).  Tile 0 also forms all 12 sums, writes
 // the record row and the factor; tile 0 of member 0 advances the step index.
-// Contains __syncthreads(); uses the block's LDS before the transforms do.
+// Contains __syncthreads(); red: kTileRedBytes of LDS outside the transform arrays.
+constexpr int kTileRedBytes = 16 * 12 * 8;
 template <int BLOCK>
 __device__ __forceinline__ cplx<double> tile_scale(const SweepArgs& a, const Member& m, int tile, double* red) {
   constexpr int kWaves = BLOCK / 64;
@@ -195,10 +208,9 @@ __device__ __forceinline__ cplx<double> tile_scale(const SweepArgs& a, const Mem
 #pragma unroll
   for (int k = 0; k < 12; ++k) r[k] = 0.0;
   for (int i = tid; i < ntiles; i += BLOCK) {
-    const double* tp = m.tilepart + size_t(i) * kTileObs;
 #pragma unroll
     for (int k = 0; k < 12; ++k)
-      if (k < nk) r[k] += tp[k];
+      if (k < nk) r[k] += gmem(m.tilepart)[size_t(k) * m.X + i];
   }
 #pragma unroll
   for (int k = 0; k < 12; ++k)
@@ -218,7 +230,6 @@ __device__ __forceinline__ cplx<double> tile_scale(const SweepArgs& a, const Mem
       r[k] = t;
     }
   }
-  __syncthreads();  // red is transform scratch from here on
   const double nr = m.dxdp * r[0], ni = m.dxdp * r[1];
   cplx<double> f{1.0, 0.0};
   if (a.fin_renorm) {
@@ -226,9 +237,9 @@ __device__ __forceinline__ cplx<double> tile_scale(const SweepArgs& a, const Mem
     f = {nr / m2, -ni / m2};
   }
   if (rec && tid == 0) {
-    const int step = *a.rec_step;
+    const int step = *gmem(a.rec_step);
     write_record(m, r, a.fin_renorm, a.fin_record, step);
-    if (&m == a.members) *a.step = step + 1;
+    if (&m == a.members) *gmem(a.step) = step + 1;
   }
   return f;
 }
@@ -245,7 +256,7 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
   constexpr int LSZ = Padded<N, R>::size;
   const int tid = threadIdx.x, team = tid / TT, t = tid % TT;
   const int row = tile * ROWS + team;
-  cplx<S>* __restrict__ W = reinterpret_cast<cplx<S>*>(m.W) + size_t(row) * N;
+  const gptr<cplx<S>> W = gmem(reinterpret_cast<cplx<S>*>(m.W)) + size_t(row) * N;
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(m.twX_hi), reinterpret_cast<const cplx<T>*>(m.twX_lo)};
   LdsRow<T> view{lds + team * LSZ};
   // a row's team inside one wave: wave-local exchanges, twiddles loaded once into registers
@@ -254,7 +265,7 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
 
   cplx<T> v[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = to_arith<T>(W[t + j * TT]);
+  for (int j = 0; j < R; ++j) v[j] = to_arith<T>(gload(W + t + j * TT));
   const auto twsrc = [&]() {
     if constexpr (kWave) {
       TwRegs<T, N, R> r;
@@ -265,7 +276,7 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
     }
   }();
   if (a.flags & kTileScale) {
-    const cplx<double> f = tile_scale<BLOCK>(a, m, tile, reinterpret_cast<double*>(lds));
+    const cplx<double> f = tile_scale<BLOCK>(a, m, tile, reinterpret_cast<double*>(lds + lds_elems<BLOCK, R>()));
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = apply(v[j], f);
   } else if (a.flags & kScaleOnLoad) {
@@ -285,15 +296,12 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
     Zp[1] = Z1;
 #pragma unroll
     for (int k = 2; k <= R / 2; ++k) Zp[k] = cmul(Zp[k / 2], Zp[k - k / 2]);
-    cplx<double> mul[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) mul[j] = cmul(Et, j < R / 2 ? Zp[j] : cplx<double>{Zp[R - j].x, -Zp[R - j].y});
+    for (int j = 0; j < R; ++j) v[j] = apply(v[j], cmul(Et, j < R / 2 ? Zp[j] : cplx<double>{Zp[R - j].x, -Zp[R - j].y}));
     if (m.lambda_decay) {
 #pragma unroll
-      for (int j = 0; j < R; ++j) mul[j] = cscale(mul[j], m.lambda_decay[t + j * TT]);
+      for (int j = 0; j < R; ++j) v[j] = apply(v[j], cplx<double>{gmem(m.lambda_decay)[t + j * TT], 0.0});
     }
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = apply(v[j], mul[j]);
   } else {
     const double hh = a.hbar_half;
 #pragma unroll
@@ -301,7 +309,7 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
       const int mm = t + j * TT;
       const double lam = m.dlambda * double(fidx(mm, N));
       const double phi = sgn * a.dt * (poly(a.kcoef, a.kdeg, p + hh * lam) - poly(a.kcoef, a.kdeg, p - hh * lam)) * a.inv_hbar;
-      const double r = m.lambda_decay ? m.lambda_decay[mm] : 1.0;
+      const double r = m.lambda_decay ? gmem(m.lambda_decay)[mm] : 1.0;
       v[j] = apply(v[j], expmi(r, phi));
     }
   }
@@ -310,7 +318,7 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     v[j] = cscale(v[j], inv);
-    W[t + j * TT] = to_arith<S>(v[j]);
+    store_stream(W + t + j * TT, to_arith<S>(v[j]));
   }
 
   if (a.flags & kColPart) {
@@ -323,7 +331,7 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
     for (int x = tid; x < N; x += BLOCK) {
       double s = 0.0;
       for (int q = 0; q < ROWS; ++q) s += red[q * N + x];
-      m.colpart[size_t(tile) * N + x] = s;
+      gmem(m.colpart)[size_t(tile) * N + x] = s;
     }
   }
 }
@@ -348,8 +356,8 @@ struct PSweep {
     const int t = opaque_v(t_in), col = opaque_v(col_in);
     const double x = m.dx * double(fidx(col, m.X));
     const int row = step < m.pot_rows ? step : m.pot_rows - 1;
-    const double* __restrict__ cf = m.pot + size_t(row) * (kMaxDeg + 1);
-    const double mask = m.xmask ? m.xmask[col] : 1.0;
+    const cptr<double> cf = cmem(m.pot) + size_t(row) * (kMaxDeg + 1);
+    const double mask = m.xmask ? gmem(m.xmask)[col] : 1.0;
     const bool gp = (a.flags & kGP) != 0;
     static_assert(kMaxDeg == 8, "odd Taylor terms u, u^3, u^5, u^7");
     // V(x - u) - V(x + u) = -2 sum_{k odd} D_k(x) u^k with the Taylor coefficients
@@ -364,32 +372,31 @@ struct PSweep {
     }
     const double f = -2.0 * a.vphase;
     const double e1 = f * d[1], e3 = f * d[3], e5 = f * d[5], e7 = f * d[7];
-    // uniform conditions outside the element loops: the multiply loop is one basic block
-    double phi[R], rr[R];
-    if (m.pot_deg >= 0) {
+    // uniform conditions outside the element loops, so that each loop is one basic block
+    // and no per-element array stays live across it
+    if (m.pot_deg >= 0 && !gp) {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         const double u = a.hbar_half * m.dtheta * double(fidx(t + j * TT, N));
         const double u2 = u * u;
-        phi[j] = u * fma(u2, fma(u2, fma(u2, e7, e5), e3), e1);
+        v[j] = apply(v[j], expmi(mask, u * fma(u2, fma(u2, fma(u2, e7, e5), e3), e1)));
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < R; ++j) phi[j] = a.vphase * m.pot_dv[size_t(t + j * TT) * m.X + col];
-    }
-    if (gp) {
-#pragma unroll
-      for (int j = 0; j < R; ++j) phi[j] -= a.gp_phase * m.G[size_t(t + j * TT) * m.X + col];
+      for (int j = 0; j < R; ++j) {
+        const int i = t + j * TT;
+        const double u = a.hbar_half * m.dtheta * double(fidx(i, N));
+        const double u2 = u * u;
+        double phi = m.pot_deg >= 0 ? u * fma(u2, fma(u2, fma(u2, e7, e5), e3), e1)
+                                    : a.vphase * gmem(m.pot_dv)[size_t(i) * m.X + col];
+        if (gp) phi -= a.gp_phase * gmem(m.G)[size_t(i) * m.X + col];
+        v[j] = apply(v[j], expmi(mask, phi));
+      }
     }
     if (m.theta_decay) {
 #pragma unroll
-      for (int j = 0; j < R; ++j) rr[j] = m.theta_decay[t + j * TT] * mask;
-    } else {
-#pragma unroll
-      for (int j = 0; j < R; ++j) rr[j] = mask;
+      for (int j = 0; j < R; ++j) v[j] = apply(v[j], cplx<double>{gmem(m.theta_decay)[t + j * TT], 0.0});
     }
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = apply(v[j], expmi(rr[j], phi[j]));
   }
 
   // D(Y) = F_p^{-1}[ theta * F_p[ sigma(p) Y ] ]   (Theta_fp_Damping, FFT.py:1135-1143)
@@ -397,10 +404,10 @@ struct PSweep {
   __device__ __forceinline__ static void cl_operator(cplx<T> (&y)[R], const Member& m, int t,
                                                      View view, const Tw& tw) {
 #pragma unroll
-    for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(m.sigma_p[t + j * TT]));
+    for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(gmem(m.sigma_p)[t + j * TT]));
     team_fft<T, N, R, -1, View, Sync>(y, t, view, tw);
 #pragma unroll
-    for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(m.theta_host[t + j * TT]));
+    for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(gmem(m.theta_host)[t + j * TT]));
     team_fft<T, N, R, +1, View, Sync>(y, t, view, tw);
     const T inv = T(1.0 / N);
 #pragma unroll
@@ -429,15 +436,15 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
   const int cc = kWave ? tid / TT : cl, tc = kWave ? tid % TT : tl;
   const int X = m.X;
   const int col = tile * C + cc;
-  cplx<S>* __restrict__ W = reinterpret_cast<cplx<S>*>(m.W);
+  const gptr<cplx<S>> W = gmem(reinterpret_cast<cplx<S>*>(m.W));
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(m.twP_hi), reinterpret_cast<const cplx<T>*>(m.twP_lo)};
   const View view_c{kWave ? lds + cc * LSZ : lds + cl};
-  const int step = *a.step + a.step_add;
+  const int step = *gmem(a.step) + a.step_add;
 
   const unsigned boff = (unsigned(tl) * unsigned(X) + unsigned(tile * C + cl)) * unsigned(sizeof(cplx<S>));
   cplx<T> v[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = to_arith<T>(at_off(W + size_t(j * TT) * X, boff));
+  for (int j = 0; j < R; ++j) v[j] = to_arith<T>(gload(at_off(W + size_t(j * TT) * X, boff)));
   const auto twsrc = [&]() {
     if constexpr (kWave) {
       TwRegs<T, N, R> r;
@@ -489,7 +496,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
     double acc[kNCol] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      const double p = m.p_host[tc + j * TT];
+      const double p = gmem(m.p_host)[tc + j * TT];
       const double w = double(v[j].x);
       acc[0] += w;
       acc[1] += double(v[j].y);
@@ -514,7 +521,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
 #pragma unroll
         for (int k = 0; k < kNCol; ++k) {
           s[k] = csum[tid * kNCol + k];
-          m.colsums[size_t(tile * C + tid) * kNCol + k] = s[k];
+          gmem(m.colsums)[size_t(tile * C + tid) * kNCol + k] = s[k];
         }
       }
     } else {
@@ -541,7 +548,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
           for (int k = 0; k < kNCol; ++k) s[k] += red[(w * kLanes + tid) * kNCol + k];
         }
 #pragma unroll
-        for (int k = 0; k < kNCol; ++k) m.colsums[size_t(tile * C + tid) * kNCol + k] = s[k];
+        for (int k = 0; k < kNCol; ++k) gmem(m.colsums)[size_t(tile * C + tid) * kNCol + k] = s[k];
       }
     }
   }
@@ -565,7 +572,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
     for (int j = 0; j < R; ++j) v[j] = *view_l.ptr(padr<R>(tl + j * TT));
   }
 #pragma unroll
-  for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = to_arith<S>(v[j]);
+  for (int j = 0; j < R; ++j) store_stream(at_off(W + size_t(j * TT) * X, boff), to_arith<S>(v[j]));
   if (a.flags & kTilePart) write_tile_partial<BLOCK, C>(a, m, tile, s);
 }
 
@@ -580,19 +587,19 @@ __device__ __forceinline__ void gpshift_body(const Member& m, int tile, cplx<dou
   LdsRow<double> view{lds + team * LSZ};
   cplx<double> v[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = {m.rho[t + j * TT], 0.0};
+  for (int j = 0; j < R; ++j) v[j] = {gmem(m.rho)[t + j * TT], 0.0};
   team_fft<double, N, R, -1>(v, t, view, tw);
-  const double half_theta = 0.5 * m.theta_host[row];
+  const double half_theta = 0.5 * cmem(m.theta_host)[row];
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     double sn, cs;
-    sincos_fp64(m.lambda_host[t + j * TT] * half_theta, &sn, &cs);
+    sincos_fp64(gmem(m.lambda_host)[t + j * TT] * half_theta, &sn, &cs);
     const double f = 2.0 * sn;  // (-2i sin) * (a + ib) = 2 sin b - 2i sin a
     v[j] = {f * v[j].y, -f * v[j].x};
   }
   team_fft<double, N, R, +1>(v, t, view, tw);
 #pragma unroll
-  for (int j = 0; j < R; ++j) m.G[size_t(row) * N + t + j * TT] = v[j].x * (1.0 / N);
+  for (int j = 0; j < R; ++j) gmem(m.G)[size_t(row) * N + t + j * TT] = v[j].x * (1.0 / N);
 }
 
 // ------------------------------------------------------------------ kernels
@@ -606,9 +613,12 @@ __device__ __forceinline__ void gpshift_body(const Member& m, int tile, cplx<dou
     default: break;                                                           \
   }
 
+// Two waves per SIMD (<= 256 VGPRs) for the plain sweeps: a batch of grids overlaps one
+// wave's loads and stores with the other's transforms.  The Caldeira-Leggett p sweep
+// holds three state copies and keeps the full register file of one wave.
 template <typename S, typename T, int R, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) xsweep_kernel(SweepArgs a) {
-  __shared__ cplx<T> lds[lds_elems<BLOCK, R>()];
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) xsweep_kernel(SweepArgs a) {
+  __shared__ cplx<T> lds[lds_elems<BLOCK, R>() + kTileRedBytes / sizeof(cplx<T>)];
   const unsigned w = a.work[blockIdx.x];
   const Member& m = a.members[w >> 16];
   const int tile = int(w & 0xffffu);
@@ -618,7 +628,7 @@ __global__ void __launch_bounds__(BLOCK) xsweep_kernel(SweepArgs a) {
 }
 
 template <typename S, typename T, int R, int BLOCK, bool CL>
-__global__ void __launch_bounds__(BLOCK) psweep_kernel(SweepArgs a) {
+__device__ __forceinline__ void psweep_entry(const SweepArgs& a) {
   __shared__ cplx<T> lds[lds_elems<BLOCK, R>() + kPTailBytes / sizeof(cplx<T>)];
   const unsigned w = a.work[blockIdx.x];
   const Member& m = a.members[w >> 16];
@@ -626,6 +636,10 @@ __global__ void __launch_bounds__(BLOCK) psweep_kernel(SweepArgs a) {
 #define BODY(n) psweep_body<S, T, n, R, BLOCK, CL>(a, m, tile, lds)
   WH_DISPATCH(m.P, BODY)
 #undef BODY
+}
+template <typename S, typename T, int R, int BLOCK, bool CL>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(CL ? 1 : 2))) psweep_kernel(SweepArgs a) {
+  psweep_entry<S, T, R, BLOCK, CL>(a);
 }
 
 template <int R, int BLOCK>

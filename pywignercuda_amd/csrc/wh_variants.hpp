@@ -10,8 +10,8 @@ struct SweepVariant {
 };
 
 // index 0 is the default
-constexpr SweepVariant kXVariants[] = {{16, 256}, {8, 256}, {8, 512}, {16, 512}, {4, 256}};
-constexpr SweepVariant kPVariants[] = {{16, 256}, {16, 512}, {8, 512}, {8, 1024}, {8, 256}};
+constexpr SweepVariant kXVariants[] = {{16, 256}, {8, 256}, {8, 512}, {4, 256}};
+constexpr SweepVariant kPVariants[] = {{16, 256}, {8, 512}, {8, 256}};
 constexpr SweepVariant kPCLVariants[] = {{16, 256}, {8, 256}};
 constexpr int kNumX = sizeof(kXVariants) / sizeof(kXVariants[0]);
 constexpr int kNumP = sizeof(kPVariants) / sizeof(kPVariants[0]);
