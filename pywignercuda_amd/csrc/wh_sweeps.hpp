@@ -165,28 +165,28 @@ __device__ __forceinline__ void write_record(const Member& m, const double (&r)[
   }
 }
 
-// kTilePart epilogue of a p-sweep tile: threads tid < C (wave 0, C <= 64) hold the
-// sums s of column tile*C + tid; the 12 observable partials of the tile go to
-// m.tilepart[tile].  Tile 0 of member 0 copies the step index to rec_step.
+// kTilePart epilogue of a p-sweep tile: threads tid < C hold the sums s of column
+// tile*C + tid; the 12 observable partials of the tile (sums over its C columns in
+// column order) go to m.tilepart[k * X + tile].  Tile 0 of member 0 copies the step
+// index to rec_step.  Every thread of the block calls it; scr: C * 12 doubles of LDS.
 template <int BLOCK, int C>
 __device__ __forceinline__ void write_tile_partial(const SweepArgs& a, const Member& m, int tile,
-                                                   const double (&s)[kNCol]) {
-  static_assert(C <= 64, "the columns of a tile are lanes of wave 0");
+                                                   const double (&s)[kNCol], double* scr) {
+  static_assert(C <= BLOCK, "one thread per column");
   const int tid = threadIdx.x;
-  if (tid >= 64) return;
-  double o[12];
   if (tid < C) {
+    double o[12];
     column_terms(m, tile * C + tid, s, o);
-  } else {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) o[k] = 0.0;
+    for (int k = 0; k < 12; ++k) scr[tid * 12 + k] = o[k];
   }
-#pragma unroll
-  for (int k = 0; k < 12; ++k) o[k] = wave_sum(o[k]);
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < 12; ++k) gmem(m.tilepart)[size_t(k) * m.X + tile] = o[k];
-    if (tile == 0 && &m == a.members) *gmem(a.rec_step) = *gmem(a.step);
+  __syncthreads();
+  if (tid < 12) {
+    double t = 0.0;
+    for (int c = 0; c < C; ++c) t += scr[c * 12 + tid];
+    gmem(m.tilepart)[size_t(tid) * m.X + tile] = t;
+  } else if (tid == 12 && tile == 0 && &m == a.members) {
+    *gmem(a.rec_step) = *gmem(a.step);
   }
 }
 
@@ -374,13 +374,39 @@ struct PSweep {
     const double e1 = f * d[1], e3 = f * d[3], e5 = f * d[5], e7 = f * d[7];
     // uniform conditions outside the element loops, so that each loop is one basic block
     // and no per-element array stays live across it
-    if (m.pot_deg >= 0 && !gp) {
+    const double hd = a.hbar_half * m.dtheta;
+    auto mult = [&](int i) {
+      const double u = hd * double(fidx(i, N));
+      const double u2 = u * u;
+      return expmi(mask, u * fma(u2, fma(u2, fma(u2, e7, e5), e3), e1));
+    };
+    if (kWave && m.pot_deg >= 0 && !gp) {
+      // theta_{N-i} = -theta_i and the phase is odd in u, so the multiplier of N - i is the
+      // exact conjugate of that of i (the reduction and the odd/even series negate exactly):
+      // lane t evaluates j < R/2 (i < N/2) and takes j >= R/2 from lane TT - t of its team
+      // (element R-1-j there); lane 0 pairs with itself (element R-j) and evaluates the
+      // self-paired i = N/2 directly.
+      cplx<double> M[R / 2];
 #pragma unroll
-      for (int j = 0; j < R; ++j) {
-        const double u = a.hbar_half * m.dtheta * double(fidx(t + j * TT, N));
-        const double u2 = u * u;
-        v[j] = apply(v[j], expmi(mask, u * fma(u2, fma(u2, fma(u2, e7, e5), e3), e1)));
-      }
+      for (int j = 0; j < R / 2; ++j) M[j] = mult(t + j * TT);
+      const cplx<double> nyq = mult(N / 2);
+      const int lane = threadIdx.x & 63;
+      const int src = lane - t + ((TT - t) & (TT - 1));
+      static_for<R / 2, R>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        cplx<double> q{__shfl(M[R - 1 - j].x, src, 64), __shfl(M[R - 1 - j].y, src, 64)};
+        if constexpr (j == R / 2) {
+          if (t == 0) q = {nyq.x, -nyq.y};  // own multiplier (conjugated back below)
+        } else {
+          if (t == 0) q = M[R - j];
+        }
+        v[j] = apply(v[j], cplx<double>{q.x, -q.y});
+      });
+#pragma unroll
+      for (int j = 0; j < R / 2; ++j) v[j] = apply(v[j], M[j]);
+    } else if (m.pot_deg >= 0 && !gp) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[j] = apply(v[j], mult(t + j * TT));
     } else {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
@@ -415,8 +441,9 @@ struct PSweep {
   }
 };
 
-// LDS bytes past the transform arrays of a p-sweep block: per-column sums (<= 64 columns)
-constexpr int kPTailBytes = 64 * kNCol * 8;
+// LDS bytes past the transform arrays of a p-sweep block: per-column sums, then the
+// per-column observable terms (<= 64 columns)
+constexpr int kPTailBytes = 64 * 12 * 8;
 
 // Lanes: the load layout (column cl = tid % C, team index tl = tid / C) makes every
 // global load and store instruction cover rows of C consecutive columns.  With wave
@@ -573,7 +600,9 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
   }
 #pragma unroll
   for (int j = 0; j < R; ++j) store_stream(at_off(W + size_t(j * TT) * X, boff), to_arith<S>(v[j]));
-  if (a.flags & kTilePart) write_tile_partial<BLOCK, C>(a, m, tile, s);
+  if (a.flags & kTilePart)
+    write_tile_partial<BLOCK, C>(a, m, tile, s,
+                                 reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + lds_elems<BLOCK, R>() * sizeof(cplx<T>)));
 }
 
 // ------------------------------------------------------------------ Gross-Pitaevskii shift
