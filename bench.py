@@ -283,6 +283,13 @@ def main():
     t_x = kernel_time(lambda: prop.kinetic(Wk), a.kernel_reps)
     bytes_per_launch = cells * 16 * 2  # read + write the complex128 state once
     achieved = bytes_per_launch / t_p / 1e9
+    # HBM bytes per launch of the same kernel and workload from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a plain copy)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_wigner2d_cl_1024_fp64.json")
+    if a.workload == "damped" and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["bytes_per_launch"]
     out = {
         "metric": "cell-steps/s at 1024^2: plain FP64 vs batched FP32 @ stated error; HBM GB/s",
         "value": value,
@@ -301,8 +308,9 @@ def main():
                                "dt=0.005, cat state, observables + renormalisation every step",
                    "grid": [prop.P, prop.X], "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "psweep_kernel<double,1024,16,4,CL>", "kernel_us": t_p * 1e6,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": ("psweep_kernel<double, double, 16, 256, true>" if a.workload == "damped"
+                                else "psweep_kernel<double, double, 16, 256, false>"), "kernel_us": t_p * 1e6,
                      "xsweep_us": t_x * 1e6,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }

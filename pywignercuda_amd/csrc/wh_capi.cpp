@@ -942,10 +942,9 @@ extern "C" int wh_wigner2d_potential(wh_wigner2d* p, void* W, int step, int fuse
   hipStream_t s = static_cast<hipStream_t>(stream);
   wh_ensemble* e = p->e;
   WH_TRY(bind_single(e, W, e->mem[0].obs, e->mem[0].obs_base, s));
-  WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), step, s), "set_step");
   const double frac = e->d.scheme == WH_SCHEME_LIE ? 1.0 : 0.5;
-  const int flags = fused ? (wh::kColSums | (e->cl() ? wh::kCL : 0)) : 0;
-  return psweep(e, flags, frac, 0, s);
+  const int flags = wh::kStepArg | (fused ? (wh::kColSums | (e->cl() ? wh::kCL : 0)) : 0);
+  return psweep(e, flags, frac, step, s);
 }
 
 extern "C" int wh_ensemble_set_potential(wh_ensemble* e, int member, const double* coef, int rows, int deg,

@@ -36,6 +36,8 @@ enum SweepFlags {
   kTileScale = 256,  // x sweep: the pending factor from the previous p sweep's tile partials;
                      // the member's tile 0 writes the record row and the factor (replaces
                      // the reduce launch between the sweeps)
+  kStepArg = 512,    // p sweep: the time index is step_add itself (single-sweep entry points),
+                     // not the device counter
 };
 
 constexpr int kTileObs = 12;  // observable partials per tile

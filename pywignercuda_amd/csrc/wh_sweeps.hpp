@@ -466,7 +466,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
   const gptr<cplx<S>> W = gmem(reinterpret_cast<cplx<S>*>(m.W));
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(m.twP_hi), reinterpret_cast<const cplx<T>*>(m.twP_lo)};
   const View view_c{kWave ? lds + cc * LSZ : lds + cl};
-  const int step = *gmem(a.step) + a.step_add;
+  const int step = (a.flags & kStepArg) ? a.step_add : *gmem(a.step) + a.step_add;
 
   const unsigned boff = (unsigned(tl) * unsigned(X) + unsigned(tile * C + cl)) * unsigned(sizeof(cplx<S>));
   cplx<T> v[R];
