@@ -69,17 +69,17 @@ __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group,
   const int tid = threadIdx.x, team = tid / TT, t = tid % TT;
   const int row = tile * ROWS + team;
   const size_t plane = size_t(d.P) * size_t(d.X);
-  cplx<S>* __restrict__ W = reinterpret_cast<cplx<S>*>(d.W) + size_t(row) * N;
+  const gptr<cplx<S>> W = gmem(reinterpret_cast<cplx<S>*>(d.W)) + size_t(row) * N;
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twX_hi), reinterpret_cast<const cplx<T>*>(d.twX_lo)};
   LdsRow<T> view{lds + team * LSZ};
-  const double s = *d.scale;
+  const double s = *gmem(d.scale);
 
   cplx<T> v[4][R];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const cplx<S>* src = W + size_t(kDiracGroups[group][q]) * plane;
+    const gptr<cplx<S>> src = W + size_t(kDiracGroups[group][q]) * plane;
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[q][j] = to_arith<T>(src[t + j * TT]);
+    for (int j = 0; j < R; ++j) v[q][j] = to_arith<T>(gload(src + t + j * TT));
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -95,7 +95,7 @@ __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group,
     const double e2 = 2.0 * sqrt(d.proj_mc2 * d.proj_mc2 + d.proj_c * d.proj_c * p * p);
     const double aa = 0.5 - d.proj_mc2 / e2, bb = -d.proj_c * p / e2, dd = 0.5 + d.proj_mc2 / e2;
     const double c = aa * double(v[0][0].x) + bb * (double(v[1][0].x) + double(v[2][0].x)) + dd * double(v[3][0].x);
-    d.rowpart[(group == 0 ? 0 : 1) * d.P + row] = c;
+    gmem(d.rowpart)[(group == 0 ? 0 : 1) * d.P + row] = c;
   }
 
 #pragma unroll
@@ -120,9 +120,9 @@ __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group,
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     team_fft<T, N, R, +1>(v[q], t, view, tw);
-    cplx<S>* dst = W + size_t(kDiracGroups[group][q]) * plane;
+    const gptr<cplx<S>> dst = W + size_t(kDiracGroups[group][q]) * plane;
 #pragma unroll
-    for (int j = 0; j < R; ++j) dst[t + j * TT] = to_arith<S>(cscale(v[q][j], inv));
+    for (int j = 0; j < R; ++j) store_stream(dst + t + j * TT, to_arith<S>(cscale(v[q][j], inv)));
   }
 }
 
@@ -151,7 +151,7 @@ __device__ __forceinline__ void dirac_accumulate(const DiracPlan& d, int k, cons
       acc[kDcBetaIm] += sb * wi;
       trre[j] += wr;
     } else {
-      const double ph = d.p_host[i];
+      const double ph = gmem(d.p_host)[i];
       acc[kDcA1Re] += wr;
       acc[kDcA1Im] += wi;
       acc[kDcPA1Re] += p * wr;
@@ -171,19 +171,19 @@ __device__ __forceinline__ void dirac_cl2(cplx<T> (&v)[R], const DiracPlan& d, i
   const T inv = T(1.0 / N);
   cplx<T> y[R], lw[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) y[j] = cscale(v[j], T(d.p_host[t + j * TT]));
+  for (int j = 0; j < R; ++j) y[j] = cscale(v[j], T(gmem(d.p_host)[t + j * TT]));
   team_fft<T, N, R, -1>(y, t, view, tw);
 #pragma unroll
-  for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(d.theta_host[t + j * TT]));
+  for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(gmem(d.theta_host)[t + j * TT]));
   team_fft<T, N, R, +1>(y, t, view, tw);
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     lw[j] = cadd(v[j], mul_i(cscale(y[j], inv), T(d.cl_a)));
-    y[j] = cscale(lw[j], T(d.p_host[t + j * TT]));
+    y[j] = cscale(lw[j], T(gmem(d.p_host)[t + j * TT]));
   }
   team_fft<T, N, R, -1>(y, t, view, tw);
 #pragma unroll
-  for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(d.theta_host[t + j * TT]));
+  for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(gmem(d.theta_host)[t + j * TT]));
   team_fft<T, N, R, +1>(y, t, view, tw);
 #pragma unroll
   for (int j = 0; j < R; ++j) v[j] = cadd(v[j], mul_i(cscale(y[j], inv), T(2.0 * d.cl_a)));
@@ -218,7 +218,7 @@ __device__ __forceinline__ void dirac_store_colsums(const DiracPlan& d, int tile
       for (int q = 0; q < kDNCol; ++q) sm[q] += red[(w * kLanes + tid) * kDNCol + q];
     }
 #pragma unroll
-    for (int q = 0; q < kDNCol; ++q) d.colsums[size_t(tile * C + tid) * kDNCol + q] = sm[q];
+    for (int q = 0; q < kDNCol; ++q) gmem(d.colsums)[size_t(tile * C + tid) * kDNCol + q] = sm[q];
   }
 }
 
@@ -231,26 +231,25 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
   const int X = d.X;
   const int col = tile * C + c;
   const size_t plane = size_t(d.P) * size_t(X);
-  cplx<S>* __restrict__ W0 = reinterpret_cast<cplx<S>*>(d.W);
+  const gptr<cplx<S>> W0 = gmem(reinterpret_cast<cplx<S>*>(d.W));
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twP_hi), reinterpret_cast<const cplx<T>*>(d.twP_lo)};
   LdsStrided<T, C> view{lds + c};
-  const int step = *d.step;
+  const int step = *gmem(d.step);
 
   // Bopp phase dt (V0(x - theta/2) - V0(x + theta/2)) of this lane's elements (shared by the 16 planes)
   const double x = d.dx * double(fidx(col, X));
-  const double mask = d.xmask ? d.xmask[col] : 1.0;
+  const double mask = d.xmask ? gmem(d.xmask)[col] : 1.0;
   double ph[R];
   {
     const int prow = step < d.pot_rows ? step : d.pot_rows - 1;
-    const double* __restrict__ cf = d.pot + size_t(prow) * (kMaxDeg + 1);
+    const BoppOdd bopp = bopp_odd(cmem(d.pot) + size_t(prow) * (kMaxDeg + 1), d.pot_deg, x, 1.0);
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int i = t + j * TT;
       const double th = d.dtheta * double(fidx(i, N));
-      const double dv = d.pot_deg >= 0 ? poly(cf, d.pot_deg, x - 0.5 * th) - poly(cf, d.pot_deg, x + 0.5 * th)
-                                       : d.pot_dv[size_t(i) * X + col];
+      const double dv = d.pot_deg >= 0 ? bopp(0.5 * th) : gmem(d.pot_dv)[size_t(i) * X + col];
       ph[j] = d.dt * dv;
-      if (d.gp_coeff != 0.0) ph[j] += d.dt * d.gp_coeff * d.G[size_t(i) * X + col];
+      if (d.gp_coeff != 0.0) ph[j] += d.dt * d.gp_coeff * gmem(d.G)[size_t(i) * X + col];
     }
   }
 
@@ -274,19 +273,19 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
       double m;
       if (d.mass_dm) {
         const size_t e = size_t(i) * X + col;
-        m = cls == 0 ? d.mass_dm[e] : cls == 1 ? d.mass_sm[e] : cls == 2 ? -d.mass_sm[e] : -d.mass_dm[e];
+        m = cls == 0 ? gmem(d.mass_dm)[e] : cls == 1 ? gmem(d.mass_sm)[e] : cls == 2 ? -gmem(d.mass_sm)[e] : -gmem(d.mass_dm)[e];
       } else {
         m = cls == 1 ? 2.0 * d.mass_phase : cls == 2 ? -2.0 * d.mass_phase : 0.0;
       }
-      const double r = (d.theta_decay ? d.theta_decay[i] : 1.0) * mask;
+      const double r = (d.theta_decay ? gmem(d.theta_decay)[i] : 1.0) * mask;
       M[j] = expmi(r, ph[j] + m);
     }
     for (int q = 0; q < 4; ++q) {
       const int k = ((cls >> 1) << 3) | ((cls & 1) << 1) | ((q >> 1) << 2) | (q & 1);
-      cplx<S>* __restrict__ W = W0 + size_t(k) * plane;
+      const gptr<cplx<S>> W = W0 + size_t(k) * plane;
       cplx<T> v[R];
 #pragma unroll
-      for (int j = 0; j < R; ++j) v[j] = to_arith<T>(at_off(W + size_t(j * TT) * X, boff));
+      for (int j = 0; j < R; ++j) v[j] = to_arith<T>(gload(at_off(W + size_t(j * TT) * X, boff)));
       team_fft<T, N, R, -1>(v, t, view, tw);
 #pragma unroll
       for (int j = 0; j < R; ++j) v[j] = apply(v[j], M[j]);
@@ -298,7 +297,7 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
 
       dirac_accumulate<T, R, TT>(d, k, v, t, acc, trre);
 #pragma unroll
-      for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = to_arith<S>(v[j]);
+      for (int j = 0; j < R; ++j) store_stream(at_off(W + size_t(j * TT) * X, boff), to_arith<S>(v[j]));
     }
   }
 #pragma unroll
@@ -350,22 +349,22 @@ __device__ __forceinline__ void dirac_pleft_body(const DiracPlan& d, int tile, c
   const int X = d.X;
   const int col = tile * C + c;
   const size_t plane = size_t(d.P) * size_t(X);
-  cplx<S>* __restrict__ W0 = reinterpret_cast<cplx<S>*>(d.W);
+  const gptr<cplx<S>> W0 = gmem(reinterpret_cast<cplx<S>*>(d.W));
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twP_hi), reinterpret_cast<const cplx<T>*>(d.twP_lo)};
   LdsStrided<T, C> view{lds + c};
-  const int step = *d.step;
+  const int step = *gmem(d.step);
   const double x = d.dx * double(fidx(col, X));
-  const double mask = d.xmask ? d.xmask[col] : 1.0;
+  const double mask = d.xmask ? gmem(d.xmask)[col] : 1.0;
   const int prow = step < d.pot_rows ? step : d.pot_rows - 1;
-  const double* __restrict__ cf = d.pot + size_t(prow) * (kMaxDeg + 1);
+  const BoppOdd bopp = bopp_odd(cmem(d.pot) + size_t(prow) * (kMaxDeg + 1), d.pot_deg, x, 1.0);
   const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<S>));
   for (int jc = 0; jc < 4; ++jc) {  // spinor column jc: planes 4 i + jc
     cplx<T> v[4][R];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      cplx<S>* W = W0 + size_t(4 * q + jc) * plane;
+      const gptr<cplx<S>> W = W0 + size_t(4 * q + jc) * plane;
 #pragma unroll
-      for (int j = 0; j < R; ++j) v[q][j] = to_arith<T>(at_off(W + size_t(j * TT) * X, boff));
+      for (int j = 0; j < R; ++j) v[q][j] = to_arith<T>(gload(at_off(W + size_t(j * TT) * X, boff)));
       team_fft<T, N, R, -1>(v[q], t, view, tw);
     }
 #pragma unroll
@@ -373,11 +372,10 @@ __device__ __forceinline__ void dirac_pleft_body(const DiracPlan& d, int tile, c
       const int i = t + j * TT;
       const double th = d.dtheta * double(fidx(i, N));
       // both scalar halves e- e+ = r e^{-i dt (V0(x - th/2) - V0(x + th/2) + a G)} applied here
-      double ph = d.pot_deg >= 0 ? poly(cf, d.pot_deg, x - 0.5 * th) - poly(cf, d.pot_deg, x + 0.5 * th)
-                                 : d.pot_dv[size_t(i) * X + col];
+      double ph = d.pot_deg >= 0 ? bopp(0.5 * th) : gmem(d.pot_dv)[size_t(i) * X + col];
       ph *= d.dt;
-      if (d.gp_coeff != 0.0) ph += d.dt * d.gp_coeff * d.G[size_t(i) * X + col];
-      const double r = (d.theta_decay ? d.theta_decay[i] : 1.0) * mask;
+      if (d.gp_coeff != 0.0) ph += d.dt * d.gp_coeff * gmem(d.G)[size_t(i) * X + col];
+      const double r = (d.theta_decay ? gmem(d.theta_decay)[i] : 1.0) * mask;
       const cplx<double> e = expmi(r, ph);
       const DiracPot u = dirac_pot_factor(d, x - 0.5 * th, -1.0);
       const cplx<double> w0 = widen(v[0][j]), w1 = widen(v[1][j]), w2 = widen(v[2][j]), w3 = widen(v[3][j]);
@@ -393,9 +391,9 @@ __device__ __forceinline__ void dirac_pleft_body(const DiracPlan& d, int tile, c
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      cplx<S>* W = W0 + size_t(4 * q + jc) * plane;
+      const gptr<cplx<S>> W = W0 + size_t(4 * q + jc) * plane;
 #pragma unroll
-      for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = to_arith<S>(v[q][j]);
+      for (int j = 0; j < R; ++j) store_stream(at_off(W + size_t(j * TT) * X, boff), to_arith<S>(v[q][j]));
     }
   }
 }
@@ -409,7 +407,7 @@ __device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, 
   const int X = d.X;
   const int col = tile * C + c;
   const size_t plane = size_t(d.P) * size_t(X);
-  cplx<S>* __restrict__ W0 = reinterpret_cast<cplx<S>*>(d.W);
+  const gptr<cplx<S>> W0 = gmem(reinterpret_cast<cplx<S>*>(d.W));
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twP_hi), reinterpret_cast<const cplx<T>*>(d.twP_lo)};
   LdsStrided<T, C> view{lds + c};
   const double x = d.dx * double(fidx(col, X));
@@ -425,9 +423,9 @@ __device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, 
     cplx<T> v[4][R];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      cplx<S>* W = W0 + size_t(4 * ir + q) * plane;
+      const gptr<cplx<S>> W = W0 + size_t(4 * ir + q) * plane;
 #pragma unroll
-      for (int j = 0; j < R; ++j) v[q][j] = to_arith<T>(at_off(W + size_t(j * TT) * X, boff));
+      for (int j = 0; j < R; ++j) v[q][j] = to_arith<T>(gload(at_off(W + size_t(j * TT) * X, boff)));
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -450,9 +448,9 @@ __device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, 
       if constexpr (CL) dirac_cl2<T, N, R>(v[q], d, t, view, tw);
       const int k = 4 * ir + q;
       dirac_accumulate<T, R, TT>(d, k, v[q], t, acc, trre);
-      cplx<S>* W = W0 + size_t(k) * plane;
+      const gptr<cplx<S>> W = W0 + size_t(k) * plane;
 #pragma unroll
-      for (int j = 0; j < R; ++j) at_off(W + size_t(j * TT) * X, boff) = to_arith<S>(v[q][j]);
+      for (int j = 0; j < R; ++j) store_stream(at_off(W + size_t(j * TT) * X, boff), to_arith<S>(v[q][j]));
     }
   }
 #pragma unroll
@@ -546,7 +544,7 @@ __global__ void __launch_bounds__(256) dirac_colsum_kernel(const DiracPlan* __re
   double acc[kDNCol];
   for (int q = 0; q < kDNCol; ++q) acc[q] = 0.0;
   for (int i = 0; i < d.P; ++i) {
-    const double p = d.dp * double(fidx(i, d.P)), ph = d.p_host[i];
+    const double p = d.dp * double(fidx(i, d.P)), ph = gmem(d.p_host)[i];
     double trr = 0.0;
     for (int k = 0; k < 16; ++k) {
       const bool diag = (k % 5) == 0, alpha = k == 3 || k == 6 || k == 9 || k == 12;
@@ -575,7 +573,7 @@ __global__ void __launch_bounds__(256) dirac_colsum_kernel(const DiracPlan* __re
     }
     acc[kDcNeg] += trr < 0.0 ? trr : 0.0;
   }
-  for (int q = 0; q < kDNCol; ++q) d.colsums[size_t(col) * kDNCol + q] = acc[q];
+  for (int q = 0; q < kDNCol; ++q) gmem(d.colsums)[size_t(col) * kDNCol + q] = acc[q];
 }
 
 // Observables of the state whose column sums are in d.colsums (state time index = step +
@@ -588,7 +586,7 @@ __global__ void __launch_bounds__(kRedThreads) dirac_reduce_kernel(const DiracPl
                                                                    int renorm, int anti, int inc) {
   const DiracPlan& d = *dp;
   __shared__ double red[kRedThreads][kRedQ + 1];
-  const int step = *d.step;
+  const int step = *gmem(d.step);
   const int tstate = step + state_add;
   const int prow = tstate < d.pot_rows ? tstate : d.pot_rows - 1;
   const double* __restrict__ cf = d.pot + size_t(prow) * (kMaxDeg + 1);
@@ -603,8 +601,8 @@ __global__ void __launch_bounds__(kRedThreads) dirac_reduce_kernel(const DiracPl
       for (int n = d.pot_deg; n >= 1; --n) dv = fma(dv, x, double(n) * cf[n]);
       vs = poly(cf, d.pot_deg, x);
     } else {
-      dv = d.dv0[col];
-      vs = d.vsum[col];
+      dv = gmem(d.dv0)[col];
+      vs = gmem(d.vsum)[col];
     }
     if (d.pot_deg >= 0 && d.general) {  // DiracEnergy adds Potential_1..3 like Potential_0 (:2801-2804)
       if (d.adeg0 >= 0) vs += poly(d.acoef[0], d.adeg0, x);
@@ -642,7 +640,7 @@ __global__ void __launch_bounds__(kRedThreads) dirac_reduce_kernel(const DiracPl
   double anti_sum = 0.0;
   if (anti && d.antiparticle) {
     double a = 0.0;
-    for (int r = threadIdx.x; r < d.P; r += kRedThreads) a += d.rowpart[r] + d.rowpart[d.P + r];
+    for (int r = threadIdx.x; r < d.P; r += kRedThreads) a += gmem(d.rowpart)[r] + gmem(d.rowpart)[d.P + r];
     red[threadIdx.x][kRedQ] = a;
     __syncthreads();
     for (int h = kRedThreads / 2; h > 0; h >>= 1) {
@@ -656,7 +654,7 @@ __global__ void __launch_bounds__(kRedThreads) dirac_reduce_kernel(const DiracPl
   __syncthreads();
   if (d.gp_coeff != 0.0) {  // density of the (normalised) state for the next Gross-Pitaevskii phase
     for (int col = threadIdx.x; col < d.X; col += kRedThreads)
-      d.rho[col] = fshare * d.colsums[size_t(col) * kDNCol + kDcTrRe];
+      gmem(d.rho)[col] = fshare * gmem(d.colsums)[size_t(col) * kDNCol + kDcTrRe];
   }
   if (threadIdx.x == 0) {
     const double raw = d.dxdp * red[0][kDoNorm];
@@ -683,7 +681,7 @@ __global__ void __launch_bounds__(256) dirac_finish_kernel(const DiracPlan* __re
   const int row = blockIdx.x;
   const size_t plane = size_t(d.P) * size_t(d.X);
   cplx<T>* W = reinterpret_cast<cplx<T>*>(d.W) + size_t(row) * d.X;
-  const double s = *d.scale;
+  const double s = *gmem(d.scale);
   __shared__ double red[256][4];
   double acc[4] = {0.0, 0.0, 0.0, 0.0};  // sum_x Re of diag-a (11, 22), diag-d (33, 44), off-diagonal b
   for (int k = 0; k < 16; ++k) {
@@ -705,8 +703,8 @@ __global__ void __launch_bounds__(256) dirac_finish_kernel(const DiracPlan* __re
     const double p = d.dp * double(fidx(row, d.P));
     const double e2 = 2.0 * sqrt(d.proj_mc2 * d.proj_mc2 + d.proj_c * d.proj_c * p * p);
     const double aa = 0.5 - d.proj_mc2 / e2, bb = -d.proj_c * p / e2, dd = 0.5 + d.proj_mc2 / e2;
-    d.rowpart[row] = aa * red[0][0] + dd * red[0][1] + bb * red[0][2];
-    d.rowpart[d.P + row] = 0.0;
+    gmem(d.rowpart)[row] = aa * red[0][0] + dd * red[0][1] + bb * red[0][2];
+    gmem(d.rowpart)[d.P + row] = 0.0;
   }
 }
 
@@ -715,7 +713,7 @@ __global__ void __launch_bounds__(kRedThreads) dirac_anti_kernel(const DiracPlan
   const DiracPlan& d = *dp;
   __shared__ double red[kRedThreads];
   double a = 0.0;
-  for (int r = threadIdx.x; r < d.P; r += kRedThreads) a += d.rowpart[r] + d.rowpart[d.P + r];
+  for (int r = threadIdx.x; r < d.P; r += kRedThreads) a += gmem(d.rowpart)[r] + gmem(d.rowpart)[d.P + r];
   red[threadIdx.x] = a;
   __syncthreads();
   for (int h = kRedThreads / 2; h > 0; h >>= 1) {

@@ -315,15 +315,17 @@ __device__ __forceinline__ void static_for(F&& f) {
 
 // Ordering of a pass's LDS write and read.  BlockSync: the team spans several waves.
 // WaveSync: every lane of the team is in one wave, whose LDS operations execute in
-// program order, so only the compiler has to keep the order (no s_barrier).
+// program order, so only the compiler has to keep the order (no s_barrier); the
+// fences name the LDS address space only, so global loads (tables, twiddles) may
+// still be scheduled across an exchange.
 struct BlockSync {
   __device__ __forceinline__ static void sync() { __syncthreads(); }
 };
 struct WaveSync {
   __device__ __forceinline__ static void sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
   }
 };
 

@@ -34,6 +34,32 @@ __device__ __forceinline__ double poly(const double* __restrict__ c, int deg, do
   return r;
 }
 
+// Bopp difference of a polynomial V of degree <= kMaxDeg as an odd polynomial in u:
+// V(x - u) - V(x + u) = -2 sum_{k odd} D_k(x) u^k, with the Taylor coefficients
+// D_k(x) = V^(k)(x) / k! from a Taylor shift of the coefficients c (evaluated once per
+// column; each cell then costs four FMAs).  Scaled by f.
+struct BoppOdd {
+  double e1, e3, e5, e7;
+  __device__ __forceinline__ double operator()(double u) const {
+    const double u2 = u * u;
+    return u * fma(u2, fma(u2, fma(u2, e7, e5), e3), e1);
+  }
+};
+template <class CP>
+__device__ __forceinline__ BoppOdd bopp_odd(CP c, int deg, double x, double f) {
+  static_assert(kMaxDeg == 8, "odd Taylor terms u, u^3, u^5, u^7");
+  double d[kMaxDeg + 1];
+#pragma unroll
+  for (int n = 0; n <= kMaxDeg; ++n) d[n] = n <= deg ? c[n] : 0.0;
+#pragma unroll
+  for (int i = 0; i < kMaxDeg; ++i) {
+#pragma unroll
+    for (int k = kMaxDeg - 1; k >= i; --k) d[k] = fma(x, d[k + 1], d[k]);
+  }
+  const double g = -2.0 * f;
+  return {g * d[1], g * d[3], g * d[5], g * d[7]};
+}
+
 // kernel-side FFT-ordered coordinate index: ((k + n/2) mod n) - n/2
 __device__ __forceinline__ int fidx(int k, int n) { return ((k + (n >> 1)) & (n - 1)) - (n >> 1); }
 
@@ -359,27 +385,11 @@ struct PSweep {
     const cptr<double> cf = cmem(m.pot) + size_t(row) * (kMaxDeg + 1);
     const double mask = m.xmask ? gmem(m.xmask)[col] : 1.0;
     const bool gp = (a.flags & kGP) != 0;
-    static_assert(kMaxDeg == 8, "odd Taylor terms u, u^3, u^5, u^7");
-    // V(x - u) - V(x + u) = -2 sum_{k odd} D_k(x) u^k with the Taylor coefficients
-    // D_k(x) = V^(k)(x) / k! of the column, from a Taylor shift of the coefficients
-    double d[kMaxDeg + 1];
-#pragma unroll
-    for (int n = 0; n <= kMaxDeg; ++n) d[n] = n <= m.pot_deg ? cf[n] : 0.0;
-#pragma unroll
-    for (int i = 0; i < kMaxDeg; ++i) {
-#pragma unroll
-      for (int k = kMaxDeg - 1; k >= i; --k) d[k] = fma(x, d[k + 1], d[k]);
-    }
-    const double f = -2.0 * a.vphase;
-    const double e1 = f * d[1], e3 = f * d[3], e5 = f * d[5], e7 = f * d[7];
+    const BoppOdd bopp = bopp_odd(cf, m.pot_deg, x, a.vphase);
     // uniform conditions outside the element loops, so that each loop is one basic block
     // and no per-element array stays live across it
     const double hd = a.hbar_half * m.dtheta;
-    auto mult = [&](int i) {
-      const double u = hd * double(fidx(i, N));
-      const double u2 = u * u;
-      return expmi(mask, u * fma(u2, fma(u2, fma(u2, e7, e5), e3), e1));
-    };
+    auto mult = [&](int i) { return expmi(mask, bopp(hd * double(fidx(i, N)))); };
     if (kWave && m.pot_deg >= 0 && !gp) {
       // theta_{N-i} = -theta_i and the phase is odd in u, so the multiplier of N - i is the
       // exact conjugate of that of i (the reduction and the odd/even series negate exactly):
@@ -411,9 +421,7 @@ struct PSweep {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         const int i = t + j * TT;
-        const double u = a.hbar_half * m.dtheta * double(fidx(i, N));
-        const double u2 = u * u;
-        double phi = m.pot_deg >= 0 ? u * fma(u2, fma(u2, fma(u2, e7, e5), e3), e1)
+        double phi = m.pot_deg >= 0 ? bopp(hd * double(fidx(i, N)))
                                     : a.vphase * gmem(m.pot_dv)[size_t(i) * m.X + col];
         if (gp) phi -= a.gp_phase * gmem(m.G)[size_t(i) * m.X + col];
         v[j] = apply(v[j], expmi(mask, phi));
