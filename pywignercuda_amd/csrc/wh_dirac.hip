@@ -72,6 +72,9 @@ __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group,
   const gptr<cplx<S>> W = gmem(reinterpret_cast<cplx<S>*>(d.W)) + size_t(row) * N;
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twX_hi), reinterpret_cast<const cplx<T>*>(d.twX_lo)};
   LdsRow<T> view{lds + team * LSZ};
+  // a team inside one wave (radix 8 up to N = 512): wave-local exchanges
+  using Sync = std::conditional_t<(TT <= 64), WaveSync, BlockSync>;
+  const TwMem<T> twm{tw};
   const double s = *gmem(d.scale);
 
   cplx<T> v[4][R];
@@ -85,7 +88,7 @@ __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group,
   for (int q = 0; q < 4; ++q) {
 #pragma unroll
     for (int j = 0; j < R; ++j) v[q][j] = cscale(v[q][j], T(s));
-    team_fft<T, N, R, -1>(v[q], t, view, tw);
+    team_fft<T, N, R, -1, LdsRow<T>, Sync>(v[q], t, view, twm);
   }
 
   const double p = d.dp * double(fidx(row, d.P));
@@ -119,7 +122,7 @@ __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group,
   const T inv = T(1.0 / N);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    team_fft<T, N, R, +1>(v[q], t, view, tw);
+    team_fft<T, N, R, +1, LdsRow<T>, Sync>(v[q], t, view, twm);
     const gptr<cplx<S>> dst = W + size_t(kDiracGroups[group][q]) * plane;
 #pragma unroll
     for (int j = 0; j < R; ++j) store_stream(dst + t + j * TT, to_arith<S>(cscale(v[q][j], inv)));
@@ -465,11 +468,16 @@ constexpr int kDiracPBlock = 256;
 // radix of the x sweep (4 planes live per lane) and of the p sweep (1 plane live)
 template <int N>
 constexpr int dirac_rx() {
-  return N >= 2048 ? 8 : 4;
+  return N >= 512 ? 8 : 4;
 }
 template <int N>
 constexpr int dirac_rp() {
   return N >= 512 ? 8 : 4;
+}
+// radix of the vector-potential p sweeps (4 planes live per lane and two factor matrices)
+template <int N>
+constexpr int dirac_rv() {
+  return N >= 2048 ? 8 : 4;
 }
 
 // LDS elements of a block over the supported sizes: teams * padded length.
@@ -519,7 +527,7 @@ template <typename S, typename T>
 __global__ void __launch_bounds__(kDiracXBlock) dirac_pleft_kernel(const DiracPlan* __restrict__ dp) {
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
   const DiracPlan& d = *dp;
-#define BODY(n) dirac_pleft_body<S, T, n, dirac_rx<n>(), kDiracXBlock>(d, int(blockIdx.x), lds)
+#define BODY(n) dirac_pleft_body<S, T, n, dirac_rv<n>(), kDiracXBlock>(d, int(blockIdx.x), lds)
   WH_DIRAC_DISPATCH(d.P, BODY)
 #undef BODY
 }
@@ -528,7 +536,7 @@ template <typename S, typename T, bool CL>
 __global__ void __launch_bounds__(kDiracXBlock) dirac_pright_kernel(const DiracPlan* __restrict__ dp) {
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
   const DiracPlan& d = *dp;
-#define BODY(n) dirac_pright_body<S, T, n, dirac_rx<n>(), kDiracXBlock, CL>(d, int(blockIdx.x), lds)
+#define BODY(n) dirac_pright_body<S, T, n, dirac_rv<n>(), kDiracXBlock, CL>(d, int(blockIdx.x), lds)
   WH_DIRAC_DISPATCH(d.P, BODY)
 #undef BODY
 }
@@ -805,7 +813,7 @@ hipError_t launch_dirac_psweep(int prec, bool cl, const DiracPlan* d, int nblock
 
 int dirac_pg_blocks(int X, int P) {
   int cols = 0;
-#define BODY(n) cols = kDiracXBlock / (n / dirac_rx<n>())
+#define BODY(n) cols = kDiracXBlock / (n / dirac_rv<n>())
   WH_DIRAC_DISPATCH(P, BODY)
 #undef BODY
   return cols > 0 && X % cols == 0 ? X / cols : -1;
