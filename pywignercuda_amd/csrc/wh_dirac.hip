@@ -494,6 +494,14 @@ constexpr int dirac_lds_elems() {
   return m;
 }
 
+// Column tile of a p-sweep block: blocks are dealt round-robin to the 8 XCDs, and two
+// neighbouring FP64 tiles (4 columns, 64 bytes) share each 128-byte line, so the tiles of
+// one XCD are made contiguous: both halves of a line then pass through the same L2.
+__device__ __forceinline__ int dirac_col_tile() {
+  const int nt = int(gridDim.x), b = int(blockIdx.x);
+  return nt % 8 == 0 ? (b % 8) * (nt / 8) + b / 8 : b;
+}
+
 #define WH_DIRAC_DISPATCH(NVAR, BODY) \
   switch (NVAR) {                     \
     case 128: BODY(128); break;       \
@@ -518,7 +526,7 @@ template <typename S, typename T, bool CL>
 __global__ void __launch_bounds__(kDiracPBlock) dirac_psweep_kernel(const DiracPlan* __restrict__ dp) {
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracPBlock>()];
   const DiracPlan& d = *dp;
-#define BODY(n) dirac_psweep_body<S, T, n, dirac_rp<n>(), kDiracPBlock, CL>(d, int(blockIdx.x), lds)
+#define BODY(n) dirac_psweep_body<S, T, n, dirac_rp<n>(), kDiracPBlock, CL>(d, dirac_col_tile(), lds)
   WH_DIRAC_DISPATCH(d.P, BODY)
 #undef BODY
 }
@@ -527,7 +535,7 @@ template <typename S, typename T>
 __global__ void __launch_bounds__(kDiracXBlock) dirac_pleft_kernel(const DiracPlan* __restrict__ dp) {
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
   const DiracPlan& d = *dp;
-#define BODY(n) dirac_pleft_body<S, T, n, dirac_rv<n>(), kDiracXBlock>(d, int(blockIdx.x), lds)
+#define BODY(n) dirac_pleft_body<S, T, n, dirac_rv<n>(), kDiracXBlock>(d, dirac_col_tile(), lds)
   WH_DIRAC_DISPATCH(d.P, BODY)
 #undef BODY
 }
@@ -536,7 +544,7 @@ template <typename S, typename T, bool CL>
 __global__ void __launch_bounds__(kDiracXBlock) dirac_pright_kernel(const DiracPlan* __restrict__ dp) {
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
   const DiracPlan& d = *dp;
-#define BODY(n) dirac_pright_body<S, T, n, dirac_rv<n>(), kDiracXBlock, CL>(d, int(blockIdx.x), lds)
+#define BODY(n) dirac_pright_body<S, T, n, dirac_rv<n>(), kDiracXBlock, CL>(d, dirac_col_tile(), lds)
   WH_DIRAC_DISPATCH(d.P, BODY)
 #undef BODY
 }
