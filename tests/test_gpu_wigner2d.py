@@ -217,3 +217,31 @@ def test_lie_odm_fp32(cuda):
     err = rel_l2(W, W_ref)
     print(f"odm fp32 rel L2 {err:.3e}")
     assert err < 1e-5
+
+
+def test_single_sweeps_match_oracle_factors(cuda):
+    """wh_wigner2d_kinetic / wh_wigner2d_potential (caller-composed loops): one K factor and
+    one V factor at time index 7 of a driven potential V = 0.5 x^2 + 0.3 sin(t) x^3, against
+    the oracle's kinetic_step / potential_step (FFT.py:131-158, 80-126)."""
+    import torch
+
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    X = P = 256
+    dt = 0.01
+    vf = lambda t, x: 0.5 * x * x + 0.3 * np.sin(t) * x ** 3  # noqa: E731
+    hip = Wigner2D(X, P, 10.0, 10.0, 1.0, 1.0, dt, potential="0.5*pow(x,2) + 0.3*sin(t)*pow(x,3)",
+                   dpotential="x + 0.9*sin(t)*pow(x,2)", mask=True, device=cuda, time_rows=32)
+    orc = ow.Wigner2DOracle(X, P, 10.0, 10.0, 1.0, 1.0, dt, potential=vf, mask=True, time_dependent=True)
+    W0 = ow.gaussian_wigner(orc.g, x0=1.0, p0=0.5, s=1.0).astype(np.complex128)
+    W = hip.to_device(W0)
+    hip.kinetic(W)
+    torch.cuda.synchronize()
+    K_ref = orc.kinetic_step(W0)
+    assert rel_l2(W.cpu().numpy(), K_ref) < 1e-12
+    hip.potential(W, step=7)
+    torch.cuda.synchronize()
+    V_ref = orc.potential_step(K_ref, 7 * dt)
+    err = rel_l2(W.cpu().numpy(), V_ref)
+    print(f"single sweeps: rel L2 {err:.3e}")
+    assert err < 1e-12
