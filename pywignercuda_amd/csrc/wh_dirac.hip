@@ -645,41 +645,46 @@ __global__ void __launch_bounds__(kRedThreads) dirac_reduce_kernel(const DiracPl
     s[kDoTrans] += x > d.transmission_x ? tr[0] : 0.0;
     s[kDoNorm] += tr[0];
   }
-  for (int q = 0; q < kRedQ; ++q) red[threadIdx.x][q] = s[q];
-  __syncthreads();
-  for (int h = kRedThreads / 2; h > 0; h >>= 1) {
-    if (int(threadIdx.x) < h)
-      for (int q = 0; q < kRedQ; ++q) red[threadIdx.x][q] += red[threadIdx.x + h][q];
-    __syncthreads();
-  }
   // antiparticle population from the row contributions of the loaded state (normalised)
-  double anti_sum = 0.0;
+  double a = 0.0;
   if (anti && d.antiparticle) {
-    double a = 0.0;
     for (int r = threadIdx.x; r < d.P; r += kRedThreads) a += gmem(d.rowpart)[r] + gmem(d.rowpart)[d.P + r];
-    red[threadIdx.x][kRedQ] = a;
-    __syncthreads();
-    for (int h = kRedThreads / 2; h > 0; h >>= 1) {
-      if (int(threadIdx.x) < h) red[threadIdx.x][kRedQ] += red[threadIdx.x + h][kRedQ];
-      __syncthreads();
-    }
-    anti_sum = red[0][kRedQ];
   }
+  // per-wave sums (fixed xor trees), then the waves in order: red[0][q] holds the totals
+  constexpr int kWaves = kRedThreads / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kRedQ; ++q) s[q] = wave_sum(s[q]);
+  a = wave_sum(a);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < kRedQ; ++q) red[wave][q] = s[q];
+    red[wave][kRedQ] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x <= kRedQ) {
+    double t = 0.0;
+    for (int w = 0; w < kWaves; ++w) t += red[w][threadIdx.x];
+    red[kWaves][threadIdx.x] = t;
+  }
+  __syncthreads();
+  const double* tot = red[kWaves];
+  const double anti_sum = tot[kRedQ];
   __shared__ double fshare;
-  if (threadIdx.x == 0) fshare = renorm ? 1.0 / (d.dxdp * red[0][kDoNorm]) : 1.0;
+  if (threadIdx.x == 0) fshare = renorm ? 1.0 / (d.dxdp * tot[kDoNorm]) : 1.0;
   __syncthreads();
   if (d.gp_coeff != 0.0) {  // density of the (normalised) state for the next Gross-Pitaevskii phase
     for (int col = threadIdx.x; col < d.X; col += kRedThreads)
       gmem(d.rho)[col] = fshare * gmem(d.colsums)[size_t(col) * kDNCol + kDcTrRe];
   }
   if (threadIdx.x == 0) {
-    const double raw = d.dxdp * red[0][kDoNorm];
+    const double raw = d.dxdp * tot[kDoNorm];
     const double f = fshare;
     double* o = d.obs + size_t(tstate - d.obs_base) * kDNObs;
-    for (int q = 0; q < kDoNeg; ++q) o[q] = f * d.dxdp * red[0][q];
+    for (int q = 0; q < kDoNeg; ++q) o[q] = f * d.dxdp * tot[q];
     if (!d.energy) o[kDoEnergy] = o[kDoEnergy + 1] = 0.0;
-    o[kDoNeg] = f * d.dxdp * red[0][kDoNeg];
-    o[kDoTrans] = f * d.dxdp * red[0][kDoTrans];
+    o[kDoNeg] = f * d.dxdp * tot[kDoNeg];
+    o[kDoTrans] = f * d.dxdp * tot[kDoTrans];
     o[kDoNorm] = f * raw;
     o[kDoRawNorm] = raw;
     if (anti && d.antiparticle) d.obs[size_t(step - d.obs_base) * kDNObs + kDoAnti] = d.dxdp * anti_sum;
