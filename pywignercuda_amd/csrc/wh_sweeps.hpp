@@ -197,7 +197,7 @@ __device__ __forceinline__ void write_record(const Member& m, const double (&r)[
 // index to rec_step.  Every thread of the block calls it; scr: C * 12 doubles of LDS.
 template <int BLOCK, int C>
 __device__ __forceinline__ void write_tile_partial(const SweepArgs& a, const Member& m, int tile,
-                                                   const double (&s)[kNCol], double* scr) {
+                                                   const double (&s)[kNCol], double* scr, int step) {
   static_assert(C <= BLOCK, "one thread per column");
   const int tid = threadIdx.x;
   if (tid < C) {
@@ -212,7 +212,7 @@ __device__ __forceinline__ void write_tile_partial(const SweepArgs& a, const Mem
     for (int c = 0; c < C; ++c) t += scr[c * 12 + tid];
     gmem(m.tilepart)[size_t(tid) * m.X + tile] = t;
   } else if (tid == 12 && tile == 0 && &m == a.members) {
-    *gmem(a.rec_step) = *gmem(a.step);
+    *gmem(a.rec_step) = step;  // the step index this sweep read at its start
   }
 }
 
@@ -230,6 +230,7 @@ __device__ __forceinline__ cplx<double> tile_scale(const SweepArgs& a, const Mem
   const int ntiles = m.X / pc;
   const bool rec = tile == 0;
   const int nk = rec ? 12 : 2;
+  const int rstep = rec ? *gmem(a.rec_step) : 0;  // issued with the partial loads
   double r[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) r[k] = 0.0;
@@ -263,9 +264,8 @@ __device__ __forceinline__ cplx<double> tile_scale(const SweepArgs& a, const Mem
     f = {nr / m2, -ni / m2};
   }
   if (rec && tid == 0) {
-    const int step = *gmem(a.rec_step);
-    write_record(m, r, a.fin_renorm, a.fin_record, step);
-    if (&m == a.members) *gmem(a.step) = step + 1;
+    write_record(m, r, a.fin_renorm, a.fin_record, rstep);
+    if (&m == a.members) *gmem(a.step) = rstep + 1;
   }
   return f;
 }
@@ -610,7 +610,8 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
   for (int j = 0; j < R; ++j) store_stream(at_off(W + size_t(j * TT) * X, boff), to_arith<S>(v[j]));
   if (a.flags & kTilePart)
     write_tile_partial<BLOCK, C>(a, m, tile, s,
-                                 reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + lds_elems<BLOCK, R>() * sizeof(cplx<T>)));
+                                 reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + lds_elems<BLOCK, R>() * sizeof(cplx<T>)),
+                                 step - a.step_add);
 }
 
 // ------------------------------------------------------------------ Gross-Pitaevskii shift
