@@ -476,10 +476,14 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
   const View view_c{kWave ? lds + cc * LSZ : lds + cl};
   const int step = (a.flags & kStepArg) ? a.step_add : *gmem(a.step) + a.step_add;
 
+  // 32-bit byte offsets from the member's base (a state is at most 2048^2 x 16 B); the
+  // store recomputes them from an opaque copy, so that no address stays live (or is
+  // spilled) across the transforms
   const unsigned boff = (unsigned(tl) * unsigned(X) + unsigned(tile * C + cl)) * unsigned(sizeof(cplx<S>));
+  const unsigned jstride = unsigned(TT) * unsigned(X) * unsigned(sizeof(cplx<S>));
   cplx<T> v[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = to_arith<T>(gload(at_off(W + size_t(j * TT) * X, boff)));
+  for (int j = 0; j < R; ++j) v[j] = to_arith<T>(gload(at_off(W, boff + unsigned(j) * jstride)));
   const auto twsrc = [&]() {
     if constexpr (kWave) {
       TwRegs<T, N, R> r;
@@ -606,8 +610,9 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = *view_l.ptr(padr<R>(tl + j * TT));
   }
+  const unsigned soff = unsigned(opaque_v(int(boff)));
 #pragma unroll
-  for (int j = 0; j < R; ++j) store_stream(at_off(W + size_t(j * TT) * X, boff), to_arith<S>(v[j]));
+  for (int j = 0; j < R; ++j) store_stream(at_off(W, soff + unsigned(j) * jstride), to_arith<S>(v[j]));
   if (a.flags & kTilePart)
     write_tile_partial<BLOCK, C>(a, m, tile, s,
                                  reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + lds_elems<BLOCK, R>() * sizeof(cplx<T>)),
