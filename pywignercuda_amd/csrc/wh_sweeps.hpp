@@ -301,13 +301,14 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
       return TwMem<T>{tw};
     }
   }();
+  // The pending factor and the 1/N of the inverse transform ride on the kinetic
+  // multiplier (the transforms are linear): one complex product per lane instead of 2R.
+  cplx<double> g{1.0 / N, 0.0};
   if (a.flags & kTileScale) {
     const cplx<double> f = tile_scale<BLOCK>(a, m, tile, reinterpret_cast<double*>(lds + lds_elems<BLOCK, R>()));
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = apply(v[j], f);
+    g = {f.x * (1.0 / N), f.y * (1.0 / N)};
   } else if (a.flags & kScaleOnLoad) {
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = scale_load(v[j], m.scale);
+    g = {cmem(m.scale)[0] * (1.0 / N), cmem(m.scale)[1] * (1.0 / N)};
   }
   team_fft<T, N, R, -1, LdsRow<T>, Sync>(v, t, view, twsrc);
 
@@ -315,7 +316,7 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
   const double sgn = (a.flags & kConj) ? -1.0 : 1.0;  // adjoint: conj(r e^{-i phi}) = r e^{+i phi}
   if (a.kin_linear) {
     const double ak = sgn * a.dt * m.dlambda * (a.k_c0 + a.k_c1 * p);
-    const cplx<double> Et = expmi(1.0, ak * double(t));
+    const cplx<double> Et = cmul(expmi(1.0, ak * double(t)), g);
     const cplx<double> Z1 = expmi(1.0, ak * double(TT));
     cplx<double> Zp[R / 2 + 1];  // Z^0 .. Z^(R/2)
     Zp[0] = {1.0, 0.0};
@@ -336,16 +337,12 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
       const double lam = m.dlambda * double(fidx(mm, N));
       const double phi = sgn * a.dt * (poly(a.kcoef, a.kdeg, p + hh * lam) - poly(a.kcoef, a.kdeg, p - hh * lam)) * a.inv_hbar;
       const double r = m.lambda_decay ? gmem(m.lambda_decay)[mm] : 1.0;
-      v[j] = apply(v[j], expmi(r, phi));
+      v[j] = apply(v[j], cmul(expmi(r, phi), g));
     }
   }
   team_fft<T, N, R, +1, LdsRow<T>, Sync>(v, t, view, twsrc);
-  const T inv = T(1.0 / N);
 #pragma unroll
-  for (int j = 0; j < R; ++j) {
-    v[j] = cscale(v[j], inv);
-    store_stream(W + t + j * TT, to_arith<S>(v[j]));
-  }
+  for (int j = 0; j < R; ++j) store_stream(W + t + j * TT, to_arith<S>(v[j]));
 
   if (a.flags & kColPart) {
     // partial column sums of Re W over this tile's rows (Gross-Pitaevskii density after K)
@@ -374,16 +371,18 @@ struct PSweep {
   using View = std::conditional_t<kWave, LdsRow<T>, LdsStrided<T, C>>;
   using Sync = std::conditional_t<kWave, WaveSync, BlockSync>;
 
-  // potential step multiplier of time index `step` (expPotential; GP phase optional)
+  // potential step multiplier of time index `step` (expPotential; GP phase optional),
+  // times the real factor `scale` (the 1/N of the inverse transform that follows)
   __device__ __forceinline__ static void potential(cplx<T> (&v)[R], const SweepArgs& a,
-                                                   const Member& m, int t_in, int col_in, int step) {
+                                                   const Member& m, int t_in, int col_in, int step,
+                                                   double scale) {
     // opaque lane indices: recomputing theta and x -/+ hbar theta/2 per call is cheaper
     // than keeping them live across the transforms between two calls
     const int t = opaque_v(t_in), col = opaque_v(col_in);
     const double x = m.dx * double(fidx(col, m.X));
     const int row = step < m.pot_rows ? step : m.pot_rows - 1;
     const cptr<double> cf = cmem(m.pot) + size_t(row) * (kMaxDeg + 1);
-    const double mask = m.xmask ? gmem(m.xmask)[col] : 1.0;
+    const double mask = (m.xmask ? gmem(m.xmask)[col] : 1.0) * scale;
     const bool gp = (a.flags & kGP) != 0;
     const BoppOdd bopp = bopp_odd(cf, m.pot_deg, x, a.vphase);
     // uniform conditions outside the element loops, so that each loop is one basic block
@@ -433,7 +432,8 @@ struct PSweep {
     }
   }
 
-  // D(Y) = F_p^{-1}[ theta * F_p[ sigma(p) Y ] ]   (Theta_fp_Damping, FFT.py:1135-1143)
+  // D(Y) = F_p^{-1}[ theta * F_p[ sigma(p) Y ] ]   (Theta_fp_Damping, FFT.py:1135-1143);
+  // the 1/N of the inverse transform is folded into theta (a power of two: exact)
   template <class Tw>
   __device__ __forceinline__ static void cl_operator(cplx<T> (&y)[R], const Member& m, int t,
                                                      View view, const Tw& tw) {
@@ -441,11 +441,8 @@ struct PSweep {
     for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(gmem(m.sigma_p)[t + j * TT]));
     team_fft<T, N, R, -1, View, Sync>(y, t, view, tw);
 #pragma unroll
-    for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(gmem(m.theta_host)[t + j * TT]));
+    for (int j = 0; j < R; ++j) y[j] = cscale(y[j], T(gmem(m.theta_host)[t + j * TT] * (1.0 / N)));
     team_fft<T, N, R, +1, View, Sync>(y, t, view, tw);
-    const T inv = T(1.0 / N);
-#pragma unroll
-    for (int j = 0; j < R; ++j) y[j] = cscale(y[j], inv);
   }
 };
 
@@ -497,16 +494,13 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = scale_load(v[j], m.scale);
   }
-  const T inv = T(1.0 / N);
   if constexpr (kWave) {
     team_fft_remap<T, N, R, -1>(v, tl, LdsRow<T>{lds + cl * LSZ}, tc, view_c, twsrc);
   } else {
     team_fft<T, N, R, -1, View, Sync>(v, tc, view_c, twsrc);
   }
-  PS::potential(v, a, m, tc, col, step);
+  PS::potential(v, a, m, tc, col, step, 1.0 / N);
   team_fft<T, N, R, +1, View, Sync>(v, tc, view_c, twsrc);
-#pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = cscale(v[j], inv);
 
   if constexpr (CL) {
     // W + 2aD W + (10/3)a^2 D^2 W + (4/3)a^3 D^3 W, a = i dt gamma, built as
@@ -594,10 +588,8 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
 
   if (a.flags & kMerge) {
     team_fft<T, N, R, -1, View, Sync>(v, tc, view_c, twsrc);
-    PS::potential(v, a, m, tc, col, step + 1);
+    PS::potential(v, a, m, tc, col, step + 1, 1.0 / N);
     team_fft<T, N, R, +1, View, Sync>(v, tc, view_c, twsrc);
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = cscale(v[j], inv);
   }
 
   if constexpr (kWave) {
