@@ -282,7 +282,12 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
   constexpr int LSZ = Padded<N, R>::size;
   const int tid = threadIdx.x, team = tid / TT, t = tid % TT;
   const int row = tile * ROWS + team;
-  const gptr<cplx<S>> W = gmem(reinterpret_cast<cplx<S>*>(m.W)) + size_t(row) * N;
+  // uniform member base + 32-bit per-lane byte offset (a state is at most 2048^2 x 16 B);
+  // the store recomputes the offset from an opaque copy, so that no per-row address
+  // stays live across the transforms
+  const gptr<cplx<S>> W = gmem(reinterpret_cast<cplx<S>*>(m.W));
+  const unsigned roff = (unsigned(row) * unsigned(N) + unsigned(t)) * unsigned(sizeof(cplx<S>));
+  constexpr unsigned kJ = unsigned(TT * sizeof(cplx<S>));
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(m.twX_hi), reinterpret_cast<const cplx<T>*>(m.twX_lo)};
   LdsRow<T> view{lds + team * LSZ};
   // a row's team inside one wave: wave-local exchanges, twiddles loaded once into registers
@@ -291,7 +296,7 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
 
   cplx<T> v[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = to_arith<T>(gload(W + t + j * TT));
+  for (int j = 0; j < R; ++j) v[j] = to_arith<T>(gload(at_off(W, roff + unsigned(j) * kJ)));
   const auto twsrc = [&]() {
     if constexpr (kWave) {
       TwRegs<T, N, R> r;
@@ -341,8 +346,9 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
     }
   }
   team_fft<T, N, R, +1, LdsRow<T>, Sync>(v, t, view, twsrc);
+  const unsigned soff = unsigned(opaque_v(int(roff)));
 #pragma unroll
-  for (int j = 0; j < R; ++j) store_stream(W + t + j * TT, to_arith<S>(v[j]));
+  for (int j = 0; j < R; ++j) store_stream(at_off(W, soff + unsigned(j) * kJ), to_arith<S>(v[j]));
 
   if (a.flags & kColPart) {
     // partial column sums of Re W over this tile's rows (Gross-Pitaevskii density after K)
