@@ -181,19 +181,33 @@ struct Padded {
 };
 
 // LDS views of one team's working array.  ptr(pb) is the address of padded
-// index pb; S is the element stride between consecutive padded indices.
-template <typename T>
+// index pb; S is the element stride between consecutive padded indices.  The element
+// type E may be narrower than the arithmetic type: exchanged values are then rounded
+// to E (complex64 exchanges of an FP64 transform halve the LDS bytes).
+template <typename E>
 struct LdsRow {  // contiguous: a[i] at base[padr(i)]
+  using elem = E;
   static constexpr int S = 1;
-  cplx<T>* base;
-  __device__ __forceinline__ cplx<T>* ptr(int pb) const { return base + pb; }
+  cplx<E>* base;
+  __device__ __forceinline__ cplx<E>* ptr(int pb) const { return base + pb; }
 };
-template <typename T, int LS>
+template <typename E, int LS>
 struct LdsStrided {  // interleaved columns: a[i] at base[padr(i) * LS]
+  using elem = E;
   static constexpr int S = LS;
-  cplx<T>* base;
-  __device__ __forceinline__ cplx<T>* ptr(int pb) const { return base + pb * LS; }
+  cplx<E>* base;
+  __device__ __forceinline__ cplx<E>* ptr(int pb) const { return base + pb * LS; }
 };
+
+// complex conversion between arithmetic and LDS element types (identity when equal)
+template <typename D, typename T>
+__device__ __forceinline__ cplx<D> ccvt(cplx<T> v) {
+  if constexpr (std::is_same_v<D, T>) {
+    return v;
+  } else {
+    return {D(v.x), D(v.y)};
+  }
+}
 
 // Hides the value of an expression from the optimiser, so that per-lane
 // addresses and indices of one transform are not merged with (and kept live
@@ -346,16 +360,17 @@ struct PassIO {
     constexpr int Nsp = ipow(R, P - 1);          // span before the previous pass
     constexpr int nbp = R / rp;
     constexpr int S = Lds::S;
+    using E = typename Lds::elem;
     static_assert(Nsp % R == 0 || (Nsp == 1 && rp == R), "separable LDS offsets");
 #pragma unroll
     for (int mm = 0; mm < nbp; ++mm) {
       const int b = tl + mm * TT;
       const int k = b % Nsp;
-      cplx<T>* p = lds.ptr(padr<R>((b - k) * rp + k));
+      cplx<E>* p = lds.ptr(padr<R>((b - k) * rp + k));
 #pragma unroll
       for (int q = 0; q < rp; ++q) {
         constexpr int unit = Nsp == 1 ? 1 : Nsp / R * (R + 1);
-        p[S * q * unit] = v[mm * rp + q];
+        p[S * q * unit] = ccvt<E>(v[mm * rp + q]);
       }
     }
   }
@@ -375,10 +390,10 @@ struct PassIO {
     const int tstep = (tl % Ns) * (N / (Ns * r));
     static_for<0, nb>([&](auto MM) {
       constexpr int mm = decltype(MM)::value;
-      const cplx<T>* p = lds.ptr(padr<R>(tl + mm * TT));
+      const auto* p = lds.ptr(padr<R>(tl + mm * TT));
       cplx<T> x[r];
 #pragma unroll
-      for (int j = 0; j < r; ++j) x[j] = p[S * j * ((N / r) / R * (R + 1))];
+      for (int j = 0; j < r; ++j) x[j] = ccvt<T>(p[S * j * ((N / r) / R * (R + 1))]);
       static_for<1, r>([&](auto J) {
         constexpr int j = decltype(J)::value;
         x[j] = tw.template mul<DIR>(x[j], TwSlots<N, R>::off(P) + j - 1, (j * tstep) & (N - 1));

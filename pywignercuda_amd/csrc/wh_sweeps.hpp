@@ -275,22 +275,23 @@ __device__ __forceinline__ cplx<double> tile_scale(const SweepArgs& a, const Mem
 // exp(-i a m_s), a = dt*dlambda*(k_c0 + k_c1 p), m_s = t + js*TT with
 // js = j (j < R/2) or j - R; computed as E_t * Z^js, E_t = exp(-i a t),
 // Z = exp(-i a TT): two sincos per lane instead of R.
-template <typename S, typename T, int N, int R, int BLOCK>
+template <typename S, typename T, int N, int R, int BLOCK, bool TP, typename E>
 __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m, int tile,
-                                            cplx<T>* lds) {
+                                            cplx<E>* lds) {
   constexpr int TT = N / R, ROWS = BLOCK / TT;
   constexpr int LSZ = Padded<N, R>::size;
   const int tid = threadIdx.x, team = tid / TT, t = tid % TT;
   const int row = tile * ROWS + team;
   // uniform member base + 32-bit per-lane byte offset (a state is at most 2048^2 x 16 B);
-  // the store recomputes the offset from an opaque copy, so that no per-row address
-  // stays live across the transforms
+  // the store recomputes the offset from an opaque copy of the lane index, so that no
+  // per-row address stays live (or is spilled) across the transforms
   const gptr<cplx<S>> W = gmem(reinterpret_cast<cplx<S>*>(m.W));
   const unsigned roff = (unsigned(row) * unsigned(N) + unsigned(t)) * unsigned(sizeof(cplx<S>));
   constexpr unsigned kJ = unsigned(TT * sizeof(cplx<S>));
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(m.twX_hi), reinterpret_cast<const cplx<T>*>(m.twX_lo)};
-  LdsRow<T> view{lds + team * LSZ};
+  LdsRow<E> view{lds + team * LSZ};
   // a row's team inside one wave: wave-local exchanges, twiddles loaded once into registers
+  // (the throughput form reads them from the table)
   constexpr bool kWave = TT <= 64;
   using Sync = std::conditional_t<kWave, WaveSync, BlockSync>;
 
@@ -298,7 +299,7 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
 #pragma unroll
   for (int j = 0; j < R; ++j) v[j] = to_arith<T>(gload(at_off(W, roff + unsigned(j) * kJ)));
   const auto twsrc = [&]() {
-    if constexpr (kWave) {
+    if constexpr (kWave && !TP) {
       TwRegs<T, N, R> r;
       load_twiddles<T, N, R>(r, t, tw);
       return r;
@@ -315,7 +316,7 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
   } else if (a.flags & kScaleOnLoad) {
     g = {cmem(m.scale)[0] * (1.0 / N), cmem(m.scale)[1] * (1.0 / N)};
   }
-  team_fft<T, N, R, -1, LdsRow<T>, Sync>(v, t, view, twsrc);
+  team_fft<T, N, R, -1, LdsRow<E>, Sync>(v, t, view, twsrc);
 
   const double p = m.dp * double(fidx(row, m.P));
   const double sgn = (a.flags & kConj) ? -1.0 : 1.0;  // adjoint: conj(r e^{-i phi}) = r e^{+i phi}
@@ -345,8 +346,9 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
       v[j] = apply(v[j], cmul(expmi(r, phi), g));
     }
   }
-  team_fft<T, N, R, +1, LdsRow<T>, Sync>(v, t, view, twsrc);
-  const unsigned soff = unsigned(opaque_v(int(roff)));
+  team_fft<T, N, R, +1, LdsRow<E>, Sync>(v, t, view, twsrc);
+  const int ts = opaque_v(int(threadIdx.x));  // offset recomputed from the lane index
+  const unsigned soff = (unsigned(tile * ROWS + ts / TT) * unsigned(N) + unsigned(ts % TT)) * unsigned(sizeof(cplx<S>));
 #pragma unroll
   for (int j = 0; j < R; ++j) store_stream(at_off(W, soff + unsigned(j) * kJ), to_arith<S>(v[j]));
 
@@ -366,7 +368,7 @@ __device__ __forceinline__ void xsweep_body(const SweepArgs& a, const Member& m,
 }
 
 // ------------------------------------------------------------------ p sweep
-template <typename T, int N, int R, int BLOCK>
+template <typename T, int N, int R, int BLOCK, typename E = T>
 struct PSweep {
   static constexpr int TT = N / R;
   static constexpr int C = BLOCK / TT;  // columns per tile
@@ -374,7 +376,7 @@ struct PSweep {
   // wave teams: a column's team lies inside one wave and the transform has an exchange
   // at which the lanes can leave the load layout (then every later exchange is wave-local)
   static constexpr bool kWave = TT <= 64 && num_passes(N, R) >= 2;
-  using View = std::conditional_t<kWave, LdsRow<T>, LdsStrided<T, C>>;
+  using View = std::conditional_t<kWave, LdsRow<E>, LdsStrided<E, C>>;
   using Sync = std::conditional_t<kWave, WaveSync, BlockSync>;
 
   // potential step multiplier of time index `step` (expPotential; GP phase optional),
@@ -461,10 +463,10 @@ constexpr int kPTailBytes = 64 * 12 * 8;
 // teams the first exchange of the first transform moves each column to one team of
 // TT consecutive lanes (column cc = tid / TT, team index tc = tid % TT) and the store
 // moves it back through LDS; otherwise the load layout is kept throughout.
-template <typename S, typename T, int N, int R, int BLOCK, bool CL>
+template <typename S, typename T, int N, int R, int BLOCK, bool CL, bool TP, typename E>
 __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m, int tile,
-                                            cplx<T>* lds) {
-  using PS = PSweep<T, N, R, BLOCK>;
+                                            cplx<E>* lds) {
+  using PS = PSweep<T, N, R, BLOCK, E>;
   using View = typename PS::View;
   using Sync = typename PS::Sync;
   constexpr int TT = PS::TT, C = PS::C, LSZ = PS::LSZ;
@@ -480,15 +482,15 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
   const int step = (a.flags & kStepArg) ? a.step_add : *gmem(a.step) + a.step_add;
 
   // 32-bit byte offsets from the member's base (a state is at most 2048^2 x 16 B); the
-  // store recomputes them from an opaque copy, so that no address stays live (or is
-  // spilled) across the transforms
+  // store recomputes them from an opaque copy of the lane index, so that no address
+  // stays live (or is spilled) across the transforms
   const unsigned boff = (unsigned(tl) * unsigned(X) + unsigned(tile * C + cl)) * unsigned(sizeof(cplx<S>));
   const unsigned jstride = unsigned(TT) * unsigned(X) * unsigned(sizeof(cplx<S>));
   cplx<T> v[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) v[j] = to_arith<T>(gload(at_off(W, boff + unsigned(j) * jstride)));
   const auto twsrc = [&]() {
-    if constexpr (kWave) {
+    if constexpr (kWave && !TP) {
       TwRegs<T, N, R> r;
       load_twiddles<T, N, R>(r, tc, tw);
       return r;
@@ -501,7 +503,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
     for (int j = 0; j < R; ++j) v[j] = scale_load(v[j], m.scale);
   }
   if constexpr (kWave) {
-    team_fft_remap<T, N, R, -1>(v, tl, LdsRow<T>{lds + cl * LSZ}, tc, view_c, twsrc);
+    team_fft_remap<T, N, R, -1>(v, tl, LdsRow<E>{lds + cl * LSZ}, tc, view_c, twsrc);
   } else {
     team_fft<T, N, R, -1, View, Sync>(v, tc, view_c, twsrc);
   }
@@ -550,7 +552,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
 #pragma unroll
         for (int k = 0; k < kNCol; ++k) acc[k] += __shfl_xor(acc[k], off, 64);
       }
-      double* csum = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + lds_elems<BLOCK, R>() * sizeof(cplx<T>));
+      double* csum = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + lds_elems<BLOCK, R>() * sizeof(cplx<E>));
       if (tc == 0) {
 #pragma unroll
         for (int k = 0; k < kNCol; ++k) csum[cc * kNCol + k] = acc[k];
@@ -602,18 +604,19 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
     // back to the load layout: natural order through the team's LDS array
     Sync::sync();
 #pragma unroll
-    for (int j = 0; j < R; ++j) *view_c.ptr(padr<R>(tc + j * TT)) = v[j];
+    for (int j = 0; j < R; ++j) *view_c.ptr(padr<R>(tc + j * TT)) = ccvt<E>(v[j]);
     __syncthreads();
-    const LdsRow<T> view_l{lds + cl * LSZ};
+    const LdsRow<E> view_l{lds + cl * LSZ};
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = *view_l.ptr(padr<R>(tl + j * TT));
+    for (int j = 0; j < R; ++j) v[j] = ccvt<T>(*view_l.ptr(padr<R>(tl + j * TT)));
   }
-  const unsigned soff = unsigned(opaque_v(int(boff)));
+  const int ts = opaque_v(int(threadIdx.x));  // offset recomputed from the lane index
+  const unsigned soff = (unsigned(ts / C) * unsigned(X) + unsigned(tile * C + ts % C)) * unsigned(sizeof(cplx<S>));
 #pragma unroll
   for (int j = 0; j < R; ++j) store_stream(at_off(W, soff + unsigned(j) * jstride), to_arith<S>(v[j]));
   if (a.flags & kTilePart)
     write_tile_partial<BLOCK, C>(a, m, tile, s,
-                                 reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + lds_elems<BLOCK, R>() * sizeof(cplx<T>)),
+                                 reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + lds_elems<BLOCK, R>() * sizeof(cplx<E>)),
                                  step - a.step_add);
 }
 
@@ -655,32 +658,37 @@ __device__ __forceinline__ void gpshift_body(const Member& m, int tile, cplx<dou
   }
 
 // Two waves per SIMD (<= 256 VGPRs) for the plain sweeps: a batch of grids overlaps one
-// wave's loads and stores with the other's transforms.  The Caldeira-Leggett p sweep
+// wave's loads and stores with the other's transforms; three for the throughput form
+// (TP: table twiddles, exchanges in the storage precision).  The Caldeira-Leggett p sweep
 // holds three state copies and keeps the full register file of one wave.
-template <typename S, typename T, int R, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) xsweep_kernel(SweepArgs a) {
-  __shared__ cplx<T> lds[lds_elems<BLOCK, R>() + kTileRedBytes / sizeof(cplx<T>)];
+template <typename S, typename T, bool TP>
+using LdsElem = std::conditional_t<TP, S, T>;
+template <typename S, typename T, int R, int BLOCK, bool TP>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TP ? 3 : 2))) xsweep_kernel(SweepArgs a) {
+  using E = LdsElem<S, T, TP>;
+  __shared__ cplx<E> lds[lds_elems<BLOCK, R>() + kTileRedBytes / sizeof(cplx<E>)];
   const unsigned w = a.work[blockIdx.x];
   const Member& m = a.members[w >> 16];
   const int tile = int(w & 0xffffu);
-#define BODY(n) xsweep_body<S, T, n, R, BLOCK>(a, m, tile, lds)
+#define BODY(n) xsweep_body<S, T, n, R, BLOCK, TP, E>(a, m, tile, lds)
   WH_DISPATCH(m.X, BODY)
 #undef BODY
 }
 
-template <typename S, typename T, int R, int BLOCK, bool CL>
+template <typename S, typename T, int R, int BLOCK, bool CL, bool TP>
 __device__ __forceinline__ void psweep_entry(const SweepArgs& a) {
-  __shared__ cplx<T> lds[lds_elems<BLOCK, R>() + kPTailBytes / sizeof(cplx<T>)];
+  using E = LdsElem<S, T, TP>;
+  __shared__ cplx<E> lds[lds_elems<BLOCK, R>() + kPTailBytes / sizeof(cplx<E>)];
   const unsigned w = a.work[blockIdx.x];
   const Member& m = a.members[w >> 16];
   const int tile = int(w & 0xffffu);
-#define BODY(n) psweep_body<S, T, n, R, BLOCK, CL>(a, m, tile, lds)
+#define BODY(n) psweep_body<S, T, n, R, BLOCK, CL, TP, E>(a, m, tile, lds)
   WH_DISPATCH(m.P, BODY)
 #undef BODY
 }
-template <typename S, typename T, int R, int BLOCK, bool CL>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(CL ? 1 : 2))) psweep_kernel(SweepArgs a) {
-  psweep_entry<S, T, R, BLOCK, CL>(a);
+template <typename S, typename T, int R, int BLOCK, bool CL, bool TP>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(CL ? 1 : TP ? 3 : 2))) psweep_kernel(SweepArgs a) {
+  psweep_entry<S, T, R, BLOCK, CL, TP>(a);
 }
 
 template <int R, int BLOCK>

@@ -5,14 +5,18 @@
 
 namespace wh {
 
+// tp: throughput form for batches of grids: twiddles read from the table instead of
+// held in registers and the exchanges between transform passes held in the state's
+// storage precision (complex64 for the complex64-state FP64-arithmetic form), so that
+// three waves fit per SIMD (<= 168 VGPRs, ~36-41 KiB of LDS per block)
 struct SweepVariant {
-  int R, block;
+  int R, block, tp;
 };
 
 // index 0 is the default
-constexpr SweepVariant kXVariants[] = {{16, 256}, {8, 256}, {8, 512}, {4, 256}};
-constexpr SweepVariant kPVariants[] = {{16, 256}, {8, 512}, {8, 256}};
-constexpr SweepVariant kPCLVariants[] = {{16, 256}, {8, 256}};
+constexpr SweepVariant kXVariants[] = {{16, 256, 0}, {8, 256, 0}, {8, 512, 0}, {4, 256, 0}, {16, 256, 1}};
+constexpr SweepVariant kPVariants[] = {{16, 256, 0}, {8, 512, 0}, {8, 256, 0}, {16, 256, 1}};
+constexpr SweepVariant kPCLVariants[] = {{16, 256, 0}, {8, 256, 0}};
 constexpr int kNumX = sizeof(kXVariants) / sizeof(kXVariants[0]);
 constexpr int kNumP = sizeof(kPVariants) / sizeof(kPVariants[0]);
 constexpr int kNumPCL = sizeof(kPCLVariants) / sizeof(kPCLVariants[0]);
