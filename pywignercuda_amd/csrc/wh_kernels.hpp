@@ -32,7 +32,7 @@ enum SweepFlags {
   kGP = 16,          // p sweep: add the Gross-Pitaevskii phase from the member's G grid
   kColPart = 32,     // x sweep: partial column sums (Gross-Pitaevskii density after K)
   kConj = 64,        // x sweep: conjugated kinetic multiplier (adjoint step)
-  kTilePart = 128,   // p sweep (with kColSums): each tile writes its 12 observable partial sums
+  kTilePart = 128,   // p sweep (requires kColSums): each tile writes its 12 observable partial sums
   kTileScale = 256,  // x sweep: the pending factor from the previous p sweep's tile partials;
                      // the member's tile 0 writes the record row and the factor (replaces
                      // the reduce launch between the sweeps)

@@ -138,6 +138,31 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// v from another lane by a DPP control (row = 16 lanes); lanes of rows outside ROWMASK get 0
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(unsigned(b)), CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(unsigned(b >> 32)), CTRL, ROWMASK, 0xF, false);
+  return __longlong_as_double((static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo));
+}
+// Sum over aligned groups of G lanes (G = 8, 16, 32 or 64) by VALU lane moves (no LDS
+// traffic, fixed order: deterministic); the last lane of each group holds the sum.
+//   quad_perm [1,0,3,2], [2,3,0,1]: quad sums;  row_half_mirror: sums of 8;  row_mirror:
+//   sums of 16;  row_bcast15 into rows 1, 3: sums of 32 there;  row_bcast31 into rows 2,
+//   3: the sum of 64 in row 3.
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+  static_assert(G == 8 || G == 16 || G == 32 || G == 64, "group of 8..64 lanes");
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  if constexpr (G >= 16) v += dpp_f64<0x140>(v);
+  if constexpr (G >= 32) v += dpp_f64<0x142, 0xA>(v);
+  if constexpr (G >= 64) v += dpp_f64<0x143, 0xC>(v);
+  return v;
+}
+
 // Observable terms of one column from its sums s = (Re S0, Im S0, sum p W, sum p^2 W,
 // sum min(W, 0)): the integrands of <x>, <x^2>, <p>, <p^2>, <xp>, <dV/dx>, <p dV/dx>,
 // <x dV/dx>, energy, negative area (the per-step record of GPU_Wigner2D_FFT.py:1856-1876).
@@ -191,25 +216,25 @@ __device__ __forceinline__ void write_record(const Member& m, const double (&r)[
   }
 }
 
-// kTilePart epilogue of a p-sweep tile: threads tid < C hold the sums s of column
-// tile*C + tid; the 12 observable partials of the tile (sums over its C columns in
-// column order) go to m.tilepart[k * X + tile].  Tile 0 of member 0 copies the step
-// index to rec_step.  Every thread of the block calls it; scr: C * 12 doubles of LDS.
+// kTilePart epilogue of a p-sweep tile: csum holds the kNCol sums of each of the tile's
+// C columns (LDS, visible to every thread: written before a barrier); thread k < 12 forms
+// observable term k of each column and sums them in column order into
+// m.tilepart[k * X + tile].  Tile 0 of member 0 copies the step index to rec_step.
 template <int BLOCK, int C>
 __device__ __forceinline__ void write_tile_partial(const SweepArgs& a, const Member& m, int tile,
-                                                   const double (&s)[kNCol], double* scr, int step) {
-  static_assert(C <= BLOCK, "one thread per column");
+                                                   const double* csum, int step) {
   const int tid = threadIdx.x;
-  if (tid < C) {
-    double o[12];
-    column_terms(m, tile * C + tid, s, o);
-#pragma unroll
-    for (int k = 0; k < 12; ++k) scr[tid * 12 + k] = o[k];
-  }
-  __syncthreads();
   if (tid < 12) {
     double t = 0.0;
-    for (int c = 0; c < C; ++c) t += scr[c * 12 + tid];
+    for (int c = 0; c < C; ++c) {
+      double s[kNCol], o[12];
+#pragma unroll
+      for (int k = 0; k < kNCol; ++k) s[k] = csum[c * kNCol + k];
+      column_terms(m, tile * C + c, s, o);
+#pragma unroll
+      for (int k = 0; k < 12; ++k)
+        if (k == tid) t += o[k];
+    }
     gmem(m.tilepart)[size_t(tid) * m.X + tile] = t;
   } else if (tid == 12 && tile == 0 && &m == a.members) {
     *gmem(a.rec_step) = step;  // the step index this sweep read at its start
@@ -241,8 +266,8 @@ __device__ __forceinline__ cplx<double> tile_scale(const SweepArgs& a, const Mem
   }
 #pragma unroll
   for (int k = 0; k < 12; ++k)
-    if (k < nk) r[k] = wave_sum(r[k]);
-  if (lane == 0) {
+    if (k < nk) r[k] = group_sum<64>(r[k]);
+  if (lane == 63) {
 #pragma unroll
     for (int k = 0; k < 12; ++k)
       if (k < nk) red[wave * 12 + k] = r[k];
@@ -532,7 +557,16 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
     for (int j = 0; j < R; ++j) v[j] = cadd(v[j], mul_i(d[j], c3));
   }
 
-  double s[kNCol] = {0.0, 0.0, 0.0, 0.0, 0.0};  // sums of column tile*C + tid (tid < C)
+  // per-column sums of the tile in the LDS tail: csum[c * kNCol + k] for column tile*C + c
+  double* csum = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + lds_elems<BLOCK, R>() * sizeof(cplx<E>));
+  // column sums to global memory, after a block barrier that follows their LDS write
+  const auto publish_colsums = [&]() {
+    if (tid < C) {
+#pragma unroll
+      for (int k = 0; k < kNCol; ++k) gmem(m.colsums)[size_t(tile * C + tid) * kNCol + k] = csum[tid * kNCol + k];
+    }
+  };
+  bool colsums_pending = false;
   if (a.flags & kColSums) {
     double acc[kNCol] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -546,24 +580,18 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
       acc[4] += w < 0.0 ? w : 0.0;
     }
     if constexpr (kWave) {
-      // a column's lanes are TT consecutive lanes of one wave
+      // a column's lanes are TT consecutive lanes of one wave; the last one gets the sums
 #pragma unroll
-      for (int off = 1; off < TT; off <<= 1) {
-#pragma unroll
-        for (int k = 0; k < kNCol; ++k) acc[k] += __shfl_xor(acc[k], off, 64);
-      }
-      double* csum = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + lds_elems<BLOCK, R>() * sizeof(cplx<E>));
-      if (tc == 0) {
+      for (int k = 0; k < kNCol; ++k) acc[k] = group_sum<TT>(acc[k]);
+      if (tc == TT - 1) {
 #pragma unroll
         for (int k = 0; k < kNCol; ++k) csum[cc * kNCol + k] = acc[k];
       }
-      __syncthreads();
-      if (tid < C) {
-#pragma unroll
-        for (int k = 0; k < kNCol; ++k) {
-          s[k] = csum[tid * kNCol + k];
-          gmem(m.colsums)[size_t(tile * C + tid) * kNCol + k] = s[k];
-        }
+      if (a.flags & kMerge) {
+        __syncthreads();
+        publish_colsums();
+      } else {
+        colsums_pending = true;  // published after the store's block barrier
       }
     } else {
       // lanes of one column inside a wave: c + C*t; fold t with xor shuffles
@@ -584,13 +612,18 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
       __syncthreads();
       if (tid < C) {
         // C <= 64: every wave holds all C columns
+        double sc[kNCol] = {0.0, 0.0, 0.0, 0.0, 0.0};
         for (int w = 0; w < kWaves; ++w) {
 #pragma unroll
-          for (int k = 0; k < kNCol; ++k) s[k] += red[(w * kLanes + tid) * kNCol + k];
+          for (int k = 0; k < kNCol; ++k) sc[k] += red[(w * kLanes + tid) * kNCol + k];
         }
 #pragma unroll
-        for (int k = 0; k < kNCol; ++k) gmem(m.colsums)[size_t(tile * C + tid) * kNCol + k] = s[k];
+        for (int k = 0; k < kNCol; ++k) {
+          gmem(m.colsums)[size_t(tile * C + tid) * kNCol + k] = sc[k];
+          csum[tid * kNCol + k] = sc[k];
+        }
       }
+      __syncthreads();  // csum complete for the tile partials; red free for the transforms
     }
   }
 
@@ -606,6 +639,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
 #pragma unroll
     for (int j = 0; j < R; ++j) *view_c.ptr(padr<R>(tc + j * TT)) = ccvt<E>(v[j]);
     __syncthreads();
+    if (colsums_pending) publish_colsums();
     const LdsRow<E> view_l{lds + cl * LSZ};
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = ccvt<T>(*view_l.ptr(padr<R>(tl + j * TT)));
@@ -614,10 +648,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
   const unsigned soff = (unsigned(ts / C) * unsigned(X) + unsigned(tile * C + ts % C)) * unsigned(sizeof(cplx<S>));
 #pragma unroll
   for (int j = 0; j < R; ++j) store_stream(at_off(W, soff + unsigned(j) * jstride), to_arith<S>(v[j]));
-  if (a.flags & kTilePart)
-    write_tile_partial<BLOCK, C>(a, m, tile, s,
-                                 reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + lds_elems<BLOCK, R>() * sizeof(cplx<E>)),
-                                 step - a.step_add);
+  if (a.flags & kTilePart) write_tile_partial<BLOCK, C>(a, m, tile, csum, step - a.step_add);
 }
 
 // ------------------------------------------------------------------ Gross-Pitaevskii shift
