@@ -312,8 +312,8 @@ def main():
                    "grid": [prop.P, prop.X], "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": ("psweep_kernel<double, double, 16, 256, true>" if a.workload == "damped"
-                                else "psweep_kernel<double, double, 16, 256, false>"), "kernel_us": t_p * 1e6,
+                     "kernel": ("psweep_kernel<double, double, 16, 256, true, false>" if a.workload == "damped"
+                                else "psweep_kernel<double, double, 16, 256, false, false>"), "kernel_us": t_p * 1e6,
                      "xsweep_us": t_x * 1e6,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
