@@ -256,7 +256,10 @@ int wh_ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_s
  * WH_INVALID_VALUE if a member's grid does not fit the variant.
  */
 int wh_ensemble_set_variants(wh_ensemble* e, int xvar, int pvar, int pclvar);
-/* Number of variants of each table and the (radix, lanes) of variant i. */
+/* Number of variants of each table and the (radix, lanes) of variant i.  The last
+   x and p variants are the batch throughput forms (same radix and lanes as variant 0;
+   twiddles from the table, the exchanges between transform passes held in the
+   state's storage precision). */
 int wh_variant_info(int table, int i, int* radix, int* lanes);
 /* Replaces member `member`'s potential coefficient rows ([rows][WH_MAX_DEG+1]
    doubles, host or device memory, copied on `stream`) -- e.g. between the runs of
