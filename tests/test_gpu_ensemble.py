@@ -101,3 +101,34 @@ def test_fp32_error_1000_steps(cuda):
         d = np.abs(oa[:, col] - ob[:, col]).max()
         print(f"  max |<{name}>_fp32 - <{name}>_fp64| = {d:.3e}")
         assert d < 1e-5, (name, d)
+
+
+def test_fp32_throughput_variants_error_1000_steps(cuda):
+    """The batch throughput sweeps (x variant 4, p variant 3: exchanges between transform
+    passes held in complex64) against the FP64 ensemble after 1000 unitary steps: the 1e-5
+    bound of the FP32 form, for the uneven sizes that select them by default."""
+    import torch
+
+    from pywignercuda_amd import _lib
+    from pywignercuda_amd.wigner2d import WignerEnsemble
+
+    L = _lib.lib()
+    xtp, ptp = L.wh_variant_info(0, -1, None, None) - 1, L.wh_variant_info(1, -1, None, None) - 1
+    members = [_member(256, 0.1, 1.0, 1.0, 1.0), _member(512, 0.05, 0.5, -2.0, 0.8),
+               _member(128, 0.2, 2.0, 0.5, 1.2)]
+    e64 = WignerEnsemble(members, 1.0, 1.0, 0.01, kinetic="0.5*p*p", precision="fp64", device=cuda)
+    e32 = WignerEnsemble(members, 1.0, 1.0, 0.01, kinetic="0.5*p*p", precision="fp32", device=cuda,
+                         variants=(xtp, ptp, 0))
+    W0 = []
+    for t, m in zip(e64.tables, members):
+        x, p = t.x_host[None, :], t.p_host[:, None]
+        W0.append(np.exp(-((x - m["x0"]) ** 2) / m["s"] ** 2 - m["s"] ** 2 * p ** 2) / np.pi)
+    A, B = e64.states_to_device(W0), e32.states_to_device(W0)
+    e64.run(A, 1000, use_graph=True)
+    e32.run(B, 1000, use_graph=True)
+    torch.cuda.synchronize()
+    for i in range(len(members)):
+        a, b = A[i].cpu().numpy(), B[i].cpu().numpy().astype(np.complex128)
+        err = np.linalg.norm(a - b) / np.linalg.norm(a)
+        print(f"member {i} ({members[i]['X']}^2): fp32 throughput-variant rel-L2 after 1000 steps {err:.3e}")
+        assert err < 1e-5, (i, err)
