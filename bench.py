@@ -19,6 +19,7 @@ usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import platform
@@ -280,7 +281,21 @@ def main():
     wall, obs = timed_run(prop, W, a.steps, a.warmup)
     cells = prop.X * prop.P
     value = cells * a.steps / wall
-    # dominant kernel: the fused p sweep (V step + CL friction + column sums)
+    # dominant kernel: the fused p sweep (V step + CL friction + column sums), in the variant
+    # the plan runs (tuned choice; the last variant of a table is its throughput form)
+    from pywignercuda_amd import _lib
+    from pywignercuda_amd.wigner2d import tuned_variants
+
+    cl = a.workload == "damped"
+    table = 2 if cl else 1
+    vi = tuned_variants("fp64", [prop.X], [prop.P], cl=cl)[table]
+    vi = max(vi, 0)
+    L = _lib.lib()
+    rr, bb = ctypes.c_int(), ctypes.c_int()
+    L.wh_variant_info(table, vi, ctypes.byref(rr), ctypes.byref(bb))
+    tp = vi == L.wh_variant_info(table, -1, None, None) - 1
+    kname = (f"psweep_kernel<double, double, {rr.value}, {bb.value}, {'true' if cl else 'false'}, "
+             f"{'true' if tp else 'false'}>")
     Wk = W.clone()
     t_p = kernel_time(lambda: prop.potential(Wk, 0, fused=True), a.kernel_reps)
     t_x = kernel_time(lambda: prop.kinetic(Wk), a.kernel_reps)
@@ -312,8 +327,7 @@ def main():
                    "grid": [prop.P, prop.X], "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": ("psweep_kernel<double, double, 16, 256, true, false>" if a.workload == "damped"
-                                else "psweep_kernel<double, double, 16, 256, false, false>"), "kernel_us": t_p * 1e6,
+                     "kernel": kname, "kernel_us": t_p * 1e6,
                      "xsweep_us": t_x * 1e6,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }

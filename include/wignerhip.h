@@ -257,7 +257,7 @@ int wh_ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_s
  */
 int wh_ensemble_set_variants(wh_ensemble* e, int xvar, int pvar, int pclvar);
 /* Number of variants of each table and the (radix, lanes) of variant i.  The last
-   x and p variants are the batch throughput forms (same radix and lanes as variant 0;
+   variant of each table is the throughput form (same radix and lanes as variant 0;
    twiddles from the table, the exchanges between transform passes held in the
    state's storage precision). */
 int wh_variant_info(int table, int i, int* radix, int* lanes);

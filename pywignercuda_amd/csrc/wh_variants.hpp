@@ -16,7 +16,7 @@ struct SweepVariant {
 // index 0 is the default
 constexpr SweepVariant kXVariants[] = {{16, 256, 0}, {8, 256, 0}, {8, 512, 0}, {4, 256, 0}, {16, 256, 1}};
 constexpr SweepVariant kPVariants[] = {{16, 256, 0}, {8, 512, 0}, {8, 256, 0}, {16, 256, 1}};
-constexpr SweepVariant kPCLVariants[] = {{16, 256, 0}, {8, 256, 0}};
+constexpr SweepVariant kPCLVariants[] = {{16, 256, 0}, {8, 256, 0}, {16, 256, 1}};
 constexpr int kNumX = sizeof(kXVariants) / sizeof(kXVariants[0]);
 constexpr int kNumP = sizeof(kPVariants) / sizeof(kPVariants[0]);
 constexpr int kNumPCL = sizeof(kPCLVariants) / sizeof(kPCLVariants[0]);

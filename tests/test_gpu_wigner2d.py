@@ -98,6 +98,34 @@ def test_lie_caldeira_leggett_fp64(cuda):
     check_obs(obs, obs_ref)
 
 
+def test_caldeira_leggett_every_variant_fp64(cuda):
+    """Every Caldeira-Leggett p-sweep variant (radix, lanes, table or register twiddles)
+    against the oracle, 256^2, 15 Lie steps."""
+    import torch
+
+    from pywignercuda_amd import _lib
+
+    fp_fn = lambda p: np.abs(p)  # noqa: E731
+    hip, orc = make_pair(cuda, amp=12.0, dt=0.005, D_theta=0.1, gamma=0.1,
+                         damping="CaldeiraLeggett", fp_str="fabs(p)", fp_fn=fp_fn)
+    W0 = ow.cat_state_wigner(orc.g, a=2.0)
+    W_ref, obs_ref, _ = orc.run(W0, 15)
+    n = _lib.lib().wh_variant_info(2, -1, None, None)
+    tried = 0
+    for v in range(n):
+        try:
+            hip.set_variants(-1, -1, v)
+        except _lib.WignerHipInvalidValue:
+            continue
+        W = hip.to_device(W0)
+        obs = hip.run(W, 16, use_graph=False).cpu().numpy()[:16]
+        torch.cuda.synchronize()
+        assert rel_l2(W.cpu().numpy(), W_ref) < 1e-10, v
+        check_obs(obs, obs_ref)
+        tried += 1
+    assert tried == n
+
+
 def test_strang_caldeira_leggett_fp64(cuda):
     fp_fn = lambda p: np.abs(p)  # noqa: E731
     hip, orc = make_pair(cuda, scheme="strang", amp=12.0, dt=0.005, D_theta=0.1, gamma=0.1,
