@@ -32,6 +32,12 @@ hipError_t launch_gpshift(const Member* members, const unsigned* work, int nwork
 hipError_t launch_colsum(int prec, const Member* members, const unsigned* work, int nwork, hipStream_t s);
 hipError_t launch_xsweep_f32w(int variant, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_psweep_f32w(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_xsweep_tp_f64(int N, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_xsweep_tp_f32(int N, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_xsweep_tp_f32w(int N, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_psweep_tp_f64(int N, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_psweep_tp_f32(int N, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_psweep_tp_f32w(int N, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_reduce(const ReduceArgs& a, int nmembers, int* done, hipStream_t s);
 hipError_t launch_colpart_reduce(const Member* members, int nmembers, int xradix, int xblock, const int* step,
                                  hipStream_t s);
@@ -300,6 +306,12 @@ struct GraphEntry {
   hipGraphExec_t exec = nullptr;
 };
 
+// run of work items whose members share the grid side N (work lists are grouped by size,
+// so that the per-size throughput kernels can be launched once per run)
+struct SizeSeg {
+  int N, off, n;
+};
+
 }  // namespace
 
 struct wh_ensemble {
@@ -311,6 +323,7 @@ struct wh_ensemble {
   std::map<int, std::unique_ptr<TwSet>> tw;  // by size
   int xvar = 0, pvar = 0, pclvar = 0;
   WorkList work_x, work_p, work_colsum, work_scale, work_gp, work_adj, work_odm_x, work_odm_p;
+  std::vector<SizeSeg> seg_x, seg_p;  // size runs of work_x (by X) and work_p (by P)
   DevBuf step, done, gcols;
   hipStream_t capture_stream = nullptr;
   std::map<std::tuple<hipStream_t, int>, GraphEntry> graphs;  // (stream, chunk)
@@ -376,12 +389,13 @@ int validate_common(const wh_ensemble_desc& d) {
   return WH_SUCCESS;
 }
 
-// variant v fits every member on the given axis (x sweep: rows of X; p sweep: columns of P)
-bool fits_all(const wh_ensemble* e, const wh::SweepVariant& v, bool x_axis) {
+// variant v fits every member on the given axis (x sweep: rows of X; p sweep: columns of P;
+// cl: a Caldeira-Leggett p variant)
+bool fits_all(const wh_ensemble* e, const wh::SweepVariant& v, bool x_axis, bool cl = false) {
   for (const auto& m : e->mem) {
     const int N = x_axis ? m.X : m.P;
     const int other = x_axis ? m.P : m.X;
-    if (!wh::variant_fits(N, v)) return false;
+    if (!wh::variant_fits(N, v, cl)) return false;
     if (other % wh::tile_lines(N, v) != 0) return false;
   }
   return true;
@@ -391,23 +405,39 @@ int build_work_lists(wh_ensemble* e) {
   const auto& xv = wh::kXVariants[e->xvar];
   const auto& pv = e->cl() ? wh::kPCLVariants[e->pclvar] : wh::kPVariants[e->pvar];
   std::vector<unsigned> wx, wp, wc, ws, wg, wox, wop;
+  // x and p items grouped by grid side (ascending), members in order within a size
+  e->seg_x.clear();
+  e->seg_p.clear();
+  for (int N = 1; N <= (1 << 20); N <<= 1) {
+    const size_t x0 = wx.size(), p0 = wp.size();
+    for (int i = 0; i < e->n; ++i) {
+      const auto& m = e->mem[i];
+      const unsigned hi = unsigned(i) << 16;
+      if (m.X == N) {
+        const int rows = wh::tile_lines(m.X, xv);
+        for (int t = 0; t < m.P / rows; ++t) wx.push_back(hi | unsigned(t));
+      }
+      if (m.P == N) {
+        // p tiles: consecutive tiles on one XCD (blocks are dealt round-robin to the 8 XCDs)
+        const int cols = wh::tile_lines(m.P, pv);
+        const int nt = m.X / cols;
+        for (int q = 0; q < nt; ++q) {
+          const int t = (nt % 8 == 0) ? (q % 8) * (nt / 8) + q / 8 : q;
+          wp.push_back(hi | unsigned(t));
+        }
+      }
+    }
+    if (wx.size() > x0) e->seg_x.push_back({N, int(x0), int(wx.size() - x0)});
+    if (wp.size() > p0) e->seg_p.push_back({N, int(p0), int(wp.size() - p0)});
+  }
   for (int i = 0; i < e->n; ++i) {
     const auto& m = e->mem[i];
     const unsigned hi = unsigned(i) << 16;
-    const int rows = wh::tile_lines(m.X, xv);
-    for (int t = 0; t < m.P / rows; ++t) wx.push_back(hi | unsigned(t));
-    // p tiles: consecutive tiles on one XCD (blocks are dealt round-robin to the 8 XCDs)
-    const int cols = wh::tile_lines(m.P, pv);
-    const int nt = m.X / cols;
-    for (int q = 0; q < nt; ++q) {
-      const int t = (nt % 8 == 0) ? (q % 8) * (nt / 8) + q / 8 : q;
-      wp.push_back(hi | unsigned(t));
-    }
     for (int t = 0; t < (m.X + 31) / 32; ++t) wc.push_back(hi | unsigned(t));
     const size_t cells = size_t(m.X) * size_t(m.P);
     for (size_t t = 0; t < (cells + 8191) / 8192; ++t) ws.push_back(hi | unsigned(t));
     if (e->gp()) {
-      const int grows = wh::tile_lines(m.X, wh::SweepVariant{16, 256});
+      const int grows = wh::tile_lines(m.X, wh::SweepVariant{16, 256, 0});
       for (int t = 0; t < m.P / grows; ++t) wg.push_back(hi | unsigned(t));
     }
     if (e->odm()) {
@@ -536,9 +566,9 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
     e->bufs.push_back(std::move(b));
   }
   // default variants: the first that fits every member
-  auto pick = [&](const wh::SweepVariant* t, int nt, bool x_axis, int& out_v) -> int {
+  auto pick = [&](const wh::SweepVariant* t, int nt, bool x_axis, int& out_v, bool cl = false) -> int {
     for (int v = 0; v < nt; ++v)
-      if (fits_all(e.get(), t[v], x_axis)) {
+      if (fits_all(e.get(), t[v], x_axis, cl)) {
         out_v = v;
         return WH_SUCCESS;
       }
@@ -546,9 +576,9 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
   };
   WH_TRY(pick(wh::kXVariants, wh::kNumX, true, e->xvar));
   WH_TRY(pick(wh::kPVariants, wh::kNumP, false, e->pvar));
-  WH_TRY(pick(wh::kPCLVariants, wh::kNumPCL, false, e->pclvar));
+  WH_TRY(pick(wh::kPCLVariants, wh::kNumPCL, false, e->pclvar, true));
   if (gp) {
-    const wh::SweepVariant g{16, 256};
+    const wh::SweepVariant g{16, 256, 0};
     for (int i = 0; i < n; ++i) {
       const auto& m = e->mem[i];
       if (!wh::variant_fits(m.X, g) || m.P % wh::tile_lines(m.X, g) != 0)
@@ -610,6 +640,16 @@ wh::SweepArgs sweep_args(const wh_ensemble* e, int flags, double frac, int step_
 int xsweep(wh_ensemble* e, int flags, hipStream_t s) {
   wh::SweepArgs a = sweep_args(e, flags, 1.0, 0);
   a.work = e->work_x.buf.as<unsigned>();
+  if (wh::kXVariants[e->xvar].tp) {
+    for (const SizeSeg& g : e->seg_x) {
+      a.work = e->work_x.buf.as<unsigned>() + g.off;
+      WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_xsweep_tp_f64(g.N, a, g.n, s)
+                : e->d.precision == WH_C64_WIDE ? wh::launch_xsweep_tp_f32w(g.N, a, g.n, s)
+                                                : wh::launch_xsweep_tp_f32(g.N, a, g.n, s),
+                "xsweep");
+    }
+    return WH_SUCCESS;
+  }
   WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_xsweep_f64(e->xvar, a, e->work_x.n, s)
             : e->d.precision == WH_C64_WIDE ? wh::launch_xsweep_f32w(e->xvar, a, e->work_x.n, s)
                                       : wh::launch_xsweep_f32(e->xvar, a, e->work_x.n, s),
@@ -622,6 +662,16 @@ int psweep(wh_ensemble* e, int flags, double frac, int step_add, hipStream_t s) 
   a.work = e->work_p.buf.as<unsigned>();
   const bool cl = (flags & wh::kCL) != 0;
   const int v = cl ? e->pclvar : e->pvar;
+  if (!cl && wh::kPVariants[v].tp) {
+    for (const SizeSeg& g : e->seg_p) {
+      a.work = e->work_p.buf.as<unsigned>() + g.off;
+      WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_psweep_tp_f64(g.N, a, g.n, s)
+                : e->d.precision == WH_C64_WIDE ? wh::launch_psweep_tp_f32w(g.N, a, g.n, s)
+                                                : wh::launch_psweep_tp_f32(g.N, a, g.n, s),
+                "psweep");
+    }
+    return WH_SUCCESS;
+  }
   WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_psweep_f64(v, cl, a, e->work_p.n, s)
             : e->d.precision == WH_C64_WIDE ? wh::launch_psweep_f32w(v, cl, a, e->work_p.n, s)
                                       : wh::launch_psweep_f32(v, cl, a, e->work_p.n, s),
@@ -834,7 +884,7 @@ extern "C" int wh_ensemble_set_variants(wh_ensemble* e, int xvar, int pvar, int 
     return fail(WH_INVALID_VALUE, "x variant %d does not fit every member", xvar);
   if (pvar >= 0 && !fits_all(e, wh::kPVariants[pvar], false))
     return fail(WH_INVALID_VALUE, "p variant %d does not fit every member", pvar);
-  if (pclvar >= 0 && !fits_all(e, wh::kPCLVariants[pclvar], false))
+  if (pclvar >= 0 && !fits_all(e, wh::kPCLVariants[pclvar], false, true))
     return fail(WH_INVALID_VALUE, "p+CL variant %d does not fit every member", pclvar);
   if (e->gp() && xvar >= 0 && xvar != e->xvar)
     return fail(WH_INVALID_VALUE, "the x variant of a Gross-Pitaevskii plan is fixed at creation");

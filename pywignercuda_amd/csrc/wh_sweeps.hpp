@@ -24,6 +24,7 @@
 #include "wh_fft.hpp"
 #include "wh_kernels.hpp"
 #include "wh_math.hpp"
+#include "wh_variants.hpp"
 
 namespace wh {
 
@@ -678,15 +679,18 @@ __device__ __forceinline__ void gpshift_body(const Member& m, int tile, cplx<dou
 }
 
 // ------------------------------------------------------------------ kernels
-#define WH_DISPATCH(NVAR, BODY)                                               \
-  switch (NVAR) {                                                             \
-    case 128: if constexpr (size_ok<128, R, BLOCK>()) { BODY(128); } break;   \
-    case 256: if constexpr (size_ok<256, R, BLOCK>()) { BODY(256); } break;   \
-    case 512: if constexpr (size_ok<512, R, BLOCK>()) { BODY(512); } break;   \
-    case 1024: if constexpr (size_ok<1024, R, BLOCK>()) { BODY(1024); } break; \
-    case 2048: if constexpr (size_ok<2048, R, BLOCK>()) { BODY(2048); } break; \
-    default: break;                                                           \
+// sizes up to MAXN (the throughput forms stop at 1024: one more size path in the same
+// kernel raises its register pressure past the 168 VGPRs of three waves per SIMD)
+#define WH_DISPATCH_TO(NVAR, BODY, MAXN)                                                     \
+  switch (NVAR) {                                                                            \
+    case 128: if constexpr (size_ok<128, R, BLOCK>() && 128 <= (MAXN)) { BODY(128); } break;   \
+    case 256: if constexpr (size_ok<256, R, BLOCK>() && 256 <= (MAXN)) { BODY(256); } break;   \
+    case 512: if constexpr (size_ok<512, R, BLOCK>() && 512 <= (MAXN)) { BODY(512); } break;   \
+    case 1024: if constexpr (size_ok<1024, R, BLOCK>() && 1024 <= (MAXN)) { BODY(1024); } break; \
+    case 2048: if constexpr (size_ok<2048, R, BLOCK>() && 2048 <= (MAXN)) { BODY(2048); } break; \
+    default: break;                                                                          \
   }
+#define WH_DISPATCH(NVAR, BODY) WH_DISPATCH_TO(NVAR, BODY, 2048)
 
 // Two waves per SIMD (<= 256 VGPRs) for the plain sweeps: a batch of grids overlaps one
 // wave's loads and stores with the other's transforms; three for the throughput form
@@ -702,7 +706,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TP ?
   const Member& m = a.members[w >> 16];
   const int tile = int(w & 0xffffu);
 #define BODY(n) xsweep_body<S, T, n, R, BLOCK, TP, E>(a, m, tile, lds)
-  WH_DISPATCH(m.X, BODY)
+  WH_DISPATCH_TO(m.X, BODY, TP ? kTpMaxN : 2048)
 #undef BODY
 }
 
@@ -714,12 +718,30 @@ __device__ __forceinline__ void psweep_entry(const SweepArgs& a) {
   const Member& m = a.members[w >> 16];
   const int tile = int(w & 0xffffu);
 #define BODY(n) psweep_body<S, T, n, R, BLOCK, CL, TP, E>(a, m, tile, lds)
-  WH_DISPATCH(m.P, BODY)
+  WH_DISPATCH_TO(m.P, BODY, TP && !CL ? kTpMaxN : 2048)
 #undef BODY
 }
 template <typename S, typename T, int R, int BLOCK, bool CL, bool TP>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(CL ? 1 : TP ? 3 : 2))) psweep_kernel(SweepArgs a) {
   psweep_entry<S, T, R, BLOCK, CL, TP>(a);
+}
+
+// Throughput forms for one grid side N: the host launches them once per size group of the
+// work list (a single kernel for all sizes raises the register pressure past the 168 VGPRs
+// of three waves per SIMD and spills in the hot path).
+template <typename S, typename T, int N, int R, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) xsweep_tp_kernel(SweepArgs a) {
+  using E = LdsElem<S, T, true>;
+  __shared__ cplx<E> lds[lds_elems<BLOCK, R>() + kTileRedBytes / sizeof(cplx<E>)];
+  const unsigned w = a.work[blockIdx.x];
+  xsweep_body<S, T, N, R, BLOCK, true, E>(a, a.members[w >> 16], int(w & 0xffffu), lds);
+}
+template <typename S, typename T, int N, int R, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) psweep_tp_kernel(SweepArgs a) {
+  using E = LdsElem<S, T, true>;
+  __shared__ cplx<E> lds[lds_elems<BLOCK, R>() + kPTailBytes / sizeof(cplx<E>)];
+  const unsigned w = a.work[blockIdx.x];
+  psweep_body<S, T, N, R, BLOCK, false, true, E>(a, a.members[w >> 16], int(w & 0xffffu), lds);
 }
 
 template <int R, int BLOCK>

@@ -21,9 +21,13 @@ constexpr int kNumX = sizeof(kXVariants) / sizeof(kXVariants[0]);
 constexpr int kNumP = sizeof(kPVariants) / sizeof(kPVariants[0]);
 constexpr int kNumPCL = sizeof(kPCLVariants) / sizeof(kPCLVariants[0]);
 
-// same rule as size_ok<N, R, BLOCK>() in wh_sweeps.hpp
-constexpr bool variant_fits(int N, SweepVariant v) {
-  return N >= v.R * 4 && N / v.R <= v.block && v.block / (N / v.R) <= 64;
+// largest grid side of the plain (x and p) throughput forms
+constexpr int kTpMaxN = 1024;
+
+// same rule as size_ok<N, R, BLOCK>() in wh_sweeps.hpp (and the throughput forms' size bound,
+// which the Caldeira-Leggett one does not have; cl: a Caldeira-Leggett p variant)
+constexpr bool variant_fits(int N, SweepVariant v, bool cl = false) {
+  return N >= v.R * 4 && N / v.R <= v.block && v.block / (N / v.R) <= 64 && (!v.tp || cl || N <= kTpMaxN);
 }
 // rows per x-sweep tile / columns per p-sweep tile
 constexpr int tile_lines(int N, SweepVariant v) { return v.block / (N / v.R); }
