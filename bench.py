@@ -69,12 +69,19 @@ def initial_state(prop, kind):
 
 
 def timed_run(prop, W, steps, warmup):
+    """warmup untimed iterations (at least two: the first graph-enabled run captures the step
+    graph, so the timed run only replays it), then `steps` timed iterations into a record
+    buffer allocated before the timing."""
     import torch
 
-    prop.run(W, warmup, use_graph=True)
+    from pywignercuda_amd._lib import WH_NOBS
+
+    warm = max(int(warmup), 2)
+    prop.run(W, warm, use_graph=True)
+    obs = torch.zeros((steps + 1, WH_NOBS), dtype=torch.float64, device=W.device)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    obs = prop.run(W, steps, first_step=warmup, normalize_initial=False, use_graph=True)
+    prop.run(W, steps, first_step=warm, normalize_initial=False, use_graph=True, obs=obs)
     torch.cuda.synchronize()
     return time.perf_counter() - t0, obs
 
@@ -105,10 +112,14 @@ def _gauss(prop_x, prop_p, dX, dP, x0=2.0, p0=0.0, s=1.0):
 def timed_ensemble(ens, states, steps, warmup):
     import torch
 
-    ens.run(states, warmup, use_graph=True)
+    from pywignercuda_amd._lib import WH_NOBS
+
+    warm = max(int(warmup), 2)
+    ens.run(states, warm, use_graph=True)
+    obs = torch.zeros((ens.n, steps + 1, WH_NOBS), dtype=torch.float64, device=ens.device)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ens.run(states, steps, first_step=warmup, normalize_initial=False, use_graph=True)
+    ens.run(states, steps, first_step=warm, normalize_initial=False, use_graph=True, obs=obs)
     torch.cuda.synchronize()
     return time.perf_counter() - t0
 
@@ -223,9 +234,23 @@ def dirac_workload(precision, steps, dev):
             / HBM_PEAK_GBS}
 
 
-def cpu_baseline(target_seconds=15.0):
-    """The oracle (NumPy complex128, single thread) on the same workload, bounded sample."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(target_seconds=12.0):
+    """The oracle (NumPy complex128) on the same workload, bounded samples: one thread with
+    numpy.fft (the reference's CPU path runs single-threaded FFTW plans), then all usable host
+    cores with scipy.fft workers (capped at 16, the host share of one GPU)."""
     import numpy as np
+    import scipy
 
     from oracle import wigner2d as ow
 
@@ -237,19 +262,39 @@ def cpu_baseline(target_seconds=15.0):
                             fp_damping=lambda p: np.abs(p), damping="CaldeiraLeggett", scheme="lie",
                             mask=True)
     W = ow.cat_state_wigner(orc.g, a=2.0)
-    orc.run(W, 0)  # multipliers cached, one iteration
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        orc.run(W, 0)  # one Lie iteration (observables + K + V + CL + renorm)
-        n += 1
-        el = time.perf_counter() - t0
-        if el > target_seconds:
-            break
-    return {"value": c["X"] * c["P"] * n / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} Lie+CL iterations of the 1024^2 damped workload with the NumPy "
-                      f"complex128 oracle (numpy.fft, 1 thread) in {el:.1f} s on "
-                      f"{platform.processor() or platform.machine()}"}
+
+    def sample(workers):
+        ow.use_fft_workers(workers)
+        try:
+            orc.run(W, 0)  # multipliers cached, one iteration
+            n = 0
+            t0 = time.perf_counter()
+            while True:
+                orc.run(W, 0)  # one Lie iteration (observables + K + V + CL + renorm)
+                n += 1
+                el = time.perf_counter() - t0
+                if el > target_seconds:
+                    return n, el
+        finally:
+            ow.use_fft_workers(None)
+
+    n1, el1 = sample(None)
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    workers = max(1, min(ncores, 16))
+    nn, eln = sample(workers)
+    cells = c["X"] * c["P"]
+    model = _cpu_model()
+    return {"value": cells * n1 / el1, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n1} Lie+CL iterations of the 1024^2 damped workload with the NumPy "
+                      f"complex128 oracle (numpy.fft, 1 thread) in {el1:.1f} s on {model}",
+            "all_cores": {"value": cells * nn / eln, "cores": workers,
+                          "sample": f"{nn} iterations, scipy.fft workers={workers} (elementwise "
+                                    f"work single-threaded numpy) in {eln:.1f} s"},
+            "cpu_model": model, "os_cpu_count": os.cpu_count(), "usable_cores": ncores,
+            "numpy": np.__version__, "scipy": scipy.__version__}
 
 
 def main():

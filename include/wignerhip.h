@@ -114,7 +114,7 @@ int wh_fft_destroy(wh_fft_plan* plan);
 typedef struct {
   int struct_size; /* sizeof(wh_wigner2d_desc) */
   int precision;   /* WH_C128 or WH_C64 */
-  int x_dim, p_dim; /* powers of two, 64..2048 */
+  int x_dim, p_dim; /* powers of two, 128..2048 */
   int scheme;      /* WH_SCHEME_LIE or WH_SCHEME_STRANG */
   /* kernel constants (expPotential / expPLambda sources, GPU_Wigner2D_FFT.py:80-200) */
   double dt, dx, dp, dtheta, dlambda, hbar, mass;

@@ -32,6 +32,33 @@ from __future__ import annotations
 
 import numpy as np
 
+# FFT backend of the oracle: numpy.fft (pocketfft, one thread: the reference's CPU path uses
+# single-threaded pyFFTW plans, CPU_GPU_Wigner2D.ipynb cells 6-11) or, for the all-cores CPU
+# baseline, scipy.fft with `workers` threads.  Same transform definition either way.
+_FFT_WORKERS = None
+
+
+def use_fft_workers(workers=None):
+    """None: numpy.fft on one thread; n: scipy.fft with n worker threads."""
+    global _FFT_WORKERS
+    _FFT_WORKERS = None if workers is None else int(workers)
+
+
+def fft(a, axis=-1):
+    if _FFT_WORKERS is None:
+        return np.fft.fft(a, axis=axis)
+    import scipy.fft
+
+    return scipy.fft.fft(a, axis=axis, workers=_FFT_WORKERS)
+
+
+def ifft(a, axis=-1):
+    if _FFT_WORKERS is None:
+        return np.fft.ifft(a, axis=axis)
+    import scipy.fft
+
+    return scipy.fft.ifft(a, axis=axis, workers=_FFT_WORKERS)
+
 
 def percent_f(v: float) -> float:
     """Value of a constant after '%f' splicing into CUDA source (6 decimals).
@@ -169,10 +196,10 @@ class Wigner2DOracle:
     def gp_shifted(self, rho: np.ndarray):
         """B-/+ (theta, x) = rho(x -/+ theta/2) by the shift theorem, FFT.py:1687-1705."""
         g = self.g
-        rt = np.fft.fft(rho.astype(np.complex128))[None, :]
+        rt = fft(rho.astype(np.complex128))[None, :]
         ph = np.exp(0.5j * g.lambda_host * g.theta_host)  # FFT.py:594-595 (no hbar)
-        Bm = np.fft.ifft(rt / ph, axis=1)
-        Bp = np.fft.ifft(rt * ph, axis=1)
+        Bm = ifft(rt / ph, axis=1)
+        Bp = ifft(rt * ph, axis=1)
         return Bm, Bp
 
     def gp_multiplier(self, Bm, Bp) -> np.ndarray:
@@ -194,8 +221,8 @@ class Wigner2DOracle:
     def cl_operator(self, W: np.ndarray) -> np.ndarray:
         """D(W) = F_p^{-1}[ theta * F_p[ sigma(p) W ] ] (Theta_fp_Damping, FFT.py:1135-1143)."""
         LW = W * self._cached("sigma", self.sigma)
-        LW = np.fft.fft(LW, axis=0) * self.g.theta_host  # host Theta grid
-        return np.fft.ifft(LW, axis=0)
+        LW = fft(LW, axis=0) * self.g.theta_host  # host Theta grid
+        return ifft(LW, axis=0)
 
     def caldeira_order3(self, W: np.ndarray) -> np.ndarray:
         """FFT.py:1145-1157: W + 2aD W + (10/3)a^2 D^2 W + (4/3)a^3 D^3 W, a = i dt gamma."""
@@ -229,8 +256,8 @@ class Wigner2DOracle:
         T = self._cached("odm", self.odm_tables)
 
         def term(gk):
-            Y = np.fft.ifft(np.fft.fft(W, axis=1) * gk, axis=1)
-            return np.fft.fft(Y, axis=0)
+            Y = ifft(fft(W, axis=1) * gk, axis=1)
+            return fft(Y, axis=0)
 
         LW = term(T["gp"] * T["gm"]) * (T["cp"] * T["cm"])
         LW = LW + term(T["gp"] * T["gsm"]) * T["cm"] * T["sp"] * (1j * sign)
@@ -238,7 +265,7 @@ class Wigner2DOracle:
         LW = LW + term(T["gsp"] * T["gsm"]) * (T["sp"] * T["sm"])
         LW = LW + term(-0.5 * T["gp"] ** 2)
         LW = LW + term(-0.5 * T["gm"] ** 2)
-        return np.fft.ifft(LW, axis=0) * (self.dt * 2.0 * self.gamma * weight / n)
+        return ifft(LW, axis=0) * (self.dt * 2.0 * self.gamma * weight / n)
 
     # ------------------------------------------------------------------ sweeps
     def _cached(self, key, fn):
@@ -248,17 +275,17 @@ class Wigner2DOracle:
 
     def kinetic_step(self, W):
         K = self._cached("K", self.kinetic_multiplier)
-        return np.fft.ifft(np.fft.fft(W, axis=1) * K, axis=1)
+        return ifft(fft(W, axis=1) * K, axis=1)
 
     def potential_step(self, W, t, gp_terms=None):
         if self.time_dependent:
             M = self.potential_multiplier(t)
         else:
             M = self._cached("V", lambda: self.potential_multiplier(0.0))
-        B = np.fft.fft(W, axis=0) * M
+        B = fft(W, axis=0) * M
         if gp_terms is not None:
             B = B * self.gp_multiplier(*gp_terms)
-        return np.fft.ifft(B, axis=0)
+        return ifft(B, axis=0)
 
     # ------------------------------------------------------------------ observables
     def observables(self, W: np.ndarray) -> dict:

@@ -223,6 +223,11 @@ def check(status: int, what: str = "") -> None:
         raise exc(f"{what}: {msg}" if what else msg)
 
 
+def last_error() -> str:
+    """Message of the calling thread's last failed library call."""
+    return lib().wh_last_error().decode(errors="replace")
+
+
 def dptr(a) -> ctypes.c_void_p:
     """Device pointer of a torch tensor (no copy)."""
     return ctypes.c_void_p(int(a.data_ptr()))

@@ -43,6 +43,9 @@ hipError_t launch_colpart_reduce(const Member* members, int nmembers, int xradix
                                  hipStream_t s);
 hipError_t launch_scale(int prec, const Member* members, const unsigned* work, int nwork, hipStream_t s);
 hipError_t launch_set_int(int* p, int v, hipStream_t s);
+hipError_t launch_bind(const BindArgs& a, hipStream_t s);
+hipError_t launch_dirac_bind(DiracPlan* d, void* W, double* obs, int obs_base, int* step, int step_v, double* scale,
+                             int reset_scale, hipStream_t s);
 hipError_t launch_set_scale(const Member* members, int nmembers, double re, double im, hipStream_t s);
 hipError_t launch_fft_axis(int prec, int columns, int N, int dir, void* d, int batch, int norm,
                            const void* tw_hi, const void* tw_lo, hipStream_t s);
@@ -322,8 +325,13 @@ struct wh_ensemble {
   DevBuf mem_dev;
   std::map<int, std::unique_ptr<TwSet>> tw;  // by size
   int xvar = 0, pvar = 0, pclvar = 0;
-  WorkList work_x, work_p, work_colsum, work_scale, work_gp, work_adj, work_odm_x, work_odm_p;
-  std::vector<SizeSeg> seg_x, seg_p;  // size runs of work_x (by X) and work_p (by P)
+  // p work lists: one for the plain p variant and one for the Caldeira-Leggett p variant
+  // (a tile spans block * R / P columns, which differs between the two variant tables;
+  // a CL plan also runs plain p sweeps: the merged Strang first half step, Strang + GP
+  // V/2, the unfused single V step)
+  WorkList work_x, work_p, work_pcl, work_colsum, work_scale, work_gp, work_adj, work_odm_x, work_odm_p;
+  std::vector<SizeSeg> seg_x, seg_p, seg_pcl;  // size runs of work_x (by X), work_p / work_pcl (by P)
+  wh::SweepVariant built_x{}, built_p{}, built_pcl{};  // the variants each list was cut for
   DevBuf step, done, gcols;
   hipStream_t capture_stream = nullptr;
   std::map<std::tuple<hipStream_t, int>, GraphEntry> graphs;  // (stream, chunk)
@@ -401,35 +409,45 @@ bool fits_all(const wh_ensemble* e, const wh::SweepVariant& v, bool x_axis, bool
   return true;
 }
 
-int build_work_lists(wh_ensemble* e) {
-  const auto& xv = wh::kXVariants[e->xvar];
-  const auto& pv = e->cl() ? wh::kPCLVariants[e->pclvar] : wh::kPVariants[e->pvar];
-  std::vector<unsigned> wx, wp, wc, ws, wg, wox, wop;
-  // x and p items grouped by grid side (ascending), members in order within a size
-  e->seg_x.clear();
-  e->seg_p.clear();
+// p tiles of one variant, grouped by grid side (ascending), members in order within a size;
+// consecutive tiles of a member go to one XCD (blocks are dealt round-robin to the 8 XCDs)
+void p_tiles(const wh_ensemble* e, const wh::SweepVariant& pv, std::vector<unsigned>& wp, std::vector<SizeSeg>& seg) {
+  seg.clear();
   for (int N = 1; N <= (1 << 20); N <<= 1) {
-    const size_t x0 = wx.size(), p0 = wp.size();
+    const size_t p0 = wp.size();
     for (int i = 0; i < e->n; ++i) {
       const auto& m = e->mem[i];
+      if (m.P != N) continue;
       const unsigned hi = unsigned(i) << 16;
-      if (m.X == N) {
-        const int rows = wh::tile_lines(m.X, xv);
-        for (int t = 0; t < m.P / rows; ++t) wx.push_back(hi | unsigned(t));
-      }
-      if (m.P == N) {
-        // p tiles: consecutive tiles on one XCD (blocks are dealt round-robin to the 8 XCDs)
-        const int cols = wh::tile_lines(m.P, pv);
-        const int nt = m.X / cols;
-        for (int q = 0; q < nt; ++q) {
-          const int t = (nt % 8 == 0) ? (q % 8) * (nt / 8) + q / 8 : q;
-          wp.push_back(hi | unsigned(t));
-        }
+      const int cols = wh::tile_lines(m.P, pv);
+      const int nt = m.X / cols;
+      for (int q = 0; q < nt; ++q) {
+        const int t = (nt % 8 == 0) ? (q % 8) * (nt / 8) + q / 8 : q;
+        wp.push_back(hi | unsigned(t));
       }
     }
-    if (wx.size() > x0) e->seg_x.push_back({N, int(x0), int(wx.size() - x0)});
-    if (wp.size() > p0) e->seg_p.push_back({N, int(p0), int(wp.size() - p0)});
+    if (wp.size() > p0) seg.push_back({N, int(p0), int(wp.size() - p0)});
   }
+}
+
+int build_work_lists(wh_ensemble* e) {
+  const auto& xv = wh::kXVariants[e->xvar];
+  std::vector<unsigned> wx, wp, wpcl, wc, ws, wg, wox, wop;
+  // x items grouped by grid side (ascending), members in order within a size
+  e->seg_x.clear();
+  for (int N = 1; N <= (1 << 20); N <<= 1) {
+    const size_t x0 = wx.size();
+    for (int i = 0; i < e->n; ++i) {
+      const auto& m = e->mem[i];
+      if (m.X != N) continue;
+      const unsigned hi = unsigned(i) << 16;
+      const int rows = wh::tile_lines(m.X, xv);
+      for (int t = 0; t < m.P / rows; ++t) wx.push_back(hi | unsigned(t));
+    }
+    if (wx.size() > x0) e->seg_x.push_back({N, int(x0), int(wx.size() - x0)});
+  }
+  p_tiles(e, wh::kPVariants[e->pvar], wp, e->seg_p);
+  if (e->cl()) p_tiles(e, wh::kPCLVariants[e->pclvar], wpcl, e->seg_pcl);
   for (int i = 0; i < e->n; ++i) {
     const auto& m = e->mem[i];
     const unsigned hi = unsigned(i) << 16;
@@ -448,11 +466,15 @@ int build_work_lists(wh_ensemble* e) {
     }
   }
   if (wx.size() > 0x7fffffff || ws.size() > 0x7fffffff) return fail(WH_INVALID_SIZE, "work list too long");
-  for (const auto& v : {&wx, &wp, &wc, &ws, &wg})
+  for (const auto& v : {&wx, &wp, &wpcl, &wc, &ws, &wg})
     for (unsigned w : *v)
       if ((w & 0xffffu) == 0xffffu) return fail(WH_INVALID_SIZE, "a member has more than 65535 tiles");
   WH_TRY(e->work_x.upload(wx));
   WH_TRY(e->work_p.upload(wp));
+  if (e->cl()) WH_TRY(e->work_pcl.upload(wpcl));
+  e->built_x = xv;
+  e->built_p = wh::kPVariants[e->pvar];
+  e->built_pcl = wh::kPCLVariants[e->pclvar];
   WH_TRY(e->work_colsum.upload(wc));
   WH_TRY(e->work_scale.upload(ws));
   if (e->gp()) WH_TRY(e->work_gp.upload(wg));
@@ -637,7 +659,14 @@ wh::SweepArgs sweep_args(const wh_ensemble* e, int flags, double frac, int step_
   return a;
 }
 
+bool same_variant(const wh::SweepVariant& a, const wh::SweepVariant& b) {
+  return a.R == b.R && a.block == b.block && a.tp == b.tp;
+}
+
 int xsweep(wh_ensemble* e, int flags, hipStream_t s) {
+  // the work list's tiles are cut for the variant launched over it
+  if (!same_variant(wh::kXVariants[e->xvar], e->built_x))
+    return fail(WH_INTERNAL_ERROR, "x work list cut for another variant");
   wh::SweepArgs a = sweep_args(e, flags, 1.0, 0);
   a.work = e->work_x.buf.as<unsigned>();
   if (wh::kXVariants[e->xvar].tp) {
@@ -659,12 +688,20 @@ int xsweep(wh_ensemble* e, int flags, hipStream_t s) {
 
 int psweep(wh_ensemble* e, int flags, double frac, int step_add, hipStream_t s) {
   wh::SweepArgs a = sweep_args(e, flags, frac, step_add);
-  a.work = e->work_p.buf.as<unsigned>();
   const bool cl = (flags & wh::kCL) != 0;
+  if (cl && !e->cl()) return fail(WH_INTERNAL_ERROR, "Caldeira-Leggett p sweep on a plan without friction");
+  // the plain and the Caldeira-Leggett p variants have their own work lists (their tiles
+  // may span different column counts)
   const int v = cl ? e->pclvar : e->pvar;
-  if (!cl && wh::kPVariants[v].tp) {
-    for (const SizeSeg& g : e->seg_p) {
-      a.work = e->work_p.buf.as<unsigned>() + g.off;
+  const wh::SweepVariant& launched = cl ? wh::kPCLVariants[v] : wh::kPVariants[v];
+  const WorkList& wl = cl ? e->work_pcl : e->work_p;
+  const std::vector<SizeSeg>& segs = cl ? e->seg_pcl : e->seg_p;
+  if (!same_variant(launched, cl ? e->built_pcl : e->built_p))
+    return fail(WH_INTERNAL_ERROR, "p work list cut for another variant");
+  a.work = wl.buf.as<unsigned>();
+  if (!cl && launched.tp) {
+    for (const SizeSeg& g : segs) {
+      a.work = wl.buf.as<unsigned>() + g.off;
       WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_psweep_tp_f64(g.N, a, g.n, s)
                 : e->d.precision == WH_C64_WIDE ? wh::launch_psweep_tp_f32w(g.N, a, g.n, s)
                                                 : wh::launch_psweep_tp_f32(g.N, a, g.n, s),
@@ -672,9 +709,9 @@ int psweep(wh_ensemble* e, int flags, double frac, int step_add, hipStream_t s) 
     }
     return WH_SUCCESS;
   }
-  WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_psweep_f64(v, cl, a, e->work_p.n, s)
-            : e->d.precision == WH_C64_WIDE ? wh::launch_psweep_f32w(v, cl, a, e->work_p.n, s)
-                                      : wh::launch_psweep_f32(v, cl, a, e->work_p.n, s),
+  WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_psweep_f64(v, cl, a, wl.n, s)
+            : e->d.precision == WH_C64_WIDE ? wh::launch_psweep_f32w(v, cl, a, wl.n, s)
+                                      : wh::launch_psweep_f32(v, cl, a, wl.n, s),
             "psweep");
   return WH_SUCCESS;
 }
@@ -760,41 +797,61 @@ int iterations(wh_ensemble* e, int count, bool merge_last, bool first, hipStream
 
 // Replays `count` middle iterations from a captured graph of graph_chunk iterations.
 // The graph reads member pointers, the step counter and the record base from
-// device memory, so one capture serves every later run of the plan.
+// device memory, so one capture serves every later run of the plan.  The capture is
+// made on the first graph-enabled run whatever its length (capturing executes
+// nothing), so that later runs replay it without capture work.
 int run_graph(wh_ensemble* e, int count, hipStream_t s) {
   const int chunk = e->graph_chunk;
-  if (count >= chunk) {
-    auto key = std::make_tuple(s, chunk);
-    auto it = e->graphs.find(key);
-    if (it == e->graphs.end()) {
-      if (!e->capture_stream)
-        WH_HIP(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking), "hipStreamCreate");
-      GraphEntry g;
-      WH_HIP(hipStreamBeginCapture(e->capture_stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
-      const int rc = iterations(e, chunk, true, false, e->capture_stream);
-      const hipError_t err = hipStreamEndCapture(e->capture_stream, &g.graph);
-      if (rc) {
-        if (g.graph) (void)hipGraphDestroy(g.graph);
-        return rc;
-      }
-      if (err != hipSuccess) return hip_fail(err, "EndCapture");
-      WH_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "GraphInstantiate");
-      it = e->graphs.emplace(key, g).first;
+  auto key = std::make_tuple(s, chunk);
+  auto it = e->graphs.find(key);
+  if (it == e->graphs.end()) {
+    if (!e->capture_stream)
+      WH_HIP(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking), "hipStreamCreate");
+    GraphEntry g;
+    WH_HIP(hipStreamBeginCapture(e->capture_stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
+    const int rc = iterations(e, chunk, true, false, e->capture_stream);
+    const hipError_t err = hipStreamEndCapture(e->capture_stream, &g.graph);
+    if (rc) {
+      if (g.graph) (void)hipGraphDestroy(g.graph);
+      return rc;
     }
-    while (count >= chunk) {
-      WH_HIP(hipGraphLaunch(it->second.exec, s), "GraphLaunch");
-      count -= chunk;
-    }
+    if (err != hipSuccess) return hip_fail(err, "EndCapture");
+    WH_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "GraphInstantiate");
+    it = e->graphs.emplace(key, g).first;
+  }
+  while (count >= chunk) {
+    WH_HIP(hipGraphLaunch(it->second.exec, s), "GraphLaunch");
+    count -= chunk;
   }
   return iterations(e, count, true, false, s);
 }
 
+// Device member table: a structural change (creation, new potential rows, adjoint
+// operands) re-uploads the whole table and waits for the copy; a change of only the
+// states and record rows of a run is written by a bind kernel in stream order.
 int sync_table(wh_ensemble* e, bool changed, hipStream_t s) {
   if (changed || e->table_dirty) {
     WH_HIP(hipMemcpyAsync(e->mem_dev.p, e->mem.data(), sizeof(wh::Member) * e->mem.size(), hipMemcpyHostToDevice, s),
            "member table upload");
     WH_HIP(hipStreamSynchronize(s), "member table upload");
     e->table_dirty = false;
+  }
+  return WH_SUCCESS;
+}
+
+int bind_states(wh_ensemble* e, hipStream_t s) {
+  if (e->table_dirty) return sync_table(e, true, s);
+  for (int first = 0; first < e->n; first += wh::kBindMax) {
+    wh::BindArgs a{};
+    a.members = e->mem_dev.as<wh::Member>();
+    a.first = first;
+    a.count = std::min(wh::kBindMax, e->n - first);
+    a.obs_base = e->mem[first].obs_base;
+    for (int i = 0; i < a.count; ++i) {
+      a.W[i] = e->mem[first + i].W;
+      a.obs[i] = e->mem[first + i].obs;
+    }
+    WH_LAUNCH(wh::launch_bind(a, s), "bind");
   }
   return WH_SUCCESS;
 }
@@ -820,7 +877,7 @@ int ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_step
     m.obs = o;
     m.obs_base = base;
   }
-  WH_TRY(sync_table(e, changed, s));
+  if (changed || e->table_dirty) WH_TRY(bind_states(e, s));
   const int prec = d.precision;
   const wh::Member* md = e->mem_dev.as<wh::Member>();
   // the state before the first iteration: norm, renormalisation factor, (Lie) row 0
@@ -974,7 +1031,8 @@ int bind_single(wh_ensemble* e, void* W, double* obs, int obs_base, hipStream_t 
   m.W = W;
   m.obs = obs;
   m.obs_base = obs_base;
-  return sync_table(e, changed, s);
+  if (changed || e->table_dirty) return bind_states(e, s);
+  return WH_SUCCESS;
 }
 }  // namespace
 
@@ -1085,6 +1143,7 @@ struct wh_dirac {
   hipStream_t capture_stream = nullptr;
   std::map<std::tuple<hipStream_t, int>, GraphEntry> graphs;
   int graph_chunk = 16;
+  bool plan_uploaded = false;
   ~wh_dirac() {
     for (auto& kv : graphs) {
       if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
@@ -1142,30 +1201,29 @@ int dirac_iterations(wh_dirac* e, int count, hipStream_t s) {
   return WH_SUCCESS;
 }
 
+// As run_graph: captured on the first graph-enabled run whatever its length.
 int dirac_run_graph(wh_dirac* e, int count, hipStream_t s) {
   const int chunk = e->graph_chunk;
-  if (count >= chunk) {
-    auto key = std::make_tuple(s, chunk);
-    auto it = e->graphs.find(key);
-    if (it == e->graphs.end()) {
-      if (!e->capture_stream)
-        WH_HIP(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking), "hipStreamCreate");
-      GraphEntry g;
-      WH_HIP(hipStreamBeginCapture(e->capture_stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
-      const int rc = dirac_iterations(e, chunk, e->capture_stream);
-      const hipError_t err = hipStreamEndCapture(e->capture_stream, &g.graph);
-      if (rc) {
-        if (g.graph) (void)hipGraphDestroy(g.graph);
-        return rc;
-      }
-      if (err != hipSuccess) return hip_fail(err, "EndCapture");
-      WH_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "GraphInstantiate");
-      it = e->graphs.emplace(key, g).first;
+  auto key = std::make_tuple(s, chunk);
+  auto it = e->graphs.find(key);
+  if (it == e->graphs.end()) {
+    if (!e->capture_stream)
+      WH_HIP(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking), "hipStreamCreate");
+    GraphEntry g;
+    WH_HIP(hipStreamBeginCapture(e->capture_stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
+    const int rc = dirac_iterations(e, chunk, e->capture_stream);
+    const hipError_t err = hipStreamEndCapture(e->capture_stream, &g.graph);
+    if (rc) {
+      if (g.graph) (void)hipGraphDestroy(g.graph);
+      return rc;
     }
-    while (count >= chunk) {
-      WH_HIP(hipGraphLaunch(it->second.exec, s), "GraphLaunch");
-      count -= chunk;
-    }
+    if (err != hipSuccess) return hip_fail(err, "EndCapture");
+    WH_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "GraphInstantiate");
+    it = e->graphs.emplace(key, g).first;
+  }
+  while (count >= chunk) {
+    WH_HIP(hipGraphLaunch(it->second.exec, s), "GraphLaunch");
+    count -= chunk;
   }
   return dirac_iterations(e, count, s);
 }
@@ -1288,16 +1346,22 @@ extern "C" int wh_dirac_destroy(wh_dirac* plan) {
 }
 
 namespace {
-// Points the device plan at W (and obs) in stream order and sets the step counter.
-int dirac_bind(wh_dirac* e, void* W, double* obs, int base, int step, hipStream_t s) {
+// Points the device plan at W (and obs) in stream order and sets the step counter (a
+// kernel writes the fields: no host synchronisation).  The first call uploads the plan.
+int dirac_bind(wh_dirac* e, void* W, double* obs, int base, int step, bool reset_scale, hipStream_t s) {
   const size_t align = state_bytes(e->d.precision);
   if (reinterpret_cast<uintptr_t>(W) % align) return fail(WH_UNALIGNED_DATA, "W %p", W);
-  WH_HIP(hipStreamSynchronize(s), "sync");
   e->host.W = W;
   e->host.obs = obs;
   e->host.obs_base = base;
-  WH_HIP(hipMemcpy(e->plan_dev.p, &e->host, sizeof(wh::DiracPlan), hipMemcpyHostToDevice), "plan upload");
-  WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), step, s), "set_step");
+  if (!e->plan_uploaded) {
+    WH_HIP(hipMemcpyAsync(e->plan_dev.p, &e->host, sizeof(wh::DiracPlan), hipMemcpyHostToDevice, s), "plan upload");
+    WH_HIP(hipStreamSynchronize(s), "plan upload");
+    e->plan_uploaded = true;
+  }
+  WH_LAUNCH(wh::launch_dirac_bind(e->plan_dev.as<wh::DiracPlan>(), W, obs, base, e->step.as<int>(), step,
+                                  e->scale.as<double>(), reset_scale ? 1 : 0, s),
+            "dirac_bind");
   return WH_SUCCESS;
 }
 }  // namespace
@@ -1306,10 +1370,7 @@ extern "C" int wh_dirac_kinetic(wh_dirac* e, void* W, void* stream) {
   if (!e) return fail(WH_INVALID_PLAN, "wh_dirac_kinetic: plan is NULL");
   if (!W) return fail(WH_INVALID_VALUE, "wh_dirac_kinetic: W is NULL");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  WH_TRY(dirac_bind(e, W, e->host.obs, e->host.obs_base, 0, s));
-  const double one = 1.0;
-  WH_HIP(hipMemcpyAsync(e->scale.p, &one, sizeof(one), hipMemcpyHostToDevice, s), "scale");
-  WH_HIP(hipStreamSynchronize(s), "sync");
+  WH_TRY(dirac_bind(e, W, e->host.obs, e->host.obs_base, 0, true, s));
   WH_LAUNCH(wh::launch_dirac_xsweep(e->d.precision, e->plan_dev.as<wh::DiracPlan>(), e->xblocks, s), "dirac_xsweep");
   return WH_SUCCESS;
 }
@@ -1320,7 +1381,7 @@ extern "C" int wh_dirac_potential(wh_dirac* e, void* W, int step, void* stream) 
   if (e->d.gp_coeff != 0.0)
     return fail(WH_INVALID_VALUE, "wh_dirac_potential: the Gross-Pitaevskii density comes from wh_dirac_run");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  WH_TRY(dirac_bind(e, W, e->host.obs, e->host.obs_base, step, s));
+  WH_TRY(dirac_bind(e, W, e->host.obs, e->host.obs_base, step, false, s));
   const auto* dp = e->plan_dev.as<wh::DiracPlan>();
   if (e->host.general)
     WH_LAUNCH(wh::launch_dirac_pgeneral(e->d.precision, e->d.gamma_dt != 0.0, dp, e->pgblocks, s), "dirac_pgeneral");
@@ -1337,7 +1398,7 @@ extern "C" int wh_dirac_run(wh_dirac* e, void* W, int first_step, int n_steps, d
                 first_step);
   hipStream_t s = static_cast<hipStream_t>(stream);
   // the device plan is read by every kernel (and by captured graphs): update it in stream order
-  WH_TRY(dirac_bind(e, W, obs, first_step, first_step, s));
+  WH_TRY(dirac_bind(e, W, obs, first_step, first_step, false, s));
   const auto* dp = e->plan_dev.as<wh::DiracPlan>();
   WH_LAUNCH(wh::launch_dirac_colsum(storage_code(e->d.precision), dp, e->d.x_dim, s), "dirac_colsum");
   WH_LAUNCH(wh::launch_dirac_reduce(dp, 0, 0, 0, 0, s), "dirac_reduce");

@@ -880,6 +880,24 @@ hipError_t launch_dirac_reduce(const DiracPlan* d, int state_add, int renorm, in
   return hipGetLastError();
 }
 
+// Per-call binding of the state and record into the device plan, in stream order (no host
+// synchronisation; captured graphs read the plan): W, obs, obs_base, the step counter and,
+// optionally, the pending factor reset to 1.
+__global__ void dirac_bind_kernel(DiracPlan* dp, void* W, double* obs, int obs_base, int* step, int step_v,
+                                  double* scale, int reset_scale) {
+  dp->W = W;
+  dp->obs = obs;
+  dp->obs_base = obs_base;
+  *step = step_v;
+  if (reset_scale) *scale = 1.0;
+}
+hipError_t launch_dirac_bind(DiracPlan* d, void* W, double* obs, int obs_base, int* step, int step_v, double* scale,
+                             int reset_scale, hipStream_t s) {
+  hipLaunchKernelGGL(dirac_bind_kernel, dim3(1), dim3(1), 0, s, d, W, obs, obs_base, step, step_v, scale,
+                     reset_scale);
+  return hipGetLastError();
+}
+
 hipError_t launch_dirac_finish(int prec, const DiracPlan* d, int P, hipStream_t s) {
   if (prec == 0)  // state storage only
     hipLaunchKernelGGL(dirac_finish_kernel<double>, dim3(P), dim3(256), 0, s, d);

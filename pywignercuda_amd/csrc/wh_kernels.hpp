@@ -106,6 +106,16 @@ struct SweepArgs {
   int fin_renorm, fin_record;
 };
 
+// Per-run binding of member states and record rows, written into the device member table
+// by a kernel in stream order (no host synchronisation; captured graphs read the table)
+constexpr int kBindMax = 60;  // members per bind launch (kernel arguments stay under 1 KiB)
+struct BindArgs {
+  Member* members;
+  int first, count, obs_base, pad;
+  void* W[kBindMax];
+  double* obs[kBindMax];
+};
+
 struct ReduceArgs {
   const Member* members;
   int renorm;     // divide by the complex norm

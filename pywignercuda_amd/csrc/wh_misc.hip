@@ -181,6 +181,15 @@ __global__ void __launch_bounds__(256) scale_kernel(const Member* members, const
 }
 
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
+__global__ void bind_kernel(BindArgs a) {
+  const int i = threadIdx.x;
+  if (i < a.count) {
+    Member& m = a.members[a.first + i];
+    m.W = a.W[i];
+    m.obs = a.obs[i];
+    m.obs_base = a.obs_base;
+  }
+}
 __global__ void set_scale_kernel(const Member* members, double re, double im) {
   const Member& m = members[blockIdx.x];
   if (threadIdx.x == 0) {
@@ -334,6 +343,10 @@ hipError_t launch_scale(int prec, const Member* members, const unsigned* work, i
 }
 hipError_t launch_set_int(int* p, int v, hipStream_t s) {
   hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, p, v);
+  return hipGetLastError();
+}
+hipError_t launch_bind(const BindArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(bind_kernel, dim3(1), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_set_scale(const Member* members, int nmembers, double re, double im, hipStream_t s) {

@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes
 import json
+import warnings
 from pathlib import Path
 
 import numpy as np
@@ -231,12 +232,20 @@ class _PlanBase:
     def _set_variants(self, ens_handle, sizes_x, sizes_p, variants):
         cells = sum(int(x) * int(p) for x, p in zip(sizes_x, sizes_p))
         batch = cells * (16 if self.precision == "fp64" else 8) >= (256 << 20)
-        xv, pv, pcl = variants if variants is not None else tuned_variants(
-            self.precision, sizes_x, sizes_p, self.cl, batch)
+        if variants is not None:
+            xv, pv, pcl = variants
+        else:
+            xv, pv, pcl = tuned_variants(self.precision, sizes_x, sizes_p, self.cl, batch)
+            if self.gp:
+                xv = -1  # the x variant of a Gross-Pitaevskii plan is fixed at creation
         if (xv, pv, pcl) != (-1, -1, -1):
             rc = _lib.lib().wh_ensemble_set_variants(ens_handle, xv, pv, pcl)
-            if rc != 0 and variants is not None:
-                _lib.check(rc, "wh_ensemble_set_variants")
+            if rc != 0:
+                if variants is not None:
+                    _lib.check(rc, "wh_ensemble_set_variants")
+                # a tuned choice that does not fit this plan: the defaults stay, say so
+                warnings.warn(f"tuned kernel variants {(xv, pv, pcl)} not applied: "
+                              f"{_lib.last_error()}", RuntimeWarning, stacklevel=3)
 
     def to_device(self, W, shape=None):
         import torch
@@ -288,6 +297,7 @@ class Wigner2D(_PlanBase):
         self.device = torch.device(device if device is not None else "cuda:0")
         self.scheme = scheme
         self.cl = damping == "CaldeiraLeggett" and gamma != 0.0
+        self.gp = gp_coeff != 0.0
         t = _Tables(X, P, X_amp, P_amp, hbar, mass, dt, potential=potential, dpotential=dpotential,
                     scheme=scheme, D_theta=D_theta, D_lambda=D_lambda, gamma=gamma, damping=damping,
                     fp_damping=fp_damping, gp_coeff=gp_coeff, gp_rho_factor=gp_rho_factor, mask=mask,
@@ -422,6 +432,7 @@ class WignerEnsemble(_PlanBase):
         self.device = torch.device(device if device is not None else "cuda:0")
         self.scheme = scheme
         self.cl = damping == "CaldeiraLeggett" and gamma != 0.0
+        self.gp = gp_coeff != 0.0
         self.dt = float(dt)
         self.tables = []
         for m in members:

@@ -1,0 +1,316 @@
+"""GPU parity at the real sizes of the benchmark configurations (libwignerhip through its C ABI
+against the oracle on the same seeded inputs).
+
+* 512^2 plain quartic oscillator, 500 steps (Lie as GPU_Wigner2D_FFT.Run, Strang as the CPU
+  notebook propagator), x,p in [-10,10), V = 0.5x^2 + 0.1x^4, W0 = exp(-(x-2)^2 - p^2)/pi;
+* 1024^2 Caldeira-Leggett damped double well (x,p in [-12,12), V = 0.1x^4 - x^2, gamma = 0.1,
+  D_theta = 2 m gamma kT = 0.1, sigma(p) = p, cat state, dt = 0.005) in FP64 and FP32;
+* the seeded uneven ensemble (sides 128..1024, V = a x^4 - b x^2) in the tuned kernels and in
+  the per-size throughput kernels, FP64 and FP32, each member against its own oracle; and the
+  full 256-member, 1000-step FP32-vs-FP64 error;
+* the Wigner-Dirac 4x4 step at 512^2 x 16 planes (soft Coulomb V0 = -2/sqrt(x^2+1), dt = 0.002,
+  positive-energy Gaussian spinor x0 = -2, px = 6) in FP64 and FP32.
+
+Bars: FP64 relative L2 <= 1e-10 in W and the observables within 1e-10 relative (1e-12
+absolute near 0); FP32 state forms relative L2 <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+from oracle import wigner2d as ow
+
+pytestmark = pytest.mark.gpu
+
+OBS_KEYS = ["X", "X2", "P", "P2", "XP", "dV", "PdV", "XdV", "H", "neg", "norm"]
+OBS_COL = {k: i for i, k in enumerate(OBS_KEYS)}
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def check_obs(obs, obs_ref, rtol=1e-10, atol=1e-12, keys=OBS_KEYS):
+    for k in keys:
+        np.testing.assert_allclose(obs[:, OBS_COL[k]], obs_ref[k], rtol=rtol, atol=atol, err_msg=k)
+
+
+# ---------------------------------------------------------------------------- plain 512^2, 500 steps
+@pytest.mark.parametrize("scheme", ["lie", "strang"])
+def test_plain_512_500_steps(cuda, scheme):
+    import torch
+
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    X = P = 512
+    args = (X, P, 10.0, 10.0, 1.0, 1.0, 0.01)
+    hip = Wigner2D(*args, potential="0.5*pow(x,2)+0.1*pow(x,4)", dpotential="x+0.4*pow(x,3)",
+                   kinetic="0.5*p*p" if scheme == "strang" else None, scheme=scheme, mask=False,
+                   device=cuda)
+    orc = ow.Wigner2DOracle(*args, potential=lambda t, x: 0.5 * x ** 2 + 0.1 * x ** 4,
+                            dpotential=lambda t, x: x + 0.4 * x ** 3, kinetic=lambda p: 0.5 * p * p,
+                            scheme=scheme, mask=False)
+    x, p = orc.g.x_host, orc.g.p_host
+    W0 = np.exp(-((x - 2.0) ** 2) - p ** 2) / np.pi
+    steps = 500
+    W_ref, obs_ref, _ = orc.run(W0, steps - 1)  # steps iterations
+    W = hip.to_device(W0)
+    obs = hip.run(W, steps).cpu().numpy()[:steps]
+    torch.cuda.synchronize()
+    err = rel_l2(W.cpu().numpy(), W_ref)
+    print(f"512^2 {scheme}, {steps} steps: rel-L2 {err:.3e}")
+    assert err < 1e-10
+    check_obs(obs, obs_ref)
+    # unitary step without mask or damping: norm conserved to 1e-12 over the run
+    drift = np.abs(obs[:, OBS_COL["norm"]] - 1.0).max()
+    assert drift < 1e-12, drift
+
+
+# ---------------------------------------------------------------------------- damped 1024^2
+def _cl_pair(cuda, precision="fp64", X=1024, variants=None, scheme="lie"):
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    args = (X, X, 12.0, 12.0, 1.0, 1.0, 0.005)
+    hip = Wigner2D(*args, potential="0.1*pow(x,4) - pow(x,2)", dpotential="0.4*pow(x,3) - 2.0*x",
+                   kinetic="0.5*p*p" if scheme == "strang" else None, scheme=scheme, D_theta=0.1, gamma=0.1,
+                   damping="CaldeiraLeggett", fp_damping="fabs(p)", mask=True, precision=precision,
+                   device=cuda, variants=variants)
+    orc = ow.Wigner2DOracle(*args, D_theta=0.1, gamma=0.1, potential=lambda t, x: 0.1 * x ** 4 - x ** 2,
+                            dpotential=lambda t, x: 0.4 * x ** 3 - 2.0 * x, kinetic=lambda p: 0.5 * p * p,
+                            fp_damping=lambda p: np.abs(p), damping="CaldeiraLeggett", scheme=scheme,
+                            mask=True)
+    return hip, orc
+
+
+@pytest.fixture(scope="module")
+def cl1024_reference():
+    args = (1024, 1024, 12.0, 12.0, 1.0, 1.0, 0.005)
+    o = ow.Wigner2DOracle(*args, D_theta=0.1, gamma=0.1, potential=lambda t, x: 0.1 * x ** 4 - x ** 2,
+                          dpotential=lambda t, x: 0.4 * x ** 3 - 2.0 * x, fp_damping=lambda p: np.abs(p),
+                          damping="CaldeiraLeggett", scheme="lie", mask=True)
+    W0 = ow.cat_state_wigner(o.g, a=2.0)
+    W_ref, obs_ref, _ = o.run(W0, 20)
+    return W0, W_ref, obs_ref
+
+
+def test_damped_1024_fp64(cuda, cl1024_reference):
+    """21 Lie iterations (K; V + Caldeira-Leggett order 3; renormalisation) in the tuned kernels,
+    graph-replayed as in the benchmark."""
+    import torch
+
+    W0, W_ref, obs_ref = cl1024_reference
+    hip, _ = _cl_pair(cuda)
+    W = hip.to_device(W0)
+    obs = hip.run(W, 21, use_graph=True).cpu().numpy()[:21]
+    torch.cuda.synchronize()
+    err = rel_l2(W.cpu().numpy(), W_ref)
+    print(f"1024^2 damped fp64: rel-L2 {err:.3e}")
+    assert err < 1e-10
+    check_obs(obs, obs_ref)
+
+
+def test_damped_1024_every_cl_variant_fp64(cuda, cl1024_reference):
+    """Every Caldeira-Leggett p-sweep variant that fits 1024 at the full size."""
+    import torch
+
+    from pywignercuda_amd import _lib
+
+    W0, W_ref, obs_ref = cl1024_reference
+    hip, _ = _cl_pair(cuda)
+    n = _lib.lib().wh_variant_info(2, -1, None, None)
+    for v in range(n):
+        try:
+            hip.set_variants(-1, -1, v)
+        except _lib.WignerHipInvalidValue:
+            continue
+        W = hip.to_device(W0)
+        obs = hip.run(W, 21, use_graph=False).cpu().numpy()[:21]
+        torch.cuda.synchronize()
+        err = rel_l2(W.cpu().numpy(), W_ref)
+        print(f"1024^2 damped CL variant {v}: rel-L2 {err:.3e}")
+        assert err < 1e-10, v
+        check_obs(obs, obs_ref)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32fast"])
+def test_damped_1024_fp32(cuda, cl1024_reference, precision):
+    """complex64 state: 'fp32' (FP64 arithmetic in the sweeps) meets 1e-5; 'fp32fast' (FP32
+    transforms) is held to its FP32-arithmetic floor (2e-6 after 21 steps here)."""
+    import torch
+
+    W0, W_ref, obs_ref = cl1024_reference
+    hip, _ = _cl_pair(cuda, precision=precision)
+    W = hip.to_device(W0)
+    obs = hip.run(W, 21).cpu().numpy()[:21]
+    torch.cuda.synchronize()
+    err = rel_l2(W.cpu().numpy().astype(np.complex128), W_ref)
+    print(f"1024^2 damped {precision}: rel-L2 {err:.3e}")
+    assert err < (1e-5 if precision == "fp32" else 2e-5)
+    for k in ["X", "P", "X2", "P2", "H", "norm"]:
+        np.testing.assert_allclose(obs[:, OBS_COL[k]], obs_ref[k], rtol=1e-5, atol=1e-5, err_msg=k)
+
+
+def test_strang_cl_every_variant_pair(cuda):
+    """Strang scheme with Caldeira-Leggett: the merged first half step runs the plain p variant and
+    the loop runs the CL p variant, each over its own work list.  Every accepted (p, p+CL) pair
+    against the oracle, 256^2, 12 iterations."""
+    import torch
+
+    from pywignercuda_amd import _lib
+
+    hip, orc = _cl_pair(cuda, X=256, scheme="strang")
+    W0 = ow.cat_state_wigner(orc.g, a=2.0)
+    W_ref, obs_ref, _ = orc.run(W0, 11)
+    L = _lib.lib()
+    npv, ncl = L.wh_variant_info(1, -1, None, None), L.wh_variant_info(2, -1, None, None)
+    tried = 0
+    for pv in range(npv):
+        for cv in range(ncl):
+            try:
+                hip.set_variants(-1, pv, cv)
+            except _lib.WignerHipInvalidValue:
+                continue
+            W = hip.to_device(W0)
+            obs = hip.run(W, 12, use_graph=True).cpu().numpy()[:12]
+            torch.cuda.synchronize()
+            err = rel_l2(W.cpu().numpy(), W_ref)
+            assert err < 1e-10, (pv, cv, err)
+            check_obs(obs, obs_ref)
+            # the unfused single V step (plain p variant on a CL plan)
+            Wp = hip.to_device(W0)
+            hip.potential(Wp, step=3, fused=False)
+            torch.cuda.synchronize()
+            Vp = orc.potential_step(W0.astype(np.complex128), 3 * orc.dt)
+            assert rel_l2(Wp.cpu().numpy(), Vp) < 1e-12, (pv, cv)
+            tried += 1
+    assert tried >= 2
+
+
+# ---------------------------------------------------------------------------- uneven ensemble
+def _seeded_subset():
+    """Two members of each side 128, 256, 512, 1024 from the seeded 256-member ensemble."""
+    import bench
+
+    members, init = bench.uneven_ensemble_members(seed=20261202)
+    pick, count = [], {}
+    for i, m in enumerate(members):
+        if count.get(m["X"], 0) < 2:
+            count[m["X"]] = count.get(m["X"], 0) + 1
+            pick.append(i)
+    assert sorted(count.values()) == [2, 2, 2, 2]
+    return [members[i] for i in pick], [init[i] for i in pick]
+
+
+@pytest.fixture(scope="module")
+def uneven_reference():
+    members, init = _seeded_subset()
+    steps = 10
+    refs = []
+    for m, (x0, p0, s) in zip(members, init):
+        a, b = m["pot_table"][0, 4], -m["pot_table"][0, 2]
+        orc = ow.Wigner2DOracle(m["X"], m["P"], 10.0, 10.0, 1.0, 1.0, 0.01, scheme="lie",
+                                potential=lambda t, x, a=a, b=b: a * x ** 4 - b * x ** 2,
+                                dpotential=lambda t, x, a=a, b=b: 4 * a * x ** 3 - 2 * b * x, mask=False)
+        g = orc.g
+        W0 = np.exp(-((g.x_host - x0) ** 2) / s ** 2 - s ** 2 * (g.p_host - p0) ** 2) / np.pi
+        W0 = W0 / (W0.sum() * g.dX * g.dP)
+        W, obs, _ = orc.run(W0, steps)
+        refs.append((W0, W, obs))
+    return members, refs, steps
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("form", ["tuned", "throughput"])
+def test_uneven_seeded_ensemble(cuda, uneven_reference, precision, form):
+    import torch
+
+    from pywignercuda_amd import _lib
+    from pywignercuda_amd.wigner2d import WignerEnsemble
+
+    members, refs, steps = uneven_reference
+    variants = None
+    if form == "throughput":
+        L = _lib.lib()
+        variants = (L.wh_variant_info(0, -1, None, None) - 1, L.wh_variant_info(1, -1, None, None) - 1, 0)
+    ens = WignerEnsemble(members, 1.0, 1.0, 0.01, scheme="lie", precision=precision, device=cuda,
+                         variants=variants)
+    states = ens.states_to_device([r[0] for r in refs])
+    obs = ens.run(states, steps + 1, use_graph=True).cpu().numpy()
+    torch.cuda.synchronize()
+    bound = 1e-10 if precision == "fp64" else 1e-5
+    for i, (W0, W_ref, obs_ref) in enumerate(refs):
+        err = rel_l2(states[i].cpu().numpy().astype(np.complex128), W_ref)
+        print(f"member {i} ({members[i]['X']}^2) {precision} {form}: rel-L2 {err:.3e}")
+        assert err < bound, (i, err)
+        tol = (1e-10, 1e-12) if precision == "fp64" else (1e-5, 1e-6)
+        for k in ["X", "P", "X2", "P2", "H", "norm"]:
+            np.testing.assert_allclose(obs[i, : steps + 1, OBS_COL[k]], obs_ref[k], rtol=tol[0], atol=tol[1],
+                                       err_msg=f"{i} {k}")
+
+
+def test_uneven_256_members_fp32_vs_fp64_1000_steps(cuda):
+    """The whole seeded 256-member ensemble, 1000 steps: FP32 state against the FP64 state of the
+    same launch sequence, per member and overall (bar 1e-5)."""
+    import torch
+
+    import bench
+    from pywignercuda_amd.wigner2d import WignerEnsemble
+
+    members, init = bench.uneven_ensemble_members(seed=20261202)
+    finals = {}
+    for prec in ("fp64", "fp32"):
+        ens = WignerEnsemble(members, 1.0, 1.0, 0.01, scheme="lie", precision=prec, device=cuda)
+        W0 = [bench._gauss(t.x_host, t.p_host, t.dX, t.dP, *ini) for t, ini in zip(ens.tables, init)]
+        states = ens.states_to_device(W0)
+        ens.run(states, 1000, use_graph=True)
+        torch.cuda.synchronize()
+        finals[prec] = [W.cpu().numpy().astype(np.complex128) for W in states]
+        del ens, states
+        torch.cuda.empty_cache()
+    worst = max(rel_l2(b, a) for a, b in zip(finals["fp64"], finals["fp32"]))
+    a = np.concatenate([w.ravel() for w in finals["fp64"]])
+    b = np.concatenate([w.ravel() for w in finals["fp32"]])
+    total = rel_l2(b, a)
+    print(f"256 members, 1000 steps: fp32 rel-L2 overall {total:.3e}, worst member {worst:.3e}")
+    assert total < 1e-5 and worst < 1e-5
+
+
+# ---------------------------------------------------------------------------- Dirac 512^2
+@pytest.fixture(scope="module")
+def dirac512_reference():
+    from oracle.dirac4x4 import Dirac4x4Oracle, gaussian_spinor_particle_up, wigner_from_spinor
+
+    zero = lambda t, x: 0.0 * x  # noqa: E731
+    orc = Dirac4x4Oracle(512, 512, 18.0, 16.0, 1.0, 1.0, 0.002,
+                         [lambda t, x: -2.0 / np.sqrt(x * x + 1.0), zero, zero, zero],
+                         dpotential0=lambda t, x: 2.0 * x / (x * x + 1.0) ** 1.5, compute_energy=True)
+    W0 = wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 6.0, 1.0, Xg, 1.0, 1.0), orc)
+    steps = 20
+    W_ref, rec = orc.run(W0, steps)
+    return W0, W_ref, rec, steps
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_dirac_512_soft_coulomb(cuda, dirac512_reference, precision):
+    import torch
+
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    W0, W_ref, rec, steps = dirac512_reference
+    hip = WignerDirac4x4(512, 512, 18.0, 16.0, 1.0, 1.0, 0.002, potentials=("-2/sqrt(x*x+1)", "0.", "0.", "0."),
+                         energy=True, precision=precision, device=cuda)
+    W = hip.to_device(W0)
+    obs = hip.run(W, steps, first_step=1, use_graph=True).cpu().numpy()
+    torch.cuda.synchronize()
+    err = rel_l2(W.cpu().numpy().reshape(4, 4, 512, 512).astype(np.complex128), W_ref)
+    print(f"Dirac 512^2 {precision}: rel-L2 {err:.3e}")
+    o = WignerDirac4x4.observables(obs)
+    if precision == "fp64":
+        assert err < 1e-10
+        rtol, atol = 1e-10, 1e-12
+    else:
+        assert err < 1e-5
+        rtol, atol = 1e-5, 1e-5
+    for k in ["X", "P", "XP", "XX", "PP", "Alpha1", "D1V0", "XD1V0", "PD1V0", "PAlpha1", "XAlpha1", "energy",
+              "negativity", "antiparticle"]:
+        ref = np.array([r[k] for r in rec])
+        np.testing.assert_allclose(o[k][:steps], ref, rtol=rtol, atol=atol, err_msg=k)

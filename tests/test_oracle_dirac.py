@@ -154,3 +154,142 @@ def test_run_normalises_and_records():
     W, rec = orc.run(initial(orc), 3)
     assert len(rec) == 3 and abs(orc.norm(W) - 1.0) < 1e-14
     assert {"X", "Alpha1", "negativity", "antiparticle", "energy"} <= set(rec[0])
+
+
+# ---------------------------------------------------------------------------- closed-form pins of
+# the record columns, the order-2 friction and the Gross-Pitaevskii shift (each fails on a wrong
+# plane, a flipped sign or a wrong coefficient)
+ALPHA1 = np.zeros((4, 4))
+ALPHA1[0, 3] = ALPHA1[3, 0] = ALPHA1[1, 2] = ALPHA1[2, 1] = 1.0
+BETA = np.diag([1.0, 1.0, -1.0, -1.0])
+
+
+def test_ehrenfest_velocity_is_c_alpha1():
+    """d<x>/dt = c <alpha_1> (Alpha_1_Average record, D44.py:3074-3100, 3435): central differences
+    of the X record against c Re Alpha1 over a run in the soft Coulomb well (no absorber).  Another
+    plane choice (alpha_2, alpha_3) or a sign slip misses by O(1)."""
+    dt = 0.0005
+    orc = make(X=128, P=128, absorb=False)
+    orc.dt = orc.k_dt = dt
+    W0 = initial(orc, x0=-2.0, px=3.0)
+    _, rec = orc.run(W0, 40)
+    X = np.array([np.real(r["X"]) for r in rec])
+    a1 = np.array([r["Alpha1"] for r in rec])
+    v = (X[2:] - X[:-2]) / (2.0 * dt)
+    np.testing.assert_allclose(v, orc.c * np.real(a1[1:-1]), rtol=2e-3)
+    assert np.abs(np.imag(a1)).max() < 1e-12  # alpha_1 is Hermitian: a real expectation value
+    assert 0.8 < np.real(a1[0]) < 1.0  # px = 3, m = c = 1: velocity ~ 3/sqrt(10) ~ 0.95
+
+    # the other record columns through their own Ehrenfest identities (Weyl symbols):
+    #   d<p>/dt = -<V0'>,  d<x^2>/dt = 2c<x alpha_1>,  d<xp>/dt = c<p alpha_1> - <x V0'>,
+    #   d<p^2>/dt = -2<p V0'>
+    def col(k):
+        return np.real(np.array([r[k] for r in rec]))
+
+    def ddt(a):
+        return (a[2:] - a[:-2]) / (2.0 * dt)
+
+    def close(lhs, rhs):
+        scale = np.abs(rhs).max()
+        assert np.abs(lhs - rhs).max() < 3e-3 * scale, (np.abs(lhs - rhs).max(), scale)
+
+    close(ddt(col("P")), -col("D1V0")[1:-1])
+    close(ddt(col("XX")), 2.0 * orc.c * col("XAlpha1")[1:-1])
+    close(ddt(col("XP")), orc.c * col("PAlpha1")[1:-1] - col("XD1V0")[1:-1])
+    close(ddt(col("PP")), -2.0 * col("PD1V0")[1:-1])
+
+
+def _spinor_energy(orc, x0, px, s, V0):
+    """<psi|c alpha_1 p + beta m c^2 + V0(x)|psi> of the Gaussian spinor, from its own 1-D
+    momentum-space and position-space representations (independent of the Wigner planes)."""
+    n = 4096
+    xs = np.linspace(-40.0, 40.0, n, endpoint=False)
+    dx = xs[1] - xs[0]
+    psi = gaussian_spinor_particle_up(x0, px, s, xs, orc.mass, orc.c)
+    norm = float(np.sum(np.abs(psi) ** 2)) * dx
+    k = 2.0 * np.pi * np.fft.fftfreq(n, dx)
+    pk = np.fft.fft(psi, axis=1)
+    nk = float(np.sum(np.abs(pk) ** 2))
+    kin = orc.c * np.real(np.einsum("ik,ij,jk->", np.conj(pk), ALPHA1, pk * k)) / nk
+    rest = orc.mass * orc.c ** 2 * np.real(np.einsum("ik,ij,jk->", np.conj(pk), BETA, pk)) / nk
+    pot = float(np.sum(np.abs(psi) ** 2 * V0(0.0, xs))) * dx / norm
+    return kin + rest + pot
+
+
+def test_energy_record_equals_spinor_hamiltonian_and_is_conserved():
+    """DiracEnergy (D44.py:2777-2805) of the initial Wigner function equals <psi|H|psi> of the
+    spinor it was formed from (c alpha_1 p + beta m c^2 + V0), and with a time-independent V0, no
+    absorber and no damping the record is conserved to O(dt^2) while the particle accelerates
+    into the well (a flipped beta or a missing V term is not conserved)."""
+    orc = make(X=128, P=128, absorb=False, compute_energy=True)
+    W0 = initial(orc, x0=-2.0, px=3.0, s=1.0)
+    E0 = np.real(orc.observables(W0, 0.0)["energy"])
+    np.testing.assert_allclose(E0, _spinor_energy(orc, -2.0, 3.0, 1.0, coulomb), rtol=1e-9)
+    orc.dt = orc.k_dt = 0.002
+    _, rec = orc.run(W0, 300)
+    E = np.array([np.real(r["energy"]) for r in rec])
+    drift = np.abs(E - E[0]).max()
+    assert drift < 2e-4, drift
+    # the kinetic part did change: the test would see a wrong split between the terms
+    kin = np.array([np.real(r["PAlpha1"]) for r in rec])
+    assert abs(kin[-1] - kin[0]) > 50 * drift
+
+
+def test_caldeira_leggett_order2_friction_factor():
+    """CaldeiraDissipatorOrder2 (D44.py:3019-3058) with sigma(p) = p: D(W) = -i d/dp (p W), so
+    W + 2aDW + 2a^2 D^2 W (a = i gamma dt) multiplies <p> by exactly 1 - 2 gamma dt + 2 (gamma dt)^2
+    per step for a free spinor (kinetic and mass steps keep <p>; the trace norm is unchanged).
+    A wrong coefficient of the second-order term shifts the factor by (gamma dt)^2 = 2.5e-5."""
+    g, dt = 0.5, 0.01
+    orc = make(X=128, P=128, potentials=[zero, zero, zero, zero], gamma=g, damping="CaldeiraLegget",
+               absorb=False)
+    orc.dt = orc.k_dt = dt
+    W0 = initial(orc, x0=-2.0, px=3.0)
+    _, rec = orc.run(W0, 5)
+    P = np.array([np.real(r["P"]) for r in rec])
+    np.testing.assert_allclose(P[1:] / P[:-1], 1.0 - 2.0 * g * dt + 2.0 * (g * dt) ** 2, rtol=1e-10)
+    # and the direct operator: D(p-Gaussian) against -i d/dp (p W) in closed form
+    p = orc.p_host
+    Y = np.exp(-((p - 1.0) ** 2)) * np.ones((1, orc.X))
+    D = orc.theta_to_p(orc.p_to_theta(Y * p) * orc.theta_host)
+    dY = -1j * (np.exp(-((p - 1.0) ** 2)) * (1.0 - 2.0 * p * (p - 1.0))) * np.ones((1, orc.X))
+    np.testing.assert_allclose(D, dY, atol=1e-10)
+
+
+def test_gp_terms_shift_theorem_on_trace_density():
+    """B-(theta,x) = rho(x - theta/2), B+ = rho(x + theta/2) with rho = Re sum_p tr W (no dP)
+    (MakeGrossPitaevskiiTerms, D44.py:2695-2715): a state whose trace density is an analytic
+    Gaussian, with unrelated content in the off-diagonal planes (which must not contribute)."""
+    orc = make(X=256, P=64, gp_coeff=1.0)
+    x, th = orc.x_host, orc.theta_host
+    W = np.zeros((4, 4, orc.P, orc.X), dtype=np.complex128)
+    rho = lambda y: np.exp(-((y - 0.7) ** 2) / 1.5)  # noqa: E731
+    # split the density over the four diagonal planes and put it in the p = 0 row only
+    for k, w in zip(range(4), (0.4, 0.3, 0.2, 0.1)):
+        W[k, k, 0] = w * rho(x[0])
+    W[0, 3] = 5.0 + 2.0j  # off-diagonal planes: ignored by the trace
+    W[1, 2, 3] = -7.0
+    Bm, Bp = orc.gp_terms(W)
+    inside = np.abs(th) < 8.0
+    sel = np.broadcast_to(inside, Bm.shape)
+    np.testing.assert_allclose(Bm[sel], np.broadcast_to(rho(x - 0.5 * th), Bm.shape)[sel], atol=1e-10)
+    np.testing.assert_allclose(Bp[sel], np.broadcast_to(rho(x + 0.5 * th), Bp.shape)[sel], atol=1e-10)
+
+
+def test_absorber_projector_and_population_pins():
+    """Absorber 1 - exp(-((j - X/2)/24)^2) over the array index (D44.py:111-155): zero at the
+    array middle (x = +-X_amp in FFT order), -> 1 far from it; the electron projectors
+    (Filter_Kernel, :674-760) are complementary orthogonal projectors for every q."""
+    orc = make(X=128, P=64)
+    a = orc.absorber()[0]
+    assert a[64] == 0.0 and a[0] > 0.99 and abs(a[64 + 24] - (1.0 - np.exp(-1.0))) < 1e-15
+    q = np.linspace(-30.0, 30.0, 61)
+    Pp, Pm = orc.projector(q, +1), orc.projector(q, -1)
+    eye = np.eye(4)[:, :, None]
+    np.testing.assert_allclose(Pp + Pm, np.broadcast_to(eye, Pp.shape), atol=1e-15)
+    np.testing.assert_allclose(np.einsum("ik...,kj...->ij...", Pp, Pp), Pp, atol=1e-14)
+    np.testing.assert_allclose(np.einsum("ik...,kj...->ij...", Pp, Pm), 0.0, atol=1e-14)
+    # the positive-energy projector commutes with the free Hamiltonian c alpha_1 q + beta m c^2
+    H = orc.c * ALPHA1[:, :, None] * q + BETA[:, :, None] * orc.mass * orc.c ** 2
+    np.testing.assert_allclose(np.einsum("ik...,kj...->ij...", H, Pp), np.einsum("ik...,kj...->ij...", Pp, H),
+                               atol=1e-12)
