@@ -81,11 +81,34 @@ int wh_version(void);
 /* ---- plain axis transforms: replaces cufft_wrapper.py ----------------- */
 typedef struct wh_fft_plan wh_fft_plan;
 
+/* cuFFT type codes, as cufft_wrapper.py:8-9 passes them */
+#define WH_CUFFT_Z2Z 0x69
+#define WH_CUFFT_C2C 0x29
+
 /*
- * Replaces cufftPlanMany (cufft_wrapper.py:118-145) for the plans the Wigner
- * path builds: rank 1, length n (a power of two, 64..2048), either
+ * Replaces cufftPlanMany (cufft_wrapper.py:118-145) with its own argument list, so
+ * that the reference's ctypes binding passes its arguments unchanged: n, inembed
+ * and onembed are addresses of int arrays (n.ctypes.data, :165-167, :199-201,
+ * :234-236), type is CUFFT_Z2Z or CUFFT_C2C.  Accepted (every length a power of
+ * two in 64..2048, in place, so onembed/ostride/odist equal the input's):
+ *   rank 1, inembed NULL            batch contiguous transforms (Plan_Z2Z((n,)),
+ *                                   :160-167, istride 1, idist 0 as passed there)
+ *   rank 1, istride 1, idist n[0]   batch contiguous rows (Plan_Z2Z_2D_Axis1, :225-236)
+ *   rank 1, istride batch, idist 1  batch interleaved columns of an [n][batch] array
+ *                                   (Plan_Z2Z_2D_Axis0, :190-201)
+ *   rank 2, inembed NULL            batch contiguous [n[0]][n[1]] 2-D transforms
+ *                                   (Plan_Z2Z((P, X)), GPU_Wigner2D_FFT.py:860)
+ * inembed[0] (the outermost extent) is not read, as in cuFFT.  Allocates the
+ * twiddle tables on the current device.
+ */
+int wh_cufft_plan_many(wh_fft_plan** plan, int rank, const int* n, const int* inembed, int istride, int idist,
+                       const int* onembed, int ostride, int odist, int type, int batch);
+/*
+ * The rank-1 plans of wh_cufft_plan_many with the length passed by value and the
+ * library's precision codes: length n (a power of two, 64..2048), either
  *   istride == 1, idist == n      batch contiguous rows  (Plan_Z2Z_2D_Axis1, :218-250;
- *                                                         Plan_Z2Z 1-D, :147-180)
+ *                                                         Plan_Z2Z 1-D, :147-180;
+ *                                                         with batch 1 any idist)
  *   istride == batch, idist == 1  batch interleaved columns of an [n][batch] array
  *                                                         (Plan_Z2Z_2D_Axis0, :183-215)
  * type: WH_C128 (Z2Z) or WH_C64 (C2C).  Allocates the twiddle table on the
@@ -99,7 +122,7 @@ int wh_fft_plan_many(wh_fft_plan** plan, int n, int istride, int idist, int type
 int wh_fft_exec(wh_fft_plan* plan, void* idata, void* odata, int direction, void* stream);
 /* As wh_fft_exec, with the 1/n of the callers' "*= 1./n" (GPU_Wigner2D_FFT.py:790-801) folded in. */
 int wh_fft_exec_normalized(wh_fft_plan* plan, void* idata, void* odata, int direction, void* stream);
-/* Replaces cufftDestroy (cufft_wrapper.py:109-115). */
+/* Replaces cufftDestroy (cufft_wrapper.py:109-115); any plan of the two creators. */
 int wh_fft_destroy(wh_fft_plan* plan);
 
 /* ---- fused Wigner2D propagator ---------------------------------------- */

@@ -171,6 +171,7 @@ def _declare(lib):
         "wh_status_string": ([ip], ctypes.c_char_p),
         "wh_version": ([], ip),
         "wh_fft_plan_many": ([POINTER(vp), ip, ip, ip, ip, ip], ip),
+        "wh_cufft_plan_many": ([POINTER(vp), ip, vp, vp, ip, ip, vp, ip, ip, ip, ip], ip),
         "wh_fft_exec": ([vp, vp, vp, ip, vp], ip),
         "wh_fft_exec_normalized": ([vp, vp, vp, ip, vp], ip),
         "wh_fft_destroy": ([vp], ip),

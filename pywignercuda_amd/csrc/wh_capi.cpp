@@ -222,11 +222,38 @@ extern "C" int wh_variant_info(int table, int i, int* radix, int* lanes) {
 }
 
 // ============================================================================ FFT plans
+// A plan is one axis transform (rank 1: batch rows, or batch interleaved columns of an
+// [n][batch] array) or a 2-D transform of batch contiguous [n][n2] arrays (rank 2: the
+// columns pass, then the rows pass).
 struct wh_fft_plan {
-  int n = 0, type = 0, batch = 0;
-  bool columns = false;  // interleaved columns of an [n][batch] array
-  TwSet tw;
+  int rank = 1;
+  int n = 0, n2 = 0, type = 0, batch = 0;
+  bool columns = false;  // rank 1: interleaved columns of an [n][batch] array
+  TwSet tw, tw2;         // rank 2: tw for the length-n columns, tw2 for the length-n2 rows
 };
+
+namespace {
+int fft_plan_make(wh_fft_plan** plan, int rank, int n, int n2, bool columns, int type, int batch) {
+  auto p = std::make_unique<wh_fft_plan>();
+  p->rank = rank;
+  p->n = n;
+  p->n2 = n2;
+  p->type = type;
+  p->batch = batch;
+  p->columns = columns;
+  WH_TRY(make_twiddles(p->tw, n, type, false));
+  if (rank == 2) WH_TRY(make_twiddles(p->tw2, n2, type, false));
+  *plan = p.release();
+  return WH_SUCCESS;
+}
+int check_column_batch(const char* who, int type, int n, int batch) {
+  const int c = wh::fft_cols_batch_multiple(type, n);
+  if (c <= 0 || batch % c != 0)
+    return fail(WH_INVALID_SIZE, "%s: a column transform of length %d needs a batch that is a multiple of %d (got %d)",
+                who, n, c, batch);
+  return WH_SUCCESS;
+}
+}  // namespace
 
 extern "C" int wh_fft_plan_many(wh_fft_plan** plan, int n, int istride, int idist, int type, int batch) {
   if (!plan) return fail(WH_INVALID_VALUE, "wh_fft_plan_many: plan is NULL");
@@ -237,26 +264,55 @@ extern "C" int wh_fft_plan_many(wh_fft_plan** plan, int n, int istride, int idis
     return fail(WH_INVALID_SIZE, "wh_fft_plan_many: length %d is not a power of two in [64, 2048]", n);
   if (batch < 1) return fail(WH_INVALID_VALUE, "wh_fft_plan_many: batch %d < 1", batch);
   bool columns;
-  if (istride == 1 && idist == n) {
-    columns = false;
+  if (istride == 1 && (idist == n || batch == 1)) {
+    columns = false;  // one transform: the distance between transforms is never used
   } else if (istride == batch && idist == 1) {
     columns = true;
-    const int c = wh::fft_cols_batch_multiple(type, n);
-    if (batch % c != 0)
-      return fail(WH_INVALID_SIZE, "wh_fft_plan_many: column batch %d is not a multiple of %d", batch, c);
+    WH_TRY(check_column_batch("wh_fft_plan_many", type, n, batch));
   } else {
     return fail(WH_INVALID_VALUE,
                 "wh_fft_plan_many: layout istride=%d idist=%d is neither rows (1, n) nor columns (batch, 1)",
                 istride, idist);
   }
-  auto p = std::make_unique<wh_fft_plan>();
-  p->n = n;
-  p->type = type;
-  p->batch = batch;
-  p->columns = columns;
-  WH_TRY(make_twiddles(p->tw, n, type, false));
-  *plan = p.release();
-  return WH_SUCCESS;
+  return fft_plan_make(plan, 1, n, 0, columns, type, batch);
+}
+
+extern "C" int wh_cufft_plan_many(wh_fft_plan** plan, int rank, const int* n, const int* inembed, int istride,
+                                  int idist, const int* onembed, int ostride, int odist, int type, int batch) {
+  const char* who = "wh_cufft_plan_many";
+  if (!plan) return fail(WH_INVALID_VALUE, "%s: plan is NULL", who);
+  *plan = nullptr;
+  int prec;
+  if (type == WH_CUFFT_Z2Z) {
+    prec = WH_C128;
+  } else if (type == WH_CUFFT_C2C) {
+    prec = WH_C64;
+  } else {
+    return fail(WH_INVALID_TYPE, "%s: type %#x is neither CUFFT_Z2Z (0x69) nor CUFFT_C2C (0x29)", who, type);
+  }
+  if (rank != 1 && rank != 2) return fail(WH_INVALID_VALUE, "%s: rank %d (1 or 2 supported)", who, rank);
+  if (!n) return fail(WH_INVALID_VALUE, "%s: n is NULL", who);
+  if (batch < 1) return fail(WH_INVALID_VALUE, "%s: batch %d < 1", who, batch);
+  for (int k = 0; k < rank; ++k)
+    if (!size_supported(n[k]))
+      return fail(WH_INVALID_SIZE, "%s: n[%d] = %d is not a power of two in [64, 2048]", who, k, n[k]);
+  if ((inembed == nullptr) != (onembed == nullptr))
+    return fail(WH_INVALID_VALUE, "%s: in-place transforms need inembed and onembed both NULL or both set", who);
+  if (!inembed) {
+    // basic layout: batch contiguous arrays, strides and distances ignored
+    if (rank == 2) {
+      WH_TRY(check_column_batch(who, prec, n[0], n[1]));
+      return fft_plan_make(plan, 2, n[0], n[1], false, prec, batch);
+    }
+    return fft_plan_make(plan, 1, n[0], 0, false, prec, batch);
+  }
+  if (rank != 1)
+    return fail(WH_INVALID_VALUE, "%s: a rank-2 plan with an advanced layout (inembed set) is not supported", who);
+  if (ostride != istride || odist != idist)
+    return fail(WH_INVALID_VALUE, "%s: output layout (%d, %d) differs from the input layout (%d, %d)", who, ostride,
+                odist, istride, idist);
+  // rank 1 with an advanced layout; inembed[0] (the outermost extent) is not used
+  return wh_fft_plan_many(plan, n[0], istride, idist, prec, batch);
 }
 
 static int fft_exec(wh_fft_plan* plan, void* idata, void* odata, int dir, int norm, void* stream) {
@@ -267,10 +323,24 @@ static int fft_exec(wh_fft_plan* plan, void* idata, void* odata, int dir, int no
   const size_t align = plan->type == WH_C128 ? 16 : 8;
   if (reinterpret_cast<uintptr_t>(idata) % align)
     return fail(WH_UNALIGNED_DATA, "wh_fft_exec: data not %zu-byte aligned", align);
-  const void* hi = plan->type == WH_C128 ? plan->tw.hi64.p : plan->tw.hi.p;
-  WH_HIP(wh::launch_fft_axis(plan->type, plan->columns, plan->n, dir, idata, plan->batch, norm, hi, plan->tw.lo.p,
-                             static_cast<hipStream_t>(stream)),
-         "wh_fft_exec launch");
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const auto hi = [&](const TwSet& t) { return plan->type == WH_C128 ? t.hi64.p : t.hi.p; };
+  if (plan->rank == 1) {
+    WH_HIP(wh::launch_fft_axis(plan->type, plan->columns, plan->n, dir, idata, plan->batch, norm, hi(plan->tw),
+                               plan->tw.lo.p, s),
+           "wh_fft_exec launch");
+    return WH_SUCCESS;
+  }
+  // rank 2, batch contiguous [n][n2] arrays: columns (length n), then rows (length n2);
+  // each pass folds in its own 1/length when normalising
+  const size_t bytes = size_t(plan->n) * size_t(plan->n2) * (plan->type == WH_C128 ? 16 : 8);
+  for (int b = 0; b < plan->batch; ++b) {
+    void* d = static_cast<char*>(idata) + size_t(b) * bytes;
+    WH_HIP(wh::launch_fft_axis(plan->type, 1, plan->n, dir, d, plan->n2, norm, hi(plan->tw), plan->tw.lo.p, s),
+           "wh_fft_exec launch (columns)");
+    WH_HIP(wh::launch_fft_axis(plan->type, 0, plan->n2, dir, d, plan->n, norm, hi(plan->tw2), plan->tw2.lo.p, s),
+           "wh_fft_exec launch (rows)");
+  }
   return WH_SUCCESS;
 }
 

@@ -1,9 +1,11 @@
 """Drop-in for the reference's cufft_wrapper.py, backed by libwignerhip's own FFT.
 
 Same names and semantics as cufft_wrapper.py: status -> exception map
-(:15-73), Plan_Z2Z / Plan_Z2Z_2D_Axis0 / Plan_Z2Z_2D_Axis1 and their C2C
-siblings (:147-250), cufftExecZ2Z / cufftExecC2C / cufftDestroy (:81-115),
-fft_Z2Z / ifft_Z2Z (:396-403).  In place, inverse unnormalised.
+(:15-73), cufftPlanMany with its own argument list (:118-145, through
+wh_cufft_plan_many), Plan_Z2Z (1-D and 2-D) / Plan_Z2Z_2D_Axis0 /
+Plan_Z2Z_2D_Axis1 and their C2C siblings (:147-250), cufftExecZ2Z /
+cufftExecC2C / cufftDestroy (:81-115), fft_Z2Z / ifft_Z2Z (:396-403).  In
+place, inverse unnormalised.
 
 Arrays are torch CUDA tensors (complex128 for Z2Z, complex64 for C2C) or raw
 integer device addresses, as the reference passed ``int(gpuarray.gpudata)``.
@@ -13,6 +15,8 @@ garbage collected.
 from __future__ import annotations
 
 import ctypes
+
+import numpy as np
 
 from . import _lib
 
@@ -40,21 +44,20 @@ def cufftCheckStatus(status):
     _lib.check(status)
 
 
-def _type_code(fft_type):
-    if fft_type == CUFFT_Z2Z:
-        return _lib.WH_C128
-    if fft_type == CUFFT_C2C:
-        return _lib.WH_C64
-    raise cufftInvalidType(f"fft type {fft_type:#x}")
-
-
 class _Plan:
-    def __init__(self, n, istride, idist, fft_type, batch):
+    """Owns one plan handle of libwignerhip (wh_cufft_plan_many's argument list, which is
+    cufftPlanMany's: cufft_wrapper.py:118-145)."""
+
+    def __init__(self, rank, n, inembed, istride, idist, fft_type, batch):
         self.fft_type = fft_type
         self.batch = batch
+        nn = np.ascontiguousarray(np.asarray(n, dtype=np.int32).reshape(-1))
+        emb = None if inembed is None else np.ascontiguousarray(np.asarray(inembed, dtype=np.int32).reshape(-1))
+        addr = lambda a: None if a is None else ctypes.c_void_p(a.ctypes.data)  # noqa: E731
         h = ctypes.c_void_p()
-        _lib.check(_lib.lib().wh_fft_plan_many(ctypes.byref(h), int(n), int(istride), int(idist),
-                                               _type_code(fft_type), int(batch)), "cufftPlanMany")
+        _lib.check(_lib.lib().wh_cufft_plan_many(ctypes.byref(h), int(rank), addr(nn), addr(emb), int(istride),
+                                                 int(idist), addr(emb), int(istride), int(idist), int(fft_type),
+                                                 int(batch)), "cufftPlanMany")
         self.handle = h
 
     def destroy(self):
@@ -69,24 +72,44 @@ class _Plan:
             pass
 
 
+class _RawPlan(_Plan):
+    """A plan created from raw cufftPlanMany arguments: n, inembed and onembed as addresses
+    of int arrays (what the reference passes, e.g. ``n.ctypes.data``) or as sequences."""
+
+    def __init__(self, rank, n, inembed, istride, idist, onembed, ostride, odist, fft_type, batch):
+        self.fft_type = fft_type
+        self.batch = batch
+        held = [_int_array(a) for a in (n, inembed, onembed)]  # kept alive over the call
+        ptr = [None if a is None else ctypes.c_void_p(a if isinstance(a, int) else a.ctypes.data) for a in held]
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().wh_cufft_plan_many(ctypes.byref(h), int(rank), ptr[0], ptr[1], int(istride), int(idist),
+                                                 ptr[2], int(ostride), int(odist), int(fft_type), int(batch)),
+                   "cufftPlanMany")
+        self.handle = h
+
+
+def _int_array(a):
+    """None, an address (int), or an int32 copy of a sequence."""
+    if a is None or isinstance(a, int):
+        return a
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(-1))
+
+
 def cufftPlanMany(rank, n, inembed, istride, idist, onembed, ostride, odist, fft_type, batch):
-    """Batched plan (cufft_wrapper.py:131-145); rank 1 and the row/column layouts only."""
-    if rank != 1:
-        raise cufftInvalidValue("only rank-1 batched plans are supported")
-    n0 = int(n[0]) if hasattr(n, "__len__") else int(n)
-    if (ostride, odist) != (istride, idist):
-        raise cufftInvalidValue("output layout must equal the input layout (in place)")
-    return _Plan(n0, istride, idist, fft_type, batch)
+    """Batched plan with cufftPlanMany's arguments (cufft_wrapper.py:131-145): n, inembed and
+    onembed as addresses of int arrays (what the reference passes) or sequences."""
+    return _RawPlan(rank, n, inembed, istride, idist, onembed, ostride, odist, fft_type, batch)
 
 
 class Plan_Z2Z(_Plan):
-    """1-D complex128 plan (cufft_wrapper.py:147-180); shape (n,), batch contiguous transforms."""
+    """complex128 plan (cufft_wrapper.py:147-180): shape (n,) or (n0, n1), batch contiguous
+    transforms, cuFFT's basic layout."""
 
     def __init__(self, shape, batch=1, fft_type=CUFFT_Z2Z):
-        if len(shape) != 1:
-            raise ValueError("only 1-D shapes are supported (the 2-D plan of the reference is unused)")
+        if len(shape) == 0:
+            raise ValueError("invalid transform size")
         self.shape = shape
-        super().__init__(shape[0], 1, shape[0], fft_type, batch)
+        super().__init__(len(shape), shape, None, 1, 0, fft_type, batch)
 
 
 class Plan_Z2Z_2D_Axis0(_Plan):
@@ -96,7 +119,7 @@ class Plan_Z2Z_2D_Axis0(_Plan):
         if len(shape) != 2:
             raise ValueError("invalid transform dimension")
         self.shape = shape
-        super().__init__(shape[0], shape[1], 1, fft_type, shape[1])
+        super().__init__(1, [shape[0]], [shape[0], shape[1]], shape[1], 1, fft_type, shape[1])
 
 
 class Plan_Z2Z_2D_Axis1(_Plan):
@@ -106,7 +129,7 @@ class Plan_Z2Z_2D_Axis1(_Plan):
         if len(shape) != 2:
             raise ValueError("invalid transform dimension")
         self.shape = shape
-        super().__init__(shape[1], 1, shape[1], fft_type, shape[0])
+        super().__init__(1, [shape[1]], list(shape), 1, shape[1], fft_type, shape[0])
 
 
 class Plan_C2C(Plan_Z2Z):

@@ -236,3 +236,40 @@ def test_dirac_plan_rejects_unsupported_physics():
         WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=("0.", "0.", " tanh(x) ", "0."))
     with pytest.raises(NotImplementedError):
         WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, gamma=0.1, damping="ODM")
+
+
+def test_reference_binding_accepts_the_reference_plan_arguments():
+    """The reference-side ctypes binding with the exact cufftPlanMany arguments of the
+    reference's plan classes at 512^2: every layout passes validation (on a host without a
+    GPU the call then fails only at the device allocation), and layouts outside the set
+    are refused with cuFFT's codes."""
+    from pywignercuda_amd import _lib
+    from refplans import StatusError, load_binding, reference_plan_calls
+
+    ns = load_binding(_lib.LIB_PATH)
+    device_codes = {0, 2, 5, 6, 7, 12}  # success or a failure of the device allocation
+    for name, args, keep in reference_plan_calls(512, 512):
+        try:
+            plan = ns["cufftPlanMany"](*args)
+            ns["cufftDestroy"](plan)
+        except StatusError as e:
+            assert e.status in device_codes, (name, e.status, _lib.last_error())
+    for name, args, keep in reference_plan_calls(512, 512, fft_type=0x29):
+        try:
+            ns["cufftDestroy"](ns["cufftPlanMany"](*args))
+        except StatusError as e:
+            assert e.status in device_codes, (name, e.status, _lib.last_error())
+    n = np.asarray((512, 512, 512), np.int32)
+    bad = [
+        ((3, n.ctypes.data, None, 1, 0, None, 1, 0, 0x69, 1), 4),  # rank 3
+        ((1, n.ctypes.data, None, 1, 0, None, 1, 0, 0x2A, 1), 3),  # C2R
+        ((1, n.ctypes.data, n.ctypes.data, 2, 7, n.ctypes.data, 2, 7, 0x69, 4), 4),  # strides
+        ((1, n.ctypes.data, n.ctypes.data, 1, 512, n.ctypes.data, 512, 1, 0x69, 4), 4),  # out layout
+        ((2, n.ctypes.data, n.ctypes.data, 1, 512, n.ctypes.data, 1, 512, 0x69, 1), 4),  # 2-D advanced
+    ]
+    m = np.asarray((100,), np.int32)
+    bad.append(((1, m.ctypes.data, None, 1, 0, None, 1, 0, 0x69, 1), 8))
+    for args, code in bad:
+        with pytest.raises(StatusError) as ei:
+            ns["cufftPlanMany"](*args)
+        assert ei.value.status == code, (args, ei.value.status)

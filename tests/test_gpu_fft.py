@@ -57,3 +57,40 @@ def test_errors_map_to_reference_exceptions(cuda):
     u = torch.zeros_like(t)
     with pytest.raises(cw.cufftInvalidValue):
         cw.cufftExecZ2Z(plan, t, u, cw.CUFFT_FORWARD)  # out of place is not supported
+
+
+def test_reference_binding_runs_the_reference_plans(cuda):
+    """The reference-side ctypes binding, called as the reference's plan classes and
+    fft_Z2Z / ifft_Z2Z call cuFFT (cufft_wrapper.py:160-236, 396-403; raw device addresses,
+    null stream), at 512^2: the 2-D plan, the 1-D plan of SetFFT_Plans and both axis plans
+    against numpy.fft; inverse unnormalised."""
+    import torch
+
+    from pywignercuda_amd import _lib
+    from refplans import load_binding, reference_plan_calls
+
+    ns = load_binding(_lib.LIB_PATH)
+    P = X = 512
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((P, X)) + 1j * rng.standard_normal((P, X))
+    fwd = {"Plan_Z2Z((P,X))": lambda z: np.fft.fft2(z), "Plan_Z2Z((X,))": lambda z: np.fft.fft(z[0]),
+           "Plan_Z2Z_2D_Axis0((P,X))": lambda z: np.fft.fft(z, axis=0),
+           "Plan_Z2Z_2D_Axis1((P,X))": lambda z: np.fft.fft(z, axis=1)}
+    norm = {"Plan_Z2Z((P,X))": P * X, "Plan_Z2Z((X,))": X, "Plan_Z2Z_2D_Axis0((P,X))": P,
+            "Plan_Z2Z_2D_Axis1((P,X))": X}
+    for name, args, keep in reference_plan_calls(P, X):
+        plan = ns["cufftPlanMany"](*args)
+        data = a[:1] if name == "Plan_Z2Z((X,))" else a
+        t = torch.tensor(data, dtype=torch.complex128, device=cuda)
+        torch.cuda.synchronize()
+        addr = int(t.data_ptr())
+        ns["cufftExecZ2Z"](plan, addr, addr, -1)
+        torch.cuda.synchronize()
+        ref = fwd[name](data).reshape(t.shape)
+        err = np.abs(t.cpu().numpy() - ref).max() / np.abs(ref).max()
+        assert err < 1e-13, (name, err)
+        ns["cufftExecZ2Z"](plan, addr, addr, 1)
+        torch.cuda.synchronize()
+        err = np.abs(t.cpu().numpy() / norm[name] - data).max() / np.abs(data).max()
+        assert err < 1e-13, (name, err)
+        ns["cufftDestroy"](plan)
