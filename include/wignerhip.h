@@ -291,6 +291,66 @@ int wh_ensemble_set_potential(wh_ensemble* e, int member, const double* coef, in
 /* The single-grid plan's underlying ensemble (one member). */
 wh_ensemble* wh_wigner2d_ensemble(wh_wigner2d* plan);
 
+/* ---- imaginary-time (Bloch) Gross-Pitaevskii solver ------------------------ */
+/*
+ * Replaces the per-instance set-up of GPU_Wigner2D_GPitaevskii_Bloch
+ * (GPU_Wigner2D_GPitaevskii.py:2064-2103: SetCUDA_Constants, the Kernel_Bloch
+ * functions of :88-110, :160-175, :498-517, the Hamiltonian grid of :2094).
+ * Host arrays, copied at creation; constants are the kernel-side values.
+ */
+#define WH_BNHIST 12 /* doubles per Bloch history row */
+/* history columns: dt used by the step, <x>, <x^2>, <p>, <p^2>, Hamiltonian_average,
+   total energy (with the non-linear term), non-linear energy, purity (re, im),
+   1 kept / 0 refused / -1 untested (first iteration), Im of the total energy */
+#define WH_BHIST_DT 0
+#define WH_BHIST_X 1
+#define WH_BHIST_X2 2
+#define WH_BHIST_P 3
+#define WH_BHIST_P2 4
+#define WH_BHIST_H 5
+#define WH_BHIST_E 6
+#define WH_BHIST_NLE 7
+#define WH_BHIST_PURITY 8
+#define WH_BHIST_PURITY_IM 9
+#define WH_BHIST_KEPT 10
+typedef struct {
+  int struct_size; /* sizeof(wh_bloch_desc) */
+  int x_dim, p_dim; /* powers of two, 128..2048 */
+  double dx, dp, dtheta, dlambda, hbar; /* kernel constants */
+  /* V(t, x) polynomial rows as in wh_wigner2d_desc, or pot_deg = WH_POT_TABLE with
+     pot_sum[P][X] = V(x - hbar theta/2) + V(x + hbar theta/2) at kernel coordinates */
+  int pot_deg, pot_rows;
+  const double* pot_coef;
+  const double* pot_sum;
+  int kinetic_deg;            /* K(p) = sum kin_coef[n] p^n */
+  double kin_coef[WH_MAX_DEG + 1];
+  double gp_coeff;            /* kernel a_GP; 0 = off */
+  const double* theta_host;   /* [P] host grids */
+  const double* lambda_host;  /* [X] */
+  const double* x_host;       /* [X] */
+  const double* p_host;       /* [P] */
+  const double* k_host;       /* [P] KineticEnergy(p) on the host grid */
+  const double* v0_host;      /* [X] Potential(0, x) on the host grid */
+  double dxdp, dx_host, dp_host, hbar_host, gp_coeff_host;
+} wh_bloch_desc;
+
+typedef struct wh_bloch wh_bloch;
+
+int wh_bloch_create(const wh_bloch_desc* desc, wh_bloch** out);
+int wh_bloch_destroy(wh_bloch* plan);
+/*
+ * Replaces GPU_Wigner2D_GPitaevskii_Bloch.Run (W_ground NULL, :2105-2340) and
+ * Run_ExitedState1 (W_ground = the ground state, :2343-2535): n_iter iterations
+ * (the reference's timeSteps + 1) from dt.  W (device, [P][X] complex128) holds W_init
+ * on entry and the final state on return; W_step (device, same size, caller-owned) is
+ * the last kept state (it starts as W_init as given).  hist (device) receives n_iter
+ * rows of WH_BNHIST doubles.  Every step is kept or refused on the device (a refused
+ * step halves dt and restores W_step), so the call launches the whole loop without
+ * waiting on the host.
+ */
+int wh_bloch_run(wh_bloch* plan, void* W, void* W_step, const void* W_ground, int n_iter, double dt, double* hist,
+                 void* stream);
+
 /* ---- Wigner-Dirac 4x4 (GPU_WignerDirac2D_4x4, GPU_WignerDirac2D_4x4.py:1880-3720) ---- */
 #define WH_DNOBS 32 /* doubles per Dirac record row */
 /* record columns (complex quantities as re, im pairs): the per-step lists of Run

@@ -156,6 +156,26 @@ class DiracDesc(ctypes.Structure):
     ]
 
 
+class BlochDesc(ctypes.Structure):
+    """wh_bloch_desc of include/wignerhip.h."""
+
+    _fields_ = [
+        ("struct_size", c_int), ("x_dim", c_int), ("p_dim", c_int),
+        ("dx", c_double), ("dp", c_double), ("dtheta", c_double), ("dlambda", c_double), ("hbar", c_double),
+        ("pot_deg", c_int), ("pot_rows", c_int), ("pot_coef", _DP), ("pot_sum", _DP),
+        ("kinetic_deg", c_int), ("kin_coef", c_double * (WH_MAX_DEG + 1)),
+        ("gp_coeff", c_double),
+        ("theta_host", _DP), ("lambda_host", _DP), ("x_host", _DP), ("p_host", _DP),
+        ("k_host", _DP), ("v0_host", _DP),
+        ("dxdp", c_double), ("dx_host", c_double), ("dp_host", c_double), ("hbar_host", c_double),
+        ("gp_coeff_host", c_double),
+    ]
+
+
+WH_BNHIST = 12
+BLOCH_HIST = {"dt": 0, "X": 1, "X2": 2, "P": 3, "P2": 4, "H": 5, "E": 6, "NLE": 7, "purity": 8, "purity_im": 9,
+              "kept": 10, "E_im": 11}
+
 WH_DNOBS = 32
 DIRAC_OBS = {"X": 0, "P": 2, "XP": 4, "XX": 6, "PP": 8, "Alpha1": 10, "D1V0": 12, "XD1V0": 14, "PD1V0": 16,
              "PAlpha1": 18, "XAlpha1": 20, "energy": 22}
@@ -189,6 +209,9 @@ def _declare(lib):
         "wh_ensemble_set_variants": ([vp, ip, ip, ip], ip),
         "wh_ensemble_set_potential": ([vp, ip, vp, ip, ip, vp], ip),
         "wh_variant_info": ([ip, ip, POINTER(ip), POINTER(ip)], ip),
+        "wh_bloch_create": ([POINTER(BlochDesc), POINTER(vp)], ip),
+        "wh_bloch_destroy": ([vp], ip),
+        "wh_bloch_run": ([vp, vp, vp, vp, ip, c_double, vp, vp], ip),
         "wh_dirac_create": ([POINTER(DiracDesc), POINTER(vp)], ip),
         "wh_dirac_destroy": ([vp], ip),
         "wh_dirac_run": ([vp, vp, ip, ip, vp, ip, vp], ip),

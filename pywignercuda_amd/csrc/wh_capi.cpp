@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/wignerhip.h"
+#include "wh_bloch.hpp"
 #include "wh_dirac.hpp"
 #include "wh_kernels.hpp"
 #include "wh_variants.hpp"
@@ -1477,5 +1478,151 @@ extern "C" int wh_dirac_run(wh_dirac* e, void* W, int first_step, int n_steps, d
   else
     WH_TRY(dirac_iterations(e, n_steps, s));
   WH_LAUNCH(wh::launch_dirac_finish(storage_code(e->d.precision), dp, e->d.p_dim, s), "dirac_finish");
+  return WH_SUCCESS;
+}
+
+// ============================================================================ Bloch solver
+struct wh_bloch {
+  wh::BlochArgs a{};
+  TwSet twx, twp;
+  DevBuf pot, vsum, rho, G, theta_host, lambda_host, x_host, p_host, k_host, v0_host, colsums, g_colsums, ctrl;
+};
+
+extern "C" int wh_bloch_create(const wh_bloch_desc* d, wh_bloch** out) {
+  if (!out) return fail(WH_INVALID_VALUE, "wh_bloch_create: out is NULL");
+  *out = nullptr;
+  if (!d) return fail(WH_INVALID_VALUE, "wh_bloch_create: desc is NULL");
+  if (d->struct_size != (int)sizeof(wh_bloch_desc))
+    return fail(WH_INVALID_DESCRIPTOR, "wh_bloch_desc.struct_size %d != %d", d->struct_size, (int)sizeof(wh_bloch_desc));
+  if (!sweep_size_supported(d->x_dim) || !sweep_size_supported(d->p_dim))
+    return fail(WH_INVALID_SIZE, "Bloch grid %d x %d: both sides must be powers of two in [128, 2048]", d->p_dim,
+                d->x_dim);
+  if (d->pot_deg == WH_POT_TABLE) {
+    if (!d->pot_sum) return fail(WH_INVALID_DESCRIPTOR, "pot_deg WH_POT_TABLE needs pot_sum");
+  } else if (d->pot_deg < 0 || d->pot_deg > WH_MAX_DEG || d->pot_rows < 1 || !d->pot_coef) {
+    return fail(WH_INVALID_DESCRIPTOR, "Bloch potential deg %d rows %d coef %p", d->pot_deg, d->pot_rows,
+                (const void*)d->pot_coef);
+  }
+  if (d->kinetic_deg < 0 || d->kinetic_deg > WH_MAX_DEG)
+    return fail(WH_INVALID_DESCRIPTOR, "Bloch kinetic degree %d", d->kinetic_deg);
+  if (!d->theta_host || !d->lambda_host || !d->x_host || !d->p_host || !d->k_host || !d->v0_host)
+    return fail(WH_INVALID_DESCRIPTOR, "Bloch: the host grids, K(p) and V(0, x) tables are required");
+  if (!(d->dxdp > 0.0) || !(d->hbar != 0.0)) return fail(WH_INVALID_DESCRIPTOR, "Bloch: dxdp %g hbar %g", d->dxdp, d->hbar);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WH_NO_DEVICE, "no HIP device");
+  auto b = std::make_unique<wh_bloch>();
+  const size_t X = size_t(d->x_dim), P = size_t(d->p_dim);
+  WH_TRY(make_twiddles(b->twx, d->x_dim, WH_C128, false));
+  WH_TRY(make_twiddles(b->twp, d->p_dim, WH_C128, false));
+  if (d->pot_deg == WH_POT_TABLE) {
+    const double zero_row[WH_MAX_DEG + 1] = {};
+    WH_TRY(b->pot.upload(zero_row, sizeof(zero_row)));
+    WH_TRY(b->vsum.upload(d->pot_sum, X * P * sizeof(double)));
+  } else {
+    WH_TRY(b->pot.upload(d->pot_coef, size_t(d->pot_rows) * (WH_MAX_DEG + 1) * sizeof(double)));
+  }
+  WH_TRY(b->theta_host.upload(d->theta_host, P * sizeof(double)));
+  WH_TRY(b->lambda_host.upload(d->lambda_host, X * sizeof(double)));
+  WH_TRY(b->x_host.upload(d->x_host, X * sizeof(double)));
+  WH_TRY(b->p_host.upload(d->p_host, P * sizeof(double)));
+  WH_TRY(b->k_host.upload(d->k_host, P * sizeof(double)));
+  WH_TRY(b->v0_host.upload(d->v0_host, X * sizeof(double)));
+  WH_TRY(b->colsums.alloc(X * wh::kBCol * sizeof(double)));
+  WH_TRY(b->g_colsums.alloc(X * wh::kBCol * sizeof(double)));
+  WH_TRY(b->ctrl.alloc(wh::kBCtrlSize * sizeof(double)));
+  if (d->gp_coeff != 0.0) {
+    WH_TRY(b->rho.alloc(X * sizeof(double)));
+    WH_TRY(b->G.alloc(X * P * sizeof(double)));
+  }
+  wh::BlochArgs& a = b->a;
+  a.twX = b->twx.hi64.as<double>();
+  a.twP = b->twp.hi64.as<double>();
+  a.X = d->x_dim;
+  a.P = d->p_dim;
+  a.kdx = d->dx;
+  a.kdp = d->dp;
+  a.kdtheta = d->dtheta;
+  a.kdlambda = d->dlambda;
+  a.khbar = d->hbar;
+  a.ka = d->gp_coeff;
+  a.pot_deg = d->pot_deg == WH_POT_TABLE ? 0 : d->pot_deg;
+  a.pot_rows = d->pot_deg == WH_POT_TABLE ? 1 : d->pot_rows;
+  a.pot = b->pot.as<double>();
+  a.vsum = b->vsum.as<double>();
+  a.kdeg = d->kinetic_deg;
+  a.gp = d->gp_coeff != 0.0;
+  for (int k = 0; k <= WH_MAX_DEG; ++k) a.kcoef[k] = d->kin_coef[k];
+  a.rho = b->rho.as<double>();
+  a.G = b->G.as<double>();
+  a.theta_host = b->theta_host.as<double>();
+  a.lambda_host = b->lambda_host.as<double>();
+  a.x_host = b->x_host.as<double>();
+  a.p_host = b->p_host.as<double>();
+  a.k_host = b->k_host.as<double>();
+  a.v0_host = b->v0_host.as<double>();
+  a.colsums = b->colsums.as<double>();
+  a.g_colsums = b->g_colsums.as<double>();
+  a.ctrl = b->ctrl.as<double>();
+  a.dxdp = d->dxdp;
+  a.dx_host = d->dx_host;
+  a.dp_host = d->dp_host;
+  a.hbar_host = d->hbar_host;
+  a.gp_host = d->gp_coeff_host;
+  a.first_test = 0;
+  *out = b.release();
+  return WH_SUCCESS;
+}
+
+extern "C" int wh_bloch_destroy(wh_bloch* b) {
+  if (!b) return fail(WH_INVALID_PLAN, "wh_bloch_destroy: NULL");
+  delete b;
+  return WH_SUCCESS;
+}
+
+extern "C" int wh_bloch_run(wh_bloch* b, void* W, void* W_step, const void* W_ground, int n_iter, double dt,
+                            double* hist, void* stream) {
+  if (!b) return fail(WH_INVALID_PLAN, "wh_bloch_run: plan is NULL");
+  if (!W || !W_step || !hist) return fail(WH_INVALID_VALUE, "wh_bloch_run: W, W_step and hist are required");
+  if (W == W_step || W == W_ground || W_step == W_ground)
+    return fail(WH_INVALID_VALUE, "wh_bloch_run: W, W_step and W_ground must be distinct buffers");
+  if (n_iter < 0) return fail(WH_INVALID_VALUE, "wh_bloch_run: n_iter %d", n_iter);
+  if (!(dt > 0.0)) return fail(WH_INVALID_VALUE, "wh_bloch_run: dt %g", dt);
+  for (const void* p : {(const void*)W, (const void*)W_step, W_ground})
+    if (reinterpret_cast<uintptr_t>(p) % 16) return fail(WH_UNALIGNED_DATA, "wh_bloch_run: %p not 16-byte aligned", p);
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  wh::BlochArgs a = b->a;
+  a.W = W;
+  a.Wstep = W_step;
+  a.Wground = nullptr;
+  a.hist = hist;
+  a.hist_base = 0;
+  const size_t bytes = size_t(a.X) * size_t(a.P) * 16;
+  // the last kept state starts as W_init as given (GPU_Wigner2D_GPitaevskii.py:2146, 2361)
+  WH_HIP(hipMemcpyAsync(W_step, W, bytes, hipMemcpyDeviceToDevice, s), "wh_bloch_run: W_step");
+  WH_LAUNCH(wh::launch_bloch_set(a.ctrl, dt, 0, s), "bloch set");
+  // the initial state: normalise, energy of reference (no W_0 removal yet)
+  WH_LAUNCH(wh::launch_bloch_colsum(a, W, a.colsums, 0, s), "bloch colsum");
+  WH_LAUNCH(wh::launch_bloch_finish(a, wh::kBInit, s), "bloch finish");
+  WH_LAUNCH(wh::launch_bloch_commit(a, s), "bloch commit");
+  WH_LAUNCH(wh::launch_bloch_set(a.ctrl, dt, 0, s), "bloch set");
+  if (W_ground) {
+    a.Wground = W_ground;
+    WH_LAUNCH(wh::launch_bloch_colsum(a, W_ground, const_cast<double*>(a.g_colsums), 0, s), "bloch colsum W_0");
+  }
+  for (int it = 0; it < n_iter; ++it) {
+    if (a.gp) {
+      WH_LAUNCH(wh::launch_bloch_colsum(a, W, nullptr, 1, s), "bloch density");
+      WH_LAUNCH(wh::launch_bloch_gshift(a, s), "bloch gshift");
+    }
+    WH_LAUNCH(wh::launch_bloch_psweep(a, 0, s), "bloch psweep");
+    WH_LAUNCH(wh::launch_bloch_xsweep(a, s), "bloch xsweep");
+    if (a.gp) {
+      WH_LAUNCH(wh::launch_bloch_colsum(a, W, nullptr, 1, s), "bloch density");
+      WH_LAUNCH(wh::launch_bloch_gshift(a, s), "bloch gshift");
+    }
+    WH_LAUNCH(wh::launch_bloch_psweep(a, 1, s), "bloch psweep");
+    WH_LAUNCH(wh::launch_bloch_finish(a, wh::kBStep, s), "bloch finish");
+    WH_LAUNCH(wh::launch_bloch_commit(a, s), "bloch commit");
+  }
   return WH_SUCCESS;
 }

@@ -22,11 +22,27 @@ class NonPolynomialPhysics(ValueError):
     """The expression is not a polynomial of degree <= WH_MAX_DEG in its variable."""
 
 
-def host_eval(expr: str, **variables):
-    """eval of a reference physics string with numpy (GPU_Wigner2D_FFT.py:674-692)."""
+def heaviside_kernel(x):
+    """The device function the reference splices into its kernels: 0 for x < 0, else 1
+    (GPU_Wigner2D_GPitaevskii.py:34-40)."""
+    return np.where(np.asarray(x) < 0.0, 0.0, 1.0)
+
+
+def heaviside_host(x):
+    """The host-side step of Propagator_Base.heaviside: 1/2 at x = 0
+    (GPU_Wigner2D_GPitaevskii.py:675-685)."""
+    x = np.asarray(x, dtype=np.float64)
+    return np.where(x > 0.0, 1.0, np.where(x == 0.0, 0.5, 0.0))
+
+
+def host_eval(expr: str, heaviside=heaviside_kernel, **variables):
+    """eval of a reference physics string with numpy (GPU_Wigner2D_FFT.py:674-692,
+    GPU_Wigner2D_GPitaevskii.py:687-697).  ``Heaviside`` follows the kernels' definition by
+    default (values that the reference computes on the device); pass
+    ``heaviside=heaviside_host`` for the values its host-side Potential computes."""
     ns = dict(variables)
     ns.update(pow=np.power, atan=np.arctan, M_E=np.e, M_PI=np.pi, sqrt=np.sqrt, exp=np.exp,
-              fabs=np.abs, abs=np.abs)
+              fabs=np.abs, abs=np.abs, Heaviside=heaviside)
     return eval(expr, np.__dict__, ns)  # noqa: S307 - the reference's own semantics
 
 
@@ -37,7 +53,7 @@ def _sympy_expr(expr: str, var: str):
     loc = {
         var: v, "t": t, "pow": sp.Pow, "sqrt": sp.sqrt, "exp": sp.exp, "sin": sp.sin,
         "cos": sp.cos, "tan": sp.tan, "tanh": sp.tanh, "atan": sp.atan, "log": sp.log,
-        "fabs": sp.Abs, "M_E": sp.E, "M_PI": sp.pi,
+        "fabs": sp.Abs, "M_E": sp.E, "M_PI": sp.pi, "Heaviside": sp.Heaviside,
     }
     return sp.sympify(expr, locals=loc), v, t
 

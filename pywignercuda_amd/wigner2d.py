@@ -26,7 +26,7 @@ from ._lib import (WH_C64, WH_C64_WIDE, WH_C128, WH_MAX_DEG, WH_NOBS, WH_POT_TAB
 # state precision names: complex128; complex64 state with FP64 arithmetic inside the sweeps
 # (one FP32 rounding of the state per sweep); complex64 state with FP32 arithmetic
 PRECISIONS = {"fp64": WH_C128, "fp32": WH_C64_WIDE, "fp32fast": WH_C64}
-from .physics import NonPolynomialPhysics, coefficient_table, depends_on_time, host_eval
+from .physics import NonPolynomialPhysics, coefficient_table, depends_on_time, heaviside_host, host_eval
 
 TUNING_FILE = Path(__file__).resolve().parent / "tuning.json"
 
@@ -171,7 +171,8 @@ class _Tables:
         else:
             dv = 0.0 * self.x_host
         self.dv = np.ascontiguousarray(np.broadcast_to(np.asarray(dv, dtype=np.float64), self.x_host.shape))
-        v0 = host_eval(potential, t=0.0, x=self.x_host)
+        # the host-side Potential of the reference (its Heaviside is 1/2 at 0: GP.py:675-697)
+        v0 = host_eval(potential, heaviside=heaviside_host, t=0.0, x=self.x_host)
         self.v0 = np.ascontiguousarray(np.broadcast_to(np.asarray(v0, dtype=np.float64), self.x_host.shape))
         self.dxdp = self.dX * self.dP
         self.inv_2m = 1.0 / (2.0 * float(mass))

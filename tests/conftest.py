@@ -8,9 +8,23 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
+def pytest_addoption(parser):
+    parser.addoption("--runslow", action="store_true", default=False,
+                     help="also run the long CPU oracle runs marked slow")
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libwignerhip on cuda:0)")
-    config.addinivalue_line("markers", "slow: long CPU oracle runs (deselected by default in the CPU suite)")
+    config.addinivalue_line("markers", "slow: long CPU oracle runs (skipped unless --runslow)")
+
+
+def pytest_collection_modifyitems(config, items):
+    if config.getoption("--runslow"):
+        return
+    skip = pytest.mark.skip(reason="long CPU oracle run: pass --runslow")
+    for item in items:
+        if "slow" in item.keywords:
+            item.add_marker(skip)
 
 
 def gpu_available():

@@ -1,0 +1,134 @@
+"""GPU: the imaginary-time (Bloch) Gross-Pitaevskii solver (GPU_Wigner2D_GPitaevskii.py:
+2064-2543) through libwignerhip, against the oracle (oracle/gp_bloch.py) and against the
+runs recorded in Wigner2D_GPitaevskii_X2_Bloch.ipynb (cells 6, 11, 20, 21: W_0 min/max,
+all 800 steps kept; W_1 min/max, four refusals at dt = 0.01 .. 0.00125 after step 700).
+
+Bars: state relative L2 1e-10 against the oracle, history columns 1e-10; the recorded
+minima/maxima (12 significant digits) at 1e-9 relative.
+"""
+import numpy as np
+import pytest
+
+from oracle.gp_bloch import GPBlochOracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _check_history(h, rec):
+    np.testing.assert_array_equal(h["kept"], rec["accepted"])
+    np.testing.assert_array_equal(h["dt"], rec["dt"])
+    for k in ("X", "X2", "P", "P2", "H", "E", "NLE", "purity"):
+        np.testing.assert_allclose(h[k], np.real(rec[k]), rtol=1e-10, atol=1e-12, err_msg=k)
+
+
+@pytest.mark.parametrize("gp", [0.0, 1.0])
+def test_bloch_matches_oracle(cuda, gp):
+    """256^2, V = 0.5 x^2 + 0.05 x^4, K = p^2/2, dt = 0.05: 120 iterations of Run, then 120 of
+    Run_ExitedState1 from the same W_init with the ground state removed."""
+    import torch
+
+    from pywignercuda_amd.bloch import GPBloch
+
+    args = (256, 256, 10.0, 10.0, 1.0, 1.0)
+    hip = GPBloch(*args, "0.5*pow(x,2) + 0.05*pow(x,4)", "0.5*p*p", gp_coeff=gp, device=cuda)
+    orc = GPBlochOracle(*args, 0.05, potential=lambda t, x: 0.5 * x * x + 0.05 * x ** 4,
+                        kinetic=lambda p: 0.5 * p * p, gp_coeff=gp)
+    g = orc.g
+    W0 = np.exp(-((g.x_host - 1.0) ** 2) / 6.0 - g.p_host ** 2 / 6.0)
+    W0 /= W0.sum() * g.dX * g.dP
+    W_ref, rec = orc.run(W0, 119)
+    W = hip.to_device(W0)
+    hist, _ = hip.run(W, 120, 0.05)
+    torch.cuda.synchronize()
+    err = rel_l2(W.cpu().numpy(), W_ref)
+    print(f"Bloch a={gp}: rel-L2 {err:.3e}, refused {int((rec['accepted'] == 0).sum())}")
+    assert err < 1e-10
+    _check_history(GPBloch.history(hist), rec)
+    # first excited state
+    W1_ref, rec1 = orc.run(W0, 119, W_ground=W_ref)
+    W1 = hip.to_device(W0)
+    hist1, _ = hip.run(W1, 120, 0.05, W_ground=W)
+    torch.cuda.synchronize()
+    err = rel_l2(W1.cpu().numpy(), W1_ref)
+    print(f"Bloch excited a={gp}: rel-L2 {err:.3e}")
+    assert err < 1e-10
+    _check_history(GPBloch.history(hist1), rec1)
+
+
+def test_bloch_heaviside_potential_table(cuda):
+    """The ATransistor potential of Wigner2D_GPitaevskii_ATransistor_Bloch.ipynb (Heaviside
+    steps: not a polynomial, so its Bopp sum is tabulated), 512 x 256 grid, a = 1."""
+    import torch
+
+    from pywignercuda_amd.bloch import GPBloch
+
+    d, vs, k = 0.75, 2.0, 5.0
+    pot = ("(-0.1+0.075*pow(x,2) + {V}*exp(-{k}*(x+{d})*(x+{d})))*Heaviside(-x-{d})"
+           " +( -0.1+4*(x+{d})+{V} )*Heaviside(x+{d})").format(V=vs, k=k, d=d)
+
+    def H(x, half):
+        return np.where(x > 0, 1.0, np.where(x == 0, 0.5 if half else 1.0, 0.0))
+
+    def vfun(half):
+        return lambda t, x: ((-0.1 + 0.075 * x ** 2 + vs * np.exp(-k * (x + d) ** 2)) * H(-x - d, half)
+                             + (-0.1 + 4 * (x + d) + vs) * H(x + d, half))
+
+    args = (512, 256, 32.0, 18.0, 1.0, 1.0)
+    hip = GPBloch(*args, pot, "0.5*p*p/1.0", gp_coeff=1.0, device=cuda)
+    orc = GPBlochOracle(*args, 0.01, potential=vfun(False), potential_host=vfun(True),
+                        kinetic=lambda p: 0.5 * p * p, gp_coeff=1.0)
+    g = orc.g
+    W0 = np.exp(-((g.x_host + 3.0) ** 2) / 4.0 - g.p_host ** 2 / 4.0)
+    W0 /= W0.sum() * g.dX * g.dP
+    W_ref, rec = orc.run(W0, 59)
+    W = hip.to_device(W0)
+    hist, _ = hip.run(W, 60, 0.01)
+    torch.cuda.synchronize()
+    err = rel_l2(W.cpu().numpy(), W_ref)
+    print(f"Bloch ATransistor potential: rel-L2 {err:.3e}")
+    assert err < 1e-10
+    _check_history(GPBloch.history(hist), rec)
+
+
+def test_x2_bloch_notebook(cuda, tmp_path):
+    """Wigner2D_GPitaevskii_X2_Bloch.ipynb as the notebook drives it."""
+    from pywignercuda_amd.GPU_Wigner2D_GPitaevskii import GPU_Wigner2D_GPitaevskii_Bloch
+
+    class frame(GPU_Wigner2D_GPitaevskii_Bloch):
+        def __init__(self):
+            X_gridDIM = P_gridDIM = 512
+            mass = 1.0
+            self.GPitaevskiiCoeff = 1.0
+            self.omega = 1.0
+            X2_constant = 0.5 * mass * self.omega ** 2
+            kinematicString = "0.5*p*p/{mass}".format(mass=mass)
+            potentialString = "{0}*pow(x,2)".format(X2_constant)
+            self.SetTimeTrack(0.01, 800, 100, fileName=str(tmp_path / "X2_Ground.hdf5"))
+            GPU_Wigner2D_GPitaevskii_Bloch.__init__(self, X_gridDIM, P_gridDIM, 16, 16, 1.0, mass,
+                                                    potentialString, kinematicString)
+
+        def Set_Initial_Condition(self):
+            self.W_init = np.exp(-self.X ** 2 / 20.0 - self.P ** 2 / 20.0)
+            norm = np.sum(self.W_init) * self.dX * self.dP
+            self.W_init /= norm
+
+    inst = frame()
+    inst.Set_Initial_Condition()
+    assert inst.Run() == 0
+    W = np.fft.fftshift(inst.W_0)
+    print(f"W_0 min {W.min()!r} max {W.max()!r}")
+    assert abs(W.min() - (-0.000683046801767)) < 1e-9 * 0.000683046801767
+    assert abs(W.max() - 0.317703452498) < 1e-9 * 0.317703452498
+    assert inst.NonLinearEnergyHistory.shape == (800,)
+    assert inst.Run_ExitedState1() == 0
+    W = np.fft.fftshift(inst.W_1)
+    print(f"W_1 min {W.min()!r} max {W.max()!r}")
+    assert abs(W.min() - (-0.303893689631)) < 1e-9 * 0.303893689631
+    assert abs(W.max() - 0.18269849465) < 1e-9 * 0.18269849465
+    refused = np.nonzero(inst.stepKept == 0)[0]
+    assert list(inst.dtHistory[refused]) == [0.01, 0.005, 0.0025, 0.00125]
+    assert refused.min() > 700
