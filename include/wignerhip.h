@@ -421,6 +421,12 @@ typedef struct {
      factors then couple all 16 planes); incompatible with mass_dm/mass_sm */
   int a_deg[3];
   double a_coef[3][WH_MAX_DEG + 1];
+  /* ODM damping (dampingModel 'ODM': DampingODM_Kernel, GPU_WignerDirac2D_4x4.py:952-1195,
+     applied between the potential factors and Theta_To_P, :3608-3615); needs A = 0 and
+     gamma_dt = 0 (the two damping models exclude each other) */
+  int odm;                    /* 1 = on */
+  double odm_gamma;           /* gammaDamping (kernel argument, full precision) */
+  double lambda_bar;          /* lambdaBar (kernel constant) */
 } wh_dirac_desc;
 
 typedef struct wh_dirac wh_dirac;

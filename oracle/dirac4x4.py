@@ -14,6 +14,7 @@ Follows GPU_WignerDirac2D_4x4.py in behaviour (not in code):
                                      s = sqrt((mass_half c)^2 + q^2)
 * potential factors ............... :357-670  W <- e^{V-} P-(x - theta/2) W P+(x + theta/2) e^{V+}
 * Caldeira-Leggett order 2 ........ :3011-3058 (dampingModel 'CaldeiraLegget'; sigma(p) = p, host grids)
+* ODM damping ..................... :952-1195, applied at :3608-3615 (dampingModel 'ODM', lambdaBar :1947)
 * edge absorber ................... :111-155 (w = 24 cells, array index), applied at :3676-3683
 * trace normalisation ............. :2740-2775
 * Gross-Pitaevskii terms .......... :1791-1840, 2695-2715 (rho = Re sum_p trace, no dP)
@@ -48,7 +49,8 @@ class Dirac4x4Oracle:
 
     def __init__(self, X, P, X_amp, P_amp, mass, c, dt, potentials, dpotential0=None, D_theta=0.0,
                  gamma=0.0, damping=None, gp_coeff=0.0, absorb=True, truncate_constants=True,
-                 antiparticle=True, compute_energy=False, mass_potential=None, kinetic_mass_half=None):
+                 antiparticle=True, compute_energy=False, mass_potential=None, kinetic_mass_half=None,
+                 lambda_bar=None):
         """mass_potential(t, x), kinetic_mass_half: the MassPotential variant
         (GPU_WignerDirac2D_MassPotential_4x4.py:387-391, 422-427, 1668-1669): Mass(t, x) in the
         potential factors, a fixed mass_half (1e-4 there) in the kinetic factors, the bare mass
@@ -80,6 +82,7 @@ class Dirac4x4Oracle:
         self.dV0 = dpotential0 if dpotential0 is not None else (lambda t, x: 0.0 * x)
         self.gamma = float(gamma)
         self.damping = damping
+        self.k_lambda_bar = tr(lambda_bar) if lambda_bar is not None else None  # ODM (:1947)
         self.gp = float(gp_coeff)
         self.absorb_on = absorb
         self.antiparticle = antiparticle
@@ -176,6 +179,26 @@ class Dirac4x4Oracle:
         j = np.arange(self.X)
         return (1.0 - np.exp(-(((j - self.X // 2) / 24.0) ** 2)))[None, :]
 
+    def odm(self, W):
+        """DampingODM_Kernel (:952-1195) in the (theta, x) picture.  Every one of its 16 outputs
+        has the form (with i' = 5 - i, j' = 5 - j, kernel coordinates, t = dt, F^2 = lambdaBar):
+          W'_ij = 1/2 [ A cx (W_ij - W_i'j') + B ct (W_ij + W_i'j')
+                        + i A sx (W_ij' - W_i'j) + i B st (W_ij' + W_i'j) ],
+          A = e^{-8 m c t g x^2 / F^2}, cx + i sx = e^{4 i m c t g x},
+          B = e^{-2 m c t g theta^2 / F^2}, ct + i st = e^{2 i m c t g theta}."""
+        k = self.k_mass * self.k_c * self.k_dt * self.gamma
+        lb = self.k_lambda_bar
+        x, th = self.kx, self.ktheta
+        A, cx, sx = np.exp(-8.0 * k * x * x / lb), np.cos(4.0 * k * x), np.sin(4.0 * k * x)
+        B, ct, st = np.exp(-2.0 * k * th * th / lb), np.cos(2.0 * k * th), np.sin(2.0 * k * th)
+        out = np.empty_like(W)
+        for i in range(4):
+            for j in range(4):
+                ib, jb = 3 - i, 3 - j
+                out[i, j] = 0.5 * (A * cx * (W[i, j] - W[ib, jb]) + B * ct * (W[i, j] + W[ib, jb])
+                                   + 1j * A * sx * (W[i, jb] - W[ib, j]) + 1j * B * st * (W[i, jb] + W[ib, j]))
+        return out
+
     def gp_terms(self, W):
         """B-(theta,x) = rho(x - theta/2), B+ = rho(x + theta/2) via the shift theorem (:2695-2715)."""
         rho = np.real(sum(W[k // 4, k % 4] for k in DIAG).sum(axis=0))
@@ -253,6 +276,8 @@ class Dirac4x4Oracle:
         W = self.lambda_to_x(W)
         W = self.p_to_theta(W)
         W = self.potential_step(W, t, Bm, Bp)
+        if self.gamma > 0.0 and self.damping == "ODM":
+            W = self.odm(W)
         W = self.theta_to_p(W)
         if self.gamma > 0.0 and self.damping == "CaldeiraLegget":
             W = self.cl_order2(W)

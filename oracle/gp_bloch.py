@@ -143,7 +143,8 @@ class GPBlochOracle:
         E_acc, _ = self.energies(W)
         dt = self.dt
         rec = {k: [] for k in ("time", "X", "X2", "P", "P2", "H", "E", "NLE", "purity", "accepted", "dt")}
-        hist = {"TotalEnergy": [], "NonLinearEnergy": [], "purity": []}
+        # the kept steps' series (TotalEnergyHistory, NonLinearEnergyHistory, purity of the reference)
+        hist = {"TotalEnergyHistory": [], "NonLinearEnergyHistory": [], "purityHistory": []}
         for ti in range(time_steps + 1):
             t = ti * self.dt
             W = self.step(W, dt, t)
@@ -165,9 +166,9 @@ class GPBlochOracle:
             if ti > 0:
                 if _lex_less(E, E_acc) and abs(pur) < 1.0:
                     W_step = W.copy()
-                    hist["purity"].append(pur)
-                    hist["TotalEnergy"].append(E)
-                    hist["NonLinearEnergy"].append(nle)
+                    hist["purityHistory"].append(pur)
+                    hist["TotalEnergyHistory"].append(E)
+                    hist["NonLinearEnergyHistory"].append(nle)
                     E_acc = E
                     acc = 1
                 else:

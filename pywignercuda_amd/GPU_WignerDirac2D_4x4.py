@@ -12,8 +12,8 @@ step, observables reduced on the device, copied once per saved frame).
 Extra attributes read if present: precision ('fp64' default or 'fp32'),
 truncateConstants (default True, the reference's '%f' splicing), useGraph
 (default True).  Not implemented (NotImplementedError): vector potentials that
-are not time-independent polynomials in x, the ODM damping model and
-antiParticleStepFiltering.
+are not time-independent polynomials in x, ODM damping together with a vector
+potential, and antiParticleStepFiltering.
 """
 from __future__ import annotations
 
@@ -64,8 +64,6 @@ class GPU_WignerDirac2D_4x4:
         self.dampingModel = getattr(self, "dampingModel", "CaldeiraLegget")
         if antiParticleStepFiltering:
             raise NotImplementedError("antiParticleStepFiltering is not implemented")
-        if self.gammaDamping > 0.0 and self.dampingModel == "ODM":
-            raise NotImplementedError("dampingModel 'ODM' is not implemented")
         self.precision = getattr(self, "precision", "fp64")
         self.truncateConstants = getattr(self, "truncateConstants", True)
         self.useGraph = getattr(self, "useGraph", True)
@@ -79,7 +77,7 @@ class GPU_WignerDirac2D_4x4:
             damping=self.dampingModel if self.gammaDamping > 0.0 else None,
             gp_coeff=self.grossPitaevskiiCoefficient, absorb=True, truncate_constants=self.truncateConstants,
             antiparticle=bool(self.antiParticleNorm), energy=(self.computeEnergy is True),
-            precision=self.precision, **variant)
+            precision=self.precision, lambda_bar=getattr(self, "lambdaBar", None), **variant)
 
     # ------------------------------------------------------------------ potentials (host, :2170-2228)
     def PotentialFunction_0(self, t, x):

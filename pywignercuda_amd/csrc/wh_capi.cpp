@@ -67,6 +67,7 @@ int dirac_x_blocks(int X, int P);
 int dirac_gp_blocks(int X, int P);
 int dirac_pg_blocks(int X, int P);
 hipError_t launch_dirac_pgeneral(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s);
+hipError_t launch_dirac_podm(int prec, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_gpshift(const DiracPlan* d, int nblocks, hipStream_t s);
 int dirac_p_blocks(int X, int P);
 }  // namespace wh
@@ -1250,6 +1251,26 @@ int dirac_validate(const wh_dirac_desc* d) {
   if (d->mass_dm && (d->a_deg[0] >= 0 || d->a_deg[1] >= 0 || d->a_deg[2] >= 0))
     return fail(WH_INVALID_DESCRIPTOR, "a position-dependent mass needs A1 = A2 = A3 = 0");
   if (!(d->dt == d->dt) || d->c == 0.0) return fail(WH_INVALID_DESCRIPTOR, "dt %g c %g", d->dt, d->c);
+  if (d->odm) {
+    if (d->a_deg[0] >= 0 || d->a_deg[1] >= 0 || d->a_deg[2] >= 0)
+      return fail(WH_INVALID_DESCRIPTOR, "ODM damping needs A1 = A2 = A3 = 0");
+    if (d->gamma_dt != 0.0) return fail(WH_INVALID_DESCRIPTOR, "ODM and Caldeira-Leggett damping exclude each other");
+    if (!(d->lambda_bar > 0.0)) return fail(WH_INVALID_DESCRIPTOR, "ODM damping needs lambda_bar > 0");
+  }
+  return WH_SUCCESS;
+}
+
+// The potential half of a step: A = 0 (diagonal factors), vector potentials (two sweeps) or
+// A = 0 with ODM damping (one sweep over plane quads).
+int dirac_potential_sweep(wh_dirac* e, hipStream_t s) {
+  const auto* dp = e->plan_dev.as<wh::DiracPlan>();
+  const int prec = e->d.precision;
+  if (e->host.odm)
+    WH_LAUNCH(wh::launch_dirac_podm(prec, dp, e->pgblocks, s), "dirac_podm");
+  else if (e->host.general)
+    WH_LAUNCH(wh::launch_dirac_pgeneral(prec, e->d.gamma_dt != 0.0, dp, e->pgblocks, s), "dirac_pgeneral");
+  else
+    WH_LAUNCH(wh::launch_dirac_psweep(prec, e->d.gamma_dt != 0.0, dp, e->pblocks, s), "dirac_psweep");
   return WH_SUCCESS;
 }
 
@@ -1259,10 +1280,7 @@ int dirac_iteration(wh_dirac* e, hipStream_t s) {
   const int prec = e->d.precision;
   if (e->d.gp_coeff != 0.0) WH_LAUNCH(wh::launch_dirac_gpshift(dp, e->gpblocks, s), "dirac_gpshift");
   WH_LAUNCH(wh::launch_dirac_xsweep(prec, dp, e->xblocks, s), "dirac_xsweep");
-  if (e->host.general)
-    WH_LAUNCH(wh::launch_dirac_pgeneral(prec, e->d.gamma_dt != 0.0, dp, e->pgblocks, s), "dirac_pgeneral");
-  else
-    WH_LAUNCH(wh::launch_dirac_psweep(prec, e->d.gamma_dt != 0.0, dp, e->pblocks, s), "dirac_psweep");
+  WH_TRY(dirac_potential_sweep(e, s));
   WH_LAUNCH(wh::launch_dirac_reduce(dp, 1, 1, 1, 1, s), "dirac_reduce");
   return WH_SUCCESS;
 }
@@ -1377,7 +1395,10 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   h.adeg2 = d.a_deg[2];
   h.general = (d.a_deg[0] >= 0 || d.a_deg[1] >= 0 || d.a_deg[2] >= 0) ? 1 : 0;
   std::memcpy(h.acoef, d.a_coef, sizeof(h.acoef));
-  if (h.general) {
+  h.odm = d.odm ? 1 : 0;
+  h.odm_k = d.mass * d.c * d.dt * d.odm_gamma;
+  h.odm_inv_lb = d.odm ? 1.0 / d.lambda_bar : 0.0;
+  if (h.general || h.odm) {
     e->pgblocks = wh::dirac_pg_blocks(d.x_dim, d.p_dim);
     if (e->pgblocks < 1) return fail(WH_INVALID_SIZE, "grid %zu x %zu", P, X);
   }
@@ -1453,11 +1474,7 @@ extern "C" int wh_dirac_potential(wh_dirac* e, void* W, int step, void* stream) 
     return fail(WH_INVALID_VALUE, "wh_dirac_potential: the Gross-Pitaevskii density comes from wh_dirac_run");
   hipStream_t s = static_cast<hipStream_t>(stream);
   WH_TRY(dirac_bind(e, W, e->host.obs, e->host.obs_base, step, false, s));
-  const auto* dp = e->plan_dev.as<wh::DiracPlan>();
-  if (e->host.general)
-    WH_LAUNCH(wh::launch_dirac_pgeneral(e->d.precision, e->d.gamma_dt != 0.0, dp, e->pgblocks, s), "dirac_pgeneral");
-  else
-    WH_LAUNCH(wh::launch_dirac_psweep(e->d.precision, e->d.gamma_dt != 0.0, dp, e->pblocks, s), "dirac_psweep");
+  WH_TRY(dirac_potential_sweep(e, s));
   return WH_SUCCESS;
 }
 

@@ -461,6 +461,116 @@ __device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, 
   dirac_store_colsums<BLOCK, C>(d, tile, acc, lds);
 }
 
+// ------------------------------------------------------------------ p sweep with ODM damping
+// DampingODM_Kernel (GPU_WignerDirac2D_4x4.py:952-1195, called at :3608-3615 between the
+// potential factors and Theta_To_P) couples, at each (theta, x) cell, the four planes
+// a = W_ij, d = W_(5-i)(5-j), b = W_i(5-j), c = W_(5-i)j (i, j in {1, 2}), one of each
+// mass-phase class.  With u = a - d, v = a + d, w = b - c, z = b + c,
+//   P = A (cx u + i sx w), Q = B (ct v + i st z), R = A (cx w + i sx u), S = B (ct z + i st v),
+//   a' = (P + Q)/2, d' = (Q - P)/2, b' = (R + S)/2, c' = (S - R)/2,
+// A = e^{-8 mc t g x^2 / lb}, cx + i sx = e^{4 i mc t g x}, B = e^{-2 mc t g th^2 / lb},
+// ct + i st = e^{2 i mc t g th} (mc = mass c, t = dt, g = gammaDamping, lb = lambdaBar, kernel
+// coordinates).  One sweep per step: each block takes the four planes of a quad through
+// FFT_p, the diagonal potential factors of A = 0, the mixing and iFFT_p.
+template <typename S, typename T, int N, int R, int BLOCK>
+__device__ __forceinline__ void dirac_podm_body(const DiracPlan& d, int tile, cplx<T>* lds) {
+  constexpr int TT = N / R;
+  constexpr int C = BLOCK / TT;
+  const int tid = threadIdx.x;
+  const int c = tid % C, t = tid / C;
+  const int X = d.X;
+  const int col = tile * C + c;
+  const size_t plane = size_t(d.P) * size_t(X);
+  const gptr<cplx<S>> W0 = gmem(reinterpret_cast<cplx<S>*>(d.W));
+  const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twP_hi), reinterpret_cast<const cplx<T>*>(d.twP_lo)};
+  LdsStrided<T, C> view{lds + c};
+  const int step = *gmem(d.step);
+  const double x = d.dx * double(fidx(col, X));
+  const double mask = d.xmask ? gmem(d.xmask)[col] : 1.0;
+  const int prow = step < d.pot_rows ? step : d.pot_rows - 1;
+  const BoppOdd bopp = bopp_odd(cmem(d.pot) + size_t(prow) * (kMaxDeg + 1), d.pot_deg, x, 1.0);
+  const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<S>));
+  const T inv = T(1.0 / N);
+  // x factors of the mixing (per column)
+  const double kx = d.odm_k * x;  // mc t g x
+  double sx, cx;
+  sincos_fp64(4.0 * kx, &sx, &cx);
+  const double Ax = exp(-8.0 * kx * x * d.odm_inv_lb);
+  double acc[kDNCol];
+#pragma unroll
+  for (int q = 0; q < kDNCol; ++q) acc[q] = 0.0;
+  double trre[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) trre[j] = 0.0;
+  for (int quad = 0; quad < 4; ++quad) {
+    const int i0 = quad >> 1, j0 = quad & 1;
+    // planes a (rows lo, cols lo: class 0), b (lo, hi: class 1), c (hi, lo: class 2), d (hi, hi: 3)
+    const int kp[4] = {4 * i0 + j0, 4 * i0 + (3 - j0), 4 * (3 - i0) + j0, 4 * (3 - i0) + (3 - j0)};
+    cplx<T> v[4][R];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const gptr<cplx<S>> W = W0 + size_t(kp[q]) * plane;
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[q][j] = to_arith<T>(gload(at_off(W + size_t(j * TT) * X, boff)));
+      team_fft<T, N, R, -1>(v[q], t, view, tw);
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int i = t + j * TT;
+      const double th = d.dtheta * double(fidx(i, N));
+      double ph = d.pot_deg >= 0 ? bopp(0.5 * th) : gmem(d.pot_dv)[size_t(i) * X + col];
+      ph *= d.dt;
+      if (d.gp_coeff != 0.0) ph += d.dt * d.gp_coeff * gmem(d.G)[size_t(i) * X + col];
+      const double r = (d.theta_decay ? gmem(d.theta_decay)[i] : 1.0) * mask;
+      double m[4];
+      if (d.mass_dm) {
+        const size_t e = size_t(i) * X + col;
+        const double dm = gmem(d.mass_dm)[e], sm = gmem(d.mass_sm)[e];
+        m[0] = dm;
+        m[1] = sm;
+        m[2] = -sm;
+        m[3] = -dm;
+      } else {
+        m[0] = 0.0;
+        m[1] = 2.0 * d.mass_phase;
+        m[2] = -2.0 * d.mass_phase;
+        m[3] = 0.0;
+      }
+      cplx<double> w[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) w[q] = cmul(widen(v[q][j]), expmi(r, ph + m[q]));
+      // ODM mixing of the quad (a = w0, b = w1, c = w2, d = w3)
+      const double kt = d.odm_k * th;  // mc t g theta
+      double st, ct;
+      sincos_fp64(2.0 * kt, &st, &ct);
+      const double Bt = exp(-2.0 * kt * th * d.odm_inv_lb);
+      const cplx<double> u = csub(w[0], w[3]), vv = cadd(w[0], w[3]);
+      const cplx<double> ww = csub(w[1], w[2]), z = cadd(w[1], w[2]);
+      const cplx<double> Pp = cscale(cadd(cscale(u, cx), mul_i(ww, sx)), 0.5 * Ax);
+      const cplx<double> Qp = cscale(cadd(cscale(vv, ct), mul_i(z, st)), 0.5 * Bt);
+      const cplx<double> Rp = cscale(cadd(cscale(ww, cx), mul_i(u, sx)), 0.5 * Ax);
+      const cplx<double> Sp = cscale(cadd(cscale(z, ct), mul_i(vv, st)), 0.5 * Bt);
+      v[0][j] = narrow<T>(cadd(Pp, Qp));
+      v[3][j] = narrow<T>(csub(Qp, Pp));
+      v[1][j] = narrow<T>(cadd(Rp, Sp));
+      v[2][j] = narrow<T>(csub(Sp, Rp));
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      team_fft<T, N, R, +1>(v[q], t, view, tw);
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[q][j] = cscale(v[q][j], inv);
+      dirac_accumulate<T, R, TT>(d, kp[q], v[q], t, acc, trre);
+      const gptr<cplx<S>> W = W0 + size_t(kp[q]) * plane;
+#pragma unroll
+      for (int j = 0; j < R; ++j) store_stream(at_off(W + size_t(j * TT) * X, boff), to_arith<S>(v[q][j]));
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j) acc[kDcNeg] += trre[j] < 0.0 ? trre[j] : 0.0;
+  dirac_store_colsums<BLOCK, C>(d, tile, acc, lds);
+}
+
 // ------------------------------------------------------------------ kernels
 constexpr int kDiracXBlock = 256;
 constexpr int kDiracPBlock = 256;
@@ -545,6 +655,15 @@ __global__ void __launch_bounds__(kDiracXBlock) dirac_pright_kernel(const DiracP
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
   const DiracPlan& d = *dp;
 #define BODY(n) dirac_pright_body<S, T, n, dirac_rv<n>(), kDiracXBlock, CL>(d, dirac_col_tile(), lds)
+  WH_DIRAC_DISPATCH(d.P, BODY)
+#undef BODY
+}
+
+template <typename S, typename T>
+__global__ void __launch_bounds__(kDiracXBlock) dirac_podm_kernel(const DiracPlan* __restrict__ dp) {
+  __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
+  const DiracPlan& d = *dp;
+#define BODY(n) dirac_podm_body<S, T, n, dirac_rv<n>(), kDiracXBlock>(d, dirac_col_tile(), lds)
   WH_DIRAC_DISPATCH(d.P, BODY)
 #undef BODY
 }
@@ -853,6 +972,17 @@ hipError_t launch_dirac_pgeneral(int prec, bool cl, const DiracPlan* d, int nblo
     else
       hipLaunchKernelGGL((dirac_pright_kernel<float, double, false>), g, b, 0, s, d);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_dirac_podm(int prec, const DiracPlan* d, int nblocks, hipStream_t s) {
+  const dim3 g(nblocks), b(kDiracXBlock);
+  if (prec == 0)
+    hipLaunchKernelGGL((dirac_podm_kernel<double, double>), g, b, 0, s, d);
+  else if (prec == 1)
+    hipLaunchKernelGGL((dirac_podm_kernel<float, float>), g, b, 0, s, d);
+  else
+    hipLaunchKernelGGL((dirac_podm_kernel<float, double>), g, b, 0, s, d);
   return hipGetLastError();
 }
 

@@ -77,6 +77,9 @@ struct DiracPlan {
   // vector potentials A1..A3 (Potential_1..3) as polynomials in x; general = any non-zero
   int general, adeg0, adeg1, adeg2;
   double acoef[3][kMaxDeg + 1];
+  // ODM damping (DampingODM_Kernel): odm_k = mass c dt gammaDamping, 1/lambdaBar; odm = on
+  int odm, pad1;
+  double odm_k, odm_inv_lb;
 };
 
 }  // namespace wh

@@ -13,7 +13,8 @@ Potential_1..3 = 0 the potential factors are diagonal in the spinor indices and
 the step is two fused sweeps; vector potentials (time-independent polynomials in
 x, e.g. the constant magnetic field of the ConstantB notebooks, A2 = -x) couple
 all 16 planes and take one more sweep.  The Gross-Pitaevskii term (density
-Re sum_p trace W, shifted by -/+ theta/2) is supported.
+Re sum_p trace W, shifted by -/+ theta/2) and both damping models (order-2
+Caldeira-Leggett, ODM) are supported.
 """
 from __future__ import annotations
 
@@ -54,7 +55,9 @@ class WignerDirac4x4:
     potentials : four reference-style C expression strings Potential_0..3(t, x).
     D_theta    : D_Theta of the potential kernels (exp(-dt D theta^2 / 2) on each side).
     D_lambda   : accepted and unused, as in the reference (spliced, read by no kernel).
-    gamma, damping : gammaDamping and dampingModel; 'CaldeiraLegget' = order-2 dissipator.
+    gamma, damping : gammaDamping and dampingModel; 'CaldeiraLegget' = order-2 dissipator,
+                 'ODM' = DampingODM_Kernel between the potential factors and Theta_To_P.
+    lambda_bar : lambdaBar of the ODM model.
     absorb     : the 24-cell edge absorber applied every step (always on in the reference).
     truncate_constants : emulate the '%f' splicing of the kernel constants.
     antiparticle, energy : record AntiParticlePopulation / DiracEnergy.
@@ -69,7 +72,7 @@ class WignerDirac4x4:
                  D_theta=0.0, D_lambda=0.0, gamma=0.0, damping=None, gp_coeff=0.0, absorb=True,
                  truncate_constants=True, antiparticle=True, energy=False, precision="fp64",
                  time_rows=None, device=None, transmission_x=10.0, mass_potential=None,
-                 kinetic_mass_half=None):
+                 kinetic_mass_half=None, lambda_bar=None):
         import torch
 
         if precision not in PRECISIONS:
@@ -88,8 +91,13 @@ class WignerDirac4x4:
             a_deg[k - 1], a_coef[k - 1] = deg, tab[0]
         if max(a_deg) >= 0 and mass_potential is not None:
             raise NotImplementedError("a position-dependent mass together with a vector potential")
-        if gamma != 0.0 and damping not in (None, "CaldeiraLegget"):
-            raise NotImplementedError(f"dampingModel {damping!r} is not supported (only 'CaldeiraLegget')")
+        if gamma != 0.0 and damping not in (None, "CaldeiraLegget", "ODM"):
+            raise NotImplementedError(f"dampingModel {damping!r} is not supported ('CaldeiraLegget' or 'ODM')")
+        if gamma > 0.0 and damping == "ODM":
+            if max(a_deg) >= 0:
+                raise NotImplementedError("ODM damping together with a vector potential")
+            if lambda_bar is None:
+                raise ValueError("ODM damping needs lambdaBar")
         self.precision = precision
         self.dtype = torch.complex128 if precision == "fp64" else torch.complex64
         self.device = torch.device(device if device is not None else "cuda:0")
@@ -175,6 +183,10 @@ class WignerDirac4x4:
                 d.a_coef[k][n] = float(a_coef[k][n])
         self.vector_potential = max(a_deg) >= 0
         d.gp_coeff = float(gp_coeff)  # passed as a kernel argument, not spliced (:3401, 3583)
+        if gamma > 0.0 and damping == "ODM":
+            # DampingODM_Kernel (:952-1195): mass, c, dt and lambdaBar spliced ('%f', :1947),
+            # gammaDamping a full-precision kernel argument (:3609-3615)
+            d.odm, d.odm_gamma, d.lambda_bar = 1, float(gamma), tr(lambda_bar)
         d.lambda_host = ptr(host_axis(self.dLambda * self.X / 2.0, self.X, self.dLambda))
         self._plan = ctypes.c_void_p()
         with torch.cuda.device(self.device):

@@ -27,7 +27,7 @@ def _check_history(h, rec):
 
 @pytest.mark.parametrize("gp", [0.0, 1.0])
 def test_bloch_matches_oracle(cuda, gp):
-    """256^2, V = 0.5 x^2 + 0.05 x^4, K = p^2/2, dt = 0.05: 120 iterations of Run, then 120 of
+    """256^2, V = 0.5 x^2 + 0.05 x^4, K = p^2/2, dt = 0.05: 120 iterations of Run, then 20 of
     Run_ExitedState1 from the same W_init with the ground state removed."""
     import torch
 
@@ -48,10 +48,12 @@ def test_bloch_matches_oracle(cuda, gp):
     print(f"Bloch a={gp}: rel-L2 {err:.3e}, refused {int((rec['accepted'] == 0).sum())}")
     assert err < 1e-10
     _check_history(GPBloch.history(hist), rec)
-    # first excited state
-    W1_ref, rec1 = orc.run(W0, 119, W_ground=W_ref)
+    # first excited state, 20 iterations: beyond them the energy sits on its plateau and each
+    # keep / refuse decision compares energies that differ by rounding only (the recorded
+    # refusals of the notebook run are checked in test_x2_bloch_notebook)
+    W1_ref, rec1 = orc.run(W0, 19, W_ground=W_ref)
     W1 = hip.to_device(W0)
-    hist1, _ = hip.run(W1, 120, 0.05, W_ground=W)
+    hist1, _ = hip.run(W1, 20, 0.05, W_ground=W)
     torch.cuda.synchronize()
     err = rel_l2(W1.cpu().numpy(), W1_ref)
     print(f"Bloch excited a={gp}: rel-L2 {err:.3e}")

@@ -233,3 +233,81 @@ def test_dirac_single_sweeps_match_oracle_factors(cuda):
     torch.cuda.synchronize()
     ref = orc.theta_to_p(orc.potential_step(orc.p_to_theta(ref), orc.dt * 3)) * orc.absorber()
     assert rel_l2(W.cpu().numpy().reshape(4, 4, hip.P, hip.X), ref) < 1e-12
+
+
+# ---------------------------------------------------------------------------- ODM damping
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("mass_potential", [False, True])
+def test_dirac_odm_matches_oracle(cuda, precision, mass_potential):
+    """dampingModel 'ODM' (DampingODM_Kernel, GPU_WignerDirac2D_4x4.py:952-1195): the quad-mixing
+    p sweep against the oracle, Klein step, D_Theta and absorber on, gamma = 0.05, lambdaBar = 4."""
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    vs, vf, dvf = CASES["klein"]
+    extra_h = dict(mass_potential="1. + 0.1*x*x", kinetic_mass_half=0.0001) if mass_potential else {}
+    extra_o = dict(mass_potential=lambda t, x: 1.0 + 0.1 * x * x, kinetic_mass_half=0.0001) if mass_potential else {}
+    hip = WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=(vs, "0.", "0.", "0."), D_theta=0.001,
+                         gamma=0.05, damping="ODM", lambda_bar=4.0, energy=True, precision=precision, device=cuda,
+                         **extra_h)
+    orc = Dirac4x4Oracle(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, [vf, zero, zero, zero], dpotential0=dvf,
+                         D_theta=0.001, gamma=0.05, damping="ODM", lambda_bar=4.0, compute_energy=True, **extra_o)
+    W0 = wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 3.0, 1.0, Xg, 1.0, 1.0), orc)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 25)
+    err = rel_l2(W, W_ref)
+    print(f"Dirac ODM {precision} mass_potential={mass_potential}: rel L2 {err:.3e}")
+    if precision == "fp64":
+        assert err < 1e-10
+        check_obs(obs, rec)
+    else:
+        assert err < 1e-5
+
+
+def _odm_notebook(tmp_path, V0, steps, pX, D_Theta, epsilon):
+    from pywignercuda_amd.GPU_WignerDirac2D_4x4 import GPU_WignerDirac2D_4x4
+
+    class Klein(GPU_WignerDirac2D_4x4):
+        def __init__(self):
+            self.gammaDamping = 0.01
+            self.fp_Damping_String = " p*p/sqrt( p*p + {epsilon}  ) ".format(epsilon=epsilon)
+            self.dampingModel = "ODM"
+            self.lambdaBar = 4.0
+            self.D_Theta = D_Theta
+            self.D_Lambda = 0.0
+            self.grossPitaevskiiCoefficient = 0.0
+            self.Potential_0_String = "{0}*0.5*( 1. + tanh((x-5.)/{1})   )".format(V0, 0.25)
+            self.Potential_1_String = " 0."
+            self.Potential_2_String = " 0."
+            self.Potential_3_String = " 0."
+            GPU_WignerDirac2D_4x4.__init__(self, 512, 512, 18, 16, 1.0, 1.0, 0.01, steps, 100,
+                                           frameSaveMode="Density", antiParticleNorm=True, computeEnergy=True)
+
+    inst = Klein()
+    inst.fileName = str(tmp_path / "Klein.hdf5")
+    W = np.empty([4, 4, 512, 512], dtype=np.complex128)
+    psiL = inst.GaussianSpinor_ParticleUp(-2, pX, 1, inst.X - 0.5 * inst.Theta)
+    psiR = inst.GaussianSpinor_ParticleUp(-2, pX, 1, inst.X + 0.5 * inst.Theta)
+    for i in range(4):
+        for j in range(4):
+            W[i, j][:, :] = psiL[i] * psiR[j].conj()
+    inst.Fourier_4X4_Theta_To_P(W)
+    W *= 1.0 / inst.Wigner_4x4_Norm(W)
+    inst.W_init = W
+    assert inst.Run() == 0
+    W0 = np.fft.fftshift(inst.Wigner_4X4__SpinTrace(10 * inst.W_end).real)
+    return float(W0.min()), float(W0.max())
+
+
+@pytest.mark.parametrize("nb", ["Free_ODM0.01", "Klein_ODM01"])
+def test_dirac_odm_notebooks(cuda, tmp_path, nb):
+    """instances/WignerDirac2D_4x4/Free_ODM0.01.ipynb (V0 = 0, px = 6, 1000 steps) and
+    Klein_ODM01.ipynb (V0 = 9, px = 7, D_Theta = 0.001, 1300 steps), 512^2, gamma = 0.01,
+    lambdaBar = 4: min / max of the spin trace of 10 W_end (cell 8 of each notebook)."""
+    if nb == "Free_ODM0.01":
+        lo, hi = _odm_notebook(tmp_path, 0.0, 1000, 6.0, 0.0, 0.01)
+        rec = (-0.000151901096206, 2.79279961979)
+    else:
+        lo, hi = _odm_notebook(tmp_path, 9.0, 1300, 7.0, 0.001, 0.5)
+        rec = (-0.0852455133993, 0.945834335445)
+    print(f"{nb}: min {lo!r} (recorded {rec[0]}), max {hi!r} (recorded {rec[1]})")
+    assert abs(hi - rec[1]) < 1e-9 * abs(rec[1])
+    assert abs(lo - rec[0]) < 1e-6 * abs(rec[0])

@@ -67,7 +67,7 @@ def test_energy_decreases_and_refusals_halve_dt():
             assert dts[k] == dts[k - 1] / 2.0
         else:
             assert dts[k] == dts[k - 1]
-    assert len(rec["TotalEnergy"]) == int((acc == 1).sum())
+    assert len(rec["TotalEnergyHistory"]) == int((acc == 1).sum())
     # the Gross-Pitaevskii term widens the ground state: <x^2> above the linear one
     lin = GPBlochOracle(128, 128, 8.0, 8.0, 1.0, 1.0, 0.05, potential=lambda t, x: 0.5 * x * x + 0.05 * x ** 4,
                         kinetic=lambda p: 0.5 * p * p, gp_coeff=0.0)
@@ -88,7 +88,7 @@ def test_x2_notebook_ground_and_first_excited_state():
     W = np.fft.fftshift(Wg.real)
     assert abs(W.min() - (-0.000683046801767)) < 1e-12
     assert abs(W.max() - 0.317703452498) < 1e-11
-    assert len(rec["NonLinearEnergy"]) == 800
+    assert len(rec["NonLinearEnergyHistory"]) == 800
     W1, rec1 = o.run(W0, 800, W_ground=Wg)
     W = np.fft.fftshift(W1.real)
     assert abs(W.min() - (-0.303893689631)) < 1e-11

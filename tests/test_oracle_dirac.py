@@ -293,3 +293,48 @@ def test_absorber_projector_and_population_pins():
     H = orc.c * ALPHA1[:, :, None] * q + BETA[:, :, None] * orc.mass * orc.c ** 2
     np.testing.assert_allclose(np.einsum("ik...,kj...->ij...", H, Pp), np.einsum("ik...,kj...->ij...", Pp, H),
                                atol=1e-12)
+
+
+# ---------------------------------------------------------------------------- ODM damping
+def _theta_state(orc):
+    W = initial(orc, x0=-1.0, px=2.0, s=1.2)
+    return orc.p_to_theta(W)  # the (theta, x) picture the damping acts in
+
+
+def test_odm_identity_at_zero_damping_and_first_order_generator():
+    """In matrix form the damping map is W' = 1/2[A cx (W - a W a) + B ct (W + a W a)
+    + i A sx (W a - a W) + i B st (W a + a W)] with a = alpha_1 (the spinor pairing i <-> 5-i).
+    gamma = 0 gives the identity; to first order in k = m c dt gamma it is
+    W + k{ -4 x^2/lb (W - a W a) - theta^2/lb (W + a W a) - 2 i x [a, W] + i theta {a, W} }
+    (a wrong partner plane or a sign slip in any of the 16 outputs fails this)."""
+    orc0 = make(X=64, P=64, gamma=0.0, damping="ODM", lambda_bar=4.0)
+    W = _theta_state(orc0)
+    np.testing.assert_allclose(orc0.odm(W), W, atol=1e-15)
+    g = 1e-6
+    orc = make(X=64, P=64, gamma=g, damping="ODM", lambda_bar=4.0)
+    k = orc.k_mass * orc.k_c * orc.k_dt * g
+    a = ALPHA1.astype(np.complex128)
+    aWa = np.einsum("ik,kj...,jl->il...", a, W, a)
+    Wa = np.einsum("ik...,kj->ij...", W, a)
+    aW = np.einsum("ik,kj...->ij...", a, W)
+    x, th = orc.kx, orc.ktheta
+    lin = W + k * (-4.0 * x * x / 4.0 * (W - aWa) - th * th / 4.0 * (W + aWa)
+                   - 2j * x * (aW - Wa) + 1j * th * (aW + Wa))
+    err = np.abs(orc.odm(W) - lin).max() / np.abs(W).max()
+    assert err < 1e-9, err  # second order: ~ (k x^2)^2 ~ 1e-14 .. 1e-10 here
+
+
+def test_odm_conserves_position_density_and_hermiticity():
+    """The map leaves the theta = 0 row of the trace (the position density sum_p tr W) exactly
+    unchanged, and keeps W_ij(theta, x) = conj W_ji(-theta, x) (the (theta, x) form of
+    W_ij(x, p) = conj W_ji(x, p))."""
+    orc = make(X=64, P=64, gamma=0.05, damping="ODM", lambda_bar=4.0)
+    W = _theta_state(orc)
+    Wp = orc.odm(W)
+    tr, trp = orc.trace(W), orc.trace(Wp)
+    np.testing.assert_allclose(trp[0], tr[0], rtol=1e-13, atol=1e-15)
+    assert np.abs(trp - tr).max() > 1e-6  # it does act away from theta = 0
+    flip = np.roll(Wp[:, :, ::-1, :], 1, axis=2)  # theta -> -theta in FFT order
+    dev = np.abs(Wp - np.conj(np.swapaxes(flip, 0, 1)))
+    dev[:, :, orc.P // 2, :] = 0.0  # the Nyquist row (-theta_max) has no +theta_max partner
+    assert dev.max() < 1e-13 * np.abs(Wp).max()
