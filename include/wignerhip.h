@@ -427,6 +427,15 @@ typedef struct {
   int odm;                    /* 1 = on */
   double odm_gamma;           /* gammaDamping (kernel argument, full precision) */
   double lambda_bar;          /* lambdaBar (kernel constant) */
+  /* the KvN variant (GPU_WignerDiracKvN2D_4x4.py): kinetic factors at q = +/- lambda/2 without
+     the mass term on the diagonal (:183-189, 283-289; mass_half = 1e-5 there, :1961), potential
+     step e^{-dt D_Theta theta^2 + i dt V0'(x) theta} on every plane (KvN_XTheta_Kernel, :356-420;
+     V0' from the polynomial rows or dv0), the step filter always on with projectors at p
+     (:775, 3599-3640); V0 only (no A, mass function, GP or damping) */
+  int kvn;
+  /* antiparticle step filter (antiParticleStepFiltering, :3636-3678): after the potential half,
+     W <- Lambda_+(p + lambda/2) W Lambda_+(p - lambda/2) in (p, lambda), then the absorber */
+  int step_filter;
 } wh_dirac_desc;
 
 typedef struct wh_dirac wh_dirac;

@@ -15,6 +15,9 @@ Follows GPU_WignerDirac2D_4x4.py in behaviour (not in code):
 * potential factors ............... :357-670  W <- e^{V-} P-(x - theta/2) W P+(x + theta/2) e^{V+}
 * Caldeira-Leggett order 2 ........ :3011-3058 (dampingModel 'CaldeiraLegget'; sigma(p) = p, host grids)
 * ODM damping ..................... :952-1195, applied at :3608-3615 (dampingModel 'ODM', lambdaBar :1947)
+* antiparticle step filter ........ :2500-2540, 3636-3678 (projectors at p -/+ lambda/2, sign +1)
+* KvN variant ..................... GPU_WignerDiracKvN2D_4x4.py:183-189, 283-289 (kinetic), 356-420
+                                    (potential), 775 (filter at lambda = 0), 1961 (mass_half 1e-5)
 * edge absorber ................... :111-155 (w = 24 cells, array index), applied at :3676-3683
 * trace normalisation ............. :2740-2775
 * Gross-Pitaevskii terms .......... :1791-1840, 2695-2715 (rho = Re sum_p trace, no dP)
@@ -50,7 +53,10 @@ class Dirac4x4Oracle:
     def __init__(self, X, P, X_amp, P_amp, mass, c, dt, potentials, dpotential0=None, D_theta=0.0,
                  gamma=0.0, damping=None, gp_coeff=0.0, absorb=True, truncate_constants=True,
                  antiparticle=True, compute_energy=False, mass_potential=None, kinetic_mass_half=None,
-                 lambda_bar=None):
+                 lambda_bar=None, kvn=False, step_filter=False):
+        """kvn: GPU_WignerDiracKvN2D_4x4 (mass_half 1e-5 unless given, kinetic factors at
+        q = +/- lambda/2 without the diagonal mass term, potential e^{-dt D theta^2 + i dt V0' theta},
+        step filter on with projectors at p).  step_filter: antiParticleStepFiltering."""
         """mass_potential(t, x), kinetic_mass_half: the MassPotential variant
         (GPU_WignerDirac2D_MassPotential_4x4.py:387-391, 422-427, 1668-1669): Mass(t, x) in the
         potential factors, a fixed mass_half (1e-4 there) in the kinetic factors, the bare mass
@@ -58,6 +64,10 @@ class Dirac4x4Oracle:
         self.X, self.P = int(X), int(P)
         self.X_amp, self.P_amp = float(X_amp), float(P_amp)
         self.mass, self.c, self.dt = float(mass), float(c), float(dt)
+        self.kvn = bool(kvn)
+        self.step_filter = bool(step_filter) or self.kvn
+        if self.kvn and kinetic_mass_half is None:
+            kinetic_mass_half = 0.00001  # GPU_WignerDiracKvN2D_4x4.py:1961
         self.dX = 2.0 * self.X_amp / self.X
         self.dP = 2.0 * self.P_amp / self.P
         self.dTheta = 2.0 * np.pi / (2.0 * self.P_amp)
@@ -116,8 +126,9 @@ class Dirac4x4Oracle:
         cs, sn = np.cos(dt * c * s), np.sin(sign * c * dt * s)
         z = np.zeros(np.broadcast(q, q).shape, dtype=np.complex128)
         U = np.zeros((4, 4) + z.shape, dtype=np.complex128)
-        U11 = cs - 1j * mh * c * sn / s
-        U44 = cs + 1j * mh * c * sn / s
+        mass_term = 0.0 if self.kvn else 1.0  # the KvN diagonal is cos alone (KvN.py:188-189)
+        U11 = cs - 1j * mass_term * mh * c * sn / s
+        U44 = cs + 1j * mass_term * mh * c * sn / s
         U14 = -1j * q * sn / s
         U[0, 0], U[3, 3], U[0, 3], U[3, 0] = U11, U44, U14, U14
         U[1, 1], U[1, 2], U[2, 1], U[2, 2] = U11, U14, U14, U44
@@ -126,8 +137,9 @@ class Dirac4x4Oracle:
     def kinetic_step(self, W):
         """In the (p, lambda) picture: W <- U(p + lambda/2) W (U(p - lambda/2) with dt -> -dt), :3537-3551."""
         if self._kin is None:
-            self._kin = (self.kinetic_factor(self.kp + 0.5 * self.klambda, +1),
-                         self.kinetic_factor(self.kp - 0.5 * self.klambda, -1))
+            q0 = 0.0 * self.kp if self.kvn else self.kp  # KvN: q = +/- lambda/2 (KvN.py:184, 284)
+            self._kin = (self.kinetic_factor(q0 + 0.5 * self.klambda, +1),
+                         self.kinetic_factor(q0 - 0.5 * self.klambda, -1))
         L, R = self._kin
         W = np.einsum("ik...,kj...->ij...", L, W)
         return np.einsum("ik...,kj...->ij...", W, R)
@@ -156,7 +168,12 @@ class Dirac4x4Oracle:
         return U, e
 
     def potential_step(self, W, t, Bm=None, Bp=None):
-        """In the (theta, x) picture: W <- e- P- W ; W <- e+ W P+, :3576-3591."""
+        """In the (theta, x) picture: W <- e- P- W ; W <- e+ W P+, :3576-3591.  KvN: every plane
+        times e^{-dt D theta^2 + i dt V0'(x) theta} (KvN_XTheta_Kernel, KvN.py:356-420)."""
+        if self.kvn:
+            dv = np.asarray(self.dV0(t, self.kx), dtype=np.float64)
+            th = self.ktheta
+            return W * np.exp(-self.k_dt * self.k_D * th * th + 1j * self.k_dt * dv * th)
         Um, em = self.potential_factor(t, -1, Bm)
         Up, ep = self.potential_factor(t, +1, Bp)
         W = em * np.einsum("ik...,kj...->ij...", Um, W)
@@ -198,6 +215,16 @@ class Dirac4x4Oracle:
                 out[i, j] = 0.5 * (A * cx * (W[i, j] - W[ib, jb]) + B * ct * (W[i, j] + W[ib, jb])
                                    + 1j * A * sx * (W[i, jb] - W[ib, j]) + 1j * B * st * (W[i, jb] + W[ib, j]))
         return out
+
+    def filter_particles(self, W):
+        """FilterAntiParticles (:2500-2540, applied at :3636-3678): in the (p, lambda) picture
+        W <- Lambda_+(p + lambda/2) W Lambda_+(p - lambda/2) (KvN: lambda = 0 in the projectors,
+        KvN.py:775), back to x."""
+        h = 0.0 if self.kvn else 0.5 * self.klambda
+        L = self.projector(self.kp + h, +1)
+        R = self.projector(self.kp - h, +1)
+        F = np.einsum("ik...,kj...->ij...", L, self.x_to_lambda(W))
+        return self.lambda_to_x(np.einsum("ik...,kj...->ij...", F, R))
 
     def gp_terms(self, W):
         """B-(theta,x) = rho(x - theta/2), B+ = rho(x + theta/2) via the shift theorem (:2695-2715)."""
@@ -281,6 +308,8 @@ class Dirac4x4Oracle:
         W = self.theta_to_p(W)
         if self.gamma > 0.0 and self.damping == "CaldeiraLegget":
             W = self.cl_order2(W)
+        if self.step_filter:
+            W = self.filter_particles(W)
         if self.absorb_on:
             W = W * self.absorber()
         return W / self.norm(W)

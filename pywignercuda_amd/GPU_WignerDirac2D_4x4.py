@@ -12,8 +12,9 @@ step, observables reduced on the device, copied once per saved frame).
 Extra attributes read if present: precision ('fp64' default or 'fp32'),
 truncateConstants (default True, the reference's '%f' splicing), useGraph
 (default True).  Not implemented (NotImplementedError): vector potentials that
-are not time-independent polynomials in x, ODM damping together with a vector
-potential, and antiParticleStepFiltering.
+are not time-independent polynomials in x and ODM damping together with a vector
+potential.  GPU_WignerDiracKvN2D_4x4 (GPU_WignerDiracKvN2D_4x4.py) is the KvN
+variant of the same class.
 """
 from __future__ import annotations
 
@@ -62,8 +63,6 @@ class GPU_WignerDirac2D_4x4:
         self.grossPitaevskiiCoefficient = getattr(self, "grossPitaevskiiCoefficient", 0.0)
         self.gammaDamping = getattr(self, "gammaDamping", 0.0)
         self.dampingModel = getattr(self, "dampingModel", "CaldeiraLegget")
-        if antiParticleStepFiltering:
-            raise NotImplementedError("antiParticleStepFiltering is not implemented")
         self.precision = getattr(self, "precision", "fp64")
         self.truncateConstants = getattr(self, "truncateConstants", True)
         self.useGraph = getattr(self, "useGraph", True)
@@ -77,7 +76,8 @@ class GPU_WignerDirac2D_4x4:
             damping=self.dampingModel if self.gammaDamping > 0.0 else None,
             gp_coeff=self.grossPitaevskiiCoefficient, absorb=True, truncate_constants=self.truncateConstants,
             antiparticle=bool(self.antiParticleNorm), energy=(self.computeEnergy is True),
-            precision=self.precision, lambda_bar=getattr(self, "lambdaBar", None), **variant)
+            precision=self.precision, lambda_bar=getattr(self, "lambdaBar", None),
+            step_filter=bool(self.antiParticleStepFiltering), **variant)
 
     # ------------------------------------------------------------------ potentials (host, :2170-2228)
     def PotentialFunction_0(self, t, x):
@@ -271,3 +271,22 @@ class GPU_WignerDirac2D_4x4:
         f.close()
         self.W_end_device = W
         return 0
+
+
+class GPU_WignerDiracKvN2D_4x4(GPU_WignerDirac2D_4x4):
+    """GPU_WignerDiracKvN2D_4x4 (GPU_WignerDiracKvN2D_4x4.py:1911-3780): the same API; kinetic
+    factors at q = +/- lambda/2 with mass_half = 1e-5 and no diagonal mass term (:183-189,
+    :283-289), the potential step e^{-dt D_Theta theta^2 + i dt V0'(x) theta} on every plane
+    (KvN_XTheta_Kernel, :356-420), the antiparticle filter with projectors at p after every
+    step (:775, :3599-3640); its loop has no damping and no Gross-Pitaevskii term."""
+
+    def __init__(self, *args, **kwargs):
+        kwargs.setdefault("antiParticleStepFiltering", True)
+        super().__init__(*args, **kwargs)
+
+    def _make_plan(self, **variant):
+        return WignerDirac4x4(
+            self.X_gridDIM, self.P_gridDIM, self.X_amplitude, self.P_amplitude, self.mass, self.c, self.dt,
+            potentials=(self.Potential_0_String, "0.", "0.", "0."), D_theta=self.D_Theta, D_lambda=self.D_Lambda,
+            absorb=True, truncate_constants=self.truncateConstants, antiparticle=bool(self.antiParticleNorm),
+            energy=(self.computeEnergy is True), precision=self.precision, kvn=True, **variant)

@@ -68,6 +68,7 @@ int dirac_gp_blocks(int X, int P);
 int dirac_pg_blocks(int X, int P);
 hipError_t launch_dirac_pgeneral(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_podm(int prec, const DiracPlan* d, int nblocks, hipStream_t s);
+hipError_t launch_dirac_fsweep(int prec, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_gpshift(const DiracPlan* d, int nblocks, hipStream_t s);
 int dirac_p_blocks(int X, int P);
 }  // namespace wh
@@ -1251,6 +1252,12 @@ int dirac_validate(const wh_dirac_desc* d) {
   if (d->mass_dm && (d->a_deg[0] >= 0 || d->a_deg[1] >= 0 || d->a_deg[2] >= 0))
     return fail(WH_INVALID_DESCRIPTOR, "a position-dependent mass needs A1 = A2 = A3 = 0");
   if (!(d->dt == d->dt) || d->c == 0.0) return fail(WH_INVALID_DESCRIPTOR, "dt %g c %g", d->dt, d->c);
+  if (d->kvn) {
+    if (d->a_deg[0] >= 0 || d->a_deg[1] >= 0 || d->a_deg[2] >= 0 || d->mass_dm || d->gp_coeff != 0.0 ||
+        d->gamma_dt != 0.0 || d->odm)
+      return fail(WH_INVALID_DESCRIPTOR,
+                  "the KvN variant takes V0 only (no vector potential, position-dependent mass, GP or damping)");
+  }
   if (d->odm) {
     if (d->a_deg[0] >= 0 || d->a_deg[1] >= 0 || d->a_deg[2] >= 0)
       return fail(WH_INVALID_DESCRIPTOR, "ODM damping needs A1 = A2 = A3 = 0");
@@ -1281,6 +1288,12 @@ int dirac_iteration(wh_dirac* e, hipStream_t s) {
   if (e->d.gp_coeff != 0.0) WH_LAUNCH(wh::launch_dirac_gpshift(dp, e->gpblocks, s), "dirac_gpshift");
   WH_LAUNCH(wh::launch_dirac_xsweep(prec, dp, e->xblocks, s), "dirac_xsweep");
   WH_TRY(dirac_potential_sweep(e, s));
+  if (e->host.filter) {
+    // antiparticle step filter and the absorber after it; the record then needs the column
+    // sums of the filtered state
+    WH_LAUNCH(wh::launch_dirac_fsweep(prec, dp, e->xblocks, s), "dirac_fsweep");
+    WH_LAUNCH(wh::launch_dirac_colsum(storage_code(prec), dp, e->d.x_dim, s), "dirac_colsum");
+  }
   WH_LAUNCH(wh::launch_dirac_reduce(dp, 1, 1, 1, 1, s), "dirac_reduce");
   return WH_SUCCESS;
 }
@@ -1395,6 +1408,12 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   h.adeg2 = d.a_deg[2];
   h.general = (d.a_deg[0] >= 0 || d.a_deg[1] >= 0 || d.a_deg[2] >= 0) ? 1 : 0;
   std::memcpy(h.acoef, d.a_coef, sizeof(h.acoef));
+  h.kvn = d.kvn ? 1 : 0;
+  h.filter = d.step_filter ? 1 : 0;
+  if (h.filter) {
+    h.fmask = h.xmask;  // the absorber follows the filter
+    h.xmask = nullptr;
+  }
   h.odm = d.odm ? 1 : 0;
   h.odm_k = d.mass * d.c * d.dt * d.odm_gamma;
   h.odm_inv_lb = d.odm ? 1.0 / d.lambda_bar : 0.0;

@@ -50,19 +50,33 @@ __device__ __forceinline__ cplx<T> narrow(cplx<double> v) { return {T(v.x), T(v.
 struct Blk {
   cplx<double> a, b, d;
 };
-__device__ __forceinline__ Blk kinetic_block(double q, double mh_c, double c_dt, double sgn) {
+// mass_on = 0: the KvN form (GPU_WignerDiracKvN2D_4x4.py:183-189), whose diagonal is cos(c dt s)
+// alone (s keeps the mass).
+__device__ __forceinline__ Blk kinetic_block(double q, double mh_c, double c_dt, double sgn, double mass_on = 1.0) {
   const double s = sqrt(mh_c * mh_c + q * q);
   double sn, cs;
   sincos_fp64(c_dt * s, &sn, &cs);
   sn *= sgn;
   const double f = sn / s;
-  return {{cs, -mh_c * f}, {0.0, -q * f}, {cs, mh_c * f}};
+  const double m = mass_on * mh_c * f;
+  return {{cs, -m}, {0.0, -q * f}, {cs, m}};
+}
+
+// Positive-energy projector Lambda_+(q) = 1/2 (1 + (m c^2 beta + c q alpha_1) / E) on one
+// {r1, r2} block (DiracElectronProjector / Filter_Kernel with sign +1, :674-760).
+__device__ __forceinline__ Blk projector_block(double q, double mc2, double c) {
+  const double e2 = 2.0 * sqrt(mc2 * mc2 + c * c * q * q);
+  return {{0.5 + mc2 / e2, 0.0}, {c * q / e2, 0.0}, {0.5 - mc2 / e2, 0.0}};
 }
 
 }  // namespace
 
 // ------------------------------------------------------------------ x sweep
-template <typename S, typename T, int N, int R, int BLOCK>
+// FILTER: the antiparticle step filter instead of the kinetic factor (FilterAntiParticles,
+// :2500-2540, applied at :3636-3678): W <- Lambda_+(q1) W Lambda_+(q2) in the (p, lambda)
+// picture, q1,2 = p +/- lambda/2 (q1 = q2 = p in the KvN form, whose filter sets lambda = 0,
+// GPU_WignerDiracKvN2D_4x4.py:775), then the edge absorber that follows it; no pending factor.
+template <typename S, typename T, int N, int R, int BLOCK, bool FILTER = false>
 __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group, int tile, cplx<T>* lds) {
   constexpr int TT = N / R, ROWS = BLOCK / TT;
   constexpr int LSZ = Padded<N, R>::size;
@@ -75,7 +89,7 @@ __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group,
   // a team inside one wave (radix 8 up to N = 512): wave-local exchanges
   using Sync = std::conditional_t<(TT <= 64), WaveSync, BlockSync>;
   const TwMem<T> twm{tw};
-  const double s = *gmem(d.scale);
+  const double s = FILTER ? 1.0 : *gmem(d.scale);
 
   cplx<T> v[4][R];
 #pragma unroll
@@ -92,7 +106,7 @@ __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group,
   }
 
   const double p = d.dp * double(fidx(row, d.P));
-  if (d.antiparticle && (group == 0 || group == 3) && t == 0) {
+  if (!FILTER && d.antiparticle && (group == 0 || group == 3) && t == 0) {
     // lambda = 0 bin (element j = 0 of lane 0): Tr[Lambda_-(p) W^(p, 0)] over this group's planes,
     // Lambda_- = [[aa, bb], [bb, dd]] on each block (DiracElectronProjector, sign -1)
     const double e2 = 2.0 * sqrt(d.proj_mc2 * d.proj_mc2 + d.proj_c * d.proj_c * p * p);
@@ -104,8 +118,19 @@ __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group,
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const double lam = d.dlambda * double(fidx(t + j * TT, N));
-    const Blk K = kinetic_block(p + 0.5 * lam, d.mh_c, d.c_dt, 1.0);
-    const Blk Rt = kinetic_block(p - 0.5 * lam, d.mh_c, d.c_dt, -1.0);
+    Blk K, Rt;
+    if constexpr (FILTER) {
+      const double h = d.kvn ? 0.0 : 0.5 * lam;
+      K = projector_block(p + h, d.proj_mc2, d.proj_c);
+      Rt = projector_block(p - h, d.proj_mc2, d.proj_c);
+    } else if (d.kvn) {
+      // KvN kinetic factors at q = +/- lambda/2 (GPU_WignerDiracKvN2D_4x4.py:183-189, 283-289)
+      K = kinetic_block(0.5 * lam, d.mh_c, d.c_dt, 1.0, 0.0);
+      Rt = kinetic_block(-0.5 * lam, d.mh_c, d.c_dt, -1.0, 0.0);
+    } else {
+      K = kinetic_block(p + 0.5 * lam, d.mh_c, d.c_dt, 1.0);
+      Rt = kinetic_block(p - 0.5 * lam, d.mh_c, d.c_dt, -1.0);
+    }
     const cplx<double> w0 = widen(v[0][j]), w1 = widen(v[1][j]), w2 = widen(v[2][j]), w3 = widen(v[3][j]);
     // K B
     const cplx<double> x0 = cadd(cmul(K.a, w0), cmul(K.b, w2));
@@ -124,8 +149,16 @@ __device__ __forceinline__ void dirac_xsweep_body(const DiracPlan& d, int group,
   for (int q = 0; q < 4; ++q) {
     team_fft<T, N, R, +1, LdsRow<T>, Sync>(v[q], t, view, twm);
     const gptr<cplx<S>> dst = W + size_t(kDiracGroups[group][q]) * plane;
+    if constexpr (FILTER) {
 #pragma unroll
-    for (int j = 0; j < R; ++j) store_stream(dst + t + j * TT, to_arith<S>(cscale(v[q][j], inv)));
+      for (int j = 0; j < R; ++j) {
+        const T f = d.fmask ? T(gmem(d.fmask)[t + j * TT] * (1.0 / N)) : inv;
+        store_stream(dst + t + j * TT, to_arith<S>(cscale(v[q][j], f)));
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < R; ++j) store_stream(dst + t + j * TT, to_arith<S>(cscale(v[q][j], inv)));
+    }
   }
 }
 
@@ -246,10 +279,24 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
   {
     const int prow = step < d.pot_rows ? step : d.pot_rows - 1;
     const BoppOdd bopp = bopp_odd(cmem(d.pot) + size_t(prow) * (kMaxDeg + 1), d.pot_deg, x, 1.0);
+    // KvN: e^{i dt V0'(x) theta} for every plane (KvN_XTheta_Kernel, GPU_WignerDiracKvN2D_4x4.py:356-420)
+    double dvx = 0.0;
+    if (d.kvn) {
+      if (d.pot_deg >= 0) {
+        const cptr<double> cf = cmem(d.pot) + size_t(prow) * (kMaxDeg + 1);
+        for (int n = d.pot_deg; n >= 1; --n) dvx = fma(dvx, x, double(n) * cf[n]);
+      } else {
+        dvx = gmem(d.dv0)[col];
+      }
+    }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int i = t + j * TT;
       const double th = d.dtheta * double(fidx(i, N));
+      if (d.kvn) {
+        ph[j] = -d.dt * dvx * th;
+        continue;
+      }
       const double dv = d.pot_deg >= 0 ? bopp(0.5 * th) : gmem(d.pot_dv)[size_t(i) * X + col];
       ph[j] = d.dt * dv;
       if (d.gp_coeff != 0.0) ph[j] += d.dt * d.gp_coeff * gmem(d.G)[size_t(i) * X + col];
@@ -280,6 +327,7 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
       } else {
         m = cls == 1 ? 2.0 * d.mass_phase : cls == 2 ? -2.0 * d.mass_phase : 0.0;
       }
+      if (d.kvn) m = 0.0;
       const double r = (d.theta_decay ? gmem(d.theta_decay)[i] : 1.0) * mask;
       M[j] = expmi(r, ph[j] + m);
     }
@@ -632,6 +680,16 @@ __global__ void __launch_bounds__(kDiracXBlock) dirac_xsweep_kernel(const DiracP
 #undef BODY
 }
 
+template <typename S, typename T>
+__global__ void __launch_bounds__(kDiracXBlock) dirac_fsweep_kernel(const DiracPlan* __restrict__ dp) {
+  __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
+  const DiracPlan& d = *dp;
+  const int group = int(blockIdx.x & 3u), tile = int(blockIdx.x >> 2);
+#define BODY(n) dirac_xsweep_body<S, T, n, dirac_rx<n>(), kDiracXBlock, true>(d, group, tile, lds)
+  WH_DIRAC_DISPATCH(d.X, BODY)
+#undef BODY
+}
+
 template <typename S, typename T, bool CL>
 __global__ void __launch_bounds__(kDiracPBlock) dirac_psweep_kernel(const DiracPlan* __restrict__ dp) {
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracPBlock>()];
@@ -972,6 +1030,16 @@ hipError_t launch_dirac_pgeneral(int prec, bool cl, const DiracPlan* d, int nblo
     else
       hipLaunchKernelGGL((dirac_pright_kernel<float, double, false>), g, b, 0, s, d);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_dirac_fsweep(int prec, const DiracPlan* d, int nblocks, hipStream_t s) {
+  if (prec == 0)
+    hipLaunchKernelGGL((dirac_fsweep_kernel<double, double>), dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
+  else if (prec == 1)
+    hipLaunchKernelGGL((dirac_fsweep_kernel<float, float>), dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
+  else
+    hipLaunchKernelGGL((dirac_fsweep_kernel<float, double>), dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
   return hipGetLastError();
 }
 

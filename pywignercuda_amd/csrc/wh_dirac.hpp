@@ -78,8 +78,12 @@ struct DiracPlan {
   int general, adeg0, adeg1, adeg2;
   double acoef[3][kMaxDeg + 1];
   // ODM damping (DampingODM_Kernel): odm_k = mass c dt gammaDamping, 1/lambdaBar; odm = on
-  int odm, pad1;
+  int odm, kvn;                     // ODM damping; the KvN variant (GPU_WignerDiracKvN2D_4x4)
   double odm_k, odm_inv_lb;
+  // antiparticle step filter (filter = 1): the edge absorber then follows the filter (fmask)
+  // instead of riding on the potential factors (xmask = null)
+  int filter, pad1;
+  const double* fmask;
 };
 
 }  // namespace wh

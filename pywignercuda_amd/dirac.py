@@ -58,6 +58,11 @@ class WignerDirac4x4:
     gamma, damping : gammaDamping and dampingModel; 'CaldeiraLegget' = order-2 dissipator,
                  'ODM' = DampingODM_Kernel between the potential factors and Theta_To_P.
     lambda_bar : lambdaBar of the ODM model.
+    kvn        : the KvN variant (GPU_WignerDiracKvN2D_4x4.py): kinetic factors at q = +/- lambda/2
+                 without the diagonal mass term (mass_half 1e-5), potential step
+                 e^{-dt D theta^2 + i dt V0'(x) theta}, the step filter always on.
+    step_filter: antiParticleStepFiltering (W <- Lambda_+ W Lambda_+ after each potential half,
+                 then the absorber).
     absorb     : the 24-cell edge absorber applied every step (always on in the reference).
     truncate_constants : emulate the '%f' splicing of the kernel constants.
     antiparticle, energy : record AntiParticlePopulation / DiracEnergy.
@@ -72,7 +77,7 @@ class WignerDirac4x4:
                  D_theta=0.0, D_lambda=0.0, gamma=0.0, damping=None, gp_coeff=0.0, absorb=True,
                  truncate_constants=True, antiparticle=True, energy=False, precision="fp64",
                  time_rows=None, device=None, transmission_x=10.0, mass_potential=None,
-                 kinetic_mass_half=None, lambda_bar=None):
+                 kinetic_mass_half=None, lambda_bar=None, kvn=False, step_filter=False):
         import torch
 
         if precision not in PRECISIONS:
@@ -89,6 +94,13 @@ class WignerDirac4x4:
                 raise NotImplementedError(
                     f"Potential_{k}_String = {potentials[k]!r}: vector potentials must be polynomials in x") from exc
             a_deg[k - 1], a_coef[k - 1] = deg, tab[0]
+        if kvn:
+            # GPU_WignerDiracKvN2D_4x4: V0 only, no damping or GP terms in its loop (:3409-3690)
+            if max(a_deg) >= 0 or mass_potential is not None:
+                raise NotImplementedError("the KvN variant takes Potential_0 only")
+            gamma, damping, gp_coeff, step_filter = 0.0, None, 0.0, True
+            if kinetic_mass_half is None:
+                kinetic_mass_half = 0.00001  # mass_half of GPU_WignerDiracKvN2D_4x4.py:1961
         if max(a_deg) >= 0 and mass_potential is not None:
             raise NotImplementedError("a position-dependent mass together with a vector potential")
         if gamma != 0.0 and damping not in (None, "CaldeiraLegget", "ODM"):
@@ -183,6 +195,8 @@ class WignerDirac4x4:
                 d.a_coef[k][n] = float(a_coef[k][n])
         self.vector_potential = max(a_deg) >= 0
         d.gp_coeff = float(gp_coeff)  # passed as a kernel argument, not spliced (:3401, 3583)
+        d.kvn, d.step_filter = int(bool(kvn)), int(bool(step_filter))
+        self.kvn, self.step_filter = bool(kvn), bool(step_filter)
         if gamma > 0.0 and damping == "ODM":
             # DampingODM_Kernel (:952-1195): mass, c, dt and lambdaBar spliced ('%f', :1947),
             # gammaDamping a full-precision kernel argument (:3609-3615)

@@ -311,3 +311,81 @@ def test_dirac_odm_notebooks(cuda, tmp_path, nb):
     print(f"{nb}: min {lo!r} (recorded {rec[0]}), max {hi!r} (recorded {rec[1]})")
     assert abs(hi - rec[1]) < 1e-9 * abs(rec[1])
     assert abs(lo - rec[0]) < 1e-6 * abs(rec[0])
+
+
+# ---------------------------------------------------------------------------- step filter, KvN
+@pytest.mark.parametrize("case", ["klein", "ramp"])
+def test_dirac_step_filter_matches_oracle(cuda, case):
+    """antiParticleStepFiltering (GPU_WignerDirac2D_4x4.py:3636-3678): the filter sweep (projectors at
+    p -/+ lambda/2), the absorber after it and the column sums of the filtered state."""
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    vs, vf, dvf = CASES[case]
+    hip = WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=(vs, "0.", "0.", "0."), D_theta=0.01,
+                         energy=True, step_filter=True, device=cuda)
+    orc = Dirac4x4Oracle(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, [vf, zero, zero, zero], dpotential0=dvf,
+                         D_theta=0.01, compute_energy=True, step_filter=True)
+    W0 = wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 3.0, 1.0, Xg, 1.0, 1.0), orc)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 20)
+    assert rel_l2(W, W_ref) < 1e-10
+    check_obs(obs, rec)
+
+
+@pytest.mark.parametrize("case", ["ramp", "klein"])
+def test_kvn_matches_oracle(cuda, case):
+    """GPU_WignerDiracKvN2D_4x4: kinetic factors at +/- lambda/2 without the mass diagonal, the
+    potential step e^{-dt D theta^2 + i dt V0' theta}, the filter at lambda = 0 every step."""
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    vs, vf, dvf = CASES[case]
+    hip = WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=(vs, "0.", "0.", "0."), D_theta=0.01,
+                         energy=True, kvn=True, device=cuda)
+    orc = Dirac4x4Oracle(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, [vf, zero, zero, zero], dpotential0=dvf,
+                         D_theta=0.01, compute_energy=True, kvn=True)
+    W0 = orc.filter_particles(
+        wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 3.0, 1.0, Xg, 1.0, 1.0), orc))
+    W0 = W0 / orc.norm(W0)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 20)
+    err = rel_l2(W, W_ref)
+    print(f"KvN {case}: rel L2 {err:.3e}")
+    assert err < 1e-10
+    check_obs(obs, rec)
+
+
+def test_kvn_x1_notebook(cuda, tmp_path):
+    """instances/WignerDiracKvN2D_4x4/X1.ipynb: 512^2, amplitudes 14, m = c = 1, dt = 0.01, 800 steps,
+    V0 = 1.5 x, spinor at x0 = -1, px = 5, s = 1 filtered to positive energy (FilterElectrons) and
+    normalised; cells 9-10: min / max of the spin trace of 10 W_init and 10 W_end."""
+    from pywignercuda_amd.GPU_WignerDirac2D_4x4 import GPU_WignerDiracKvN2D_4x4
+
+    class Klein(GPU_WignerDiracKvN2D_4x4):
+        def __init__(self):
+            self.gammaDamping = 0.0
+            self.D_Theta = 0.0
+            self.D_Lambda = 0.0
+            self.Potential_0_String = "1.5*x"
+            self.Potential_1_String = " 0.*x"
+            self.Potential_2_String = " 0.*x"
+            self.Potential_3_String = " 0.*x"
+            GPU_WignerDiracKvN2D_4x4.__init__(self, 512, 512, 14, 14, 1.0, 1.0, 0.01, 800, 10,
+                                              frameSaveMode="Density", antiParticleNorm=True, computeEnergy=True)
+
+    inst = Klein()
+    inst.fileName = str(tmp_path / "X1.hdf5")
+    W = np.empty([4, 4, 512, 512], dtype=np.complex128)
+    psiL = inst.GaussianSpinor_ParticleUp(-1, 5, 1.0, inst.X - 0.5 * inst.Theta)
+    psiR = inst.GaussianSpinor_ParticleUp(-1, 5, 1.0, inst.X + 0.5 * inst.Theta)
+    for i in range(4):
+        for j in range(4):
+            W[i, j][:, :] = psiL[i] * psiR[j].conj()
+    inst.Fourier_4X4_Theta_To_P(W)
+    inst.FilterElectrons(W, 1)
+    W *= 1.0 / inst.Wigner_4x4_Norm(W)
+    inst.W_init = W
+    Wi = np.fft.fftshift(inst.Wigner_4X4__SpinTrace(10 * W).real)
+    assert abs(Wi.max() - 3.17920519628) < 1e-9 * 3.17920519628
+    assert inst.Run() == 0
+    We = np.fft.fftshift(inst.Wigner_4X4__SpinTrace(10 * inst.W_end).real)
+    print(f"KvN X1: W_end min {We.min()!r} max {We.max()!r}")
+    assert abs(We.max() - 3.15212339853) < 1e-9 * 3.15212339853
+    assert abs(We.min() - (-7.04369343969e-09)) < 1e-3 * 7.04369343969e-09

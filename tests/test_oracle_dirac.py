@@ -338,3 +338,45 @@ def test_odm_conserves_position_density_and_hermiticity():
     dev = np.abs(Wp - np.conj(np.swapaxes(flip, 0, 1)))
     dev[:, :, orc.P // 2, :] = 0.0  # the Nyquist row (-theta_max) has no +theta_max partner
     assert dev.max() < 1e-13 * np.abs(Wp).max()
+
+
+# ---------------------------------------------------------------------------- step filter and KvN
+def test_step_filter_is_an_idempotent_projection():
+    """W <- Lambda_+(p + lambda/2) W Lambda_+(p - lambda/2): applying it twice equals once (the
+    projectors are idempotent); it keeps a positive-energy spinor state almost whole, and its
+    trace norm and the antiparticle population add up to the norm."""
+    orc = make(X=128, P=128, step_filter=True)
+    W = initial(orc)
+    F1 = orc.filter_particles(W)
+    F2 = orc.filter_particles(F1)
+    assert np.abs(F2 - F1).max() < 1e-13 * np.abs(F1).max()
+    assert abs(orc.norm(F1) - 1.0) < 5e-3  # the positive-energy Gaussian spinor (~0.2% antiparticle)
+    # the trace norm splits: tr(L+ W L+) + tr(L- W L-) = tr W at lambda = 0 (L+ + L- = 1, L^2 = L)
+    anti = orc.antiparticle_population(orc.x_to_lambda(W), -1)
+    assert abs(orc.norm(F1) + anti - orc.norm(W)) < 1e-13
+
+
+def test_kvn_potential_step_is_the_classical_force():
+    """KvN potential step e^{-dt D theta^2 + i dt V0' theta}: for V0 = F x it shifts the
+    momentum marginal by -F dt exactly (dp/dt = -V0'); the trace norm is unchanged.  Exact
+    constants: the '%f'-truncated dtheta would scale the shift by k_dtheta / dtheta."""
+    F = 1.5
+    orc = make(X=128, P=128, potentials=[lambda t, x: F * x, zero, zero, zero], kvn=True,
+               truncate_constants=False)
+    orc.dV0 = lambda t, x: F + 0.0 * x
+    W = initial(orc, x0=-1.0, px=2.0)
+    p0 = np.real(orc.observables(W, 0.0)["P"])
+    W1 = orc.theta_to_p(orc.potential_step(orc.p_to_theta(W), 0.0))
+    p1 = np.real(orc.observables(W1, 0.0)["P"])
+    assert abs((p1 - p0) - (-F * orc.k_dt)) < 1e-10
+    assert abs(orc.norm(W1) - orc.norm(W)) < 1e-13
+
+
+def test_kvn_kinetic_factor_unitary_without_mass_term():
+    orc = make(X=64, P=64, kvn=True)
+    assert abs(orc.k_mh - 0.00001) < 1e-18
+    q = np.linspace(-5, 5, 41)
+    for sgn in (+1, -1):
+        U = orc.kinetic_factor(q, sgn)
+        assert unitary_err(U) < 1e-13
+        np.testing.assert_allclose(U[0, 0].imag, 0.0, atol=0)  # cos(c dt s) alone on the diagonal
