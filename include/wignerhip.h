@@ -187,6 +187,11 @@ typedef struct {
   const double* odm_x;
   const double* odm_th;
   double odm_sign;            /* 1 damping, -1 pumping */
+  /* ODM channels: 0 or 1 = one (Lindbladian_ODM_Order1, weight odm_weight1 -- 1 in
+     GPU_Wigner2D_FFT.Run); 2 = GPU_Wigner2D_ODM2 (FFT.py:1973-2230): W += L(W) with
+     (odm_weight1, sign 1), then with (odm_weight2, sign -1) on the result (:2173-2181) */
+  int odm_channels;
+  double odm_weight1, odm_weight2;
 } wh_wigner2d_desc;
 
 typedef struct wh_wigner2d wh_wigner2d;
@@ -246,6 +251,8 @@ typedef struct {
   double gamma_dt;
   int renormalize;
   double gp_coeff;
+  int odm_channels;           /* as in wh_wigner2d_desc */
+  double odm_weight1, odm_weight2;
 } wh_ensemble_desc;
 
 typedef struct {

@@ -17,7 +17,8 @@ Follows, line by line in behaviour (not in code):
 * Gross-Pitaevskii .............. GPU_Wigner2D_FFT.py:435-519, 1687-1705, 1845-1848;
                                   GPU_Wigner2D_GPitaevskii.py:466-496, 1743-1748, 1904-1956
 * Caldeira-Leggett order 3 ...... GPU_Wigner2D_FFT.py:399-431, 1135-1157, 1903-1913
-* ODM Lindblad (order 1) ........ GPU_Wigner2D_FFT.py:1196-1244, 1516-1634, 1907-1910
+* ODM Lindblad (order 1) ........ GPU_Wigner2D_FFT.py:1196-1244, 1516-1634, 1907-1910; two
+                                  channels of GPU_Wigner2D_ODM2, :1973-2230
 * Lie driver / observables ...... GPU_Wigner2D_FFT.py:1754-1964
 * Strang driver / observables ... GPU_Wigner2D_GPitaevskii.py:1809-2055
 * '%f' constant splicing ........ GPU_Wigner2D_FFT.py:833-842 (optional emulation)
@@ -125,7 +126,8 @@ class Wigner2DOracle:
     def __init__(self, X, P, X_amp, P_amp, hbar, mass, dt, D_theta=0.0, D_lambda=0.0,
                  gamma=0.0, potential=None, dpotential=None, kinetic=None,
                  fp_damping=None, damping=None, gp_coeff=0.0, scheme="lie",
-                 mask=True, truncate_constants=False, time_dependent=False, L_material=None):
+                 mask=True, truncate_constants=False, time_dependent=False, L_material=None,
+                 odm_weights=None):
         self.g = Grid(X, P, X_amp, P_amp)
         self.hbar, self.mass, self.dt = float(hbar), float(mass), float(dt)
         self.D_theta, self.D_lambda, self.gamma = float(D_theta), float(D_lambda), float(gamma)
@@ -136,6 +138,9 @@ class Wigner2DOracle:
         self.damping = damping
         self.gp_coeff = float(gp_coeff)
         self.L_material = L_material  # ODM (FFT.py:1196-1197)
+        # ODM channels: (1,) for GPU_Wigner2D_FFT.Run (weight 1, sign 1, FFT.py:1907-1910);
+        # GPU_Wigner2D_ODM2 applies (w1, sign 1) then (w2, sign -1) (FFT.py:2173-2181)
+        self.odm_weights = (1.0,) if odm_weights is None else tuple(odm_weights)
         self.scheme = scheme
         self.mask = mask
         self.time_dependent = bool(time_dependent)
@@ -344,8 +349,9 @@ class Wigner2DOracle:
             if self.gamma != 0.0:
                 if self.damping == "CaldeiraLeggett":
                     W = self.caldeira_order3(W)
-                elif self.damping == "ODM":  # Lindbladian_ODM_Order1, weight 1, sign 1 (FFT.py:1907-1910)
-                    W = W + self.odm_operator(W)
+                elif self.damping == "ODM":  # Lindbladian_ODM_Order1 per channel (FFT.py:1907-1910, 2173-2181)
+                    for k, w in enumerate(self.odm_weights):
+                        W = W + self.odm_operator(W, weight=w, sign=1.0 if k == 0 else -1.0)
                 W = W / (W.sum() * dxdp)
             if self.scheme == "strang" and record:
                 for k, v in self.observables(W).items():

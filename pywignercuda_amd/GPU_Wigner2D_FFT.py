@@ -249,7 +249,7 @@ class Propagator_Base:
         self.W_end = f["W_end"] + 0j
 
     # ---------------------------------------------------------------- device plan
-    def _make_plan(self, scheme, gp_coeff, kinetic):
+    def _make_plan(self, scheme, gp_coeff, kinetic, odm_weights=None):
         if self.gammaDamping != 0.0 and self.dampingFunction not in ("CaldeiraLeggett", "ODM"):
             raise NotImplementedError(f"dampingFunction {self.dampingFunction!r} is not implemented")
         if self.gammaDamping != 0.0 and self.dampingFunction == "ODM" and scheme != "lie":
@@ -262,7 +262,8 @@ class Propagator_Base:
             L_material=getattr(self, "L_material", None),
             fp_damping=self.fp_Damping_String, gp_coeff=gp_coeff, mask=self.absorbingMask,
             truncate_constants=self.truncateConstants, precision=self.precision,
-            time_rows=self.timeSteps + 3)
+            time_rows=self.timeSteps + 3,
+            odm_weights=odm_weights if self.dampingFunction == "ODM" else None)
 
     def _run_loop(self, plan, post_step):
         """Runs timeSteps+1 iterations in chunks between saved frames; returns the record."""
@@ -359,7 +360,37 @@ class GPU_Wigner2D_FFT(Propagator_Base):
 
 
 class GPU_Wigner2D_ODM2(Propagator_Base):
-    """GPU_Wigner2D_ODM2 (GPU_Wigner2D_FFT.py:1973-2230): two ODM Lindblad channels -- not built."""
+    """Wigner propagator with two ODM Lindblad channels (GPU_Wigner2D_FFT.py:1973-2230): the Lie
+    step of GPU_Wigner2D_FFT, then W += L(W) with (weightLindbland1, sign 1) and with
+    (weightLindbland2, sign -1) on the result, then the renormalisation.  Requires
+    ``dampingFunction = 'ODM'`` (or 'CaldeiraLeggett', which takes the order-3 composite),
+    ``L_material`` and ``epsilon``/``fp_Damping_String`` as in the reference."""
 
-    def __init__(self, *args, **kwargs):
-        raise NotImplementedError("the ODM Lindblad dissipators are not implemented on the GPU path")
+    def __init__(self, X_gridDIM, P_gridDIM, X_amplitude, P_amplitude, hBar, mass, D_Theta, D_Lambda,
+                 gammaDamping, weightLindbland1, weightLindbland2, potentialString, dPotentialString,
+                 kinematicString, normalization="Wigner"):
+        self.normalization = normalization
+        self.D_Theta = D_Theta
+        self.D_Lambda = D_Lambda
+        self.gammaDamping = gammaDamping
+        self.weightLindbland1 = weightLindbland1
+        self.weightLindbland2 = weightLindbland2
+        self.potentialString = potentialString
+        self.dPotentialString = dPotentialString
+        self.kinematicString = kinematicString
+        self.SetPhaseSpaceBox2D(X_gridDIM, P_gridDIM, X_amplitude, P_amplitude)
+        self.hBar = hBar
+        self.mass = mass
+        self.plan = self._make_plan("lie", self.grossPitaevskiiCoefficient, None,
+                                    odm_weights=(weightLindbland1, weightLindbland2))
+        self.Hamiltonian = self.P ** 2 / (2.0 * self.mass) + self.Potential(0, self.X)
+        self.Hamiltonian = self.fft_shift2D(self.Hamiltonian)
+
+    def Run(self):
+        """GPU_Wigner2D_FFT.py:2016-2230; returns 0."""
+        self.file = _io.RunFile(self.fileName or "Wigner2D_ODM2_run")
+        self.WriteHDF5_variables()
+        self.file.create_dataset("Hamiltonian", data=self.Hamiltonian.real)
+        rec, saved = self._run_loop(self.plan, post_step=False)
+        self._store_results(rec, saved)
+        return 0

@@ -110,6 +110,7 @@ class Wigner2DDesc(ctypes.Structure):
         ("x_host", _DP), ("p_host", _DP), ("dv_host", _DP), ("v0_host", _DP),
         ("dxdp", c_double), ("inv_2m", c_double), ("dx_host", c_double),
         ("pot_dv", _DP), ("odm_g", _DP), ("odm_x", _DP), ("odm_th", _DP), ("odm_sign", c_double),
+        ("odm_channels", c_int), ("odm_weight1", c_double), ("odm_weight2", c_double),
     ]
 
 
@@ -121,6 +122,7 @@ class EnsembleDesc(ctypes.Structure):
         ("dt", c_double), ("hbar", c_double), ("mass", c_double),
         ("kinetic_poly", c_int), ("kinetic_deg", c_int), ("kin_coef", c_double * (WH_MAX_DEG + 1)),
         ("damping", c_int), ("gamma_dt", c_double), ("renormalize", c_int), ("gp_coeff", c_double),
+        ("odm_channels", c_int), ("odm_weight1", c_double), ("odm_weight2", c_double),
     ]
 
 

@@ -57,7 +57,7 @@ int adj_cols(int P);
 int odm_rows_per_tile(int X);
 int odm_cols_per_tile(int P);
 hipError_t launch_odm(int prec, const Member* members, const unsigned* work_rows, int nrows,
-                      const unsigned* work_cols, int ncols, hipStream_t s);
+                      const unsigned* work_cols, int ncols, double weight, double sgn, hipStream_t s);
 hipError_t launch_dirac_xsweep(int prec, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_psweep(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_colsum(int prec, const DiracPlan* d, int X, hipStream_t s);
@@ -467,6 +467,8 @@ int validate_common(const wh_ensemble_desc& d) {
     return fail(WH_INVALID_DESCRIPTOR, "damping %d (0 none, 1 Caldeira-Leggett, 2 ODM)", d.damping);
   if (d.damping == 2 && d.scheme != WH_SCHEME_LIE)
     return fail(WH_INVALID_DESCRIPTOR, "the ODM dissipator is part of the Lie scheme (GPU_Wigner2D_FFT)");
+  if (d.damping == 2 && (d.odm_channels < 0 || d.odm_channels > 2))
+    return fail(WH_INVALID_DESCRIPTOR, "odm_channels %d (0 or 1: one channel, 2: two)", d.odm_channels);
   return WH_SUCCESS;
 }
 
@@ -830,9 +832,13 @@ int iteration(wh_ensemble* e, bool merge, bool first, hipStream_t s) {
     if (e->gp()) WH_TRY(gpshift(e, s));
     WH_TRY(xsweep(e, wh::kScaleOnLoad, s));
     WH_TRY(psweep(e, e->gp() ? wh::kGP : 0, 1.0, 0, s));
-    WH_LAUNCH(wh::launch_odm(d.precision, e->mem_dev.as<wh::Member>(), e->work_odm_x.buf.as<unsigned>(),
-                             e->work_odm_x.n, e->work_odm_p.buf.as<unsigned>(), e->work_odm_p.n, s),
-              "odm");
+    // one Lindblad channel (GPU_Wigner2D_FFT, weight 1, sign 1) or two applied in turn
+    // (GPU_Wigner2D_ODM2: weightLindbland1 with sign 1, then weightLindbland2 with sign -1)
+    for (int ch = 0; ch < (d.odm_channels == 2 ? 2 : 1); ++ch)
+      WH_LAUNCH(wh::launch_odm(d.precision, e->mem_dev.as<wh::Member>(), e->work_odm_x.buf.as<unsigned>(),
+                               e->work_odm_x.n, e->work_odm_p.buf.as<unsigned>(), e->work_odm_p.n,
+                               ch == 0 ? d.odm_weight1 : d.odm_weight2, ch == 0 ? 1.0 : -1.0, s),
+                "odm");
     WH_LAUNCH(wh::launch_colsum(storage_code(d.precision), e->mem_dev.as<wh::Member>(),
                                 e->work_colsum.buf.as<unsigned>(),
                                 e->work_colsum.n, s),
@@ -1045,6 +1051,9 @@ extern "C" int wh_wigner2d_create(const wh_wigner2d_desc* d, wh_wigner2d** out) 
   c.gamma_dt = d->gamma_dt;
   c.renormalize = d->renormalize;
   c.gp_coeff = d->gp_coeff;
+  c.odm_channels = d->odm_channels;
+  c.odm_weight1 = d->odm_weight1;
+  c.odm_weight2 = d->odm_weight2;
   wh_member_desc m{};
   m.struct_size = sizeof(m);
   m.x_dim = d->x_dim;

@@ -40,8 +40,10 @@ __device__ __forceinline__ cplx<U> nar(cplx<double> v) {
 
 }  // namespace
 
+// weight, sgn: the channel's weight and sign (GPU_Wigner2D_ODM2 runs two channels,
+// (weightLindbland1, +1) then (weightLindbland2, -1), GPU_Wigner2D_FFT.py:2173-2181)
 template <typename S, typename T, int N, int R, int BLOCK>
-__device__ __forceinline__ void odm_x_body(const Member& m, int tile, cplx<T>* lds) {
+__device__ __forceinline__ void odm_x_body(const Member& m, int tile, cplx<T>* lds, double weight, double sgn) {
   constexpr int TT = N / R, ROWS = BLOCK / TT;
   constexpr int LSZ = Padded<N, R>::size;
   const int tid = threadIdx.x, team = tid / TT, t = tid % TT;
@@ -90,7 +92,7 @@ __device__ __forceinline__ void odm_x_body(const Member& m, int tile, cplx<T>* l
         const double h = 0.5 * m.odm_x[xi] * inv;  // cos 2x / 2
         acc[j] = {h * u.x, h * u.y};
       } else if (term == 1) {
-        const double h = 0.5 * m.odm_sign * m.odm_x[N + xi] * inv;  // i s sin 2x / 2
+        const double h = 0.5 * m.odm_sign * sgn * m.odm_x[N + xi] * inv;  // i s sin 2x / 2
         acc[j] = {acc[j].x - h * u.y, acc[j].y + h * u.x};
       } else {
         acc[j] = {acc[j].x + inv * u.x, acc[j].y + inv * u.y};
@@ -98,7 +100,7 @@ __device__ __forceinline__ void odm_x_body(const Member& m, int tile, cplx<T>* l
     }
   }
   // W += c Lx (re-read the row: it stays in cache, and f no longer holds it)
-  const double c = m.odm_c;
+  const double c = m.odm_c * weight;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const cplx<double> w = wide(W[t + j * TT]);
@@ -107,7 +109,7 @@ __device__ __forceinline__ void odm_x_body(const Member& m, int tile, cplx<T>* l
 }
 
 template <typename S, typename T, int N, int R, int BLOCK>
-__device__ __forceinline__ void odm_p_body(const Member& m, int tile, cplx<T>* lds) {
+__device__ __forceinline__ void odm_p_body(const Member& m, int tile, cplx<T>* lds, double weight, double sgn) {
   constexpr int TT = N / R;
   constexpr int C = BLOCK / TT;
   const int tid = threadIdx.x;
@@ -128,7 +130,7 @@ __device__ __forceinline__ void odm_p_body(const Member& m, int tile, cplx<T>* l
   }
   team_fft<T, N, R, -1>(a, t, view, tw);
   team_fft<T, N, R, -1>(b, t, view, tw);
-  const double half_c = 0.5 * m.odm_c, s = m.odm_sign;
+  const double half_c = 0.5 * m.odm_c * weight, s = m.odm_sign * sgn;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const int i = t + j * TT;
@@ -166,18 +168,19 @@ __device__ __forceinline__ void odm_back_body(const Member& m, int tile, cplx<T>
 }
 
 template <typename S, typename T, int KIND>
-__global__ void __launch_bounds__(kOdmBlock) odm_kernel(const Member* members, const unsigned* work) {
+__global__ void __launch_bounds__(kOdmBlock) odm_kernel(const Member* members, const unsigned* work, double weight,
+                                                        double sgn) {
   __shared__ cplx<T> lds[lds_elems<kOdmBlock, kOdmR>()];
   const unsigned w = work[blockIdx.x];
   const Member& m = members[w >> 16];
   const int tile = int(w & 0xffffu);
   constexpr int R = kOdmR, BLOCK = kOdmBlock;
   if constexpr (KIND == 0) {
-#define BODY(n) odm_x_body<S, T, n, R, BLOCK>(m, tile, lds)
+#define BODY(n) odm_x_body<S, T, n, R, BLOCK>(m, tile, lds, weight, sgn)
     WH_DISPATCH(m.X, BODY)
 #undef BODY
   } else if constexpr (KIND == 1) {
-#define BODY(n) odm_p_body<S, T, n, R, BLOCK>(m, tile, lds)
+#define BODY(n) odm_p_body<S, T, n, R, BLOCK>(m, tile, lds, weight, sgn)
     WH_DISPATCH(m.P, BODY)
 #undef BODY
   } else {
@@ -192,20 +195,20 @@ int odm_rows_per_tile(int X) { return kOdmBlock / (X / kOdmR); }
 int odm_cols_per_tile(int P) { return kOdmBlock / (P / kOdmR); }
 
 hipError_t launch_odm(int prec, const Member* members, const unsigned* work_rows, int nrows,
-                      const unsigned* work_cols, int ncols, hipStream_t s) {
+                      const unsigned* work_cols, int ncols, double weight, double sgn, hipStream_t s) {
   const dim3 b(kOdmBlock);
   if (prec == 0) {
-    hipLaunchKernelGGL((odm_kernel<double, double, 0>), dim3(nrows), b, 0, s, members, work_rows);
-    hipLaunchKernelGGL((odm_kernel<double, double, 1>), dim3(ncols), b, 0, s, members, work_cols);
-    hipLaunchKernelGGL((odm_kernel<double, double, 2>), dim3(nrows), b, 0, s, members, work_rows);
+    hipLaunchKernelGGL((odm_kernel<double, double, 0>), dim3(nrows), b, 0, s, members, work_rows, weight, sgn);
+    hipLaunchKernelGGL((odm_kernel<double, double, 1>), dim3(ncols), b, 0, s, members, work_cols, weight, sgn);
+    hipLaunchKernelGGL((odm_kernel<double, double, 2>), dim3(nrows), b, 0, s, members, work_rows, weight, sgn);
   } else if (prec == 1) {
-    hipLaunchKernelGGL((odm_kernel<float, float, 0>), dim3(nrows), b, 0, s, members, work_rows);
-    hipLaunchKernelGGL((odm_kernel<float, float, 1>), dim3(ncols), b, 0, s, members, work_cols);
-    hipLaunchKernelGGL((odm_kernel<float, float, 2>), dim3(nrows), b, 0, s, members, work_rows);
+    hipLaunchKernelGGL((odm_kernel<float, float, 0>), dim3(nrows), b, 0, s, members, work_rows, weight, sgn);
+    hipLaunchKernelGGL((odm_kernel<float, float, 1>), dim3(ncols), b, 0, s, members, work_cols, weight, sgn);
+    hipLaunchKernelGGL((odm_kernel<float, float, 2>), dim3(nrows), b, 0, s, members, work_rows, weight, sgn);
   } else {
-    hipLaunchKernelGGL((odm_kernel<float, double, 0>), dim3(nrows), b, 0, s, members, work_rows);
-    hipLaunchKernelGGL((odm_kernel<float, double, 1>), dim3(ncols), b, 0, s, members, work_cols);
-    hipLaunchKernelGGL((odm_kernel<float, double, 2>), dim3(nrows), b, 0, s, members, work_rows);
+    hipLaunchKernelGGL((odm_kernel<float, double, 0>), dim3(nrows), b, 0, s, members, work_rows, weight, sgn);
+    hipLaunchKernelGGL((odm_kernel<float, double, 1>), dim3(ncols), b, 0, s, members, work_cols, weight, sgn);
+    hipLaunchKernelGGL((odm_kernel<float, double, 2>), dim3(nrows), b, 0, s, members, work_rows, weight, sgn);
   }
   return hipGetLastError();
 }

@@ -207,7 +207,7 @@ class _Tables:
         d.gp_rho_factor = self.rho_factor
 
 
-def _common(d, *, precision, scheme, t: _Tables, dt, gamma, damping, kinetic, gp_coeff):
+def _common(d, *, precision, scheme, t: _Tables, dt, gamma, damping, kinetic, gp_coeff, odm_weights=None):
     d.precision = PRECISIONS[precision]
     d.scheme = WH_SCHEME_LIE if scheme == "lie" else WH_SCHEME_STRANG
     d.dt, d.hbar, d.mass = t.k_dt, t.k_hbar, t.k_mass
@@ -227,6 +227,11 @@ def _common(d, *, precision, scheme, t: _Tables, dt, gamma, damping, kinetic, gp
     d.gamma_dt = float(dt) * float(gamma)  # host constants (FFT.py:1150-1156)
     d.renormalize = 1 if gamma != 0.0 else 0
     d.gp_coeff = t.k_agp
+    # ODM channels: one with weight 1 (GPU_Wigner2D_FFT.Run) or the two of GPU_Wigner2D_ODM2
+    w = (1.0,) if odm_weights is None else tuple(float(v) for v in odm_weights)
+    d.odm_channels = len(w)
+    d.odm_weight1 = w[0]
+    d.odm_weight2 = w[1] if len(w) > 1 else 0.0
 
 
 class _PlanBase:
@@ -271,6 +276,8 @@ class Wigner2D(_PlanBase):
     kinetic     : ignored by the Lie scheme (the reference hard-wires dt*lambda*p/m).
     damping     : None, 'CaldeiraLeggett' (order-3 composite) or 'ODM' (Lindblad dissipator of
                   order 1, Lie scheme; needs L_material and fp_damping; odm_sign -1 pumps).
+    odm_weights : ODM channel weights: None = one channel of weight 1 (GPU_Wigner2D_FFT);
+                  (w1, w2) = GPU_Wigner2D_ODM2 (W += L(W) with (w1, +1), then (w2, -1)).
     gp_coeff    : Gross-Pitaevskii coefficient a_GP (0 = off).
     mask        : absorbing edge mask of the potential kernel (on in the reference).
     truncate_constants : emulate the '%f' splicing of kernel constants.
@@ -288,9 +295,11 @@ class Wigner2D(_PlanBase):
                  kinetic=None, scheme="lie", D_theta=0.0, D_lambda=0.0, gamma=0.0,
                  damping=None, fp_damping=None, gp_coeff=0.0, gp_rho_factor=None, mask=True,
                  truncate_constants=False, precision="fp64", time_rows=None, pot_table=None,
-                 device=None, variants=None, L_material=None, odm_sign=1.0):
+                 device=None, variants=None, L_material=None, odm_sign=1.0, odm_weights=None):
         import torch
 
+        if odm_weights is not None and not 1 <= len(odm_weights) <= 2:
+            raise ValueError("odm_weights: one or two Lindblad channel weights")
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         self.precision = precision
@@ -315,7 +324,7 @@ class Wigner2D(_PlanBase):
         d = _lib.Wigner2DDesc()
         d.struct_size = ctypes.sizeof(_lib.Wigner2DDesc)
         _common(d, precision=precision, scheme=scheme, t=t, dt=dt, gamma=gamma, damping=damping,
-                kinetic=kinetic, gp_coeff=gp_coeff)
+                kinetic=kinetic, gp_coeff=gp_coeff, odm_weights=odm_weights)
         t.fill(d)
         self._plan = ctypes.c_void_p()
         with torch.cuda.device(self.device):

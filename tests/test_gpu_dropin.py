@@ -132,3 +132,31 @@ def test_odm_x2_notebook(cuda, tmp_path):
     inst.W_init = inst.Wigner_HarmonicOscillator(0, inst.omega, 0.0, 3.0)
     assert inst.Run() == 0
     _check(inst, -1.28963238389e-05, 0.313738969442)
+
+
+def test_odm2_x2_notebook(cuda, tmp_path):
+    """instances/Wigner2D/Wigner2D_ODM2_X2_D0.ipynb: 512^2, amplitudes 12, dt = 0.005, 20000 steps,
+    gamma = 0.07/4, L_material = 3, epsilon = 0.5, weights 1 and 0.9, HO ground state at p = 3;
+    cell 11: min / max of W_end."""
+    from pywignercuda_amd.GPU_Wigner2D_FFT import GPU_Wigner2D_ODM2
+
+    class frame(GPU_Wigner2D_ODM2):
+        def __init__(self):
+            mass = 1.0
+            self.L_material = 3.0
+            epsilon = 0.5
+            self.epsilon = epsilon
+            self.dampingFunction = "ODM"
+            self.grossPitaevskiiCoefficient = 0.0
+            self.omega = 1.0
+            X2_constant = 0.5 * mass * self.omega ** 2
+            self.fp_Damping_String = " p*p/sqrt( p*p + {epsilon}  ) ".format(epsilon=epsilon)
+            self.SetTimeTrack(0.005, 20000, 2000, fileName=str(tmp_path / "X2_ODM_D0.hdf5"))
+            GPU_Wigner2D_ODM2.__init__(self, 512, 512, 12, 12, 1.0, mass, 0.0, 0.0, 0.07 / 4.0, 1.0, 0.9,
+                                       "{0}*pow(x,2)".format(X2_constant), "2*{0}*x".format(X2_constant),
+                                       "p*p/( 2.*{m} )".format(m=mass))
+
+    inst = frame()
+    inst.W_init = inst.Wigner_HarmonicOscillator(0, inst.omega, 0.0, 3.0)
+    assert inst.Run() == 0
+    _check(inst, -1.56718191353e-06, 0.310080647085)

@@ -30,7 +30,7 @@ def rel_l2(a, b):
 def make_pair(cuda, X=256, P=256, amp=10.0, dt=0.01, scheme="lie", mask=True, truncate=False,
               D_theta=0.0, D_lambda=0.0, gamma=0.0, damping=None, fp_str=None, fp_fn=None, gp=0.0,
               precision="fp64", pot=("0.5*pow(x,2)+0.1*pow(x,4)", "x+0.4*pow(x,3)", V_quartic, dV_quartic),
-              L_material=None, hbar=1.0):
+              L_material=None, hbar=1.0, odm_weights=None):
     from pywignercuda_amd.wigner2d import Wigner2D
 
     vs, dvs, vf, dvf = pot
@@ -38,12 +38,12 @@ def make_pair(cuda, X=256, P=256, amp=10.0, dt=0.01, scheme="lie", mask=True, tr
                    kinetic="0.5*p*p" if scheme == "strang" else None, scheme=scheme,
                    D_theta=D_theta, D_lambda=D_lambda, gamma=gamma, damping=damping,
                    fp_damping=fp_str, gp_coeff=gp, mask=mask, truncate_constants=truncate,
-                   precision=precision, device=cuda, L_material=L_material)
+                   precision=precision, device=cuda, L_material=L_material, odm_weights=odm_weights)
     orc = ow.Wigner2DOracle(X, P, amp, amp, hbar, 1.0, dt, D_theta=D_theta, D_lambda=D_lambda,
                             gamma=gamma, potential=vf, dpotential=dvf,
                             kinetic=(lambda p: 0.5 * p * p), fp_damping=fp_fn, damping=damping,
                             gp_coeff=gp, scheme=scheme, mask=mask, truncate_constants=truncate,
-                            L_material=L_material)
+                            L_material=L_material, odm_weights=odm_weights)
     return hip, orc
 
 
@@ -233,6 +233,26 @@ def test_lie_odm_fp64(cuda, gp, hbar):
     W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 30)
     assert rel_l2(W, W_ref) < 1e-10
     check_obs(obs, obs_ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_lie_odm2_two_channels(cuda, precision):
+    """GPU_Wigner2D_ODM2 (GPU_Wigner2D_FFT.py:1973-2230): W += L(W) with (weight 1, sign 1), then
+    with (weight 0.9, sign -1) on the result, as the ODM2 notebook sets them."""
+    fp_str = " p*p/sqrt( p*p + 0.5  ) "
+    fp_fn = lambda p: p * p / np.sqrt(p * p + 0.5)  # noqa: E731
+    hip, orc = make_pair(cuda, amp=12.0, dt=0.005, gamma=0.07 / 4.0, damping="ODM", fp_str=fp_str, fp_fn=fp_fn,
+                         truncate=True, L_material=3.0, odm_weights=(1.0, 0.9), precision=precision,
+                         pot=("0.5*pow(x,2)", "x", lambda t, x: 0.5 * x * x, lambda t, x: x))
+    W0 = ow.harmonic_oscillator_wigner(orc.g, 0, 1.0, 0.0, 3.0)
+    W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 30)
+    err = rel_l2(W.astype(np.complex128), W_ref)
+    print(f"ODM2 {precision}: rel L2 {err:.3e}")
+    if precision == "fp64":
+        assert err < 1e-10
+        check_obs(obs, obs_ref)
+    else:
+        assert err < 1e-5
 
 
 def test_lie_odm_fp32(cuda):
