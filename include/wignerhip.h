@@ -234,6 +234,30 @@ int wh_wigner2d_adjoint_step(wh_wigner2d* plan, void* Lambda, const void* W_next
 /* Observables of W (FFT.py:1857-1878) into one row of WH_NOBS doubles (device). */
 int wh_wigner2d_observe(wh_wigner2d* plan, const void* W, double* obs_row, void* stream);
 
+/*
+ * No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under hipGraph stream capture, serving loops that forbid hipMalloc): the
+ * observables of wh_wigner2d_observe without a plan and without any allocation.  The
+ * caller passes the state and the host grids (all device memory); the column sums go
+ * to a fixed device array of the library (one per GPU, about 81 KiB, held until the
+ * process ends), so x_dim is at most WH_NOALLOC_MAX_X (WH_INVALID_SIZE beyond).  Calls
+ * that overlap on one GPU share that array and race.  The first call on a device looks
+ * the array up (hipGetSymbolAddress, not a stream operation): make it outside capture.
+ */
+#define WH_NOALLOC_MAX_X 2048
+typedef struct wh_observe_desc {
+  int struct_size;            /* sizeof(wh_observe_desc) */
+  int precision;              /* storage of W: WH_C128, or WH_C64 / WH_C64_WIDE (complex64) */
+  int x_dim, p_dim;           /* W is [p_dim][x_dim], FFT ordered */
+  const double* x_host;       /* [X] host x grid, as wh_member_desc */
+  const double* p_host;       /* [P] */
+  const double* dv_host;      /* [X] dPotential(0,x) */
+  const double* v0_host;      /* [X] Potential(0,x) */
+  double dxdp;                /* dX*dP */
+  double inv_2m;              /* 1/(2 mass) */
+} wh_observe_desc;
+int wh_wigner2d_observe_noalloc(const wh_observe_desc* desc, const void* W, double* obs_row, void* stream);
+
 /* ---- ensembles: many Wigner functions, uneven sizes, one launch per sweep --- */
 /*
  * No counterpart in the reference (one grid per class instance there): the

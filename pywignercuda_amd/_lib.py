@@ -19,6 +19,7 @@ WH_SCHEME_LIE, WH_SCHEME_STRANG = 0, 1
 WH_MAX_DEG = 8
 WH_POT_TABLE = -1
 WH_NOBS = 16
+WH_NOALLOC_MAX_X = 2048
 OBS_COLUMNS = ["X", "X2", "P", "P2", "XP", "dV", "PdV", "XdV", "H", "neg", "norm",
                "rawnorm_re", "rawnorm_im", "probsq"]
 
@@ -160,6 +161,16 @@ class DiracDesc(ctypes.Structure):
     ]
 
 
+class ObserveDesc(ctypes.Structure):
+    """wh_observe_desc of include/wignerhip.h (pointers are device addresses)."""
+
+    _fields_ = [
+        ("struct_size", c_int), ("precision", c_int), ("x_dim", c_int), ("p_dim", c_int),
+        ("x_host", c_void_p), ("p_host", c_void_p), ("dv_host", c_void_p), ("v0_host", c_void_p),
+        ("dxdp", c_double), ("inv_2m", c_double),
+    ]
+
+
 class BlochDesc(ctypes.Structure):
     """wh_bloch_desc of include/wignerhip.h."""
 
@@ -205,6 +216,7 @@ def _declare(lib):
         "wh_wigner2d_kinetic": ([vp, vp, vp], ip),
         "wh_wigner2d_potential": ([vp, vp, ip, ip, vp], ip),
         "wh_wigner2d_observe": ([vp, vp, vp, vp], ip),
+        "wh_wigner2d_observe_noalloc": ([POINTER(ObserveDesc), vp, vp, vp], ip),
         "wh_wigner2d_ensemble": ([vp], vp),
         "wh_wigner2d_adjoint_step": ([vp, vp, vp, ip, vp, vp], ip),
         "wh_ensemble_create": ([POINTER(EnsembleDesc), POINTER(MemberDesc), ip, POINTER(vp)], ip),

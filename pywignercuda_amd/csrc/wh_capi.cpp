@@ -31,6 +31,8 @@ hipError_t launch_psweep_f64(int variant, bool cl, const SweepArgs& a, int nwork
 hipError_t launch_psweep_f32(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_gpshift(const Member* members, const unsigned* work, int nwork, hipStream_t s);
 hipError_t launch_colsum(int prec, const Member* members, const unsigned* work, int nwork, hipStream_t s);
+hipError_t launch_observe_noalloc(int prec, Member m, hipStream_t s);
+int noalloc_max_x();
 hipError_t launch_xsweep_f32w(int variant, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_psweep_f32w(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_xsweep_tp_f64(int N, const SweepArgs& a, int nwork, hipStream_t s);
@@ -1212,6 +1214,35 @@ extern "C" int wh_wigner2d_observe(wh_wigner2d* p, const void* W, double* obs_ro
                               e->work_colsum.n, s),
             "colsum");
   return reduce(e, 0, 1, 0, 0, false, s);
+}
+
+extern "C" int wh_wigner2d_observe_noalloc(const wh_observe_desc* d, const void* W, double* obs_row,
+                                           void* stream) {
+  if (!d) return fail(WH_INVALID_VALUE, "wh_wigner2d_observe_noalloc: desc is NULL");
+  if (d->struct_size != (int)sizeof(wh_observe_desc))
+    return fail(WH_INVALID_DESCRIPTOR, "struct_size %d != %d", d->struct_size, (int)sizeof(wh_observe_desc));
+  if (d->precision != WH_C128 && d->precision != WH_C64 && d->precision != WH_C64_WIDE)
+    return fail(WH_INVALID_TYPE, "precision %d is not WH_C128, WH_C64 or WH_C64_WIDE", d->precision);
+  if (d->x_dim < 1 || d->x_dim > wh::noalloc_max_x() || d->p_dim < 1)
+    return fail(WH_INVALID_SIZE, "wh_wigner2d_observe_noalloc: grid %d x %d: x_dim must lie in [1, %d]",
+                d->p_dim, d->x_dim, wh::noalloc_max_x());
+  if (!d->x_host || !d->p_host || !d->dv_host || !d->v0_host)
+    return fail(WH_INVALID_DESCRIPTOR, "wh_wigner2d_observe_noalloc: x_host, p_host, dv_host and v0_host are needed");
+  if (!W || !obs_row) return fail(WH_INVALID_VALUE, "wh_wigner2d_observe_noalloc: W %p obs %p", W, (void*)obs_row);
+  wh::Member m{};
+  m.W = const_cast<void*>(W);
+  m.obs = obs_row;
+  m.x_host = d->x_host;
+  m.p_host = d->p_host;
+  m.dv_host = d->dv_host;
+  m.v0_host = d->v0_host;
+  m.X = d->x_dim;
+  m.P = d->p_dim;
+  m.dxdp = d->dxdp;
+  m.inv_2m = d->inv_2m;
+  WH_LAUNCH(wh::launch_observe_noalloc(storage_code(d->precision), m, static_cast<hipStream_t>(stream)),
+            "observe_noalloc");
+  return WH_SUCCESS;
 }
 
 // ============================================================================ Wigner-Dirac 4x4

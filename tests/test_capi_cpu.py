@@ -278,3 +278,34 @@ def test_reference_binding_accepts_the_reference_plan_arguments():
         with pytest.raises(StatusError) as ei:
             ns["cufftPlanMany"](*args)
         assert ei.value.status == code, (args, ei.value.status)
+
+
+def test_observe_noalloc_validation_before_device():
+    """wh_wigner2d_observe_noalloc checks the descriptor and the size limit before any
+    device call: struct size, precision, x_dim beyond WH_NOALLOC_MAX_X, missing grids."""
+    from pywignercuda_amd import _lib
+
+    L = _lib.lib()
+    d = _lib.ObserveDesc()
+    d.struct_size = ctypes.sizeof(d)
+    d.precision = _lib.WH_C128
+    d.x_dim, d.p_dim = 256, 256
+    d.x_host = d.p_host = d.dv_host = d.v0_host = 0x1000  # never dereferenced on these paths
+    d.dxdp, d.inv_2m = 0.01, 0.5
+    W, obs = ctypes.c_void_p(0x2000), ctypes.c_void_p(0x3000)
+    assert L.wh_wigner2d_observe_noalloc(None, W, obs, None) == 4
+    bad = _lib.ObserveDesc.from_buffer_copy(d)
+    bad.struct_size -= 8
+    assert L.wh_wigner2d_observe_noalloc(ctypes.byref(bad), W, obs, None) == 10
+    bad = _lib.ObserveDesc.from_buffer_copy(d)
+    bad.precision = 7
+    assert L.wh_wigner2d_observe_noalloc(ctypes.byref(bad), W, obs, None) == 3
+    for x_dim in (0, _lib.WH_NOALLOC_MAX_X + 1, 4096):
+        bad = _lib.ObserveDesc.from_buffer_copy(d)
+        bad.x_dim = x_dim
+        assert L.wh_wigner2d_observe_noalloc(ctypes.byref(bad), W, obs, None) == 8
+        assert b"x_dim" in L.wh_last_error()
+    bad = _lib.ObserveDesc.from_buffer_copy(d)
+    bad.v0_host = None
+    assert L.wh_wigner2d_observe_noalloc(ctypes.byref(bad), W, obs, None) == 10
+    assert L.wh_wigner2d_observe_noalloc(ctypes.byref(d), None, obs, None) == 4
