@@ -69,16 +69,18 @@ def initial_state(prop, kind):
 
 
 def timed_run(prop, W, steps, warmup):
-    """warmup untimed iterations (at least two: the first graph-enabled run captures the step
-    graph, so the timed run only replays it), then `steps` timed iterations into a record
-    buffer allocated before the timing."""
+    """warmup untimed iterations (at least two: the first graph-enabled run captures and
+    uploads the step graph, so the timed run only replays it), then `steps` timed iterations
+    into the same record buffer, allocated before the timing."""
     import torch
 
     from pywignercuda_amd._lib import WH_NOBS
 
     warm = max(int(warmup), 2)
-    prop.run(W, warm, use_graph=True)
-    obs = torch.zeros((steps + 1, WH_NOBS), dtype=torch.float64, device=W.device)
+    # one record buffer and one time origin for the warm-up and the timed run, as a caller's
+    # steady-state loop has them: the timed call re-binds nothing
+    obs = torch.zeros((max(steps, warm) + 1, WH_NOBS), dtype=torch.float64, device=W.device)
+    prop.run(W, warm, first_step=warm, use_graph=True, obs=obs)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     prop.run(W, steps, first_step=warm, normalize_initial=False, use_graph=True, obs=obs)

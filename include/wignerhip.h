@@ -239,7 +239,7 @@ int wh_wigner2d_observe(wh_wigner2d* plan, const void* W, double* obs_row, void*
  * (code under hipGraph stream capture, serving loops that forbid hipMalloc): the
  * observables of wh_wigner2d_observe without a plan and without any allocation.  The
  * caller passes the state and the host grids (all device memory); the column sums go
- * to a fixed device array of the library (one per GPU, about 81 KiB, held until the
+ * to a fixed device array of the library (one per GPU, about 82 KiB, held until the
  * process ends), so x_dim is at most WH_NOALLOC_MAX_X (WH_INVALID_SIZE beyond).  Calls
  * that overlap on one GPU share that array and race.  The first call on a device looks
  * the array up (hipGetSymbolAddress, not a stream operation): make it outside capture.

@@ -528,9 +528,16 @@ int build_work_lists(wh_ensemble* e) {
   for (int i = 0; i < e->n; ++i) {
     const auto& m = e->mem[i];
     const unsigned hi = unsigned(i) << 16;
-    for (int t = 0; t < (m.X + 31) / 32; ++t) wc.push_back(hi | unsigned(t));
+    // column-sum tiles: with a multiple of 8 tiles, block q (on XCD q % 8) takes tile
+    // (q % 8) * (nt / 8) + q / 8, so that neighbouring tiles (halves of one 128-byte line)
+    // are read on one XCD
+    const int nc = (m.X + wh::kColsumCols - 1) / wh::kColsumCols;
+    for (int q = 0; q < nc; ++q) {
+      const int t = (nc % 8 == 0) ? (q % 8) * (nc / 8) + q / 8 : q;
+      wc.push_back(hi | unsigned(t));
+    }
     const size_t cells = size_t(m.X) * size_t(m.P);
-    for (size_t t = 0; t < (cells + 8191) / 8192; ++t) ws.push_back(hi | unsigned(t));
+    for (size_t t = 0; t < (cells + wh::kScaleChunk - 1) / wh::kScaleChunk; ++t) ws.push_back(hi | unsigned(t));
     if (e->gp()) {
       const int grows = wh::tile_lines(m.X, wh::SweepVariant{16, 256, 0});
       for (int t = 0; t < m.P / grows; ++t) wg.push_back(hi | unsigned(t));
@@ -793,8 +800,14 @@ int psweep(wh_ensemble* e, int flags, double frac, int step_add, hipStream_t s) 
   return WH_SUCCESS;
 }
 
-int reduce(wh_ensemble* e, int renorm, int record, int rho, int update_scale, bool inc, hipStream_t s) {
+// step_arg >= -1 with use_step_arg: the time index of the record row is step_arg (the
+// device counter is not read); next_step >= 0: the counter is set to it (needs inc)
+int reduce(wh_ensemble* e, int renorm, int record, int rho, int update_scale, bool inc, hipStream_t s,
+           bool use_step_arg = false, int step_arg = 0, int next_step = -1) {
   wh::ReduceArgs r{};
+  r.use_step_arg = use_step_arg ? 1 : 0;
+  r.step_arg = step_arg;
+  r.next_step = next_step;
   r.members = e->mem_dev.as<wh::Member>();
   r.renorm = renorm;
   r.record = record;
@@ -898,6 +911,8 @@ int run_graph(wh_ensemble* e, int count, hipStream_t s) {
     }
     if (err != hipSuccess) return hip_fail(err, "EndCapture");
     WH_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "GraphInstantiate");
+    // uploaded now, so that the first replay does not carry the upload
+    WH_HIP(hipGraphUpload(g.exec, s), "GraphUpload");
     it = e->graphs.emplace(key, g).first;
   }
   while (count >= chunk) {
@@ -964,10 +979,9 @@ int ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_step
   // the state before the first iteration: norm, renormalisation factor, (Lie) row 0
   WH_LAUNCH(wh::launch_colsum(storage_code(prec), md, e->work_colsum.buf.as<unsigned>(), e->work_colsum.n, s),
             "colsum");
-  WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), d.scheme == WH_SCHEME_LIE ? first_step - 1 : first_step, s),
-            "set_step");
-  WH_TRY(reduce(e, normalize_initial, d.scheme == WH_SCHEME_LIE, e->gp(), 1, false, s));
-  WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), first_step, s), "set_step");
+  // record row of time index first_step - 1 (Lie) or first_step; the counter then reads first_step
+  WH_TRY(reduce(e, normalize_initial, d.scheme == WH_SCHEME_LIE, e->gp(), 1, true, s, true,
+                d.scheme == WH_SCHEME_LIE ? first_step - 1 : first_step, first_step));
   const bool merged_strang = d.scheme == WH_SCHEME_STRANG && !e->gp();
   if (n_steps > 0 && merged_strang) {
     // first half step of the first iteration, with the pending factor folded in
@@ -1208,12 +1222,11 @@ extern "C" int wh_wigner2d_observe(wh_wigner2d* p, const void* W, double* obs_ro
   hipStream_t s = static_cast<hipStream_t>(stream);
   wh_ensemble* e = p->e;
   WH_TRY(bind_single(e, const_cast<void*>(W), obs_row, 0, s));
-  WH_LAUNCH(wh::launch_set_int(e->step.as<int>(), 0, s), "set_step");
   WH_LAUNCH(wh::launch_colsum(storage_code(e->d.precision), e->mem_dev.as<wh::Member>(),
                               e->work_colsum.buf.as<unsigned>(),
                               e->work_colsum.n, s),
             "colsum");
-  return reduce(e, 0, 1, 0, 0, false, s);
+  return reduce(e, 0, 1, 0, 0, false, s, true, 0);
 }
 
 extern "C" int wh_wigner2d_observe_noalloc(const wh_observe_desc* d, const void* W, double* obs_row,
@@ -1361,6 +1374,8 @@ int dirac_run_graph(wh_dirac* e, int count, hipStream_t s) {
     }
     if (err != hipSuccess) return hip_fail(err, "EndCapture");
     WH_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "GraphInstantiate");
+    // uploaded now, so that the first replay does not carry the upload
+    WH_HIP(hipGraphUpload(g.exec, s), "GraphUpload");
     it = e->graphs.emplace(key, g).first;
   }
   while (count >= chunk) {

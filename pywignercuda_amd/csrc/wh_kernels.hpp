@@ -13,6 +13,8 @@ namespace wh {
 constexpr int kMaxDeg = 8;  // polynomial degree of V(t,x) and K(p)
 constexpr int kNCol = 5;    // column sums: Re S0, Im S0, sum p W, sum p^2 W, sum min(W,0)
 constexpr int kNObs = 16;   // per-step observable record
+constexpr int kColsumCols = 4;     // columns per column-sum tile (64 row groups of 4 lanes)
+constexpr int kScaleChunk = 2048;  // elements per block of the scale kernel
 
 enum ObsIndex {
   kObsX = 0, kObsX2, kObsP, kObsP2, kObsXP, kObsDV, kObsPDV, kObsXDV, kObsH, kObsNeg,
@@ -123,7 +125,11 @@ struct ReduceArgs {
   int rho;        // write rho(x) from the column sums (Gross-Pitaevskii)
   int update_scale;
   const int* step;
-  int* step_inc;  // incremented by one by block 0 (after all blocks read it: separate kernel)
+  int* step_inc;  // set to step + 1 (or next_step) by the last block to finish
+  int use_step_arg;  // 1: the time index is step_arg, the device counter is not read
+  int step_arg;
+  int next_step;     // >= 0: the value written to *step_inc instead of step + 1
+  int pad;
 };
 
 }  // namespace wh

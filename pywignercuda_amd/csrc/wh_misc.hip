@@ -15,18 +15,23 @@
 namespace wh {
 
 // ------------------------------------------------------------------ column sums
+// One tile = kColsumCols consecutive columns of one member; 256 lanes = 64 row groups of
+// kColsumCols lanes (each load instruction of a wave covers 16 rows x 4 columns), the 64
+// group sums combined by a fixed tree in LDS.
 template <typename T>
 __global__ void __launch_bounds__(256) colsum_kernel(const Member* members, const unsigned* work) {
-  __shared__ double red[8][32][kNCol];
+  constexpr int CC = kColsumCols, G = 256 / CC;
+  __shared__ double red[G][CC][kNCol];
   const unsigned w = work[blockIdx.x];
   const Member& m = members[w >> 16];
   const int tile = int(w & 0xffffu);
   const cplx<T>* W = reinterpret_cast<const cplx<T>*>(m.W);
-  const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
-  const int col = tile * 32 + c;
+  const int c = threadIdx.x % CC, g = threadIdx.x / CC;
+  const int col = tile * CC + c;
   double acc[kNCol] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (col < m.X) {
-    for (int i = g; i < m.P; i += 8) {
+#pragma unroll 4
+    for (int i = g; i < m.P; i += G) {
       const cplx<T> v = W[size_t(i) * m.X + col];
       const double p = m.p_host[i], wr = double(v.x);
       acc[0] += wr;
@@ -39,13 +44,16 @@ __global__ void __launch_bounds__(256) colsum_kernel(const Member* members, cons
 #pragma unroll
   for (int k = 0; k < kNCol; ++k) red[g][c][k] = acc[k];
   __syncthreads();
+  for (int h = G / 2; h > 0; h >>= 1) {
+    if (g < h) {
+#pragma unroll
+      for (int k = 0; k < kNCol; ++k) red[g][c][k] += red[g + h][c][k];
+    }
+    __syncthreads();
+  }
   if (g == 0 && col < m.X) {
 #pragma unroll
-    for (int k = 0; k < kNCol; ++k) {
-      double s = 0.0;
-      for (int q = 0; q < 8; ++q) s += red[q][c][k];
-      m.colsums[size_t(col) * kNCol + k] = s;
-    }
+    for (int k = 0; k < kNCol; ++k) m.colsums[size_t(col) * kNCol + k] = red[0][c][k];
   }
 }
 
@@ -56,7 +64,7 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a, int* done, in
   __shared__ double scale_sh[2];
   const int tid = threadIdx.x;
   const Member& m = a.members[blockIdx.x];
-  const int step = *a.step;
+  const int step = a.use_step_arg ? a.step_arg : *a.step;
   double acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = 0.0;
@@ -136,7 +144,7 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a, int* done, in
     const int prev = atomicAdd(done, 1);
     if (prev == nblocks - 1) {
       *done = 0;
-      *a.step_inc = step + 1;
+      *a.step_inc = a.next_step >= 0 ? a.next_step : step + 1;
       __threadfence();
     }
   }
@@ -170,11 +178,11 @@ template <typename T>
 __global__ void __launch_bounds__(256) scale_kernel(const Member* members, const unsigned* work) {
   const unsigned w = work[blockIdx.x];
   const Member& m = members[w >> 16];
-  const size_t chunk = size_t(w & 0xffffu) * 8192;
+  const size_t chunk = size_t(w & 0xffffu) * kScaleChunk;
   const size_t n = size_t(m.X) * size_t(m.P);
   cplx<T>* W = reinterpret_cast<cplx<T>*>(m.W);
   const double sr = m.scale[0], si = m.scale[1];
-  for (size_t i = chunk + threadIdx.x; i < chunk + 8192 && i < n; i += 256) {
+  for (size_t i = chunk + threadIdx.x; i < chunk + kScaleChunk && i < n; i += 256) {
     const double x = double(W[i].x), y = double(W[i].y);
     W[i] = {T(x * sr - y * si), T(x * si + y * sr)};
   }

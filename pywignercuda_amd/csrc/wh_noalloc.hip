@@ -23,7 +23,7 @@ constexpr int kNoAllocMaxX = 2048;  // X of the largest grid (column sums of 204
 struct NoAllocScratch {
   Member member;
   int step, pad[3];
-  unsigned work[kNoAllocMaxX / 32];
+  unsigned work[kNoAllocMaxX / kColsumCols];
   double colsums[size_t(kNoAllocMaxX) * kNCol];
 };
 
@@ -34,13 +34,12 @@ constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
 void* g_scratch_ptr[kMaxDevices] = {};
 
-__global__ void __launch_bounds__(64) noalloc_bind_kernel(NoAllocScratch* sc, Member m, int ntiles) {
-  const int i = threadIdx.x;
-  if (i == 0) {
+__global__ void __launch_bounds__(256) noalloc_bind_kernel(NoAllocScratch* sc, Member m, int ntiles) {
+  if (threadIdx.x == 0) {
     sc->member = m;
     sc->step = 0;
   }
-  if (i < ntiles) sc->work[i] = unsigned(i);  // member 0, column tile i
+  for (int i = threadIdx.x; i < ntiles; i += 256) sc->work[i] = unsigned(i);  // member 0, column tile i
 }
 }  // namespace
 
@@ -80,8 +79,8 @@ hipError_t launch_observe_noalloc(int prec, Member m, hipStream_t s) {
   unsigned* work = reinterpret_cast<unsigned*>(b + offsetof(NoAllocScratch, work));
   m.colsums = reinterpret_cast<double*>(b + offsetof(NoAllocScratch, colsums));
   m.obs_base = 0;
-  const int ntiles = (m.X + 31) / 32;
-  hipLaunchKernelGGL(noalloc_bind_kernel, dim3(1), dim3(64), 0, s, sc, m, ntiles);
+  const int ntiles = (m.X + kColsumCols - 1) / kColsumCols;
+  hipLaunchKernelGGL(noalloc_bind_kernel, dim3(1), dim3(256), 0, s, sc, m, ntiles);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   e = launch_colsum(prec, mem, work, ntiles, s);
@@ -94,6 +93,7 @@ hipError_t launch_observe_noalloc(int prec, Member m, hipStream_t s) {
   r.update_scale = 0;
   r.step = step;
   r.step_inc = nullptr;  // the done counter is only touched with step_inc
+  r.next_step = -1;
   return launch_reduce(r, 1, nullptr, s);
 }
 
