@@ -166,7 +166,7 @@ def uneven_ensemble_members(seed=20261202, n=256):
     return members, init
 
 
-def uneven_ensemble(steps, warmup, dev):
+def uneven_ensemble(steps, warmup, dev, precisions=("fp32", "fp32fast", "fp64")):
     """FP32 run of the seeded uneven ensemble (one launch per sweep for all 256 members) and its
     relative L2 error against the FP64 run of the same ensemble."""
     import numpy as np
@@ -177,7 +177,7 @@ def uneven_ensemble(steps, warmup, dev):
     members, init = uneven_ensemble_members()
     out = {}
     finals = {}
-    for prec in ("fp32", "fp32fast", "fp64"):
+    for prec in precisions:
         ens = WignerEnsemble(members, 1.0, 1.0, 0.01, scheme="lie", precision=prec, device=dev)
         W0 = [_gauss(t.x_host, t.p_host, t.dX, t.dP, *ini) for t, ini in zip(ens.tables, init)]
         warm = ens.states_to_device(W0)  # graph capture and first launches outside the timing
@@ -194,6 +194,8 @@ def uneven_ensemble(steps, warmup, dev):
         finals[prec] = torch.cat([W.reshape(-1).to(torch.complex128) for W in states]).cpu().numpy()
         del ens, states
     for prec in ("fp32", "fp32fast"):
+        if prec not in finals or "fp64" not in finals:
+            continue
         d = finals[prec] - finals["fp64"]
         out[prec]["rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(finals["fp64"]))
     out["cells"] = int(sum(m["X"] * m["P"] for m in members))
@@ -234,6 +236,41 @@ def dirac_workload(precision, steps, dev):
     return {"cell_steps_per_s": cells * steps / wall, "steps": steps,
             "hbm_frac_algorithmic": cells * steps * (1024 if precision == "fp64" else 512) / wall / 1e9
             / HBM_PEAK_GBS}
+
+
+def calibrate_copy(dev):
+    """8 plain device copies of 256 MiB: the scale of the HBM byte counters (FETCH_SIZE,
+    WRITE_SIZE) against known traffic."""
+    import torch
+
+    src = torch.empty(256 << 20, dtype=torch.uint8, device=dev).fill_(1)
+    dst = torch.empty_like(src)
+    for _ in range(8):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"bytes_read_per_copy": 256 << 20, "bytes_written_per_copy": 256 << 20, "copies": 8}
+
+
+def _uneven_one(prec):
+    def run(steps, warmup, dev):
+        r = uneven_ensemble(steps, warmup, dev, precisions=(prec,))
+        return dict(r[prec], cells=r["cells"], members=r["members"])
+    return run
+
+
+# extra workloads runnable alone (--workload NAME): (runner(steps, warmup, dev), description)
+SIDE_WORKLOADS = {
+    "batch-fp64": (lambda st, w, d: hbm_batch("fp64", 64, st, w, d), "64 x 1024^2 plain quartic Lie step, FP64"),
+    "batch-fp32": (lambda st, w, d: hbm_batch("fp32", 128, st, w, d),
+                   "128 x 1024^2 plain quartic Lie step, complex64 state with FP64 arithmetic"),
+    "batch-fp32fast": (lambda st, w, d: hbm_batch("fp32fast", 128, st, w, d),
+                       "128 x 1024^2 plain quartic Lie step, FP32 arithmetic"),
+    "uneven-fp32": (_uneven_one("fp32"), "256-member uneven ensemble (sides 128..1024, seeded), 'fp32'"),
+    "uneven-fp32fast": (_uneven_one("fp32fast"), "256-member uneven ensemble, 'fp32fast'"),
+    "uneven-fp64": (_uneven_one("fp64"), "256-member uneven ensemble, FP64"),
+    "dirac-fp64": (lambda st, w, d: dirac_workload("fp64", st, d), "Wigner-Dirac 4x4, 512^2 x 16 planes, FP64"),
+    "dirac-fp32": (lambda st, w, d: dirac_workload("fp32", st, d), "Wigner-Dirac 4x4, 512^2 x 16 planes, FP32"),
+}
 
 
 def _cpu_model():
@@ -307,8 +344,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=200)
-    ap.add_argument("--workload", choices=("damped", "plain"), default="damped",
-                    help="timed workload: the damped 1024^2 step (default) or the undamped quartic step")
+    ap.add_argument("--workload", choices=("damped", "plain") + tuple(SIDE_WORKLOADS), default="damped",
+                    help="timed workload: the damped 1024^2 step (default) or the undamped quartic step; "
+                         "the others run one of the extra workloads alone for --steps steps (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 8 plain device copies of 256 MiB (scale for the HBM byte counters)")
     a = ap.parse_args()
@@ -319,6 +357,16 @@ def main():
     import torch
 
     dev = torch.device("cuda:0")
+    if a.workload in SIDE_WORKLOADS:
+        fn, desc = SIDE_WORKLOADS[a.workload]
+        r = fn(a.steps, a.warmup, dev)
+        side = {"metric": "cell-steps/s (one extra workload alone)", "value": r["cell_steps_per_s"],
+                "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+                "higher_is_better": True, "data": "synthetic", "config": {"workload": desc}, "detail": r}
+        if a.calibrate_copy:
+            side["copy_calibration"] = calibrate_copy(dev)
+        print(json.dumps(side))
+        return
     if a.workload == "plain":
         prop = make(plain("fp64"), dev)
         W = prop.to_device(initial_state(prop, "gauss"))
@@ -424,13 +472,7 @@ def main():
         out["extra"]["dirac_512^2_fp64"] = dirac_workload("fp64", 1000, dev)
         out["extra"]["dirac_512^2_fp32"] = dirac_workload("fp32", 1000, dev)
     if a.calibrate_copy:
-        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev).fill_(1)
-        dst = torch.empty_like(src)
-        for _ in range(8):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        out["copy_calibration"] = {"bytes_read_per_copy": 256 << 20, "bytes_written_per_copy": 256 << 20,
-                                   "copies": 8}
+        out["copy_calibration"] = calibrate_copy(dev)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

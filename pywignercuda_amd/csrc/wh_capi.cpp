@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -35,12 +36,6 @@ hipError_t launch_observe_noalloc(int prec, Member m, hipStream_t s);
 int noalloc_max_x();
 hipError_t launch_xsweep_f32w(int variant, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_psweep_f32w(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
-hipError_t launch_xsweep_tp_f64(int N, const SweepArgs& a, int nwork, hipStream_t s);
-hipError_t launch_xsweep_tp_f32(int N, const SweepArgs& a, int nwork, hipStream_t s);
-hipError_t launch_xsweep_tp_f32w(int N, const SweepArgs& a, int nwork, hipStream_t s);
-hipError_t launch_psweep_tp_f64(int N, const SweepArgs& a, int nwork, hipStream_t s);
-hipError_t launch_psweep_tp_f32(int N, const SweepArgs& a, int nwork, hipStream_t s);
-hipError_t launch_psweep_tp_f32w(int N, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_reduce(const ReduceArgs& a, int nmembers, int* done, hipStream_t s);
 hipError_t launch_colpart_reduce(const Member* members, int nmembers, int xradix, int xblock, const int* step,
                                  hipStream_t s);
@@ -60,17 +55,17 @@ int odm_rows_per_tile(int X);
 int odm_cols_per_tile(int P);
 hipError_t launch_odm(int prec, const Member* members, const unsigned* work_rows, int nrows,
                       const unsigned* work_cols, int ncols, double weight, double sgn, hipStream_t s);
-hipError_t launch_dirac_xsweep(int prec, const DiracPlan* d, int nblocks, hipStream_t s);
+hipError_t launch_dirac_xsweep(int prec, int X, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_psweep(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_colsum(int prec, const DiracPlan* d, int X, hipStream_t s);
 hipError_t launch_dirac_reduce(const DiracPlan* d, int state_add, int renorm, int anti, int inc, hipStream_t s);
 hipError_t launch_dirac_finish(int prec, const DiracPlan* d, int P, hipStream_t s);
-int dirac_x_blocks(int X, int P);
+int dirac_x_blocks(int prec, int X, int P);
 int dirac_gp_blocks(int X, int P);
 int dirac_pg_blocks(int X, int P);
 hipError_t launch_dirac_pgeneral(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_podm(int prec, const DiracPlan* d, int nblocks, hipStream_t s);
-hipError_t launch_dirac_fsweep(int prec, const DiracPlan* d, int nblocks, hipStream_t s);
+hipError_t launch_dirac_fsweep(int prec, int X, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_gpshift(const DiracPlan* d, int nblocks, hipStream_t s);
 int dirac_p_blocks(int X, int P);
 }  // namespace wh
@@ -384,12 +379,6 @@ struct GraphEntry {
   hipGraphExec_t exec = nullptr;
 };
 
-// run of work items whose members share the grid side N (work lists are grouped by size,
-// so that the per-size throughput kernels can be launched once per run)
-struct SizeSeg {
-  int N, off, n;
-};
-
 }  // namespace
 
 struct wh_ensemble {
@@ -405,7 +394,6 @@ struct wh_ensemble {
   // a CL plan also runs plain p sweeps: the merged Strang first half step, Strang + GP
   // V/2, the unfused single V step)
   WorkList work_x, work_p, work_pcl, work_colsum, work_scale, work_gp, work_adj, work_odm_x, work_odm_p;
-  std::vector<SizeSeg> seg_x, seg_p, seg_pcl;  // size runs of work_x (by X), work_p / work_pcl (by P)
   wh::SweepVariant built_x{}, built_p{}, built_pcl{};  // the variants each list was cut for
   DevBuf step, done, gcols;
   hipStream_t capture_stream = nullptr;
@@ -488,10 +476,8 @@ bool fits_all(const wh_ensemble* e, const wh::SweepVariant& v, bool x_axis, bool
 
 // p tiles of one variant, grouped by grid side (ascending), members in order within a size;
 // consecutive tiles of a member go to one XCD (blocks are dealt round-robin to the 8 XCDs)
-void p_tiles(const wh_ensemble* e, const wh::SweepVariant& pv, std::vector<unsigned>& wp, std::vector<SizeSeg>& seg) {
-  seg.clear();
+void p_tiles(const wh_ensemble* e, const wh::SweepVariant& pv, std::vector<unsigned>& wp) {
   for (int N = 1; N <= (1 << 20); N <<= 1) {
-    const size_t p0 = wp.size();
     for (int i = 0; i < e->n; ++i) {
       const auto& m = e->mem[i];
       if (m.P != N) continue;
@@ -503,7 +489,6 @@ void p_tiles(const wh_ensemble* e, const wh::SweepVariant& pv, std::vector<unsig
         wp.push_back(hi | unsigned(t));
       }
     }
-    if (wp.size() > p0) seg.push_back({N, int(p0), int(wp.size() - p0)});
   }
 }
 
@@ -511,9 +496,7 @@ int build_work_lists(wh_ensemble* e) {
   const auto& xv = wh::kXVariants[e->xvar];
   std::vector<unsigned> wx, wp, wpcl, wc, ws, wg, wox, wop;
   // x items grouped by grid side (ascending), members in order within a size
-  e->seg_x.clear();
   for (int N = 1; N <= (1 << 20); N <<= 1) {
-    const size_t x0 = wx.size();
     for (int i = 0; i < e->n; ++i) {
       const auto& m = e->mem[i];
       if (m.X != N) continue;
@@ -521,10 +504,9 @@ int build_work_lists(wh_ensemble* e) {
       const int rows = wh::tile_lines(m.X, xv);
       for (int t = 0; t < m.P / rows; ++t) wx.push_back(hi | unsigned(t));
     }
-    if (wx.size() > x0) e->seg_x.push_back({N, int(x0), int(wx.size() - x0)});
   }
-  p_tiles(e, wh::kPVariants[e->pvar], wp, e->seg_p);
-  if (e->cl()) p_tiles(e, wh::kPCLVariants[e->pclvar], wpcl, e->seg_pcl);
+  p_tiles(e, wh::kPVariants[e->pvar], wp);
+  if (e->cl()) p_tiles(e, wh::kPCLVariants[e->pclvar], wpcl);
   for (int i = 0; i < e->n; ++i) {
     const auto& m = e->mem[i];
     const unsigned hi = unsigned(i) << 16;
@@ -753,16 +735,6 @@ int xsweep(wh_ensemble* e, int flags, hipStream_t s) {
     return fail(WH_INTERNAL_ERROR, "x work list cut for another variant");
   wh::SweepArgs a = sweep_args(e, flags, 1.0, 0);
   a.work = e->work_x.buf.as<unsigned>();
-  if (wh::kXVariants[e->xvar].tp) {
-    for (const SizeSeg& g : e->seg_x) {
-      a.work = e->work_x.buf.as<unsigned>() + g.off;
-      WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_xsweep_tp_f64(g.N, a, g.n, s)
-                : e->d.precision == WH_C64_WIDE ? wh::launch_xsweep_tp_f32w(g.N, a, g.n, s)
-                                                : wh::launch_xsweep_tp_f32(g.N, a, g.n, s),
-                "xsweep");
-    }
-    return WH_SUCCESS;
-  }
   WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_xsweep_f64(e->xvar, a, e->work_x.n, s)
             : e->d.precision == WH_C64_WIDE ? wh::launch_xsweep_f32w(e->xvar, a, e->work_x.n, s)
                                       : wh::launch_xsweep_f32(e->xvar, a, e->work_x.n, s),
@@ -779,20 +751,9 @@ int psweep(wh_ensemble* e, int flags, double frac, int step_add, hipStream_t s) 
   const int v = cl ? e->pclvar : e->pvar;
   const wh::SweepVariant& launched = cl ? wh::kPCLVariants[v] : wh::kPVariants[v];
   const WorkList& wl = cl ? e->work_pcl : e->work_p;
-  const std::vector<SizeSeg>& segs = cl ? e->seg_pcl : e->seg_p;
   if (!same_variant(launched, cl ? e->built_pcl : e->built_p))
     return fail(WH_INTERNAL_ERROR, "p work list cut for another variant");
   a.work = wl.buf.as<unsigned>();
-  if (!cl && launched.tp) {
-    for (const SizeSeg& g : segs) {
-      a.work = wl.buf.as<unsigned>() + g.off;
-      WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_psweep_tp_f64(g.N, a, g.n, s)
-                : e->d.precision == WH_C64_WIDE ? wh::launch_psweep_tp_f32w(g.N, a, g.n, s)
-                                                : wh::launch_psweep_tp_f32(g.N, a, g.n, s),
-                "psweep");
-    }
-    return WH_SUCCESS;
-  }
   WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_psweep_f64(v, cl, a, wl.n, s)
             : e->d.precision == WH_C64_WIDE ? wh::launch_psweep_f32w(v, cl, a, wl.n, s)
                                       : wh::launch_psweep_f32(v, cl, a, wl.n, s),
@@ -1339,12 +1300,12 @@ int dirac_iteration(wh_dirac* e, hipStream_t s) {
   const auto* dp = e->plan_dev.as<wh::DiracPlan>();
   const int prec = e->d.precision;
   if (e->d.gp_coeff != 0.0) WH_LAUNCH(wh::launch_dirac_gpshift(dp, e->gpblocks, s), "dirac_gpshift");
-  WH_LAUNCH(wh::launch_dirac_xsweep(prec, dp, e->xblocks, s), "dirac_xsweep");
+  WH_LAUNCH(wh::launch_dirac_xsweep(prec, e->d.x_dim, dp, e->xblocks, s), "dirac_xsweep");
   WH_TRY(dirac_potential_sweep(e, s));
   if (e->host.filter) {
     // antiparticle step filter and the absorber after it; the record then needs the column
     // sums of the filtered state
-    WH_LAUNCH(wh::launch_dirac_fsweep(prec, dp, e->xblocks, s), "dirac_fsweep");
+    WH_LAUNCH(wh::launch_dirac_fsweep(prec, e->d.x_dim, dp, e->xblocks, s), "dirac_fsweep");
     WH_LAUNCH(wh::launch_dirac_colsum(storage_code(prec), dp, e->d.x_dim, s), "dirac_colsum");
   }
   WH_LAUNCH(wh::launch_dirac_reduce(dp, 1, 1, 1, 1, s), "dirac_reduce");
@@ -1397,7 +1358,7 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   e->d = *desc;
   const auto& d = *desc;
   const size_t X = size_t(d.x_dim), P = size_t(d.p_dim);
-  e->xblocks = wh::dirac_x_blocks(d.x_dim, d.p_dim);
+  e->xblocks = wh::dirac_x_blocks(d.precision, d.x_dim, d.p_dim);
   e->pblocks = wh::dirac_p_blocks(d.x_dim, d.p_dim);
   if (e->xblocks < 1 || e->pblocks < 1) return fail(WH_INVALID_SIZE, "grid %zu x %zu", P, X);
   WH_TRY(make_twiddles(e->twx, d.x_dim, d.precision, d.gp_coeff != 0.0));
@@ -1537,7 +1498,7 @@ extern "C" int wh_dirac_kinetic(wh_dirac* e, void* W, void* stream) {
   if (!W) return fail(WH_INVALID_VALUE, "wh_dirac_kinetic: W is NULL");
   hipStream_t s = static_cast<hipStream_t>(stream);
   WH_TRY(dirac_bind(e, W, e->host.obs, e->host.obs_base, 0, true, s));
-  WH_LAUNCH(wh::launch_dirac_xsweep(e->d.precision, e->plan_dev.as<wh::DiracPlan>(), e->xblocks, s), "dirac_xsweep");
+  WH_LAUNCH(wh::launch_dirac_xsweep(e->d.precision, e->d.x_dim, e->plan_dev.as<wh::DiracPlan>(), e->xblocks, s), "dirac_xsweep");
   return WH_SUCCESS;
 }
 

@@ -628,6 +628,14 @@ template <int N>
 constexpr int dirac_rx() {
   return N >= 512 ? 8 : 4;
 }
+// radix of the kinetic x sweep for the state type S: at 512 a complex128 state takes radix 4
+// (two waves per row, 203 VGPRs, two waves per SIMD); radix 8 needs more than the 512
+// registers of a lane there and spilled 40 VGPRs to scratch (+35% memory traffic, 6% slower
+// per step).  The complex64 state keeps radix 8 (no scratch, 5% faster than radix 4).
+template <int N, typename S>
+constexpr int dirac_rx_state() {
+  return (N == 512 && sizeof(S) == 8) ? 4 : dirac_rx<N>();
+}
 template <int N>
 constexpr int dirac_rp() {
   return N >= 512 ? 8 : 4;
@@ -670,24 +678,15 @@ __device__ __forceinline__ int dirac_col_tile() {
     default: break;                   \
   }
 
-template <typename S, typename T>
+// One kernel per grid side: the register demand differs widely between sides (up to two
+// waves per SIMD at 512 and below, one above), and one kernel for all sides would take the
+// largest one's occupancy.
+template <typename S, typename T, int N, bool FILTER>
 __global__ void __launch_bounds__(kDiracXBlock) dirac_xsweep_kernel(const DiracPlan* __restrict__ dp) {
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
   const DiracPlan& d = *dp;
   const int group = int(blockIdx.x & 3u), tile = int(blockIdx.x >> 2);
-#define BODY(n) dirac_xsweep_body<S, T, n, dirac_rx<n>(), kDiracXBlock>(d, group, tile, lds)
-  WH_DIRAC_DISPATCH(d.X, BODY)
-#undef BODY
-}
-
-template <typename S, typename T>
-__global__ void __launch_bounds__(kDiracXBlock) dirac_fsweep_kernel(const DiracPlan* __restrict__ dp) {
-  __shared__ cplx<T> lds[dirac_lds_elems<kDiracXBlock>()];
-  const DiracPlan& d = *dp;
-  const int group = int(blockIdx.x & 3u), tile = int(blockIdx.x >> 2);
-#define BODY(n) dirac_xsweep_body<S, T, n, dirac_rx<n>(), kDiracXBlock, true>(d, group, tile, lds)
-  WH_DIRAC_DISPATCH(d.X, BODY)
-#undef BODY
+  dirac_xsweep_body<S, T, N, dirac_rx_state<N, S>(), kDiracXBlock, FILTER>(d, group, tile, lds);
 }
 
 template <typename S, typename T, bool CL>
@@ -945,18 +944,19 @@ __global__ void __launch_bounds__(256) dirac_gpshift_kernel(const DiracPlan* __r
 }
 
 // ------------------------------------------------------------------ launchers
-template <int N>
+template <int N, typename S>
 constexpr int xsweep_rows() {
-  return kDiracXBlock / (N / dirac_rx<N>());
+  return kDiracXBlock / (N / dirac_rx_state<N, S>());
 }
 template <int N>
 constexpr int psweep_cols() {
   return kDiracPBlock / (N / dirac_rp<N>());
 }
 
-int dirac_x_blocks(int X, int P) {
+// blocks of the x sweep for a state of precision code prec (0: complex128)
+int dirac_x_blocks(int prec, int X, int P) {
   int rows = 0;
-#define BODY(n) rows = xsweep_rows<n>()
+#define BODY(n) rows = prec == 0 ? xsweep_rows<n, double>() : xsweep_rows<n, float>()
   WH_DIRAC_DISPATCH(X, BODY)
 #undef BODY
   return rows > 0 && P % rows == 0 ? 4 * (P / rows) : -1;
@@ -970,14 +970,29 @@ int dirac_p_blocks(int X, int P) {
   return cols > 0 && X % cols == 0 ? X / cols : -1;
 }
 
-hipError_t launch_dirac_xsweep(int prec, const DiracPlan* d, int nblocks, hipStream_t s) {
-  if (prec == 0)
-    hipLaunchKernelGGL((dirac_xsweep_kernel<double, double>), dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
-  else if (prec == 1)
-    hipLaunchKernelGGL((dirac_xsweep_kernel<float, float>), dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
-  else
-    hipLaunchKernelGGL((dirac_xsweep_kernel<float, double>), dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
+template <typename S, typename T, bool FILTER>
+hipError_t dirac_x_launch(int X, const DiracPlan* d, int nblocks, hipStream_t s) {
+  const dim3 g(nblocks), b(kDiracXBlock);
+  switch (X) {
+    case 128: hipLaunchKernelGGL((dirac_xsweep_kernel<S, T, 128, FILTER>), g, b, 0, s, d); break;
+    case 256: hipLaunchKernelGGL((dirac_xsweep_kernel<S, T, 256, FILTER>), g, b, 0, s, d); break;
+    case 512: hipLaunchKernelGGL((dirac_xsweep_kernel<S, T, 512, FILTER>), g, b, 0, s, d); break;
+    case 1024: hipLaunchKernelGGL((dirac_xsweep_kernel<S, T, 1024, FILTER>), g, b, 0, s, d); break;
+    case 2048: hipLaunchKernelGGL((dirac_xsweep_kernel<S, T, 2048, FILTER>), g, b, 0, s, d); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
+}
+
+template <bool FILTER>
+hipError_t dirac_x_launch_prec(int prec, int X, const DiracPlan* d, int nblocks, hipStream_t s) {
+  if (prec == 0) return dirac_x_launch<double, double, FILTER>(X, d, nblocks, s);
+  if (prec == 1) return dirac_x_launch<float, float, FILTER>(X, d, nblocks, s);
+  return dirac_x_launch<float, double, FILTER>(X, d, nblocks, s);
+}
+
+hipError_t launch_dirac_xsweep(int prec, int X, const DiracPlan* d, int nblocks, hipStream_t s) {
+  return dirac_x_launch_prec<false>(prec, X, d, nblocks, s);
 }
 
 hipError_t launch_dirac_psweep(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s) {
@@ -1033,14 +1048,8 @@ hipError_t launch_dirac_pgeneral(int prec, bool cl, const DiracPlan* d, int nblo
   return hipGetLastError();
 }
 
-hipError_t launch_dirac_fsweep(int prec, const DiracPlan* d, int nblocks, hipStream_t s) {
-  if (prec == 0)
-    hipLaunchKernelGGL((dirac_fsweep_kernel<double, double>), dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
-  else if (prec == 1)
-    hipLaunchKernelGGL((dirac_fsweep_kernel<float, float>), dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
-  else
-    hipLaunchKernelGGL((dirac_fsweep_kernel<float, double>), dim3(nblocks), dim3(kDiracXBlock), 0, s, d);
-  return hipGetLastError();
+hipError_t launch_dirac_fsweep(int prec, int X, const DiracPlan* d, int nblocks, hipStream_t s) {
+  return dirac_x_launch_prec<true>(prec, X, d, nblocks, s);
 }
 
 hipError_t launch_dirac_podm(int prec, const DiracPlan* d, int nblocks, hipStream_t s) {

@@ -105,6 +105,17 @@ __device__ __forceinline__ void store_stream(gptr<cplx<S>> p, cplx<S> v) {
   __builtin_nontemporal_store(vec2{v.x, v.y}, reinterpret_cast<gptr<vec2>>(p));
 }
 
+// Write-back store of a state element, for sweeps whose store instruction covers half of
+// each 128-byte line (the p sweep's C-column rows): the neighbouring tile, on the same XCD at
+// about the same time, fills the other half in L2 before the line is written back.  With
+// streaming stores the half lines went out separately (+39% WRITE_SIZE on a 64 x 1024^2
+// FP64 batch; +1.2% with this store).
+template <typename S>
+__device__ __forceinline__ void store_cached(gptr<cplx<S>> p, cplx<S> v) {
+  typedef S vec2 __attribute__((ext_vector_type(2)));
+  *reinterpret_cast<gptr<vec2>>(p) = vec2{v.x, v.y};
+}
+
 // state storage S <-> arithmetic T (complex64 states with FP64 arithmetic convert at the
 // sweep's load and store; every other combination is the identity)
 template <typename T, typename S>
@@ -648,7 +659,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
   const int ts = opaque_v(int(threadIdx.x));  // offset recomputed from the lane index
   const unsigned soff = (unsigned(ts / C) * unsigned(X) + unsigned(tile * C + ts % C)) * unsigned(sizeof(cplx<S>));
 #pragma unroll
-  for (int j = 0; j < R; ++j) store_stream(at_off(W, soff + unsigned(j) * jstride), to_arith<S>(v[j]));
+  for (int j = 0; j < R; ++j) store_cached(at_off(W, soff + unsigned(j) * jstride), to_arith<S>(v[j]));
   if (a.flags & kTilePart) write_tile_partial<BLOCK, C>(a, m, tile, csum, step - a.step_add);
 }
 
@@ -679,8 +690,10 @@ __device__ __forceinline__ void gpshift_body(const Member& m, int tile, cplx<dou
 }
 
 // ------------------------------------------------------------------ kernels
-// sizes up to MAXN (the throughput forms stop at 1024: one more size path in the same
-// kernel raises its register pressure past the 168 VGPRs of three waves per SIMD)
+// sizes up to MAXN.  One launch covers members of every size: the throughput forms (sides
+// up to 1024) spill a few VGPRs at three waves per SIMD, in the general-polynomial kinetic
+// branch, and still run an uneven 128..1024 ensemble 2% faster than one launch per size
+// (and a uniform batch within 0.5%)
 #define WH_DISPATCH_TO(NVAR, BODY, MAXN)                                                     \
   switch (NVAR) {                                                                            \
     case 128: if constexpr (size_ok<128, R, BLOCK>() && 128 <= (MAXN)) { BODY(128); } break;   \
@@ -724,24 +737,6 @@ __device__ __forceinline__ void psweep_entry(const SweepArgs& a) {
 template <typename S, typename T, int R, int BLOCK, bool CL, bool TP>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(CL ? 1 : TP ? 3 : 2))) psweep_kernel(SweepArgs a) {
   psweep_entry<S, T, R, BLOCK, CL, TP>(a);
-}
-
-// Throughput forms for one grid side N: the host launches them once per size group of the
-// work list (a single kernel for all sizes raises the register pressure past the 168 VGPRs
-// of three waves per SIMD and spills in the hot path).
-template <typename S, typename T, int N, int R, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) xsweep_tp_kernel(SweepArgs a) {
-  using E = LdsElem<S, T, true>;
-  __shared__ cplx<E> lds[lds_elems<BLOCK, R>() + kTileRedBytes / sizeof(cplx<E>)];
-  const unsigned w = a.work[blockIdx.x];
-  xsweep_body<S, T, N, R, BLOCK, true, E>(a, a.members[w >> 16], int(w & 0xffffu), lds);
-}
-template <typename S, typename T, int N, int R, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) psweep_tp_kernel(SweepArgs a) {
-  using E = LdsElem<S, T, true>;
-  __shared__ cplx<E> lds[lds_elems<BLOCK, R>() + kPTailBytes / sizeof(cplx<E>)];
-  const unsigned w = a.work[blockIdx.x];
-  psweep_body<S, T, N, R, BLOCK, false, true, E>(a, a.members[w >> 16], int(w & 0xffffu), lds);
 }
 
 template <int R, int BLOCK>
