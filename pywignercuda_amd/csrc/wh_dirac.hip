@@ -227,8 +227,14 @@ __device__ __forceinline__ void dirac_cl2(cplx<T> (&v)[R], const DiracPlan& d, i
 
 // Column sums of a tile: lanes of one column inside a wave are c + C*t; fold t with xor
 // shuffles, then across waves through LDS.
+// part: -1 all column sums; 0 the trace, beta and negativity sums; 1 the alpha_1 sums
+__device__ __forceinline__ bool dirac_col_owned(int part, int q) {
+  const bool alpha = q >= kDcA1Re && q <= kDcHA1Im;
+  return part < 0 || (part == 1) == alpha;
+}
 template <int BLOCK, int C, typename L>
-__device__ __forceinline__ void dirac_store_colsums(const DiracPlan& d, int tile, double (&acc)[kDNCol], L* lds) {
+__device__ __forceinline__ void dirac_store_colsums(const DiracPlan& d, int tile, double (&acc)[kDNCol], L* lds,
+                                                    int part = -1) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int off = C; off < 64; off <<= 1) {
@@ -254,12 +260,17 @@ __device__ __forceinline__ void dirac_store_colsums(const DiracPlan& d, int tile
       for (int q = 0; q < kDNCol; ++q) sm[q] += red[(w * kLanes + tid) * kDNCol + q];
     }
 #pragma unroll
-    for (int q = 0; q < kDNCol; ++q) gmem(d.colsums)[size_t(tile * C + tid) * kDNCol + q] = sm[q];
+    for (int q = 0; q < kDNCol; ++q)
+      if (dirac_col_owned(part, q)) gmem(d.colsums)[size_t(tile * C + tid) * kDNCol + q] = sm[q];
   }
 }
 
+// Two blocks per column tile: half 0 takes the mass-phase classes 0 and 3 (planes 0, 1, 4, 5,
+// 10, 11, 14, 15: every diagonal plane, so the trace, beta and negativity sums), half 1 the
+// classes 1 and 2 (planes 2, 3, 6, 7, 8, 9, 12, 13: every alpha_1 plane); each writes its own
+// column sums.  One block per tile left half of the CUs idle at 512^2 (128 tiles).
 template <typename S, typename T, int N, int R, int BLOCK, bool CL>
-__device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, cplx<T>* lds) {
+__device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, int half, cplx<T>* lds) {
   constexpr int TT = N / R;
   constexpr int C = BLOCK / TT;  // columns per tile
   const int tid = threadIdx.x;
@@ -315,7 +326,8 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
   // planes by mass-phase class: spinor rows {1,2} or {3,4} x columns {1,2} or {3,4}.  Class
   // multiplier r e^{-i (ph + m)}: m = F- - F+ (lo, lo), F- + F+ (lo, hi), -(F- + F+) (hi, lo),
   // -(F- - F+) (hi, hi), F -/+ = Mass(x -/+ theta/2) c^2 dt; a constant mass gives 0, 2F, -2F, 0.
-  for (int cls = 0; cls < 4; ++cls) {
+  for (int ci = 0; ci < 2; ++ci) {
+    const int cls = half == 0 ? 3 * ci : 1 + ci;
     cplx<double> M[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -354,7 +366,7 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
 #pragma unroll
   for (int j = 0; j < R; ++j) acc[kDcNeg] += trre[j] < 0.0 ? trre[j] : 0.0;
 
-  dirac_store_colsums<BLOCK, C>(d, tile, acc, lds);
+  dirac_store_colsums<BLOCK, C>(d, tile, acc, lds, half);
 }
 
 
@@ -693,7 +705,8 @@ template <typename S, typename T, bool CL>
 __global__ void __launch_bounds__(kDiracPBlock) dirac_psweep_kernel(const DiracPlan* __restrict__ dp) {
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracPBlock>()];
   const DiracPlan& d = *dp;
-#define BODY(n) dirac_psweep_body<S, T, n, dirac_rp<n>(), kDiracPBlock, CL>(d, dirac_col_tile(), lds)
+  const int u = dirac_col_tile();
+#define BODY(n) dirac_psweep_body<S, T, n, dirac_rp<n>(), kDiracPBlock, CL>(d, u >> 1, u & 1, lds)
   WH_DIRAC_DISPATCH(d.P, BODY)
 #undef BODY
 }
@@ -967,7 +980,7 @@ int dirac_p_blocks(int X, int P) {
 #define BODY(n) cols = psweep_cols<n>()
   WH_DIRAC_DISPATCH(P, BODY)
 #undef BODY
-  return cols > 0 && X % cols == 0 ? X / cols : -1;
+  return cols > 0 && X % cols == 0 ? 2 * (X / cols) : -1;  // two blocks (halves) per tile
 }
 
 template <typename S, typename T, bool FILTER>
