@@ -56,7 +56,7 @@ int odm_cols_per_tile(int P);
 hipError_t launch_odm(int prec, const Member* members, const unsigned* work_rows, int nrows,
                       const unsigned* work_cols, int ncols, double weight, double sgn, hipStream_t s);
 hipError_t launch_dirac_xsweep(int prec, int X, const DiracPlan* d, int nblocks, hipStream_t s);
-hipError_t launch_dirac_psweep(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s);
+hipError_t launch_dirac_psweep(int prec, int P, bool cl, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_colsum(int prec, const DiracPlan* d, int X, hipStream_t s);
 hipError_t launch_dirac_reduce(const DiracPlan* d, int state_add, int renorm, int anti, int inc, hipStream_t s);
 hipError_t launch_dirac_finish(int prec, const DiracPlan* d, int P, hipStream_t s);
@@ -1291,7 +1291,7 @@ int dirac_potential_sweep(wh_dirac* e, hipStream_t s) {
   else if (e->host.general)
     WH_LAUNCH(wh::launch_dirac_pgeneral(prec, e->d.gamma_dt != 0.0, dp, e->pgblocks, s), "dirac_pgeneral");
   else
-    WH_LAUNCH(wh::launch_dirac_psweep(prec, e->d.gamma_dt != 0.0, dp, e->pblocks, s), "dirac_psweep");
+    WH_LAUNCH(wh::launch_dirac_psweep(prec, e->d.p_dim, e->d.gamma_dt != 0.0, dp, e->pblocks, s), "dirac_psweep");
   return WH_SUCCESS;
 }
 

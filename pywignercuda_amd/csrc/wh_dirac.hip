@@ -650,7 +650,7 @@ constexpr int dirac_rx_state() {
 }
 template <int N>
 constexpr int dirac_rp() {
-  return N >= 512 ? 8 : 4;
+  return N >= 512 ? 8 : 4;  // at 512 radix 4 (three waves per SIMD) is 8% slower per step
 }
 // radix of the vector-potential p sweeps (4 planes live per lane and two factor matrices)
 template <int N>
@@ -701,14 +701,12 @@ __global__ void __launch_bounds__(kDiracXBlock) dirac_xsweep_kernel(const DiracP
   dirac_xsweep_body<S, T, N, dirac_rx_state<N, S>(), kDiracXBlock, FILTER>(d, group, tile, lds);
 }
 
-template <typename S, typename T, bool CL>
+template <typename S, typename T, int N, bool CL>
 __global__ void __launch_bounds__(kDiracPBlock) dirac_psweep_kernel(const DiracPlan* __restrict__ dp) {
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracPBlock>()];
   const DiracPlan& d = *dp;
   const int u = dirac_col_tile();
-#define BODY(n) dirac_psweep_body<S, T, n, dirac_rp<n>(), kDiracPBlock, CL>(d, u >> 1, u & 1, lds)
-  WH_DIRAC_DISPATCH(d.P, BODY)
-#undef BODY
+  dirac_psweep_body<S, T, N, dirac_rp<N>(), kDiracPBlock, CL>(d, u >> 1, u & 1, lds);
 }
 
 template <typename S, typename T>
@@ -1008,25 +1006,26 @@ hipError_t launch_dirac_xsweep(int prec, int X, const DiracPlan* d, int nblocks,
   return dirac_x_launch_prec<false>(prec, X, d, nblocks, s);
 }
 
-hipError_t launch_dirac_psweep(int prec, bool cl, const DiracPlan* d, int nblocks, hipStream_t s) {
+template <typename S, typename T, bool CL>
+hipError_t dirac_p_launch(int P, const DiracPlan* d, int nblocks, hipStream_t s) {
   const dim3 g(nblocks), b(kDiracPBlock);
-  if (prec == 0) {
-    if (cl)
-      hipLaunchKernelGGL((dirac_psweep_kernel<double, double, true>), g, b, 0, s, d);
-    else
-      hipLaunchKernelGGL((dirac_psweep_kernel<double, double, false>), g, b, 0, s, d);
-  } else if (prec == 1) {
-    if (cl)
-      hipLaunchKernelGGL((dirac_psweep_kernel<float, float, true>), g, b, 0, s, d);
-    else
-      hipLaunchKernelGGL((dirac_psweep_kernel<float, float, false>), g, b, 0, s, d);
-  } else {
-    if (cl)
-      hipLaunchKernelGGL((dirac_psweep_kernel<float, double, true>), g, b, 0, s, d);
-    else
-      hipLaunchKernelGGL((dirac_psweep_kernel<float, double, false>), g, b, 0, s, d);
+  switch (P) {
+    case 128: hipLaunchKernelGGL((dirac_psweep_kernel<S, T, 128, CL>), g, b, 0, s, d); break;
+    case 256: hipLaunchKernelGGL((dirac_psweep_kernel<S, T, 256, CL>), g, b, 0, s, d); break;
+    case 512: hipLaunchKernelGGL((dirac_psweep_kernel<S, T, 512, CL>), g, b, 0, s, d); break;
+    case 1024: hipLaunchKernelGGL((dirac_psweep_kernel<S, T, 1024, CL>), g, b, 0, s, d); break;
+    case 2048: hipLaunchKernelGGL((dirac_psweep_kernel<S, T, 2048, CL>), g, b, 0, s, d); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_dirac_psweep(int prec, int P, bool cl, const DiracPlan* d, int nblocks, hipStream_t s) {
+  if (prec == 0) return cl ? dirac_p_launch<double, double, true>(P, d, nblocks, s)
+                           : dirac_p_launch<double, double, false>(P, d, nblocks, s);
+  if (prec == 1) return cl ? dirac_p_launch<float, float, true>(P, d, nblocks, s)
+                           : dirac_p_launch<float, float, false>(P, d, nblocks, s);
+  return cl ? dirac_p_launch<float, double, true>(P, d, nblocks, s) : dirac_p_launch<float, double, false>(P, d, nblocks, s);
 }
 
 int dirac_pg_blocks(int X, int P) {
