@@ -37,6 +37,8 @@ int noalloc_max_x();
 hipError_t launch_xsweep_f32w(int variant, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_psweep_f32w(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_reduce(const ReduceArgs& a, int nmembers, int* done, hipStream_t s);
+hipError_t launch_colsum_reduce(int prec, const Member* members, const unsigned* work, int nwork,
+                                const ReduceArgs& a, int* member_done, hipStream_t s);
 hipError_t launch_colpart_reduce(const Member* members, int nmembers, int xradix, int xblock, const int* step,
                                  hipStream_t s);
 hipError_t launch_scale(int prec, const Member* members, const unsigned* work, int nwork, hipStream_t s);
@@ -396,6 +398,7 @@ struct wh_ensemble {
   WorkList work_x, work_p, work_pcl, work_colsum, work_scale, work_gp, work_adj, work_odm_x, work_odm_p;
   wh::SweepVariant built_x{}, built_p{}, built_pcl{};  // the variants each list was cut for
   DevBuf step, done, gcols;
+  DevBuf member_done;  // [n] tile counters of the column-sum + reduce kernel (zero between launches)
   hipStream_t capture_stream = nullptr;
   std::map<std::tuple<hipStream_t, int>, GraphEntry> graphs;  // (stream, chunk)
   int graph_chunk = 16;
@@ -680,6 +683,10 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
   WH_TRY(e->step.alloc(16));
   const int zero[4] = {0, 0, 0, 0};
   WH_TRY(e->done.upload(zero, sizeof(zero)));
+  {
+    const std::vector<int> zeros(size_t(n), 0);
+    WH_TRY(e->member_done.upload(zeros.data(), zeros.size() * sizeof(int)));
+  }
 
   WH_TRY(build_work_lists(e.get()));
   *out = e.release();
@@ -850,12 +857,17 @@ int iterations(wh_ensemble* e, int count, bool merge_last, bool first, hipStream
   return WH_SUCCESS;
 }
 
-// Replays `count` middle iterations from a captured graph of graph_chunk iterations.
-// The graph reads member pointers, the step counter and the record base from
-// device memory, so one capture serves every later run of the plan.  The capture is
-// made on the first graph-enabled run whatever its length (capturing executes
-// nothing), so that later runs replay it without capture work.
-int run_graph(wh_ensemble* e, int count, hipStream_t s) {
+// Graph replay for runs with at least two chunks of middle iterations: a shorter run
+// launches its iterations directly, since the host stays ahead of the GPU there and the
+// first replay of a graph in a process costs ~60 us more than direct launches.
+bool graph_pays(int chunk, int use_graph, int count) { return use_graph && count >= 2 * chunk; }
+
+// The graph reads member pointers, the step counter and the record base from device
+// memory, so one capture serves every later run of the plan.
+// The plan's step graph for stream s (graph_chunk iterations): captured (capturing
+// executes nothing), instantiated and uploaded on the first graph-enabled run whatever its
+// length, so that later runs replay it without capture or upload work.
+int ensure_graph(wh_ensemble* e, hipStream_t s, hipGraphExec_t* exec) {
   const int chunk = e->graph_chunk;
   auto key = std::make_tuple(s, chunk);
   auto it = e->graphs.find(key);
@@ -872,13 +884,20 @@ int run_graph(wh_ensemble* e, int count, hipStream_t s) {
     }
     if (err != hipSuccess) return hip_fail(err, "EndCapture");
     WH_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "GraphInstantiate");
-    // uploaded now, so that the first replay does not carry the upload
     WH_HIP(hipGraphUpload(g.exec, s), "GraphUpload");
     it = e->graphs.emplace(key, g).first;
   }
-  while (count >= chunk) {
-    WH_HIP(hipGraphLaunch(it->second.exec, s), "GraphLaunch");
-    count -= chunk;
+  *exec = it->second.exec;
+  return WH_SUCCESS;
+}
+
+// Replays `count` iterations from the step graph (whole chunks), the rest launched directly.
+int run_graph(wh_ensemble* e, int count, hipStream_t s) {
+  hipGraphExec_t exec = nullptr;
+  WH_TRY(ensure_graph(e, s, &exec));
+  while (count >= e->graph_chunk) {
+    WH_HIP(hipGraphLaunch(exec, s), "GraphLaunch");
+    count -= e->graph_chunk;
   }
   return iterations(e, count, true, false, s);
 }
@@ -938,16 +957,33 @@ int ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_step
   const int prec = d.precision;
   const wh::Member* md = e->mem_dev.as<wh::Member>();
   // the state before the first iteration: norm, renormalisation factor, (Lie) row 0
-  WH_LAUNCH(wh::launch_colsum(storage_code(prec), md, e->work_colsum.buf.as<unsigned>(), e->work_colsum.n, s),
-            "colsum");
-  // record row of time index first_step - 1 (Lie) or first_step; the counter then reads first_step
-  WH_TRY(reduce(e, normalize_initial, d.scheme == WH_SCHEME_LIE, e->gp(), 1, true, s, true,
-                d.scheme == WH_SCHEME_LIE ? first_step - 1 : first_step, first_step));
+  {
+    // column sums and reduce in one launch: record row of time index first_step - 1 (Lie) or
+    // first_step, the pending factor, the Gross-Pitaevskii density; the counter then reads first_step
+    wh::ReduceArgs r{};
+    r.members = md;
+    r.renorm = normalize_initial;
+    r.record = d.scheme == WH_SCHEME_LIE;
+    r.rho = e->gp();
+    r.update_scale = 1;
+    r.step = e->step.as<int>();
+    r.step_inc = e->step.as<int>();
+    r.use_step_arg = 1;
+    r.step_arg = d.scheme == WH_SCHEME_LIE ? first_step - 1 : first_step;
+    r.next_step = first_step;
+    WH_LAUNCH(wh::launch_colsum_reduce(storage_code(prec), md, e->work_colsum.buf.as<unsigned>(), e->work_colsum.n,
+                                       r, e->member_done.as<int>(), s),
+              "colsum_reduce");
+  }
   const bool merged_strang = d.scheme == WH_SCHEME_STRANG && !e->gp();
   if (n_steps > 0 && merged_strang) {
     // first half step of the first iteration, with the pending factor folded in
     WH_TRY(psweep(e, wh::kScaleOnLoad, 0.5, 0, s));
     WH_LAUNCH(wh::launch_set_scale(md, e->n, 1.0, 0.0, s), "set_scale");
+  }
+  if (use_graph) {
+    hipGraphExec_t exec = nullptr;
+    WH_TRY(ensure_graph(e, s, &exec));  // ready for a later, longer run
   }
   if (n_steps > 0) {
     // the first iteration takes the factor of the initial reduce; the graph replays the
@@ -955,11 +991,13 @@ int ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_step
     WH_TRY(iterations(e, 1, merged_strang && n_steps > 1, true, s));
     if (merged_strang) {
       if (n_steps > 1) {
-        WH_TRY(use_graph ? run_graph(e, n_steps - 2, s) : iterations(e, n_steps - 2, true, false, s));
+        WH_TRY(graph_pays(e->graph_chunk, use_graph, n_steps - 2) ? run_graph(e, n_steps - 2, s)
+                                                                   : iterations(e, n_steps - 2, true, false, s));
         WH_TRY(iterations(e, 1, false, false, s));  // last iteration: no next half step
       }
     } else {
-      WH_TRY(use_graph ? run_graph(e, n_steps - 1, s) : iterations(e, n_steps - 1, false, false, s));
+      WH_TRY(graph_pays(e->graph_chunk, use_graph, n_steps - 1) ? run_graph(e, n_steps - 1, s)
+                                                                   : iterations(e, n_steps - 1, false, false, s));
     }
     if (tiles_pending(e)) WH_TRY(reduce(e, renorm_flag(e), 1, 0, 1, true, s));
   }
@@ -1318,7 +1356,10 @@ int dirac_iterations(wh_dirac* e, int count, hipStream_t s) {
 }
 
 // As run_graph: captured on the first graph-enabled run whatever its length.
-int dirac_run_graph(wh_dirac* e, int count, hipStream_t s) {
+// The plan's step graph for stream s (graph_chunk iterations): captured (capturing
+// executes nothing), instantiated and uploaded on the first graph-enabled run whatever its
+// length, so that later runs replay it without capture or upload work.
+int dirac_ensure_graph(wh_dirac* e, hipStream_t s, hipGraphExec_t* exec) {
   const int chunk = e->graph_chunk;
   auto key = std::make_tuple(s, chunk);
   auto it = e->graphs.find(key);
@@ -1335,13 +1376,20 @@ int dirac_run_graph(wh_dirac* e, int count, hipStream_t s) {
     }
     if (err != hipSuccess) return hip_fail(err, "EndCapture");
     WH_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "GraphInstantiate");
-    // uploaded now, so that the first replay does not carry the upload
     WH_HIP(hipGraphUpload(g.exec, s), "GraphUpload");
     it = e->graphs.emplace(key, g).first;
   }
-  while (count >= chunk) {
-    WH_HIP(hipGraphLaunch(it->second.exec, s), "GraphLaunch");
-    count -= chunk;
+  *exec = it->second.exec;
+  return WH_SUCCESS;
+}
+
+// Replays `count` iterations from the step graph (whole chunks), the rest launched directly.
+int dirac_run_graph(wh_dirac* e, int count, hipStream_t s) {
+  hipGraphExec_t exec = nullptr;
+  WH_TRY(dirac_ensure_graph(e, s, &exec));
+  while (count >= e->graph_chunk) {
+    WH_HIP(hipGraphLaunch(exec, s), "GraphLaunch");
+    count -= e->graph_chunk;
   }
   return dirac_iterations(e, count, s);
 }
@@ -1525,7 +1573,11 @@ extern "C" int wh_dirac_run(wh_dirac* e, void* W, int first_step, int n_steps, d
   const auto* dp = e->plan_dev.as<wh::DiracPlan>();
   WH_LAUNCH(wh::launch_dirac_colsum(storage_code(e->d.precision), dp, e->d.x_dim, s), "dirac_colsum");
   WH_LAUNCH(wh::launch_dirac_reduce(dp, 0, 0, 0, 0, s), "dirac_reduce");
-  if (use_graph)
+  if (use_graph) {
+    hipGraphExec_t exec = nullptr;
+    WH_TRY(dirac_ensure_graph(e, s, &exec));  // ready for a later, longer run
+  }
+  if (graph_pays(e->graph_chunk, use_graph, n_steps))
     WH_TRY(dirac_run_graph(e, n_steps, s));
   else
     WH_TRY(dirac_iterations(e, n_steps, s));

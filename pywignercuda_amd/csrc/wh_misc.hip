@@ -19,12 +19,8 @@ namespace wh {
 // kColsumCols lanes (each load instruction of a wave covers 16 rows x 4 columns), the 64
 // group sums combined by a fixed tree in LDS.
 template <typename T>
-__global__ void __launch_bounds__(256) colsum_kernel(const Member* members, const unsigned* work) {
+__device__ __forceinline__ void colsum_tile(const Member& m, int tile, double (*red)[kColsumCols][kNCol]) {
   constexpr int CC = kColsumCols, G = 256 / CC;
-  __shared__ double red[G][CC][kNCol];
-  const unsigned w = work[blockIdx.x];
-  const Member& m = members[w >> 16];
-  const int tile = int(w & 0xffffu);
   const cplx<T>* W = reinterpret_cast<const cplx<T>*>(m.W);
   const int c = threadIdx.x % CC, g = threadIdx.x / CC;
   const int col = tile * CC + c;
@@ -57,14 +53,20 @@ __global__ void __launch_bounds__(256) colsum_kernel(const Member* members, cons
   }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) colsum_kernel(const Member* members, const unsigned* work) {
+  __shared__ double red[256 / kColsumCols][kColsumCols][kNCol];
+  const unsigned w = work[blockIdx.x];
+  colsum_tile<T>(members[w >> 16], int(w & 0xffffu), red);
+}
+
 // ------------------------------------------------------------------ reduce (one block per member)
-__global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a, int* done, int nblocks) {
+// One member's record row, pending factor and (Gross-Pitaevskii) density from its column
+// sums, by a block of 256 threads.
+__device__ __forceinline__ void reduce_member(const ReduceArgs& a, const Member& m, int step, double (*red)[256],
+                                              double* scale_sh) {
   constexpr int K = 12;
-  __shared__ double red[K][256];
-  __shared__ double scale_sh[2];
   const int tid = threadIdx.x;
-  const Member& m = a.members[blockIdx.x];
-  const int step = a.use_step_arg ? a.step_arg : *a.step;
   double acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = 0.0;
@@ -138,7 +140,15 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a, int* done, in
       m.rho[j] = (cs[0] * sr - cs[1] * si) * m.rho_factor;
     }
   }
-  if (a.step_inc && tid == 0) {
+}
+
+// ------------------------------------------------------------------ reduce (one block per member)
+__global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a, int* done, int nblocks) {
+  __shared__ double red[12][256];
+  __shared__ double scale_sh[2];
+  const int step = a.use_step_arg ? a.step_arg : *a.step;
+  reduce_member(a, a.members[blockIdx.x], step, red, scale_sh);
+  if (a.step_inc && threadIdx.x == 0) {
     // the last block to finish advances the step counter: every block has read it by then
     __threadfence();
     const int prev = atomicAdd(done, 1);
@@ -148,6 +158,38 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a, int* done, in
       __threadfence();
     }
   }
+}
+
+// ------------------------------------------------------------------ column sums + reduce
+// The column sums of every member and, in the block that finishes a member's last tile,
+// that member's reduce (one launch instead of two at the start of a run).  The time index
+// is a.step_arg (a.use_step_arg = 1: the counter is not read in this kernel); the last
+// block of member 0 sets *a.step_inc = a.next_step.  member_done: one counter per member,
+// zero between launches (the last block resets it).
+template <typename T>
+__global__ void __launch_bounds__(256) colsum_reduce_kernel(const Member* members, const unsigned* work,
+                                                            ReduceArgs a, int* member_done) {
+  __shared__ double cred[256 / kColsumCols][kColsumCols][kNCol];
+  __shared__ double red[12][256];
+  __shared__ double scale_sh[2];
+  __shared__ int last;
+  const unsigned w = work[blockIdx.x];
+  const int mi = int(w >> 16);
+  const Member& m = members[mi];
+  colsum_tile<T>(m, int(w & 0xffffu), cred);
+  __threadfence();  // this block's column sums, before the count below
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int ntiles = (m.X + kColsumCols - 1) / kColsumCols;
+    const int prev = atomicAdd(member_done + mi, 1);
+    last = prev == ntiles - 1;
+    if (last) member_done[mi] = 0;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();  // the other blocks' column sums of this member
+  reduce_member(a, m, a.step_arg, red, scale_sh);
+  if (a.step_inc && mi == 0 && threadIdx.x == 0) *a.step_inc = a.next_step;
 }
 
 // ------------------------------------------------------------------ density after K
@@ -331,6 +373,15 @@ hipError_t launch_colsum(int prec, const Member* members, const unsigned* work, 
     hipLaunchKernelGGL(colsum_kernel<double>, dim3(nwork), dim3(256), 0, s, members, work);
   else
     hipLaunchKernelGGL(colsum_kernel<float>, dim3(nwork), dim3(256), 0, s, members, work);
+  return hipGetLastError();
+}
+hipError_t launch_colsum_reduce(int prec, const Member* members, const unsigned* work, int nwork,
+                                const ReduceArgs& a, int* member_done, hipStream_t s) {
+  if (!a.use_step_arg || (a.step_inc && a.next_step < 0)) return hipErrorInvalidValue;
+  if (prec == 0)
+    hipLaunchKernelGGL(colsum_reduce_kernel<double>, dim3(nwork), dim3(256), 0, s, members, work, a, member_done);
+  else
+    hipLaunchKernelGGL(colsum_reduce_kernel<float>, dim3(nwork), dim3(256), 0, s, members, work, a, member_done);
   return hipGetLastError();
 }
 hipError_t launch_reduce(const ReduceArgs& a, int nmembers, int* done, hipStream_t s) {
