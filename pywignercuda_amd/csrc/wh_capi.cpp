@@ -37,8 +37,6 @@ int noalloc_max_x();
 hipError_t launch_xsweep_f32w(int variant, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_psweep_f32w(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_reduce(const ReduceArgs& a, int nmembers, int* done, hipStream_t s);
-hipError_t launch_colsum_reduce(int prec, const Member* members, const unsigned* work, int nwork,
-                                const ReduceArgs& a, int* member_done, hipStream_t s);
 hipError_t launch_colpart_reduce(const Member* members, int nmembers, int xradix, int xblock, const int* step,
                                  hipStream_t s);
 hipError_t launch_scale(int prec, const Member* members, const unsigned* work, int nwork, hipStream_t s);
@@ -398,7 +396,6 @@ struct wh_ensemble {
   WorkList work_x, work_p, work_pcl, work_colsum, work_scale, work_gp, work_adj, work_odm_x, work_odm_p;
   wh::SweepVariant built_x{}, built_p{}, built_pcl{};  // the variants each list was cut for
   DevBuf step, done, gcols;
-  DevBuf member_done;  // [n] tile counters of the column-sum + reduce kernel (zero between launches)
   hipStream_t capture_stream = nullptr;
   std::map<std::tuple<hipStream_t, int>, GraphEntry> graphs;  // (stream, chunk)
   int graph_chunk = 16;
@@ -683,10 +680,6 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
   WH_TRY(e->step.alloc(16));
   const int zero[4] = {0, 0, 0, 0};
   WH_TRY(e->done.upload(zero, sizeof(zero)));
-  {
-    const std::vector<int> zeros(size_t(n), 0);
-    WH_TRY(e->member_done.upload(zeros.data(), zeros.size() * sizeof(int)));
-  }
 
   WH_TRY(build_work_lists(e.get()));
   *out = e.release();
@@ -957,24 +950,13 @@ int ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_step
   const int prec = d.precision;
   const wh::Member* md = e->mem_dev.as<wh::Member>();
   // the state before the first iteration: norm, renormalisation factor, (Lie) row 0
-  {
-    // column sums and reduce in one launch: record row of time index first_step - 1 (Lie) or
-    // first_step, the pending factor, the Gross-Pitaevskii density; the counter then reads first_step
-    wh::ReduceArgs r{};
-    r.members = md;
-    r.renorm = normalize_initial;
-    r.record = d.scheme == WH_SCHEME_LIE;
-    r.rho = e->gp();
-    r.update_scale = 1;
-    r.step = e->step.as<int>();
-    r.step_inc = e->step.as<int>();
-    r.use_step_arg = 1;
-    r.step_arg = d.scheme == WH_SCHEME_LIE ? first_step - 1 : first_step;
-    r.next_step = first_step;
-    WH_LAUNCH(wh::launch_colsum_reduce(storage_code(prec), md, e->work_colsum.buf.as<unsigned>(), e->work_colsum.n,
-                                       r, e->member_done.as<int>(), s),
-              "colsum_reduce");
-  }
+  WH_LAUNCH(wh::launch_colsum(storage_code(prec), md, e->work_colsum.buf.as<unsigned>(), e->work_colsum.n, s),
+            "colsum");
+  // record row of time index first_step - 1 (Lie) or first_step; the counter then reads first_step.
+  // (One launch for both, the member's last column-sum block doing its reduce, took 30 us at
+  // 1024^2 instead of 13: its device-scope fences write back and invalidate the L2 per block.)
+  WH_TRY(reduce(e, normalize_initial, d.scheme == WH_SCHEME_LIE, e->gp(), 1, true, s, true,
+                d.scheme == WH_SCHEME_LIE ? first_step - 1 : first_step, first_step));
   const bool merged_strang = d.scheme == WH_SCHEME_STRANG && !e->gp();
   if (n_steps > 0 && merged_strang) {
     // first half step of the first iteration, with the pending factor folded in
@@ -1355,7 +1337,6 @@ int dirac_iterations(wh_dirac* e, int count, hipStream_t s) {
   return WH_SUCCESS;
 }
 
-// As run_graph: captured on the first graph-enabled run whatever its length.
 // The plan's step graph for stream s (graph_chunk iterations): captured (capturing
 // executes nothing), instantiated and uploaded on the first graph-enabled run whatever its
 // length, so that later runs replay it without capture or upload work.

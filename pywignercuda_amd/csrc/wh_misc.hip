@@ -160,38 +160,6 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a, int* done, in
   }
 }
 
-// ------------------------------------------------------------------ column sums + reduce
-// The column sums of every member and, in the block that finishes a member's last tile,
-// that member's reduce (one launch instead of two at the start of a run).  The time index
-// is a.step_arg (a.use_step_arg = 1: the counter is not read in this kernel); the last
-// block of member 0 sets *a.step_inc = a.next_step.  member_done: one counter per member,
-// zero between launches (the last block resets it).
-template <typename T>
-__global__ void __launch_bounds__(256) colsum_reduce_kernel(const Member* members, const unsigned* work,
-                                                            ReduceArgs a, int* member_done) {
-  __shared__ double cred[256 / kColsumCols][kColsumCols][kNCol];
-  __shared__ double red[12][256];
-  __shared__ double scale_sh[2];
-  __shared__ int last;
-  const unsigned w = work[blockIdx.x];
-  const int mi = int(w >> 16);
-  const Member& m = members[mi];
-  colsum_tile<T>(m, int(w & 0xffffu), cred);
-  __threadfence();  // this block's column sums, before the count below
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int ntiles = (m.X + kColsumCols - 1) / kColsumCols;
-    const int prev = atomicAdd(member_done + mi, 1);
-    last = prev == ntiles - 1;
-    if (last) member_done[mi] = 0;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();  // the other blocks' column sums of this member
-  reduce_member(a, m, a.step_arg, red, scale_sh);
-  if (a.step_inc && mi == 0 && threadIdx.x == 0) *a.step_inc = a.next_step;
-}
-
 // ------------------------------------------------------------------ density after K
 __global__ void __launch_bounds__(256) colpart_reduce_kernel(const Member* members, int xradix, int xblock,
                                                              const int* step) {
@@ -373,15 +341,6 @@ hipError_t launch_colsum(int prec, const Member* members, const unsigned* work, 
     hipLaunchKernelGGL(colsum_kernel<double>, dim3(nwork), dim3(256), 0, s, members, work);
   else
     hipLaunchKernelGGL(colsum_kernel<float>, dim3(nwork), dim3(256), 0, s, members, work);
-  return hipGetLastError();
-}
-hipError_t launch_colsum_reduce(int prec, const Member* members, const unsigned* work, int nwork,
-                                const ReduceArgs& a, int* member_done, hipStream_t s) {
-  if (!a.use_step_arg || (a.step_inc && a.next_step < 0)) return hipErrorInvalidValue;
-  if (prec == 0)
-    hipLaunchKernelGGL(colsum_reduce_kernel<double>, dim3(nwork), dim3(256), 0, s, members, work, a, member_done);
-  else
-    hipLaunchKernelGGL(colsum_reduce_kernel<float>, dim3(nwork), dim3(256), 0, s, members, work, a, member_done);
   return hipGetLastError();
 }
 hipError_t launch_reduce(const ReduceArgs& a, int nmembers, int* done, hipStream_t s) {
