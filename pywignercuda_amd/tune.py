@@ -13,6 +13,7 @@ from __future__ import annotations
 import argparse
 import json
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -141,6 +142,7 @@ def main():
     ap.add_argument("--sizes", default="128,256,512,1024,2048")
     ap.add_argument("--reps", type=int, default=50)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--out", default=None, help="write the table here instead of the package's tuning.json")
     a = ap.parse_args()
     sizes = [int(s) for s in a.sizes.split(",")]
     table = json.loads(TUNING_FILE.read_text()) if TUNING_FILE.exists() else {}
@@ -164,8 +166,9 @@ def main():
                         batch.setdefault(k + "_us", {})[str(N)] = round(v[1], 2)
                 print(f"{prec} {N}: batch {rb}", flush=True)
     table["_meta"] = {"seconds": round(time.time() - t0, 1)}
-    TUNING_FILE.write_text(json.dumps(table, indent=1, sort_keys=True))
-    print(f"wrote {TUNING_FILE}")
+    out = Path(a.out) if a.out else TUNING_FILE
+    out.write_text(json.dumps(table, indent=1, sort_keys=True))
+    print(f"wrote {out}")
 
 
 if __name__ == "__main__":
