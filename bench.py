@@ -270,7 +270,50 @@ SIDE_WORKLOADS = {
     "uneven-fp64": (_uneven_one("fp64"), "256-member uneven ensemble, FP64"),
     "dirac-fp64": (lambda st, w, d: dirac_workload("fp64", st, d), "Wigner-Dirac 4x4, 512^2 x 16 planes, FP64"),
     "dirac-fp32": (lambda st, w, d: dirac_workload("fp32", st, d), "Wigner-Dirac 4x4, 512^2 x 16 planes, FP32"),
+    "adjoint": (lambda st, w, d: adjoint_workload(st, w, d),
+                "512^2 differentiable control: forward + adjoint backward (64 Fourier parameters, checkpoints every 50)"),
 }
+
+
+def adjoint_workload(steps, warmup, dev, n_params=64, K=50):
+    """The differentiable-control configuration: 512^2, V = 0.25x^4 - x^2 - E(t)x with E(t) a
+    64-term Fourier series (N(0, 0.1) coefficients, T = 20), dt = 0.01, loss 1 - overlap with a
+    Gaussian at x = +2, checkpoints every K steps.  Times one forward + backward (torch.autograd
+    over the adjoint step) after an untimed one."""
+    import numpy as np
+    import torch
+
+    from pywignercuda_amd.adjoint import fourier_control_table, propagate
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    dt, T, X = 0.01, 20.0, 512
+    base = np.zeros(9)
+    base[4], base[2] = 0.25, -1.0
+    prop = Wigner2D(X, X, 10.0, 10.0, 1.0, 1.0, dt, potential="0.25*pow(x,4) - pow(x,2)",
+                    dpotential="pow(x,3) - 2*x", mask=False, device=dev, pot_table=np.tile(base, (steps + 2, 1)))
+    rng = np.random.default_rng(20261203)
+    params = torch.tensor(rng.normal(0.0, 0.1, n_params), dtype=torch.float64, device=dev, requires_grad=True)
+    x, p = prop.x_host[None, :], prop.p_host[:, None]
+    W0 = np.exp(-((x + 1.0) ** 2) - p ** 2) / np.pi
+    W0 = prop.to_device(W0 / (W0.sum() * prop.dX * prop.dP))
+    Wt = torch.tensor(np.exp(-((x - 2.0) ** 2) - p ** 2) / np.pi, dtype=torch.float64, device=dev)
+    c = 2 * np.pi * prop.dX * prop.dP
+
+    def grad_once():
+        L = 1.0 - c * (propagate(prop, fourier_control_table(params, steps, dt, T, base), W0, steps,
+                                 checkpoint_every=K) * Wt).sum()
+        (g,) = torch.autograd.grad(L, params)
+        return g
+
+    grad_once()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grad_once()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = X * X
+    return {"cell_steps_per_s": cells * steps / wall, "forward_backward_s": wall, "steps": steps,
+            "params": n_params, "checkpoint_every": K}
 
 
 def _cpu_model():
@@ -474,6 +517,7 @@ def main():
             out["extra"]["uneven_ensemble_256"]["fp32"]["cell_steps_per_s"] / plain64)
         out["extra"]["dirac_512^2_fp64"] = dirac_workload("fp64", 1000, dev)
         out["extra"]["dirac_512^2_fp32"] = dirac_workload("fp32", 1000, dev)
+        out["extra"]["adjoint_512^2_2000_steps"] = adjoint_workload(2000, 0, dev)
     if a.calibrate_copy:
         out["copy_calibration"] = calibrate_copy(dev)
     if not a.no_cpu:

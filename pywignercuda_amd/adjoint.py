@@ -9,8 +9,8 @@ Gradients flow to coef_table (dL/da_k(n)) and to W0.
 
 Forward: the fused run, keeping the state every `checkpoint_every` steps.
 Backward: per segment, last to first, the forward states are recomputed from the
-segment's checkpoint and the adjoint state is carried back through the
-reverse-time step with conjugated phases (wh_wigner2d_adjoint_step), which also
+segment's checkpoint (two single-sweep launches per step) and the adjoint state is carried
+back through the reverse-time step with conjugated phases (wh_wigner2d_adjoint_step), which also
 accumulates dL/da_k(n).  Memory: n_steps/K checkpoints + K states.
 
 The reference has no gradients; there is no reference counterpart to this module.
@@ -68,7 +68,10 @@ def _make_function():
                 W = ctx.ckpts[s].clone()
                 traj = []
                 for k in range(n):
-                    prop.run(W, 1, first_step=n0 + k, normalize_initial=False, use_graph=False)
+                    # one Lie step as two single sweeps (bit-identical to run(W, 1) without
+                    # renormalisation, without its observable launches)
+                    prop.kinetic(W)
+                    prop.potential(W, n0 + k, fused=False)
                     traj.append(W.clone())
                 for k in reversed(range(n)):
                     prop.adjoint_step(Lam, traj[k], n0 + k, grads[n0 + k])
