@@ -737,46 +737,71 @@ __global__ void __launch_bounds__(kDiracXBlock) dirac_podm_kernel(const DiracPla
 }
 
 // Column sums of a state without propagating it (the record of the first state).
+// Column sums of the state (no sweep): a block takes kDColsumCols consecutive columns, its
+// 256 lanes 64 row groups of kDColsumCols lanes (each load instruction covers 16 rows of the
+// tile's columns); the group sums are combined by a fixed tree in LDS.  One thread per column
+// over all rows took 440 us at 512^2 x 16 planes (two blocks); this form fills the GPU.
+constexpr int kDColsumCols = 4;
 template <typename T>
 __global__ void __launch_bounds__(256) dirac_colsum_kernel(const DiracPlan* __restrict__ dp) {
+  constexpr int CC = kDColsumCols, G = 256 / CC;
+  __shared__ double red[G][CC][kDNCol];
   const DiracPlan& d = *dp;
-  const int col = blockIdx.x * 256 + threadIdx.x;
-  if (col >= d.X) return;
+  const int c = threadIdx.x % CC, g = threadIdx.x / CC;
+  const int col = blockIdx.x * CC + c;
   const size_t plane = size_t(d.P) * size_t(d.X);
   const cplx<T>* W = reinterpret_cast<const cplx<T>*>(d.W);
   double acc[kDNCol];
+#pragma unroll
   for (int q = 0; q < kDNCol; ++q) acc[q] = 0.0;
-  for (int i = 0; i < d.P; ++i) {
-    const double p = d.dp * double(fidx(i, d.P)), ph = gmem(d.p_host)[i];
-    double trr = 0.0;
-    for (int k = 0; k < 16; ++k) {
-      const bool diag = (k % 5) == 0, alpha = k == 3 || k == 6 || k == 9 || k == 12;
-      if (!diag && !alpha) continue;
-      const cplx<T> w = W[size_t(k) * plane + size_t(i) * d.X + col];
-      const double wr = double(w.x), wi = double(w.y);
-      if (diag) {
-        const double sb = k < 8 ? 1.0 : -1.0;
-        acc[kDcTrRe] += wr;
-        acc[kDcTrIm] += wi;
-        acc[kDcPTrRe] += p * wr;
-        acc[kDcPTrIm] += p * wi;
-        acc[kDcP2TrRe] += p * p * wr;
-        acc[kDcP2TrIm] += p * p * wi;
-        acc[kDcBetaRe] += sb * wr;
-        acc[kDcBetaIm] += sb * wi;
-        trr += wr;
-      } else {
-        acc[kDcA1Re] += wr;
-        acc[kDcA1Im] += wi;
-        acc[kDcPA1Re] += p * wr;
-        acc[kDcPA1Im] += p * wi;
-        acc[kDcHA1Re] += ph * wr;
-        acc[kDcHA1Im] += ph * wi;
+  if (col < d.X) {
+    for (int i = g; i < d.P; i += G) {
+      const double p = d.dp * double(fidx(i, d.P)), ph = gmem(d.p_host)[i];
+      const size_t e = size_t(i) * d.X + col;
+      double trr = 0.0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const bool diag = (k % 5) == 0, alpha = k == 3 || k == 6 || k == 9 || k == 12;
+        if (!diag && !alpha) continue;
+        const cplx<T> w = W[size_t(k) * plane + e];
+        const double wr = double(w.x), wi = double(w.y);
+        if (diag) {
+          const double sb = k < 8 ? 1.0 : -1.0;
+          acc[kDcTrRe] += wr;
+          acc[kDcTrIm] += wi;
+          acc[kDcPTrRe] += p * wr;
+          acc[kDcPTrIm] += p * wi;
+          acc[kDcP2TrRe] += p * p * wr;
+          acc[kDcP2TrIm] += p * p * wi;
+          acc[kDcBetaRe] += sb * wr;
+          acc[kDcBetaIm] += sb * wi;
+          trr += wr;
+        } else {
+          acc[kDcA1Re] += wr;
+          acc[kDcA1Im] += wi;
+          acc[kDcPA1Re] += p * wr;
+          acc[kDcPA1Im] += p * wi;
+          acc[kDcHA1Re] += ph * wr;
+          acc[kDcHA1Im] += ph * wi;
+        }
       }
+      acc[kDcNeg] += trr < 0.0 ? trr : 0.0;
     }
-    acc[kDcNeg] += trr < 0.0 ? trr : 0.0;
   }
-  for (int q = 0; q < kDNCol; ++q) gmem(d.colsums)[size_t(col) * kDNCol + q] = acc[q];
+#pragma unroll
+  for (int q = 0; q < kDNCol; ++q) red[g][c][q] = acc[q];
+  __syncthreads();
+  for (int h = G / 2; h > 0; h >>= 1) {
+    if (g < h) {
+#pragma unroll
+      for (int q = 0; q < kDNCol; ++q) red[g][c][q] += red[g + h][c][q];
+    }
+    __syncthreads();
+  }
+  if (g == 0 && col < d.X) {
+#pragma unroll
+    for (int q = 0; q < kDNCol; ++q) gmem(d.colsums)[size_t(col) * kDNCol + q] = red[0][c][q];
+  }
 }
 
 // Observables of the state whose column sums are in d.colsums (state time index = step +
@@ -1076,7 +1101,7 @@ hipError_t launch_dirac_podm(int prec, const DiracPlan* d, int nblocks, hipStrea
 }
 
 hipError_t launch_dirac_colsum(int prec, const DiracPlan* d, int X, hipStream_t s) {
-  const int nb = (X + 255) / 256;
+  const int nb = (X + kDColsumCols - 1) / kDColsumCols;
   if (prec == 0)  // state storage only: complex128 or complex64
     hipLaunchKernelGGL(dirac_colsum_kernel<double>, dim3(nb), dim3(256), 0, s, d);
   else
