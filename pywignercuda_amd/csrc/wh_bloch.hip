@@ -141,18 +141,23 @@ __global__ void __launch_bounds__(kBlock) bloch_xsweep_kernel(BlochArgs a) {
 // ------------------------------------------------------------------ column sums of a state
 // 32 columns per block, 8 row groups.  gp_density: also rho(x) = dP Re sum_p W
 // (ProbabilityX, GP.py:1743-1748).  colsums == nullptr: the density only.
+// 4-column tiles over 64 row groups of 4 lanes (one load instruction covers 16 rows of the
+// tile's columns), group sums combined by a fixed tree in LDS
+constexpr int kBColsumCols = 4;
 __global__ void __launch_bounds__(kBlock) bloch_colsum_kernel(BlochArgs a, const void* state, double* colsums,
                                                              int density) {
-  __shared__ double red[8][32][kBCol];
-  const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
-  const int col = blockIdx.x * 32 + c;
+  constexpr int CC = kBColsumCols, G = kBlock / CC;
+  __shared__ double red[G][CC][kBCol];
+  const int c = threadIdx.x % CC, g = threadIdx.x / CC;
+  const int col = blockIdx.x * CC + c;
   const int X = a.X;
   const cplx<double>* W = reinterpret_cast<const cplx<double>*>(state);
   double acc[kBCol];
 #pragma unroll
   for (int k = 0; k < kBCol; ++k) acc[k] = 0.0;
   if (col < X) {
-    for (int i = g; i < a.P; i += 8) {
+#pragma unroll 4
+    for (int i = g; i < a.P; i += G) {
       const cplx<double> v = W[size_t(i) * X + col];
       const double p = a.p_host[i];
       acc[0] += v.x;
@@ -167,18 +172,19 @@ __global__ void __launch_bounds__(kBlock) bloch_colsum_kernel(BlochArgs a, const
 #pragma unroll
   for (int k = 0; k < kBCol; ++k) red[g][c][k] = acc[k];
   __syncthreads();
-  if (g == 0 && col < X) {
-    double s[kBCol];
+  for (int h = G / 2; h > 0; h >>= 1) {
+    if (g < h) {
 #pragma unroll
-    for (int k = 0; k < kBCol; ++k) {
-      s[k] = 0.0;
-      for (int q = 0; q < 8; ++q) s[k] += red[q][c][k];
+      for (int k = 0; k < kBCol; ++k) red[g][c][k] += red[g + h][c][k];
     }
+    __syncthreads();
+  }
+  if (g == 0 && col < X) {
     if (colsums) {
 #pragma unroll
-      for (int k = 0; k < kBCol; ++k) colsums[size_t(col) * kBCol + k] = s[k];
+      for (int k = 0; k < kBCol; ++k) colsums[size_t(col) * kBCol + k] = red[0][c][k];
     }
-    if (density) a.rho[col] = a.dp_host * s[0];
+    if (density) a.rho[col] = a.dp_host * red[0][c][0];
   }
 }
 
@@ -423,7 +429,8 @@ hipError_t launch_bloch_gshift(const BlochArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t launch_bloch_colsum(const BlochArgs& a, const void* state, double* colsums, int density, hipStream_t s) {
-  hipLaunchKernelGGL(bloch_colsum_kernel, dim3((a.X + 31) / 32), dim3(kBlock), 0, s, a, state, colsums, density);
+  hipLaunchKernelGGL(bloch_colsum_kernel, dim3((a.X + kBColsumCols - 1) / kBColsumCols), dim3(kBlock), 0, s, a, state,
+                     colsums, density);
   return hipGetLastError();
 }
 hipError_t launch_bloch_finish(const BlochArgs& a, int mode, hipStream_t s) {
