@@ -265,12 +265,18 @@ __device__ __forceinline__ void dirac_store_colsums(const DiracPlan& d, int tile
   }
 }
 
-// Two blocks per column tile: half 0 takes the mass-phase classes 0 and 3 (planes 0, 1, 4, 5,
-// 10, 11, 14, 15: every diagonal plane, so the trace, beta and negativity sums), half 1 the
-// classes 1 and 2 (planes 2, 3, 6, 7, 8, 9, 12, 13: every alpha_1 plane); each writes its own
-// column sums.  One block per tile left half of the CUs idle at 512^2 (128 tiles).
+// Four blocks per column tile, four planes each, two of each of two mass-phase classes
+// (class 0: planes 0 1 4 5, class 1: 2 3 6 7, class 2: 8 9 12 13, class 3: 10 11 14 15):
+//   quarter 0: 0 5 | 10 15 (every diagonal plane: the trace, beta and negativity sums)
+//   quarter 1: 3 6 | 9 12  (every alpha_1 plane: the alpha sums)
+//   quarter 2: 1 4 | 11 14,  quarter 3: 2 7 | 8 13 (no column sums)
+// Each quarter writes only its own column sums.  With 16 planes per tile (one block) the 512^2
+// sweep had one wave per SIMD on half of the CUs; two class pairs per tile filled the CUs, four
+// quarters give two waves per SIMD (at two multiplier evaluations per block instead of one).
+__constant__ int kDiracQuarter[4][2][2] = {{{0, 5}, {10, 15}}, {{3, 6}, {9, 12}}, {{1, 4}, {11, 14}}, {{2, 7}, {8, 13}}};
+__constant__ int kDiracQuarterClass[4][2] = {{0, 3}, {1, 2}, {0, 3}, {1, 2}};
 template <typename S, typename T, int N, int R, int BLOCK, bool CL>
-__device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, int half, cplx<T>* lds) {
+__device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, int quarter, cplx<T>* lds) {
   constexpr int TT = N / R;
   constexpr int C = BLOCK / TT;  // columns per tile
   const int tid = threadIdx.x;
@@ -327,7 +333,7 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
   // multiplier r e^{-i (ph + m)}: m = F- - F+ (lo, lo), F- + F+ (lo, hi), -(F- + F+) (hi, lo),
   // -(F- - F+) (hi, hi), F -/+ = Mass(x -/+ theta/2) c^2 dt; a constant mass gives 0, 2F, -2F, 0.
   for (int ci = 0; ci < 2; ++ci) {
-    const int cls = half == 0 ? 3 * ci : 1 + ci;
+    const int cls = kDiracQuarterClass[quarter][ci];
     cplx<double> M[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -343,8 +349,8 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
       const double r = (d.theta_decay ? gmem(d.theta_decay)[i] : 1.0) * mask;
       M[j] = expmi(r, ph[j] + m);
     }
-    for (int q = 0; q < 4; ++q) {
-      const int k = ((cls >> 1) << 3) | ((cls & 1) << 1) | ((q >> 1) << 2) | (q & 1);
+    for (int q = 0; q < 2; ++q) {
+      const int k = kDiracQuarter[quarter][ci][q];
       const gptr<cplx<S>> W = W0 + size_t(k) * plane;
       cplx<T> v[R];
 #pragma unroll
@@ -366,7 +372,7 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
 #pragma unroll
   for (int j = 0; j < R; ++j) acc[kDcNeg] += trre[j] < 0.0 ? trre[j] : 0.0;
 
-  dirac_store_colsums<BLOCK, C>(d, tile, acc, lds, half);
+  if (quarter < 2) dirac_store_colsums<BLOCK, C>(d, tile, acc, lds, quarter);
 }
 
 
@@ -706,7 +712,7 @@ __global__ void __launch_bounds__(kDiracPBlock) dirac_psweep_kernel(const DiracP
   __shared__ cplx<T> lds[dirac_lds_elems<kDiracPBlock>()];
   const DiracPlan& d = *dp;
   const int u = dirac_col_tile();
-  dirac_psweep_body<S, T, N, dirac_rp<N>(), kDiracPBlock, CL>(d, u >> 1, u & 1, lds);
+  dirac_psweep_body<S, T, N, dirac_rp<N>(), kDiracPBlock, CL>(d, u >> 2, u & 3, lds);
 }
 
 template <typename S, typename T>
@@ -1003,7 +1009,7 @@ int dirac_p_blocks(int X, int P) {
 #define BODY(n) cols = psweep_cols<n>()
   WH_DIRAC_DISPATCH(P, BODY)
 #undef BODY
-  return cols > 0 && X % cols == 0 ? 2 * (X / cols) : -1;  // two blocks (halves) per tile
+  return cols > 0 && X % cols == 0 ? 4 * (X / cols) : -1;  // four blocks (quarters) per tile
 }
 
 template <typename S, typename T, bool FILTER>
