@@ -52,6 +52,38 @@ extern "C" {
 
 #define WH_MAX_DEG 8 /* degree bound of the polynomial V(t,x) and K(p) */
 #define WH_POT_TABLE (-1) /* pot_deg value: V given as a [P][X] Bopp-difference table */
+#define WH_POT_EXPR (-2)  /* pot_deg value: V(t, x) given as an expression program (below) */
+
+/* Expression programs (pot_deg = WH_POT_EXPR): V(t, x) as WH_EXPR_MAX or fewer (op, value)
+   pairs of doubles for a stack machine, evaluated per cell by the p sweep at time
+   t = step * pot_t_step -- the in-kernel counterpart of the reference splicing its Potential
+   string into expPotential (GPU_Wigner2D_FFT.py:88-94, 869-873).  Ops: push value / x / t,
+   negate, the binary + - * / pow (second operand on top), and the functions exp sin cos tan
+   sqrt fabs tanh sinh cosh log atan Heaviside (0 below 0, else 1, as the reference's device
+   Heaviside).  The stack holds at most WH_EXPR_STACK values. */
+#define WH_EXPR_MAX 64
+#define WH_EXPR_STACK 16
+#define WH_EXPR_CONST 0
+#define WH_EXPR_X 1
+#define WH_EXPR_T 2
+#define WH_EXPR_NEG 3
+#define WH_EXPR_ADD 4
+#define WH_EXPR_SUB 5
+#define WH_EXPR_MUL 6
+#define WH_EXPR_DIV 7
+#define WH_EXPR_POW 8
+#define WH_EXPR_EXP 9
+#define WH_EXPR_SIN 10
+#define WH_EXPR_COS 11
+#define WH_EXPR_TAN 12
+#define WH_EXPR_SQRT 13
+#define WH_EXPR_FABS 14
+#define WH_EXPR_TANH 15
+#define WH_EXPR_SINH 16
+#define WH_EXPR_COSH 17
+#define WH_EXPR_LOG 18
+#define WH_EXPR_ATAN 19
+#define WH_EXPR_HEAVISIDE 20
 #define WH_NOBS 16   /* doubles per observable record */
 
 /* observable record columns (one row per recorded step) */
@@ -149,7 +181,9 @@ typedef struct {
   /* potential V(t,x) = sum_n pot_coef[row*(WH_MAX_DEG+1)+n] x^n, row = time index
      (clamped to pot_rows-1); pot_rows = 1 for a time-independent V.
      pot_deg = WH_POT_TABLE: V is not a polynomial; the p sweep reads the Bopp
-     difference from pot_dv instead (pot_coef is then unused) */
+     difference from pot_dv instead (pot_coef is then unused).
+     pot_deg = WH_POT_EXPR: V(t, x) is the program pot_expr (pot_coef unused); the p
+     sweeps of such a plan run p variant 0 */
   int pot_deg;
   int pot_rows;
   const double* pot_coef;
@@ -192,6 +226,11 @@ typedef struct {
      (odm_weight1, sign 1), then with (odm_weight2, sign -1) on the result (:2173-2181) */
   int odm_channels;
   double odm_weight1, odm_weight2;
+  /* pot_deg = WH_POT_EXPR: the program of V(t, x) (pot_expr_len pairs, host memory) and the
+     time per step index (the host dt, as the reference's t = tIndex*dt) */
+  const double* pot_expr;
+  int pot_expr_len;
+  double pot_t_step;
 } wh_wigner2d_desc;
 
 typedef struct wh_wigner2d wh_wigner2d;
@@ -291,6 +330,9 @@ typedef struct {
   const double* pot_dv; /* [P][X] Bopp difference of V for pot_deg = WH_POT_TABLE */
   const double *odm_g, *odm_x, *odm_th; /* ODM tables, as in wh_wigner2d_desc */
   double odm_sign;
+  const double* pot_expr; /* pot_deg = WH_POT_EXPR: as in wh_wigner2d_desc */
+  int pot_expr_len;
+  double pot_t_step;
 } wh_member_desc;
 
 typedef struct wh_ensemble wh_ensemble;
@@ -307,7 +349,9 @@ int wh_ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_s
 /*
  * Launch variants (index into the kernel variant tables; -1 keeps the current
  * choice): x sweep, p sweep, p sweep with Caldeira-Leggett.  Rejected with
- * WH_INVALID_VALUE if a member's grid does not fit the variant.
+ * WH_INVALID_VALUE if a member's grid does not fit the variant.  A plan with a
+ * member whose V is an expression program (WH_POT_EXPR) keeps p variant 0: its p
+ * choices are ignored.
  */
 int wh_ensemble_set_variants(wh_ensemble* e, int xvar, int pvar, int pclvar);
 /* Number of variants of each table and the (radix, lanes) of variant i.  The last

@@ -18,6 +18,8 @@ WH_FORWARD, WH_INVERSE = -1, 1
 WH_SCHEME_LIE, WH_SCHEME_STRANG = 0, 1
 WH_MAX_DEG = 8
 WH_POT_TABLE = -1
+WH_POT_EXPR = -2
+WH_EXPR_MAX, WH_EXPR_STACK = 64, 16
 WH_NOBS = 16
 WH_NOALLOC_MAX_X = 2048
 OBS_COLUMNS = ["X", "X2", "P", "P2", "XP", "dV", "PdV", "XdV", "H", "neg", "norm",
@@ -112,6 +114,7 @@ class Wigner2DDesc(ctypes.Structure):
         ("dxdp", c_double), ("inv_2m", c_double), ("dx_host", c_double),
         ("pot_dv", _DP), ("odm_g", _DP), ("odm_x", _DP), ("odm_th", _DP), ("odm_sign", c_double),
         ("odm_channels", c_int), ("odm_weight1", c_double), ("odm_weight2", c_double),
+        ("pot_expr", _DP), ("pot_expr_len", c_int), ("pot_t_step", c_double),
     ]
 
 
@@ -139,6 +142,7 @@ class MemberDesc(ctypes.Structure):
         ("x_host", _DP), ("p_host", _DP), ("dv_host", _DP), ("v0_host", _DP),
         ("dxdp", c_double), ("inv_2m", c_double), ("dx_host", c_double), ("gp_rho_factor", c_double),
         ("pot_dv", _DP), ("odm_g", _DP), ("odm_x", _DP), ("odm_th", _DP), ("odm_sign", c_double),
+        ("pot_expr", _DP), ("pot_expr_len", c_int), ("pot_t_step", c_double),
     ]
 
 

@@ -28,14 +28,14 @@
 namespace wh {
 hipError_t launch_xsweep_f64(int variant, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_xsweep_f32(int variant, const SweepArgs& a, int nwork, hipStream_t s);
-hipError_t launch_psweep_f64(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
-hipError_t launch_psweep_f32(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_psweep_f64(int variant, bool cl, bool expr, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_psweep_f32(int variant, bool cl, bool expr, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_gpshift(const Member* members, const unsigned* work, int nwork, hipStream_t s);
 hipError_t launch_colsum(int prec, const Member* members, const unsigned* work, int nwork, hipStream_t s);
 hipError_t launch_observe_noalloc(int prec, Member m, hipStream_t s);
 int noalloc_max_x();
 hipError_t launch_xsweep_f32w(int variant, const SweepArgs& a, int nwork, hipStream_t s);
-hipError_t launch_psweep_f32w(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_psweep_f32w(int variant, bool cl, bool expr, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_reduce(const ReduceArgs& a, int nmembers, int* done, hipStream_t s);
 hipError_t launch_colpart_reduce(const Member* members, int nmembers, int xradix, int xblock, const int* step,
                                  hipStream_t s);
@@ -362,7 +362,7 @@ namespace {
 struct MemberBufs {
   DevBuf pot, theta_decay, lambda_decay, x_mask, sigma_p, theta_host, lambda_host;
   DevBuf x_host, p_host, dv_host, v0_host, colsums, tilepart, scale, rho, G, colpart, pot_dv;
-  DevBuf odm_g, odm_x, odm_th, odm_s;
+  DevBuf odm_g, odm_x, odm_th, odm_s, pot_expr;
 };
 
 struct WorkList {
@@ -389,6 +389,7 @@ struct wh_ensemble {
   DevBuf mem_dev;
   std::map<int, std::unique_ptr<TwSet>> tw;  // by size
   int xvar = 0, pvar = 0, pclvar = 0;
+  bool expr = false;  // a member's V is an expression program: p sweeps run kExprPVariant
   // p work lists: one for the plain p variant and one for the Caldeira-Leggett p variant
   // (a tile spans block * R / P columns, which differs between the two variant tables;
   // a CL plan also runs plain p sweeps: the merged Strang first half step, Strang + GP
@@ -420,6 +421,28 @@ struct wh_wigner2d {
 
 namespace {
 
+// An expression program: length, op codes and the stack depth it reaches (the device
+// evaluator trusts all three).
+int check_expr(const double* code, int n, int member) {
+  if (!code || n < 1 || n > WH_EXPR_MAX)
+    return fail(WH_INVALID_DESCRIPTOR, "member %d: expression program %p of %d ops (1..%d)", member,
+                (const void*)code, n, WH_EXPR_MAX);
+  int depth = 0;
+  for (int i = 0; i < n; ++i) {
+    const double opd = code[2 * i];
+    // range first (NaN fails it), then integrality
+    if (!(opd >= WH_EXPR_CONST && opd <= WH_EXPR_HEAVISIDE) || opd != double(int(opd)))
+      return fail(WH_INVALID_DESCRIPTOR, "member %d: expression op %g at %d", member, opd, i);
+    const int op = int(opd);
+    if (op <= WH_EXPR_T) ++depth;
+    else if (op >= WH_EXPR_ADD && op <= WH_EXPR_POW) --depth;
+    if (depth < 1 || depth > WH_EXPR_STACK)
+      return fail(WH_INVALID_DESCRIPTOR, "member %d: expression stack depth %d at op %d", member, depth, i);
+  }
+  if (depth != 1) return fail(WH_INVALID_DESCRIPTOR, "member %d: expression leaves %d values", member, depth);
+  return WH_SUCCESS;
+}
+
 int validate_member(const wh_ensemble_desc& d, const wh_member_desc& m, int i) {
   if (m.struct_size != (int)sizeof(wh_member_desc))
     return fail(WH_INVALID_DESCRIPTOR, "member %d: struct_size %d != %d", i, m.struct_size, (int)sizeof(wh_member_desc));
@@ -428,6 +451,8 @@ int validate_member(const wh_ensemble_desc& d, const wh_member_desc& m, int i) {
                 m.p_dim, m.x_dim);
   if (m.pot_deg == WH_POT_TABLE) {
     if (!m.pot_dv) return fail(WH_INVALID_DESCRIPTOR, "member %d: pot_deg WH_POT_TABLE needs pot_dv", i);
+  } else if (m.pot_deg == WH_POT_EXPR) {
+    WH_TRY(check_expr(m.pot_expr, m.pot_expr_len, i));
   } else if (m.pot_deg < 0 || m.pot_deg > WH_MAX_DEG || m.pot_rows < 1 || !m.pot_coef) {
     return fail(WH_INVALID_DESCRIPTOR, "member %d: potential deg %d (max %d), rows %d, coef %p", i, m.pot_deg,
                 WH_MAX_DEG, m.pot_rows, (const void*)m.pot_coef);
@@ -576,10 +601,11 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
       if (!e->tw.count(N)) e->tw[N] = std::make_unique<TwSet>();
     WH_TRY(make_twiddles(*e->tw[md.x_dim], md.x_dim, prec, gp));
     WH_TRY(make_twiddles(*e->tw[md.p_dim], md.p_dim, prec, false));
-    if (md.pot_deg == WH_POT_TABLE) {
+    if (md.pot_deg == WH_POT_TABLE || md.pot_deg == WH_POT_EXPR) {
       const double zero_row[WH_MAX_DEG + 1] = {};
       WH_TRY(b->pot.upload(zero_row, sizeof(zero_row)));
-      WH_TRY(b->pot_dv.upload(md.pot_dv, X * P * sizeof(double)));
+      if (md.pot_deg == WH_POT_TABLE) WH_TRY(b->pot_dv.upload(md.pot_dv, X * P * sizeof(double)));
+      else WH_TRY(b->pot_expr.upload(md.pot_expr, size_t(md.pot_expr_len) * 2 * sizeof(double)));
     } else {
       WH_TRY(b->pot.upload(md.pot_coef, size_t(md.pot_rows) * (WH_MAX_DEG + 1) * sizeof(double)));
     }
@@ -635,7 +661,10 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
     m.twX64 = tx.hi64.p;
     m.X = md.x_dim;
     m.P = md.p_dim;
-    m.pot_rows = md.pot_deg == WH_POT_TABLE ? 1 : md.pot_rows;
+    m.pot_rows = md.pot_deg < 0 ? 1 : md.pot_rows;
+    m.pot_expr = b->pot_expr.as<double>();
+    m.pot_expr_len = md.pot_deg == WH_POT_EXPR ? md.pot_expr_len : 0;
+    m.pot_t_step = md.pot_t_step;
     m.pot_deg = md.pot_deg;
     m.dx = md.dx;
     m.dp = md.dp;
@@ -665,6 +694,13 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
   WH_TRY(pick(wh::kXVariants, wh::kNumX, true, e->xvar));
   WH_TRY(pick(wh::kPVariants, wh::kNumP, false, e->pvar));
   WH_TRY(pick(wh::kPCLVariants, wh::kNumPCL, false, e->pclvar, true));
+  for (int i = 0; i < n; ++i) e->expr = e->expr || e->mem[i].pot_deg == WH_POT_EXPR;
+  if (e->expr) {
+    if (!fits_all(e.get(), wh::kPVariants[wh::kExprPVariant], false) ||
+        !fits_all(e.get(), wh::kPCLVariants[wh::kExprPVariant], false, true))
+      return fail(WH_INVALID_SIZE, "a grid does not fit the p variant of expression-program potentials");
+    e->pvar = e->pclvar = wh::kExprPVariant;
+  }
   if (gp) {
     const wh::SweepVariant g{16, 256, 0};
     for (int i = 0; i < n; ++i) {
@@ -754,9 +790,9 @@ int psweep(wh_ensemble* e, int flags, double frac, int step_add, hipStream_t s) 
   if (!same_variant(launched, cl ? e->built_pcl : e->built_p))
     return fail(WH_INTERNAL_ERROR, "p work list cut for another variant");
   a.work = wl.buf.as<unsigned>();
-  WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_psweep_f64(v, cl, a, wl.n, s)
-            : e->d.precision == WH_C64_WIDE ? wh::launch_psweep_f32w(v, cl, a, wl.n, s)
-                                      : wh::launch_psweep_f32(v, cl, a, wl.n, s),
+  WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_psweep_f64(v, cl, e->expr, a, wl.n, s)
+            : e->d.precision == WH_C64_WIDE ? wh::launch_psweep_f32w(v, cl, e->expr, a, wl.n, s)
+                                      : wh::launch_psweep_f32(v, cl, e->expr, a, wl.n, s),
             "psweep");
   return WH_SUCCESS;
 }
@@ -1022,6 +1058,7 @@ extern "C" int wh_ensemble_set_variants(wh_ensemble* e, int xvar, int pvar, int 
   if (e->gp() && xvar >= 0 && xvar != e->xvar)
     return fail(WH_INVALID_VALUE, "the x variant of a Gross-Pitaevskii plan is fixed at creation");
   if (xvar >= 0) e->xvar = xvar;
+  if (e->expr) pvar = pclvar = -1;  // expression-program plans keep their p variant
   if (pvar >= 0) e->pvar = pvar;
   if (pclvar >= 0) e->pclvar = pclvar;
   return build_work_lists(e);
@@ -1063,6 +1100,9 @@ extern "C" int wh_wigner2d_create(const wh_wigner2d_desc* d, wh_wigner2d** out) 
   m.pot_rows = d->pot_rows;
   m.pot_coef = d->pot_coef;
   m.pot_dv = d->pot_dv;
+  m.pot_expr = d->pot_expr;
+  m.pot_expr_len = d->pot_expr_len;
+  m.pot_t_step = d->pot_t_step;
   m.odm_g = d->odm_g;
   m.odm_x = d->odm_x;
   m.odm_th = d->odm_th;
@@ -1143,8 +1183,8 @@ extern "C" int wh_ensemble_set_potential(wh_ensemble* e, int member, const doubl
   hipStream_t s = static_cast<hipStream_t>(stream);
   wh::Member& m = e->mem[member];
   MemberBufs& b = *e->bufs[member];
-  if (m.pot_deg == WH_POT_TABLE)
-    return fail(WH_INVALID_VALUE, "wh_ensemble_set_potential: member %d has a tabulated (non-polynomial) V", member);
+  if (m.pot_deg == WH_POT_TABLE || m.pot_deg == WH_POT_EXPR)
+    return fail(WH_INVALID_VALUE, "wh_ensemble_set_potential: member %d has a non-polynomial V", member);
   const size_t bytes = size_t(rows) * (WH_MAX_DEG + 1) * sizeof(double);
   bool changed = false;
   if (b.pot.bytes != bytes) {
@@ -1169,7 +1209,7 @@ extern "C" int wh_wigner2d_adjoint_step(wh_wigner2d* p, void* Lambda, const void
   wh_ensemble* e = p->e;
   if (e->d.scheme != WH_SCHEME_LIE || e->cl() || e->odm() || e->gp())
     return fail(WH_INVALID_VALUE, "the adjoint step covers the Lie scheme without damping or Gross-Pitaevskii");
-  if (e->mem[0].pot_deg == WH_POT_TABLE)
+  if (e->mem[0].pot_deg < 0)
     return fail(WH_INVALID_VALUE, "the adjoint step needs V as polynomial coefficients (gradients are per coefficient)");
   if (e->d.precision == WH_C64_WIDE)
     return fail(WH_INVALID_TYPE, "the adjoint step runs on WH_C128 or WH_C64 states");

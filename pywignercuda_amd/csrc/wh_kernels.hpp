@@ -11,6 +11,9 @@
 namespace wh {
 
 constexpr int kMaxDeg = 8;  // polynomial degree of V(t,x) and K(p)
+constexpr int kPotTable = -1;  // pot_deg: V given as a [P][X] Bopp-difference table
+constexpr int kPotExpr = -2;   // pot_deg: V(t, x) given as an expression program
+constexpr int kExprMax = 64, kExprStack = 16;
 constexpr int kNCol = 5;    // column sums: Re S0, Im S0, sum p W, sum p^2 W, sum min(W,0)
 constexpr int kNObs = 16;   // per-step observable record
 constexpr int kColsumCols = 4;     // columns per column-sum tile (64 row groups of 4 lanes)
@@ -83,6 +86,10 @@ struct Member {
   // adjoint step (single-grid plans): forward state after the step, gradient column sums
   const void* W2;              // [P][X] W_{n+1}
   double* gcols;               // [X][kMaxDeg+1]
+  // V(t, x) as an expression program (pot_deg = kPotExpr): (op, value) pairs, t = step * pot_t_step
+  const double* pot_expr;
+  int pot_expr_len, pad3;
+  double pot_t_step;
   int obs_base, pad0, pad1, pad2;
 };
 

@@ -29,6 +29,45 @@
 namespace wh {
 
 // ------------------------------------------------------------------ helpers
+// V(t, x) of an expression program (the op codes of include/wignerhip.h, WH_EXPR_*): a
+// stack machine over (op, value) pairs; the host checks the program (length, stack depth)
+// when the plan is created.  Device math functions of the HIP runtime.
+__device__ __noinline__ double eval_expr(const double* __restrict__ code, int n, double x, double t) {
+  double st[kExprStack];
+  int sp = 0;
+  for (int i = 0; i < n; ++i) {
+    const int op = int(code[2 * i]);
+    const double v = code[2 * i + 1];
+    if (op <= 2) {
+      st[sp++] = op == 0 ? v : op == 1 ? x : t;
+    } else if (op <= 3) {
+      st[sp - 1] = -st[sp - 1];
+    } else if (op <= 8) {
+      const double b = st[--sp], a = st[sp - 1];
+      st[sp - 1] = op == 4 ? a + b : op == 5 ? a - b : op == 6 ? a * b : op == 7 ? a / b : pow(a, b);
+    } else {
+      const double a = st[sp - 1];
+      double r;
+      switch (op) {
+        case 9: r = exp(a); break;
+        case 10: r = sin(a); break;
+        case 11: r = cos(a); break;
+        case 12: r = tan(a); break;
+        case 13: r = sqrt(a); break;
+        case 14: r = fabs(a); break;
+        case 15: r = tanh(a); break;
+        case 16: r = sinh(a); break;
+        case 17: r = cosh(a); break;
+        case 18: r = log(a); break;
+        case 19: r = atan(a); break;
+        default: r = a < 0.0 ? 0.0 : 1.0; break;  // Heaviside
+      }
+      st[sp - 1] = r;
+    }
+  }
+  return st[0];
+}
+
 __device__ __forceinline__ double poly(const double* __restrict__ c, int deg, double x) {
   double r = c[deg];
   for (int n = deg - 1; n >= 0; --n) r = fma(r, x, c[n]);
@@ -417,7 +456,11 @@ struct PSweep {
   using Sync = std::conditional_t<kWave, WaveSync, BlockSync>;
 
   // potential step multiplier of time index `step` (expPotential; GP phase optional),
-  // times the real factor `scale` (the 1/N of the inverse transform that follows)
+  // times the real factor `scale` (the 1/N of the inverse transform that follows).
+  // EXPR: the kernel also evaluates expression programs (pot_deg = kPotExpr); the call to
+  // the evaluator makes the compiler save the live state around it, so only the kernels
+  // launched for plans with such a V carry it
+  template <bool EXPR>
   __device__ __forceinline__ static void potential(cplx<T> (&v)[R], const SweepArgs& a,
                                                    const Member& m, int t_in, int col_in, int step,
                                                    double scale) {
@@ -465,8 +508,16 @@ struct PSweep {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         const int i = t + j * TT;
-        double phi = m.pot_deg >= 0 ? bopp(hd * double(fidx(i, N)))
-                                    : a.vphase * gmem(m.pot_dv)[size_t(i) * m.X + col];
+        double phi;
+        if (m.pot_deg >= 0) {
+          phi = bopp(hd * double(fidx(i, N)));
+        } else if (EXPR && m.pot_deg == kPotExpr) {
+          const double u = hd * double(fidx(i, N)), tt = double(step) * m.pot_t_step;
+          phi = a.vphase * (eval_expr(m.pot_expr, m.pot_expr_len, x - u, tt) -
+                            eval_expr(m.pot_expr, m.pot_expr_len, x + u, tt));
+        } else {
+          phi = a.vphase * gmem(m.pot_dv)[size_t(i) * m.X + col];
+        }
         if (gp) phi -= a.gp_phase * gmem(m.G)[size_t(i) * m.X + col];
         v[j] = apply(v[j], expmi(mask, phi));
       }
@@ -500,7 +551,7 @@ constexpr int kPTailBytes = 64 * 12 * 8;
 // teams the first exchange of the first transform moves each column to one team of
 // TT consecutive lanes (column cc = tid / TT, team index tc = tid % TT) and the store
 // moves it back through LDS; otherwise the load layout is kept throughout.
-template <typename S, typename T, int N, int R, int BLOCK, bool CL, bool TP, typename E>
+template <typename S, typename T, int N, int R, int BLOCK, bool CL, bool TP, bool EXPR, typename E>
 __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m, int tile,
                                             cplx<E>* lds) {
   using PS = PSweep<T, N, R, BLOCK, E>;
@@ -544,7 +595,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
   } else {
     team_fft<T, N, R, -1, View, Sync>(v, tc, view_c, twsrc);
   }
-  PS::potential(v, a, m, tc, col, step, 1.0 / N);
+  PS::template potential<EXPR>(v, a, m, tc, col, step, 1.0 / N);
   team_fft<T, N, R, +1, View, Sync>(v, tc, view_c, twsrc);
 
   if constexpr (CL) {
@@ -641,7 +692,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
 
   if (a.flags & kMerge) {
     team_fft<T, N, R, -1, View, Sync>(v, tc, view_c, twsrc);
-    PS::potential(v, a, m, tc, col, step + 1, 1.0 / N);
+    PS::template potential<EXPR>(v, a, m, tc, col, step + 1, 1.0 / N);
     team_fft<T, N, R, +1, View, Sync>(v, tc, view_c, twsrc);
   }
 
@@ -723,20 +774,20 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TP ?
 #undef BODY
 }
 
-template <typename S, typename T, int R, int BLOCK, bool CL, bool TP>
+template <typename S, typename T, int R, int BLOCK, bool CL, bool TP, bool EXPR>
 __device__ __forceinline__ void psweep_entry(const SweepArgs& a) {
   using E = LdsElem<S, T, TP>;
   __shared__ cplx<E> lds[lds_elems<BLOCK, R>() + kPTailBytes / sizeof(cplx<E>)];
   const unsigned w = a.work[blockIdx.x];
   const Member& m = a.members[w >> 16];
   const int tile = int(w & 0xffffu);
-#define BODY(n) psweep_body<S, T, n, R, BLOCK, CL, TP, E>(a, m, tile, lds)
+#define BODY(n) psweep_body<S, T, n, R, BLOCK, CL, TP, EXPR, E>(a, m, tile, lds)
   WH_DISPATCH_TO(m.P, BODY, TP && !CL ? kTpMaxN : 2048)
 #undef BODY
 }
-template <typename S, typename T, int R, int BLOCK, bool CL, bool TP>
+template <typename S, typename T, int R, int BLOCK, bool CL, bool TP, bool EXPR = false>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(CL ? 1 : TP ? 3 : 2))) psweep_kernel(SweepArgs a) {
-  psweep_entry<S, T, R, BLOCK, CL, TP>(a);
+  psweep_entry<S, T, R, BLOCK, CL, TP, EXPR>(a);
 }
 
 template <int R, int BLOCK>

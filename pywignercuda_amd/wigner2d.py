@@ -20,13 +20,14 @@ from pathlib import Path
 import numpy as np
 
 from . import _lib
-from ._lib import (WH_C64, WH_C64_WIDE, WH_C128, WH_MAX_DEG, WH_NOBS, WH_POT_TABLE, WH_SCHEME_LIE,
+from ._lib import (WH_C64, WH_C64_WIDE, WH_C128, WH_MAX_DEG, WH_NOBS, WH_POT_EXPR, WH_POT_TABLE, WH_SCHEME_LIE,
                    WH_SCHEME_STRANG)
 
 # state precision names: complex128; complex64 state with FP64 arithmetic inside the sweeps
 # (one FP32 rounding of the state per sweep); complex64 state with FP32 arithmetic
 PRECISIONS = {"fp64": WH_C128, "fp32": WH_C64_WIDE, "fp32fast": WH_C64}
-from .physics import NonPolynomialPhysics, coefficient_table, depends_on_time, heaviside_host, host_eval
+from .physics import (NonPolynomialPhysics, coefficient_table, compile_expression, depends_on_time, heaviside_host,
+                      host_eval)
 
 TUNING_FILE = Path(__file__).resolve().parent / "tuning.json"
 
@@ -107,6 +108,8 @@ class _Tables:
             return (1.0 - np.exp(-((kx - xm) ** 2) / w2)) * (1.0 - np.exp(-((kx + xm) ** 2) / w2))
 
         self.pot_dv = None
+        self.pot_expr = None
+        self.pot_t_step = float(dt)
         self.x_mask = None
         if mask:
             self.x_mask = edge(10.0)  # expPotential, GPU_Wigner2D_FFT.py:121-123
@@ -126,17 +129,24 @@ class _Tables:
                 deg, tab = coefficient_table(potential, "x", times)
             except NonPolynomialPhysics:
                 if self.time_dependent:
-                    raise NonPolynomialPhysics(
-                        f"{potential!r}: a time-dependent V must be a polynomial in x (degree <= {WH_MAX_DEG})")
-                # arbitrary time-independent V: the Bopp difference V(x - hbar theta/2) - V(x + hbar theta/2)
-                # at kernel coordinates, as expPotential evaluates it per cell (FFT.py:88-94)
-                u = 0.5 * self.k_hbar * ktheta[:, None]
-                vm = host_eval(potential, t=0.0, x=kx[None, :] - u)
-                vp = host_eval(potential, t=0.0, x=kx[None, :] + u)
-                self.pot_dv = np.ascontiguousarray(
-                    np.broadcast_to(np.asarray(vm - vp, dtype=np.float64), (self.P, self.X)))
-                deg, tab = WH_POT_TABLE, np.zeros((1, WH_MAX_DEG + 1))
+                    # arbitrary V(t, x): a program the p sweep evaluates per cell at t = step * dt, as
+                    # expPotential evaluates the spliced string (FFT.py:88-94); NonPolynomialPhysics for
+                    # syntax the device evaluator does not have
+                    self.pot_expr = compile_expression(potential, "x")
+                    deg = WH_POT_EXPR
+                else:
+                    # arbitrary time-independent V: the Bopp difference V(x - hbar theta/2) - V(x + hbar theta/2)
+                    # at kernel coordinates, as expPotential evaluates it per cell (FFT.py:88-94)
+                    u = 0.5 * self.k_hbar * ktheta[:, None]
+                    vm = host_eval(potential, t=0.0, x=kx[None, :] - u)
+                    vp = host_eval(potential, t=0.0, x=kx[None, :] + u)
+                    self.pot_dv = np.ascontiguousarray(
+                        np.broadcast_to(np.asarray(vm - vp, dtype=np.float64), (self.P, self.X)))
+                    deg = WH_POT_TABLE
+                tab = np.zeros((1, WH_MAX_DEG + 1))
         self.pot_deg, self.pot_table = deg, np.ascontiguousarray(tab)
+        # a time-dependent V read from coefficient rows limits the run to the rows present
+        self.time_table = self.time_dependent and self.pot_expr is None
         # Caldeira-Leggett friction sigma(p) = sign(p) f(p) at kernel p (theta_fp, FFT.py:424-429)
         self.sigma = None
         if damping == "CaldeiraLeggett" and gamma != 0.0:
@@ -193,6 +203,9 @@ class _Tables:
         d.pot_deg, d.pot_rows = self.pot_deg, self.pot_table.shape[0]
         d.pot_coef = self.ptr(self.pot_table)
         d.pot_dv = self.ptr(self.pot_dv)
+        d.pot_expr = self.ptr(self.pot_expr)
+        d.pot_expr_len = 0 if self.pot_expr is None else int(self.pot_expr.shape[0])
+        d.pot_t_step = self.pot_t_step
         d.odm_g, d.odm_x, d.odm_th = self.ptr(self.odm_g), self.ptr(self.odm_x), self.ptr(self.odm_th)
         d.odm_sign = self.odm_sign
         d.theta_decay = self.ptr(self.theta_decay)
@@ -319,6 +332,7 @@ class Wigner2D(_PlanBase):
         self.x_host, self.p_host = t.x_host, t.p_host
         self.dt = float(dt)
         self.time_dependent = t.time_dependent
+        self.time_table = t.time_table
         self.pot_table = t.pot_table
         self.renormalize = gamma != 0.0
         d = _lib.Wigner2DDesc()
@@ -364,7 +378,7 @@ class Wigner2D(_PlanBase):
         import torch
 
         self._check_state(W)
-        if self.time_dependent and first_step + n_steps + 1 >= self.pot_table.shape[0]:
+        if self.time_table and first_step + n_steps + 1 >= self.pot_table.shape[0]:
             raise ValueError(f"time-dependent potential table has {self.pot_table.shape[0]} rows; "
                              f"need {first_step + n_steps + 2} (pass time_rows)")
         if obs is None:
@@ -400,7 +414,7 @@ class Wigner2D(_PlanBase):
                    "wh_ensemble_set_potential")
         self._pot_device = tab  # the copy is asynchronous: keep the source alive
         self.pot_table = tab.cpu().numpy()
-        self.time_dependent = tab.shape[0] > 1
+        self.time_dependent = self.time_table = tab.shape[0] > 1
 
     def adjoint_step(self, Lam, W_next, step: int, grad_row):
         """Lambda <- U_step^dagger Lambda; grad_row (device, 9 doubles) <- dL/da_k(step)."""
@@ -456,7 +470,8 @@ class WignerEnsemble(_PlanBase):
         self.n = len(self.tables)
         self.shapes = [(t.P, t.X) for t in self.tables]
         self.time_dependent = any(t.time_dependent for t in self.tables)
-        self.time_rows = min(t.pot_table.shape[0] for t in self.tables)
+        self.time_table = any(t.time_table for t in self.tables)
+        self.time_rows = min((t.pot_table.shape[0] for t in self.tables if t.time_table), default=1)
         d = _lib.EnsembleDesc()
         d.struct_size = ctypes.sizeof(_lib.EnsembleDesc)
         _common(d, precision=precision, scheme=scheme, t=self.tables[0], dt=dt, gamma=gamma,
@@ -501,7 +516,7 @@ class WignerEnsemble(_PlanBase):
         for W, s in zip(states, self.shapes):
             if W.dtype != self.dtype or not W.is_cuda or not W.is_contiguous() or tuple(W.shape) != s:
                 raise ValueError(f"states must be contiguous {self.dtype} CUDA tensors of shapes {self.shapes}")
-        if self.time_dependent and first_step + n_steps + 1 >= self.time_rows:
+        if self.time_table and first_step + n_steps + 1 >= self.time_rows:
             raise ValueError("time-dependent potential table too short (pass time_rows)")
         if obs is None:
             obs = torch.zeros((self.n, n_steps + 1, WH_NOBS), dtype=torch.float64, device=self.device)

@@ -107,6 +107,42 @@ def test_descriptor_validation_errors(field, value, code):
     assert L.wh_wigner2d_create(ctypes.byref(d), ctypes.byref(h)) == code, L.wh_last_error()
 
 
+def test_expression_program_validation_before_device():
+    """The library checks an expression program (length, op codes, stack depth) before it
+    touches the device: the evaluator on the device trusts all three."""
+    from pywignercuda_amd import _lib
+    from pywignercuda_amd.physics import compile_expression
+
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+
+    def create(code):
+        d, keep = _valid_wigner_desc()
+        d.pot_deg = _lib.WH_POT_EXPR
+        if code is not None:
+            code = np.ascontiguousarray(code, dtype=np.float64)
+            keep.append(code)
+            d.pot_expr = code.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            d.pot_expr_len = code.shape[0]
+        d.pot_t_step = 0.01
+        rc = L.wh_wigner2d_create(ctypes.byref(d), ctypes.byref(h))
+        if rc == 0:
+            L.wh_wigner2d_destroy(h)
+        return rc, L.wh_last_error()
+
+    rc, msg = create(compile_expression("0.5*x*x*cos(t)"))
+    assert rc in (0, 12), msg  # 12 = no device (CPU container)
+    for code, what in [(None, b"ops"), (np.zeros((0, 2)), b"ops"), (np.zeros((65, 2)), b"ops"),
+                       ([[1, 0], [4, 0]], b"depth"),             # x + <nothing>
+                       ([[1, 0], [1, 0]], b"leaves 2"),          # two values left
+                       ([[3, 0]], b"depth"),                     # negate an empty stack
+                       ([[1, 0], [21, 0]], b"op"),               # no op 21
+                       ([[1.5, 0]], b"op"), ([[np.nan, 0]], b"op"),
+                       ([[1, 0]] * 17 + [[4, 0]] * 16, b"depth")]:  # 17 values on a 16-value stack
+        rc, msg = create(code)
+        assert rc == 10 and b"expression" in msg and what in msg, (code, rc, msg)
+
+
 def test_ensemble_validation_before_device():
     from pywignercuda_amd import _lib
 
@@ -189,12 +225,51 @@ def test_non_polynomial_potential_tabulated_like_oracle():
     m = orc.potential_multiplier(0.0)
     phase = t.k_dt / t.k_hbar * t.pot_dv
     np.testing.assert_allclose(np.cos(phase) - 1j * np.sin(phase), m, rtol=0, atol=1e-14)
-    # time-dependent non-polynomial V is refused with a clear error
+    # a time-dependent non-polynomial V becomes an expression program (no time rows)
+    t2 = _Tables(X, P, 12.0, 9.0, hbar, 1.0, dt, potential="tanh(x)*cos(t)", dpotential=None, scheme="lie",
+                 D_theta=0, D_lambda=0, gamma=0, damping=None, fp_damping=None, gp_coeff=0, gp_rho_factor=None,
+                 mask=False, truncate_constants=True, time_rows=None, pot_table=None)
+    assert t2.pot_deg == -2 and t2.pot_table.shape[0] == 1 and t2.time_dependent and not t2.time_table
+    assert t2.pot_t_step == dt and t2.pot_expr.shape == (5, 2)
+    # syntax the device evaluator does not have is refused with a clear error
     from pywignercuda_amd.physics import NonPolynomialPhysics
     with pytest.raises(NonPolynomialPhysics):
-        _Tables(X, P, 12.0, 9.0, hbar, 1.0, dt, potential="tanh(x)*cos(t)", dpotential=None, scheme="lie",
+        _Tables(X, P, 12.0, 9.0, hbar, 1.0, dt, potential="erf(x)*cos(t)", dpotential=None, scheme="lie",
                 D_theta=0, D_lambda=0, gamma=0, damping=None, fp_damping=None, gp_coeff=0, gp_rho_factor=None,
                 mask=False, truncate_constants=True, time_rows=None, pot_table=None)
+
+
+@pytest.mark.parametrize("expr", [
+    "0.5*x*x + 0.2*sin(3*t)*exp(-x*x)",
+    "1.5*0.5*( 1. + tanh((x-5.)/0.3)   ) - 2/sqrt(x*x+1)*cos(2*M_PI*t)",
+    "abs(x)*log(1+t) + atan(x) + sinh(x/4) + cosh(x/5) + tan(x/9) - pow(x, 3)*t/50",
+    "Heaviside(x - 1.0)*(1 - exp(-t)) - -x/(2.0+t)",
+])
+def test_expression_programs_evaluate_like_the_string(expr):
+    """The stack program of a physics string, run op by op, gives what the string's numpy
+    evaluation gives (the reference splices the string into C and evaluates it per cell)."""
+    from pywignercuda_amd.physics import EXPR_OPS, compile_expression, eval_program, host_eval
+
+    code = compile_expression(expr)
+    assert code.ndim == 2 and code.shape[1] == 2 and code.shape[0] <= 64
+    assert set(code[:, 0].astype(int)) <= set(EXPR_OPS.values())
+    x = np.linspace(-7.0, 7.0, 257)
+    for t in (0.0, 0.013, 1.7):
+        want = np.broadcast_to(host_eval(expr, t=t, x=x), x.shape)
+        np.testing.assert_allclose(np.broadcast_to(eval_program(code, x, t), x.shape), want, rtol=1e-13, atol=1e-13)
+
+
+def test_expression_programs_fold_constants_and_bound_the_stack():
+    from pywignercuda_amd.physics import EXPR_OPS, NonPolynomialPhysics, compile_expression
+
+    code = compile_expression("2*M_PI*sqrt(4.0)*x*cos(t)")
+    # 2*pi*sqrt(4) folds to one constant: const, x, mul, t, cos, mul
+    assert code.shape[0] == 6 and code[0, 0] == EXPR_OPS["const"] and code[0, 1] == pytest.approx(4 * np.pi)
+    deep = "x" + "".join(f"+(x*{i}" for i in range(1, 20)) + ")" * 19
+    with pytest.raises(NonPolynomialPhysics):
+        compile_expression(deep + "*t")  # nesting beyond the device stack
+    with pytest.raises(NonPolynomialPhysics):
+        compile_expression("+".join(["x*t"] * 40))  # beyond the program length
 
 
 def test_dirac_descriptor_validation_before_device():

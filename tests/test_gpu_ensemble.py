@@ -48,6 +48,39 @@ def test_uneven_ensemble_matches_oracle(cuda, scheme):
         np.testing.assert_allclose(obs[i, : steps + 1, 8], obs_ref["H"], rtol=1e-10, atol=1e-12)
 
 
+def test_ensemble_with_an_expression_potential_member(cuda):
+    """An ensemble whose second member has a time-dependent non-polynomial V (expression
+    program) beside polynomial members: the plan runs the expression p kernels for all of
+    them, every member against its own oracle."""
+    import torch
+
+    from pywignercuda_amd.wigner2d import WignerEnsemble
+
+    members = [_member(128, 0.1, 1.0, 1.0, 1.0), _member(256, 0.05, 0.5, -2.0, 0.8)]
+    expr = "0.5*x*x + 0.2*cos(3*t)*tanh(x)"
+    members[1].update(potential=expr, dpotential="x")
+    dt, steps = 0.01, 25
+    ens = WignerEnsemble(members, 1.0, 1.0, dt, scheme="strang", kinetic="0.5*p*p", device=cuda)
+    assert ens.tables[1].pot_deg == -2 and ens.tables[0].pot_deg >= 0
+    refs = [_oracle(members[0], dt, steps, "strang")]
+    m = members[1]
+    orc = ow.Wigner2DOracle(m["X"], m["P"], m["X_amp"], m["P_amp"], 1.0, 1.0, dt, scheme="strang",
+                            potential=lambda t, x: 0.5 * x * x + 0.2 * np.cos(3 * t) * np.tanh(x),
+                            dpotential=lambda t, x: x, kinetic=lambda p: 0.5 * p * p, mask=False,
+                            time_dependent=True)
+    W0 = ow.gaussian_wigner(orc.g, x0=m["x0"], s=m["s"])
+    W_ref, obs_ref, _ = orc.run(W0, steps)
+    refs.append((W0, W_ref, obs_ref))
+    states = ens.states_to_device([r[0] for r in refs])
+    obs = ens.run(states, steps + 1, use_graph=True).cpu().numpy()
+    torch.cuda.synchronize()
+    for i, (W0, W_ref, obs_ref) in enumerate(refs):
+        W = states[i].cpu().numpy()
+        err = np.linalg.norm(W - W_ref) / np.linalg.norm(W_ref)
+        assert err < 1e-10, (i, err)
+        np.testing.assert_allclose(obs[i, : steps + 1, 8], obs_ref["H"], rtol=1e-10, atol=1e-12)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32fast"])
 def test_every_variant_matches_oracle(cuda, prec):
     import ctypes

@@ -221,6 +221,81 @@ def test_non_polynomial_potential_table(cuda, scheme):
     check_obs(obs, obs_ref)
 
 
+DRIVEN_EXPR = "0.5*x*x + 0.3*sin(2*t)*exp(-0.5*x*x) + 0.05*Heaviside(x - 1.5)*cos(t)"
+
+
+def driven_V(t, x):
+    return 0.5 * x * x + 0.3 * np.sin(2 * t) * np.exp(-0.5 * x * x) + 0.05 * np.where(x - 1.5 < 0, 0.0, 1.0) * np.cos(t)
+
+
+@pytest.mark.parametrize("scheme,damping", [("lie", None), ("strang", None), ("lie", "CaldeiraLeggett"),
+                                            ("strang", "CaldeiraLeggett")])
+def test_time_dependent_expression_potential(cuda, scheme, damping):
+    """A time-dependent non-polynomial V(t, x) (a driven Gaussian bump on a harmonic well with
+    a switched step): the p sweep evaluates its program per cell at t = step * dt, as the
+    reference evaluates the spliced string (FFT.py:88-94), against the time-dependent oracle."""
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    X = P = 256
+    dt, gamma = 0.01, (0.05 if damping else 0.0)
+    fp_str = " p*p/sqrt( p*p + 0.5  ) " if damping else None
+    fp_fn = (lambda p: p * p / np.sqrt(p * p + 0.5)) if damping else None
+    hip = Wigner2D(X, P, 10.0, 10.0, 1.0, 1.0, dt, potential=DRIVEN_EXPR, dpotential="x",
+                   kinetic="0.5*p*p" if scheme == "strang" else None, scheme=scheme, D_theta=0.01,
+                   gamma=gamma, damping=damping, fp_damping=fp_str, mask=True, device=cuda)
+    assert hip.tables.pot_deg == -2 and not hip.tables.time_table
+    orc = ow.Wigner2DOracle(X, P, 10.0, 10.0, 1.0, 1.0, dt, D_theta=0.01, gamma=gamma, potential=driven_V,
+                            dpotential=lambda t, x: x, kinetic=(lambda p: 0.5 * p * p), fp_damping=fp_fn,
+                            damping=damping, scheme=scheme, mask=True, time_dependent=True)
+    W0 = ow.gaussian_wigner(orc.g, x0=1.0, p0=1.0, s=1.0)
+    W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 40)
+    err = rel_l2(W, W_ref)
+    print(f"expression V {scheme} {damping}: rel L2 {err:.3e}")
+    assert err < 1e-10
+    check_obs(obs, obs_ref)
+    # the graph path and a run past any coefficient-table length (no time rows needed)
+    W2 = hip.to_device(W0)
+    hip.run(W2, 41, use_graph=True)
+    assert rel_l2(W2.cpu().numpy(), W_ref) < 1e-10
+    hip.run(W2, 5000, first_step=41, use_graph=True)
+    assert np.isfinite(W2.cpu().numpy()).all()
+
+
+@pytest.mark.parametrize("precision,bound", [("fp32", 2e-5), ("fp32fast", 2e-5)])
+def test_time_dependent_expression_potential_fp32(cuda, precision, bound):
+    """The complex64-state forms run the same program (arithmetic in FP64 / FP32)."""
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    X = P = 256
+    dt = 0.01
+    hip = Wigner2D(X, P, 10.0, 10.0, 1.0, 1.0, dt, potential=DRIVEN_EXPR, dpotential="x", mask=True,
+                   precision=precision, device=cuda)
+    orc = ow.Wigner2DOracle(X, P, 10.0, 10.0, 1.0, 1.0, dt, potential=driven_V, dpotential=lambda t, x: x,
+                            mask=True, time_dependent=True)
+    W0 = ow.gaussian_wigner(orc.g, x0=1.0, p0=1.0, s=1.0)
+    W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 40)
+    err = rel_l2(W.astype(np.complex128), W_ref)
+    print(f"expression V {precision}: rel L2 {err:.3e}")
+    assert err < bound
+
+
+def test_expression_potential_single_sweep(cuda):
+    """wh_wigner2d_potential at time index 7 of the driven expression V against the oracle's
+    potential_step at t = 7 dt (FFT.py:80-126)."""
+    import torch
+
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    X, P, dt = 256, 128, 0.01
+    hip = Wigner2D(X, P, 10.0, 10.0, 1.0, 1.0, dt, potential=DRIVEN_EXPR, dpotential="x", mask=True, device=cuda)
+    orc = ow.Wigner2DOracle(X, P, 10.0, 10.0, 1.0, 1.0, dt, potential=driven_V, mask=True, time_dependent=True)
+    W0 = ow.gaussian_wigner(orc.g, x0=1.0, p0=0.5, s=1.0).astype(np.complex128)
+    W = hip.to_device(W0)
+    hip.potential(W, step=7)
+    torch.cuda.synchronize()
+    assert rel_l2(W.cpu().numpy(), orc.potential_step(W0, 7 * dt)) < 1e-12
+
+
 @pytest.mark.parametrize("gp,hbar", [(0.0, 1.0), (0.5, 0.8)])
 def test_lie_odm_fp64(cuda, gp, hbar):
     """ODM Lindblad dissipator (order 1) after the V step, with and without the GP term."""
