@@ -121,13 +121,17 @@ class Wigner2DOracle:
         reference; exposed so closed-form tests can switch it off).
     truncate_constants: emulate the '%f' splicing of the kernel constants.
     time_dependent: V depends on t (otherwise the multipliers are cached).
+    kvn: the classical KvN potential step of GPU_FokkerPlank2D_FFT (exp_PotentialKvN,
+        FFT.py:27-75, called at :2401): phase dt * (-dPotential(t, x) * theta), the same
+        D_Theta decay and edge mask; with renormalize (its 'Wigner' normalization, :2408-2410).
+    renormalize: None = after every step when gamma != 0 (FFT.py:1912-1913).
     """
 
     def __init__(self, X, P, X_amp, P_amp, hbar, mass, dt, D_theta=0.0, D_lambda=0.0,
                  gamma=0.0, potential=None, dpotential=None, kinetic=None,
                  fp_damping=None, damping=None, gp_coeff=0.0, scheme="lie",
                  mask=True, truncate_constants=False, time_dependent=False, L_material=None,
-                 odm_weights=None):
+                 odm_weights=None, kvn=False, renormalize=None):
         self.g = Grid(X, P, X_amp, P_amp)
         self.hbar, self.mass, self.dt = float(hbar), float(mass), float(dt)
         self.D_theta, self.D_lambda, self.gamma = float(D_theta), float(D_lambda), float(gamma)
@@ -144,6 +148,8 @@ class Wigner2DOracle:
         self.scheme = scheme
         self.mask = mask
         self.time_dependent = bool(time_dependent)
+        self.kvn = bool(kvn)
+        self.renormalize = renormalize
         self._cache = {}
         tr = percent_f if truncate_constants else float
         g = self.g
@@ -182,7 +188,10 @@ class Wigner2DOracle:
         """Multiplier on the (theta, x) grid (FFT.py:80-126; GP.py:60-86)."""
         th, x, h = self.ktheta, self.kx, self.k_hbar
         frac = 1.0 if self.scheme == "lie" else 0.5
-        dV = (self.potential(t, x - h * th / 2.0) - self.potential(t, x + h * th / 2.0)) / h
+        if self.kvn:  # Potential_XTheta = -dPotential(t, x) * theta (FFT.py:42-47, 60)
+            dV = -self.dpotential(t, x) * th
+        else:
+            dV = (self.potential(t, x - h * th / 2.0) - self.potential(t, x + h * th / 2.0)) / h
         phase = frac * self.k_dt * dV
         r = np.exp(-frac * self.k_Dtheta * th * th * self.k_dt)
         m = r * np.cos(phase) - 1j * (r * np.sin(phase))
@@ -352,6 +361,7 @@ class Wigner2DOracle:
                 elif self.damping == "ODM":  # Lindbladian_ODM_Order1 per channel (FFT.py:1907-1910, 2173-2181)
                     for k, w in enumerate(self.odm_weights):
                         W = W + self.odm_operator(W, weight=w, sign=1.0 if k == 0 else -1.0)
+            if self.gamma != 0.0 if self.renormalize is None else self.renormalize:
                 W = W / (W.sum() * dxdp)
             if self.scheme == "strang" and record:
                 for k, v in self.observables(W).items():

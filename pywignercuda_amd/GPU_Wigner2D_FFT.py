@@ -1,4 +1,5 @@
-"""Drop-in for GPU_Wigner2D_FFT.py: Propagator_Base and GPU_Wigner2D_FFT.
+"""Drop-in for GPU_Wigner2D_FFT.py: Propagator_Base, GPU_Wigner2D_FFT, GPU_Wigner2D_ODM2 and
+the classical KvN propagator GPU_FokkerPlank2D_FFT.
 
 Usage is the reference's (instances/Wigner2D/*.ipynb): subclass, set
 ``dampingFunction``, ``grossPitaevskiiCoefficient``, ``fp_Damping_String``, call
@@ -249,17 +250,22 @@ class Propagator_Base:
         self.W_end = f["W_end"] + 0j
 
     # ---------------------------------------------------------------- device plan
-    def _make_plan(self, scheme, gp_coeff, kinetic, odm_weights=None):
-        if self.gammaDamping != 0.0 and self.dampingFunction not in ("CaldeiraLeggett", "ODM"):
-            raise NotImplementedError(f"dampingFunction {self.dampingFunction!r} is not implemented")
-        if self.gammaDamping != 0.0 and self.dampingFunction == "ODM" and scheme != "lie":
-            raise NotImplementedError("ODM damping is part of GPU_Wigner2D_FFT (Lie scheme) only")
+    def _make_plan(self, scheme, gp_coeff, kinetic, odm_weights=None, kvn=False, renormalize=None):
+        if kvn:
+            # the Fokker-Planck Run applies only the Caldeira-Leggett dissipator (FFT.py:2404-2406)
+            damping = "CaldeiraLeggett" if self.gammaDamping != 0.0 and \
+                self.dampingFunction == "CaldeiraLeggett" else None
+        else:
+            if self.gammaDamping != 0.0 and self.dampingFunction not in ("CaldeiraLeggett", "ODM"):
+                raise NotImplementedError(f"dampingFunction {self.dampingFunction!r} is not implemented")
+            if self.gammaDamping != 0.0 and self.dampingFunction == "ODM" and scheme != "lie":
+                raise NotImplementedError("ODM damping is part of GPU_Wigner2D_FFT (Lie scheme) only")
+            damping = self.dampingFunction if self.gammaDamping != 0.0 else None
         return Wigner2D(
             self.X_gridDIM, self.P_gridDIM, self.X_amplitude, self.P_amplitude, self.hBar, self.mass, self.dt,
             potential=self.potentialString, dpotential=self.dPotentialString, kinetic=kinetic, scheme=scheme,
-            D_theta=self.D_Theta, D_lambda=self.D_Lambda, gamma=self.gammaDamping,
-            damping=self.dampingFunction if self.gammaDamping != 0.0 else None,
-            L_material=getattr(self, "L_material", None),
+            D_theta=self.D_Theta, D_lambda=self.D_Lambda, gamma=self.gammaDamping, damping=damping,
+            L_material=getattr(self, "L_material", None), kvn=kvn, renormalize=renormalize,
             fp_damping=self.fp_Damping_String, gp_coeff=gp_coeff, mask=self.absorbingMask,
             truncate_constants=self.truncateConstants, precision=self.precision,
             time_rows=self.timeSteps + 3,
@@ -389,6 +395,43 @@ class GPU_Wigner2D_ODM2(Propagator_Base):
     def Run(self):
         """GPU_Wigner2D_FFT.py:2016-2230; returns 0."""
         self.file = _io.RunFile(self.fileName or "Wigner2D_ODM2_run")
+        self.WriteHDF5_variables()
+        self.file.create_dataset("Hamiltonian", data=self.Hamiltonian.real)
+        rec, saved = self._run_loop(self.plan, post_step=False)
+        self._store_results(rec, saved)
+        return 0
+
+
+class GPU_FokkerPlank2D_FFT(Propagator_Base):
+    """Classical Koopman-von Neumann Fokker-Planck propagator in 2D phase space
+    (GPU_Wigner2D_FFT.py:2242-2459): the Lie step of GPU_Wigner2D_FFT with the potential factor
+    exp(i dt dV(x) theta) exp(-D_Theta theta^2 dt) and the edge mask of exp_PotentialKvN
+    (:27-75) in place of the Bopp difference of V, the Caldeira-Leggett dissipator when
+    gammaDamping != 0 and dampingFunction == 'CaldeiraLeggett', and the renormalisation when
+    gammaDamping != 0 and normalization == 'Wigner' (:2403-2410).  dPotentialString must not
+    depend on t.  The fields the reference reads off ``self`` (dampingFunction,
+    fp_Damping_String, SetTimeTrack) are set as for GPU_Wigner2D_FFT; no kinematicString is
+    needed."""
+
+    def __init__(self, X_gridDIM, P_gridDIM, X_amplitude, P_amplitude, hBar, mass, D_Theta, D_Lambda,
+                 gammaDamping, potentialString, dPotentialString, normalization="Wigner"):
+        self.normalization = normalization
+        self.D_Theta = D_Theta
+        self.D_Lambda = D_Lambda
+        self.gammaDamping = gammaDamping
+        self.potentialString = potentialString
+        self.dPotentialString = dPotentialString
+        self.SetPhaseSpaceBox2D(X_gridDIM, P_gridDIM, X_amplitude, P_amplitude)
+        self.hBar = hBar
+        self.mass = mass
+        self.plan = self._make_plan("lie", 0.0, None, kvn=True,
+                                    renormalize=gammaDamping != 0.0 and normalization == "Wigner")
+        self.Hamiltonian = self.P ** 2 / (2.0 * self.mass) + self.Potential(0, self.X)
+        self.Hamiltonian = self.fft_shift2D(self.Hamiltonian)
+
+    def Run(self):
+        """GPU_Wigner2D_FFT.py:2282-2459; returns 0."""
+        self.file = _io.RunFile(self.fileName or "FokkerPlanck2D_run")
         self.WriteHDF5_variables()
         self.file.create_dataset("Hamiltonian", data=self.Hamiltonian.real)
         rec, saved = self._run_loop(self.plan, post_step=False)

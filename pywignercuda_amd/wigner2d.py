@@ -75,7 +75,7 @@ class _Tables:
 
     def __init__(self, X, P, X_amp, P_amp, hbar, mass, dt, *, potential, dpotential, scheme,
                  D_theta, D_lambda, gamma, damping, fp_damping, gp_coeff, gp_rho_factor, mask,
-                 truncate_constants, time_rows, pot_table, L_material=None, odm_sign=1.0):
+                 truncate_constants, time_rows, pot_table, L_material=None, odm_sign=1.0, kvn=False):
         self.X, self.P = int(X), int(P)
         self.X_amp, self.P_amp = float(X_amp), float(P_amp)
         tr = percent_f if truncate_constants else float
@@ -116,7 +116,18 @@ class _Tables:
             if gp_coeff != 0.0 and scheme == "lie":
                 self.x_mask = self.x_mask * edge(20.0)  # expPotential_GrossPitaevskii, :514-516
         # potential coefficient rows
-        if pot_table is not None:
+        if kvn:
+            # classical KvN (Fokker-Planck) phase -dt dV(x) theta (exp_PotentialKvN, FFT.py:37-64), as the
+            # table read by the p sweep: phase = dt/hbar * pot_dv, pot_dv = -hbar theta dV(x)
+            if scheme != "lie" or pot_table is not None or not dpotential:
+                raise ValueError("the KvN phase belongs to the Lie scheme and needs dpotential")
+            if depends_on_time(dpotential, "x"):
+                raise NotImplementedError("a time-dependent dPotential in the KvN (Fokker-Planck) propagator")
+            dvk = np.broadcast_to(np.asarray(host_eval(dpotential, t=0.0, x=kx), dtype=np.float64), kx.shape)
+            self.pot_dv = np.ascontiguousarray(-self.k_hbar * ktheta[:, None] * dvk[None, :])
+            self.time_dependent = False
+            deg, tab = WH_POT_TABLE, np.zeros((1, WH_MAX_DEG + 1))
+        elif pot_table is not None:
             # explicit device coefficients: the degree is the highest non-zero column
             tab = np.ascontiguousarray(pot_table, dtype=np.float64)
             nz = np.nonzero(np.any(tab != 0, axis=0))[0]
@@ -220,7 +231,8 @@ class _Tables:
         d.gp_rho_factor = self.rho_factor
 
 
-def _common(d, *, precision, scheme, t: _Tables, dt, gamma, damping, kinetic, gp_coeff, odm_weights=None):
+def _common(d, *, precision, scheme, t: _Tables, dt, gamma, damping, kinetic, gp_coeff, odm_weights=None,
+            renormalize=None):
     d.precision = PRECISIONS[precision]
     d.scheme = WH_SCHEME_LIE if scheme == "lie" else WH_SCHEME_STRANG
     d.dt, d.hbar, d.mass = t.k_dt, t.k_hbar, t.k_mass
@@ -238,7 +250,7 @@ def _common(d, *, precision, scheme, t: _Tables, dt, gamma, damping, kinetic, gp
     cl = damping == "CaldeiraLeggett" and gamma != 0.0
     d.damping = 1 if cl else (2 if (damping == "ODM" and gamma != 0.0) else 0)
     d.gamma_dt = float(dt) * float(gamma)  # host constants (FFT.py:1150-1156)
-    d.renormalize = 1 if gamma != 0.0 else 0
+    d.renormalize = int(gamma != 0.0 if renormalize is None else bool(renormalize))
     d.gp_coeff = t.k_agp
     # ODM channels: one with weight 1 (GPU_Wigner2D_FFT.Run) or the two of GPU_Wigner2D_ODM2
     w = (1.0,) if odm_weights is None else tuple(float(v) for v in odm_weights)
@@ -301,6 +313,10 @@ class Wigner2D(_PlanBase):
     time_rows   : rows of the coefficient table for a time-dependent V (>= steps + 2).
     pot_table   : explicit [rows, 9] coefficient table for the device (the string
                   is still used for the host observables).
+    kvn         : classical Koopman-von Neumann (Fokker-Planck) potential step
+                  exp(i dt dV(x) theta) in place of the Bopp difference of V
+                  (GPU_FokkerPlank2D_FFT, FFT.py:27-75, 2242-2459); Lie scheme, time-independent dV.
+    renormalize : divide by dXdP sum(W) after each step; None = when gamma != 0.
     variants    : (x, p, p+CL) kernel variant indices; None = tuned choice.
     """
 
@@ -308,7 +324,8 @@ class Wigner2D(_PlanBase):
                  kinetic=None, scheme="lie", D_theta=0.0, D_lambda=0.0, gamma=0.0,
                  damping=None, fp_damping=None, gp_coeff=0.0, gp_rho_factor=None, mask=True,
                  truncate_constants=False, precision="fp64", time_rows=None, pot_table=None,
-                 device=None, variants=None, L_material=None, odm_sign=1.0, odm_weights=None):
+                 device=None, variants=None, L_material=None, odm_sign=1.0, odm_weights=None,
+                 kvn=False, renormalize=None):
         import torch
 
         if odm_weights is not None and not 1 <= len(odm_weights) <= 2:
@@ -325,7 +342,7 @@ class Wigner2D(_PlanBase):
                     scheme=scheme, D_theta=D_theta, D_lambda=D_lambda, gamma=gamma, damping=damping,
                     fp_damping=fp_damping, gp_coeff=gp_coeff, gp_rho_factor=gp_rho_factor, mask=mask,
                     truncate_constants=truncate_constants, time_rows=time_rows, pot_table=pot_table,
-                    L_material=L_material, odm_sign=odm_sign)
+                    L_material=L_material, odm_sign=odm_sign, kvn=kvn)
         self.tables = t
         self.X, self.P = t.X, t.P
         self.dX, self.dP = t.dX, t.dP
@@ -334,11 +351,11 @@ class Wigner2D(_PlanBase):
         self.time_dependent = t.time_dependent
         self.time_table = t.time_table
         self.pot_table = t.pot_table
-        self.renormalize = gamma != 0.0
+        self.renormalize = gamma != 0.0 if renormalize is None else bool(renormalize)
         d = _lib.Wigner2DDesc()
         d.struct_size = ctypes.sizeof(_lib.Wigner2DDesc)
         _common(d, precision=precision, scheme=scheme, t=t, dt=dt, gamma=gamma, damping=damping,
-                kinetic=kinetic, gp_coeff=gp_coeff, odm_weights=odm_weights)
+                kinetic=kinetic, gp_coeff=gp_coeff, odm_weights=odm_weights, renormalize=self.renormalize)
         t.fill(d)
         self._plan = ctypes.c_void_p()
         with torch.cuda.device(self.device):

@@ -239,6 +239,32 @@ def test_non_polynomial_potential_tabulated_like_oracle():
                 mask=False, truncate_constants=True, time_rows=None, pot_table=None)
 
 
+def test_kvn_phase_tabulated_like_oracle():
+    """The Fokker-Planck (KvN) potential step is the p sweep's table path with the phase
+    -dt dV(x) theta of exp_PotentialKvN (FFT.py:37-64): table times dt/hbar against the oracle's
+    KvN multiplier, '%f' constants and D_Theta decay included."""
+    from oracle import wigner2d as ow
+    from pywignercuda_amd.wigner2d import _Tables
+
+    X, P, hbar, dt = 128, 64, 0.8, 0.005
+    t = _Tables(X, P, 12.0, 9.0, hbar, 1.0, dt, potential="0.1*pow(x,4) - x*x", dpotential="0.4*pow(x,3) - 2*x",
+                scheme="lie", D_theta=0.0143, D_lambda=0, gamma=0, damping=None, fp_damping=None, gp_coeff=0,
+                gp_rho_factor=None, mask=True, truncate_constants=True, time_rows=None, pot_table=None, kvn=True)
+    assert t.pot_deg == -1 and not t.time_dependent
+    orc = ow.Wigner2DOracle(X, P, 12.0, 9.0, hbar, 1.0, dt, D_theta=0.0143,
+                            potential=lambda t_, x: 0.1 * x ** 4 - x * x,
+                            dpotential=lambda t_, x: 0.4 * x ** 3 - 2 * x, mask=True, truncate_constants=True,
+                            kvn=True)
+    phase = t.k_dt / t.k_hbar * t.pot_dv
+    m = np.cos(phase) - 1j * np.sin(phase)
+    m = m * t.theta_decay[:, None] * t.x_mask[None, :]
+    np.testing.assert_allclose(m, orc.potential_multiplier(0.0), rtol=0, atol=1e-14)
+    with pytest.raises(NotImplementedError):
+        _Tables(X, P, 12.0, 9.0, hbar, 1.0, dt, potential="x*x", dpotential="2*x*cos(t)", scheme="lie", D_theta=0,
+                D_lambda=0, gamma=0, damping=None, fp_damping=None, gp_coeff=0, gp_rho_factor=None, mask=True,
+                truncate_constants=True, time_rows=None, pot_table=None, kvn=True)
+
+
 @pytest.mark.parametrize("expr", [
     "0.5*x*x + 0.2*sin(3*t)*exp(-x*x)",
     "1.5*0.5*( 1. + tanh((x-5.)/0.3)   ) - 2/sqrt(x*x+1)*cos(2*M_PI*t)",

@@ -12,6 +12,8 @@ Sources (raw notebook JSON lines):
   Caldeira + GP          instances/Wigner2D/Wigner2D_FFT_Caldeira_GPitaevskii_X2.ipynb:999
   GP Strang              instances/Wigner2D_GPitaevskii/Wigner2D_GPitaevskii_X2.ipynb:405
   ODM X2                 instances/Wigner2D/Wigner2D_FFT_ODM_X2_D0.ipynb:453
+  Fokker-Planck X2       instances/FokkerPlanck2D/Damping_XX_PInit3_Omega1_Gamma0.07_DTheta0.0143.ipynb
+                         (cells 12 and 15 outputs)
 """
 import numpy as np
 import pytest
@@ -160,3 +162,35 @@ def test_odm2_x2_notebook(cuda, tmp_path):
     inst.W_init = inst.Wigner_HarmonicOscillator(0, inst.omega, 0.0, 3.0)
     assert inst.Run() == 0
     _check(inst, -1.56718191353e-06, 0.310080647085)
+
+
+def test_fokker_planck_x2_notebook(cuda, tmp_path):
+    """GPU_FokkerPlank2D_FFT driven as Damping_XX_PInit3_Omega1_Gamma0.07_DTheta0.0143.ipynb drives
+    it: 512^2, amplitudes 12, dt 0.005, 20000 steps, D_Theta 0.0143, gamma 0.07, epsilon 0.5
+    (the string gets epsilon^2), HO ground state at p = 3.  Recorded: W_end min / max (cell 12),
+    <x> and <p> at the last record (cell 15)."""
+    from pywignercuda_amd.GPU_Wigner2D_FFT import GPU_FokkerPlank2D_FFT
+
+    class frame(GPU_FokkerPlank2D_FFT):
+        def __init__(self):
+            self.mass = 1.0
+            self.omega = 1.0
+            kappa = 1.0
+            self.dampingFunction = "CaldeiraLeggett"
+            self.epsilon = 0.5
+            self.grossPitaevskiiCoefficient = 0.0
+            X2_constant = 0.5 * self.mass * self.omega ** 2
+            self.fp_Damping_String = " p*p/sqrt( p*p + {epsilon}  ) ".format(epsilon=self.epsilon ** 2)
+            self.SetTimeTrack(0.005, 20000, 500, fileName=str(tmp_path / "X2.hdf5"))
+            GPU_FokkerPlank2D_FFT.__init__(self, 512, 512, 12, 12, kappa, 1.0, 0.0143, 0.0, 0.07,
+                                           "{0}*pow(x,2)".format(X2_constant), "2*{0}*x".format(X2_constant))
+
+    inst = frame()
+    inst.W_init = inst.Wigner_HarmonicOscillator(0, 1.0, 0.0, 3.0)
+    assert abs(inst.W_init.max() - 0.318309886184) < 1e-12  # recorded for W_init (cell 11)
+    assert inst.Run() == 0
+    _check(inst, -2.33870293025e-07, 0.906308254344)
+    print(f"<x> {inst.X_average[-1]!r} <p> {inst.P_average[-1]!r}")
+    assert abs(inst.X_average[-1] - (-0.00914105730639)) < 1e-12
+    assert abs(inst.P_average[-1] - 0.0116785228734) < 1e-12
+    assert abs(inst.P_average[0] - 3.0) < 1e-12 and len(inst.X_average) == 20001

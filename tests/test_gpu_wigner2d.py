@@ -296,6 +296,34 @@ def test_expression_potential_single_sweep(cuda):
     assert rel_l2(W.cpu().numpy(), orc.potential_step(W0, 7 * dt)) < 1e-12
 
 
+@pytest.mark.parametrize("precision,bound", [("fp64", 1e-10), ("fp32", 2e-6)])
+def test_kvn_fokker_planck(cuda, precision, bound):
+    """The classical KvN (Fokker-Planck) step of GPU_FokkerPlank2D_FFT (FFT.py:27-75, 2242-2459)
+    on a quartic double well, where it differs from the Wigner step: Caldeira-Leggett friction,
+    diffusion, renormalisation, '%f' constants, against the KvN oracle."""
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    X = P = 256
+    dt = 0.005
+    fp_str = " p*p/sqrt( p*p + 0.25  ) "
+    hip = Wigner2D(X, P, 10.0, 10.0, 1.0, 1.0, dt, potential="0.1*pow(x,4) - x*x",
+                   dpotential="0.4*pow(x,3) - 2*x", D_theta=0.0143, gamma=0.07, damping="CaldeiraLeggett",
+                   fp_damping=fp_str, mask=True, truncate_constants=True, precision=precision, kvn=True,
+                   device=cuda)
+    orc = ow.Wigner2DOracle(X, P, 10.0, 10.0, 1.0, 1.0, dt, D_theta=0.0143, gamma=0.07,
+                            potential=lambda t, x: 0.1 * x ** 4 - x * x,
+                            dpotential=lambda t, x: 0.4 * x ** 3 - 2 * x,
+                            fp_damping=lambda p: p * p / np.sqrt(p * p + 0.25), damping="CaldeiraLeggett",
+                            mask=True, truncate_constants=True, kvn=True)
+    W0 = ow.gaussian_wigner(orc.g, x0=1.5, p0=1.0, s=0.8)
+    W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 40)
+    err = rel_l2(W.astype(np.complex128), W_ref)
+    print(f"KvN {precision}: rel L2 {err:.3e}")
+    assert err < bound
+    if precision == "fp64":
+        check_obs(obs, obs_ref)
+
+
 @pytest.mark.parametrize("gp,hbar", [(0.0, 1.0), (0.5, 0.8)])
 def test_lie_odm_fp64(cuda, gp, hbar):
     """ODM Lindblad dissipator (order 1) after the V step, with and without the GP term."""

@@ -12,6 +12,9 @@ Pins (each chosen so that a plausible slip fails it):
   * Caldeira-Leggett: the friction sign and factor via d<p>/dt = -2 gamma <p>, and the
     reference's nested accumulation expanded as a polynomial in D;
   * Gross-Pitaevskii: the shifted densities equal rho(x -/+ theta/2) (shift theorem);
+  * KvN (Fokker-Planck) potential step: the classical momentum kick
+    W(x, p) -> W(x, p + dt V'(x)) of a Gaussian in closed form, and for quadratic V the
+    classical and quantum propagators coincide;
   * '%f' splicing of constants.
 """
 import numpy as np
@@ -199,3 +202,45 @@ def test_odm_six_terms_split_into_x_and_theta_factors():
                 pot_table=None, L_material=3.0)
     for k, name in enumerate(["gp", "gm", "gsp", "gsm"]):
         np.testing.assert_array_equal(t.odm_g[k], np.broadcast_to(T[name], t.odm_g[k].shape))
+
+
+# ----------------------------------------------------------------------------- KvN Fokker-Planck
+def test_kvn_potential_step_is_the_classical_momentum_kick():
+    """exp_PotentialKvN (FFT.py:27-75) multiplies the theta picture by exp(i dt V'(x) theta):
+    the state becomes W(x, p + dt V'(x)), the kick p -> p - dt V'(x) of every classical
+    trajectory.  For a Gaussian the kicked state is known in closed form; a wrong sign, a
+    factor 2 or hbar (the Bopp difference's expansion) fails it."""
+    X = P = 128
+    dt = 0.01
+    dV = lambda t, x: 0.4 * x ** 3 - 2.0 * x  # noqa: E731  (V = 0.1 x^4 - x^2)
+    orc = ow.Wigner2DOracle(X, P, 8.0, 8.0, 0.7, 1.0, dt, potential=lambda t, x: 0.1 * x ** 4 - x ** 2,
+                            dpotential=dV, mask=False, kvn=True)
+    g = orc.g
+    x0, p0, s = 0.5, 0.3, 1.0
+    W0 = ow.gaussian_wigner(g, x0=x0, p0=p0, s=s).astype(np.complex128)
+    W1 = orc.potential_step(W0, 0.0)
+    x, p = g.x_host, g.p_host
+    want = (1.0 / np.pi) * np.exp(-((x - x0) ** 2) / s ** 2 - s ** 2 * (p + dt * dV(0, x) - p0) ** 2)
+    assert np.abs(W1 - want).max() < 1e-12
+    # the quantum (Bopp) step differs from the kick for a quartic V: the pin separates the two
+    Wq = ow.Wigner2DOracle(X, P, 8.0, 8.0, 0.7, 1.0, dt, potential=lambda t, x: 0.1 * x ** 4 - x ** 2,
+                           dpotential=dV, mask=False).potential_step(W0, 0.0)
+    assert np.abs(Wq - want).max() > 1e-6
+
+
+def test_kvn_equals_the_wigner_propagator_for_quadratic_v():
+    """For quadratic V the Bopp difference is exactly -hbar theta V'(x): the Fokker-Planck run
+    (damped, diffusive, renormalised, edge mask on) equals the Wigner run step for step."""
+    X = P = 128
+    V = lambda t, x: 0.5 * x * x  # noqa: E731
+    dV = lambda t, x: x  # noqa: E731
+    fp = lambda p: p * p / np.sqrt(p * p + 0.25)  # noqa: E731
+    kw = dict(D_theta=0.0143, gamma=0.07, potential=V, dpotential=dV, fp_damping=fp, damping="CaldeiraLeggett",
+              truncate_constants=True)
+    a = ow.Wigner2DOracle(X, P, 10.0, 10.0, 1.0, 1.0, 0.005, kvn=True, **kw)
+    b = ow.Wigner2DOracle(X, P, 10.0, 10.0, 1.0, 1.0, 0.005, **kw)
+    W0 = ow.gaussian_wigner(a.g, x0=0.0, p0=3.0)
+    Wa, oa, _ = a.run(W0, 30)
+    Wb, ob, _ = b.run(W0, 30)
+    assert rel_l2(Wa, Wb) < 1e-12
+    np.testing.assert_allclose(oa["P"], ob["P"], rtol=1e-11, atol=1e-13)
