@@ -491,9 +491,11 @@ typedef struct {
      phase dt*a*Re(B- - B+), B-/+ = rho(x -/+ theta/2), rho = Re sum_p trace of the state */
   double gp_coeff;            /* grossPitaevskiiCoefficient (host value); 0 = off */
   const double* lambda_host;  /* [X] host Lambda grid (exp(i lambda theta/2)) */
-  /* vector potentials Potential_1..3 as time-independent polynomials in x (a_deg[k] = -1:
-     identically zero).  Any non-zero one switches the potential step to two sweeps (the
-     factors then couple all 16 planes); incompatible with mass_dm/mass_sm */
+  /* vector potentials Potential_1..3 as polynomials in x (a_deg[k] = -1: identically zero;
+     a_deg[k] bounds the degree over all rows).  Any non-zero one switches the potential step
+     to two sweeps (the factors then couple all 16 planes); incompatible with mass_dm/mass_sm.
+     Time-independent: a_coef, a_rows = 0.  Time-dependent: a_rows rows of [3][WH_MAX_DEG + 1]
+     coefficients in a_coef_rows (row = time index, clamped to a_rows - 1; host memory). */
   int a_deg[3];
   double a_coef[3][WH_MAX_DEG + 1];
   /* ODM damping (dampingModel 'ODM': DampingODM_Kernel, GPU_WignerDirac2D_4x4.py:952-1195,
@@ -511,6 +513,8 @@ typedef struct {
   /* antiparticle step filter (antiParticleStepFiltering, :3636-3678): after the potential half,
      W <- Lambda_+(p + lambda/2) W Lambda_+(p - lambda/2) in (p, lambda), then the absorber */
   int step_filter;
+  const double* a_coef_rows;  /* [a_rows][3][WH_MAX_DEG + 1], see a_deg */
+  int a_rows;
 } wh_dirac_desc;
 
 typedef struct wh_dirac wh_dirac;

@@ -12,7 +12,8 @@ step, observables reduced on the device, copied once per saved frame).
 Extra attributes read if present: precision ('fp64' default or 'fp32'),
 truncateConstants (default True, the reference's '%f' splicing), useGraph
 (default True).  Not implemented (NotImplementedError): vector potentials that
-are not time-independent polynomials in x and ODM damping together with a vector
+are not polynomials in x (their coefficients may depend on t), a position-dependent
+mass together with a vector potential, and ODM damping together with a vector
 potential.  GPU_WignerDiracKvN2D_4x4 (GPU_WignerDiracKvN2D_4x4.py) is the KvN
 variant of the same class.
 """
@@ -77,7 +78,8 @@ class GPU_WignerDirac2D_4x4:
             gp_coeff=self.grossPitaevskiiCoefficient, absorb=True, truncate_constants=self.truncateConstants,
             antiparticle=bool(self.antiParticleNorm), energy=(self.computeEnergy is True),
             precision=self.precision, lambda_bar=getattr(self, "lambdaBar", None),
-            step_filter=bool(self.antiParticleStepFiltering), **variant)
+            step_filter=bool(self.antiParticleStepFiltering),
+            time_rows=(self.timeSteps + 3) if getattr(self, "timeSteps", None) else None, **variant)
 
     # ------------------------------------------------------------------ potentials (host, :2170-2228)
     def PotentialFunction_0(self, t, x):

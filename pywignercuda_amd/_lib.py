@@ -161,7 +161,7 @@ class DiracDesc(ctypes.Structure):
         ("mass_dm", _DP), ("mass_sm", _DP), ("gp_coeff", c_double), ("lambda_host", _DP),
         ("a_deg", c_int * 3), ("a_coef", (c_double * (WH_MAX_DEG + 1)) * 3),
         ("odm", c_int), ("odm_gamma", c_double), ("lambda_bar", c_double),
-        ("kvn", c_int), ("step_filter", c_int),
+        ("kvn", c_int), ("step_filter", c_int), ("a_coef_rows", _DP), ("a_rows", c_int),
     ]
 
 

@@ -1284,7 +1284,7 @@ struct wh_dirac {
   wh_dirac_desc d{};
   wh::DiracPlan host{};  // host mirror of the device plan
   DevBuf plan_dev, pot, pot_dv, theta_decay, xmask, x_host, p_host, theta_host, dv0, vsum, mass_dm, mass_sm;
-  DevBuf colsums, rowpart, scale, step, rho, G, lambda_host;
+  DevBuf colsums, rowpart, scale, step, rho, G, lambda_host, acoef_rows;
   TwSet twx, twp;
   int xblocks = 0, pblocks = 0, gpblocks = 0, pgblocks = 0;
   hipStream_t capture_stream = nullptr;
@@ -1325,6 +1325,9 @@ int dirac_validate(const wh_dirac_desc* d) {
       return fail(WH_INVALID_DESCRIPTOR, "vector potential %d: degree %d", k + 1, d->a_deg[k]);
   if (d->mass_dm && (d->a_deg[0] >= 0 || d->a_deg[1] >= 0 || d->a_deg[2] >= 0))
     return fail(WH_INVALID_DESCRIPTOR, "a position-dependent mass needs A1 = A2 = A3 = 0");
+  if (d->a_rows < 0 || (d->a_rows > 0 && !d->a_coef_rows))
+    return fail(WH_INVALID_DESCRIPTOR, "vector potential rows %d, coefficients %p", d->a_rows,
+                (const void*)d->a_coef_rows);
   if (!(d->dt == d->dt) || d->c == 0.0) return fail(WH_INVALID_DESCRIPTOR, "dt %g c %g", d->dt, d->c);
   if (d->kvn) {
     if (d->a_deg[0] >= 0 || d->a_deg[1] >= 0 || d->a_deg[2] >= 0 || d->mass_dm || d->gp_coeff != 0.0 ||
@@ -1493,6 +1496,11 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   h.adeg2 = d.a_deg[2];
   h.general = (d.a_deg[0] >= 0 || d.a_deg[1] >= 0 || d.a_deg[2] >= 0) ? 1 : 0;
   std::memcpy(h.acoef, d.a_coef, sizeof(h.acoef));
+  if (h.general && d.a_rows > 0) {
+    WH_TRY(e->acoef_rows.upload(d.a_coef_rows, size_t(d.a_rows) * 3 * (WH_MAX_DEG + 1) * sizeof(double)));
+    h.acoef_rows = e->acoef_rows.as<double>();
+    h.a_rows = d.a_rows;
+  }
   h.kvn = d.kvn ? 1 : 0;
   h.filter = d.step_filter ? 1 : 0;
   if (h.filter) {

@@ -391,10 +391,19 @@ __device__ __forceinline__ void dirac_psweep_body(const DiracPlan& d, int tile, 
 struct DiracPot {
   cplx<double> u11, u33, u13, u14, u23;
 };
-__device__ __forceinline__ DiracPot dirac_pot_factor(const DiracPlan& d, double y, double s) {
-  const double a1 = d.adeg0 >= 0 ? poly(d.acoef[0], d.adeg0, y) : 0.0;
-  const double a2 = d.adeg1 >= 0 ? poly(d.acoef[1], d.adeg1, y) : 0.0;
-  const double a3 = d.adeg2 >= 0 ? poly(d.acoef[2], d.adeg2, y) : 0.0;
+// Coefficients [3][kMaxDeg + 1] of A1..A3 at time index `step`.
+__device__ __forceinline__ const double* dirac_acoef(const DiracPlan& d, int step) {
+  if (d.a_rows <= 0) return &d.acoef[0][0];
+  const int row = step < d.a_rows ? step : d.a_rows - 1;
+  return d.acoef_rows + size_t(row) * 3 * (kMaxDeg + 1);
+}
+
+__device__ __forceinline__ DiracPot dirac_pot_factor(const DiracPlan& d, const double* __restrict__ ac, double y,
+                                                     double s) {
+  constexpr int K = kMaxDeg + 1;
+  const double a1 = d.adeg0 >= 0 ? poly(ac, d.adeg0, y) : 0.0;
+  const double a2 = d.adeg1 >= 0 ? poly(ac + K, d.adeg1, y) : 0.0;
+  const double a3 = d.adeg2 >= 0 ? poly(ac + 2 * K, d.adeg2, y) : 0.0;
   const double mdt = d.mass_phase;  // mass_half c^2 dt
   const double F = sqrt(mdt * mdt + (a1 * a1 + a2 * a2 + a3 * a3) * d.dt * d.dt);
   double sn, cs;
@@ -422,6 +431,7 @@ __device__ __forceinline__ void dirac_pleft_body(const DiracPlan& d, int tile, c
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twP_hi), reinterpret_cast<const cplx<T>*>(d.twP_lo)};
   LdsStrided<T, C> view{lds + c};
   const int step = *gmem(d.step);
+  const double* __restrict__ ac = dirac_acoef(d, step);  // A1..A3 at this time index
   const double x = d.dx * double(fidx(col, X));
   const double mask = d.xmask ? gmem(d.xmask)[col] : 1.0;
   const int prow = step < d.pot_rows ? step : d.pot_rows - 1;
@@ -446,7 +456,7 @@ __device__ __forceinline__ void dirac_pleft_body(const DiracPlan& d, int tile, c
       if (d.gp_coeff != 0.0) ph += d.dt * d.gp_coeff * gmem(d.G)[size_t(i) * X + col];
       const double r = (d.theta_decay ? gmem(d.theta_decay)[i] : 1.0) * mask;
       const cplx<double> e = expmi(r, ph);
-      const DiracPot u = dirac_pot_factor(d, x - 0.5 * th, -1.0);
+      const DiracPot u = dirac_pot_factor(d, ac, x - 0.5 * th, -1.0);
       const cplx<double> w0 = widen(v[0][j]), w1 = widen(v[1][j]), w2 = widen(v[2][j]), w3 = widen(v[3][j]);
       const cplx<double> mu13{-u.u13.x, -u.u13.y};
       const cplx<double> o0 = cadd(cmul(u.u11, w0), cadd(cmul(u.u13, w2), cmul(u.u14, w3)));
@@ -480,6 +490,7 @@ __device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, 
   const Twiddles<T> tw{reinterpret_cast<const cplx<T>*>(d.twP_hi), reinterpret_cast<const cplx<T>*>(d.twP_lo)};
   LdsStrided<T, C> view{lds + c};
   const double x = d.dx * double(fidx(col, X));
+  const double* __restrict__ ac = dirac_acoef(d, *gmem(d.step));  // the time index of dirac_pleft
   const unsigned boff = (unsigned(t) * unsigned(X) + unsigned(col)) * unsigned(sizeof(cplx<S>));
   const T inv = T(1.0 / N);
   double acc[kDNCol];
@@ -500,7 +511,7 @@ __device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, 
     for (int j = 0; j < R; ++j) {
       const int i = t + j * TT;
       const double th = d.dtheta * double(fidx(i, N));
-      const DiracPot u = dirac_pot_factor(d, x + 0.5 * th, 1.0);
+      const DiracPot u = dirac_pot_factor(d, ac, x + 0.5 * th, 1.0);
       const cplx<double> w0 = widen(v[0][j]), w1 = widen(v[1][j]), w2 = widen(v[2][j]), w3 = widen(v[3][j]);
       const cplx<double> mu13{-u.u13.x, -u.u13.y};
       // (W U)_{ij} = sum_k W_ik U_kj with U31 = U13, U32 = U14, U41 = U23, U42 = U24 = -U13
@@ -838,10 +849,13 @@ __global__ void __launch_bounds__(kRedThreads) dirac_reduce_kernel(const DiracPl
       dv = gmem(d.dv0)[col];
       vs = gmem(d.vsum)[col];
     }
-    if (d.pot_deg >= 0 && d.general) {  // DiracEnergy adds Potential_1..3 like Potential_0 (:2801-2804)
-      if (d.adeg0 >= 0) vs += poly(d.acoef[0], d.adeg0, x);
-      if (d.adeg1 >= 0) vs += poly(d.acoef[1], d.adeg1, x);
-      if (d.adeg2 >= 0) vs += poly(d.acoef[2], d.adeg2, x);
+    // DiracEnergy adds Potential_1..3 like Potential_0 (:2801-2804); with a V0 table, vsum holds
+    // them already unless they depend on time
+    if (d.general && (d.pot_deg >= 0 || d.a_rows > 0)) {
+      const double* ac = dirac_acoef(d, tstate);
+      if (d.adeg0 >= 0) vs += poly(ac, d.adeg0, x);
+      if (d.adeg1 >= 0) vs += poly(ac + (kMaxDeg + 1), d.adeg1, x);
+      if (d.adeg2 >= 0) vs += poly(ac + 2 * (kMaxDeg + 1), d.adeg2, x);
     }
     const double tr[2] = {cs[kDcTrRe], cs[kDcTrIm]}, ptr[2] = {cs[kDcPTrRe], cs[kDcPTrIm]};
     const double a1[2] = {cs[kDcA1Re], cs[kDcA1Im]};

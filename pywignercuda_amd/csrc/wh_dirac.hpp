@@ -84,6 +84,10 @@ struct DiracPlan {
   // instead of riding on the potential factors (xmask = null)
   int filter, pad1;
   const double* fmask;
+  // time-dependent vector potentials: a_rows rows of [3][kMaxDeg + 1] (row = time index,
+  // clamped); a_rows = 0: acoef above
+  const double* acoef_rows;
+  int a_rows, pad2;
 };
 
 }  // namespace wh

@@ -68,7 +68,8 @@ class WignerDirac4x4:
     antiparticle, energy : record AntiParticlePopulation / DiracEnergy.
     precision  : 'fp64' (complex128 planes), 'fp32' (complex64 planes, FP64 arithmetic in the
                  sweeps) or 'fp32fast' (complex64 planes, FP32 transforms, FP64 factors and sums).
-    time_rows  : rows of the coefficient table for a time-dependent polynomial V0.
+    time_rows  : rows of the coefficient tables for a time-dependent polynomial V0 or
+                 time-dependent polynomial vector potentials (>= steps + 2).
     mass_potential, kinetic_mass_half : the MassPotential variant -- Mass(t, x) string of the
                  potential factors and the fixed mass_half of the kinetic factors.
     """
@@ -83,17 +84,21 @@ class WignerDirac4x4:
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         a_deg, a_coef = [-1, -1, -1], np.zeros((3, WH_MAX_DEG + 1))
+        # time-dependent vector potentials: one coefficient row per time index t = dt * index,
+        # as V0's rows (the reference evaluates Potential_k(t, x) per cell, :417-442, 575-601)
+        a_time = any(not _is_zero(potentials[k]) and depends_on_time(potentials[k], "x") for k in (1, 2, 3))
+        a_times = np.arange(int(time_rows) if time_rows else 4096) * float(dt) if a_time else [0.0]
+        a_rows = np.zeros((len(a_times), 3, WH_MAX_DEG + 1))
         for k in (1, 2, 3):
             if _is_zero(potentials[k]):
                 continue
-            if depends_on_time(potentials[k], "x"):
-                raise NotImplementedError(f"Potential_{k}_String = {potentials[k]!r}: time-dependent vector potential")
             try:
-                deg, tab = coefficient_table(potentials[k], "x", [0.0])
+                deg, tab = coefficient_table(potentials[k], "x", a_times)
             except NonPolynomialPhysics as exc:
                 raise NotImplementedError(
                     f"Potential_{k}_String = {potentials[k]!r}: vector potentials must be polynomials in x") from exc
             a_deg[k - 1], a_coef[k - 1] = deg, tab[0]
+            a_rows[:, k - 1, :] = tab
         if kvn:
             # GPU_WignerDiracKvN2D_4x4: V0 only, no damping or GP terms in its loop (:3409-3690)
             if max(a_deg) >= 0 or mass_potential is not None:
@@ -158,6 +163,9 @@ class WignerDirac4x4:
             d.mass_dm, d.mass_sm = ptr(Fm - Fp), ptr(Fm + Fp)
         V0 = potentials[0]
         self.time_dependent = depends_on_time(V0, "x")
+        self.a_table = a_rows if a_time else None
+        if a_time:
+            d.a_coef_rows, d.a_rows = ptr(a_rows), a_rows.shape[0]
         try:
             times = np.arange(int(time_rows) if time_rows else 4096) * self.dt if self.time_dependent else [0.0]
             deg, tab = coefficient_table(V0, "x", times)
@@ -175,8 +183,9 @@ class WignerDirac4x4:
             d.pot_dv = ptr(np.broadcast_to(np.asarray(vm - vp, dtype=np.float64), (self.P, self.X)))
             dv0 = host_eval(_sympy_derivative(V0) + " + 0.*x", t=0.0, x=kx)
             d.dv0 = ptr(np.broadcast_to(np.asarray(dv0, dtype=np.float64), kx.shape))
+            # V0 + A1 + A2 + A3 of the energy record; time-dependent A are added per step on the device
             vs = sum(np.broadcast_to(np.asarray(host_eval(v, t=0.0, x=kx), dtype=np.float64), kx.shape)
-                     for v in potentials)
+                     for v in (potentials[:1] if a_time else potentials))
             d.vsum = ptr(vs)
         k_D = tr(D_theta)
         d.theta_decay = ptr(np.exp(-k_dt * k_D * ktheta * ktheta)) if k_D != 0.0 else None
@@ -237,8 +246,10 @@ class WignerDirac4x4:
         import torch
 
         self._check_state(W)
-        if self.time_dependent and first_step + n_steps + 1 >= self.pot_table.shape[0]:
-            raise ValueError(f"time-dependent potential table has {self.pot_table.shape[0]} rows; "
+        rows = [t.shape[0] for t in ((self.pot_table if self.time_dependent else None), self.a_table)
+                if t is not None]
+        if rows and first_step + n_steps + 1 >= min(rows):
+            raise ValueError(f"time-dependent potential table has {min(rows)} rows; "
                              f"need {first_step + n_steps + 2} (pass time_rows)")
         if obs is None:
             obs = torch.zeros((n_steps + 1, WH_DNOBS), dtype=torch.float64, device=W.device)

@@ -218,6 +218,36 @@ def test_dirac_vector_potential_constant_field(cuda, precision):
         assert err < 1e-5
 
 
+@pytest.mark.parametrize("case,precision", [("klein", "fp64"), ("ramp", "fp64"), ("ramp", "fp32")])
+def test_dirac_time_dependent_vector_potential(cuda, case, precision):
+    """Vector potentials whose polynomial coefficients depend on t (a driving field): the two
+    potential sweeps read the coefficient row of the step's time index, the energy record adds
+    A1..A3 at the state's time (D44.py:417-442, 575-601, 2801-2804); V0 tabulated (klein) or
+    polynomial (ramp)."""
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    vs, vf, dvf = CASES[case]
+    pots = (vs, "0.05*x*cos(3*t)", " -x + 0.2*sin(2*t) ", "0.1*t + 0.02*pow(x,2)")
+    hip = WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=pots, energy=True, D_theta=0.01,
+                         precision=precision, time_rows=64, device=cuda)
+    assert hip.vector_potential and hip.a_table is not None
+    orc = Dirac4x4Oracle(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01,
+                         [vf, lambda t, x: 0.05 * x * np.cos(3 * t), lambda t, x: -x + 0.2 * np.sin(2 * t),
+                          lambda t, x: 0.1 * t + 0.02 * x * x],
+                         dpotential0=dvf, compute_energy=True, D_theta=0.01)
+    W0 = wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 3.0, 1.0, Xg, 1.0, 1.0), orc)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 20)
+    err = rel_l2(W, W_ref)
+    print(f"time-dependent A {case} {precision}: rel L2 {err:.3e}")
+    if precision == "fp64":
+        assert err < 1e-10
+        check_obs(obs, rec)
+    else:
+        assert err < 1e-5
+    with pytest.raises(ValueError):  # beyond the coefficient rows
+        hip.run(hip.to_device(W0), 80)
+
+
 def test_dirac_single_sweeps_match_oracle_factors(cuda):
     """wh_dirac_kinetic / wh_dirac_potential (one sweep each, no normalisation) against the
     oracle's kinetic and potential steps between the transforms."""
