@@ -493,7 +493,8 @@ typedef struct {
   const double* lambda_host;  /* [X] host Lambda grid (exp(i lambda theta/2)) */
   /* vector potentials Potential_1..3 as polynomials in x (a_deg[k] = -1: identically zero;
      a_deg[k] bounds the degree over all rows).  Any non-zero one switches the potential step
-     to two sweeps (the factors then couple all 16 planes); incompatible with mass_dm/mass_sm.
+     to two sweeps (the factors then couple all 16 planes; with mass_dm/mass_sm their mass
+     term is Mass(x -/+ theta/2) c^2 dt from those tables).
      Time-independent: a_coef, a_rows = 0.  Time-dependent: a_rows rows of [3][WH_MAX_DEG + 1]
      coefficients in a_coef_rows (row = time index, clamped to a_rows - 1; host memory). */
   int a_deg[3];

@@ -1323,8 +1323,6 @@ int dirac_validate(const wh_dirac_desc* d) {
   for (int k = 0; k < 3; ++k)
     if (d->a_deg[k] < -1 || d->a_deg[k] > WH_MAX_DEG)
       return fail(WH_INVALID_DESCRIPTOR, "vector potential %d: degree %d", k + 1, d->a_deg[k]);
-  if (d->mass_dm && (d->a_deg[0] >= 0 || d->a_deg[1] >= 0 || d->a_deg[2] >= 0))
-    return fail(WH_INVALID_DESCRIPTOR, "a position-dependent mass needs A1 = A2 = A3 = 0");
   if (d->a_rows < 0 || (d->a_rows > 0 && !d->a_coef_rows))
     return fail(WH_INVALID_DESCRIPTOR, "vector potential rows %d, coefficients %p", d->a_rows,
                 (const void*)d->a_coef_rows);

@@ -398,13 +398,13 @@ __device__ __forceinline__ const double* dirac_acoef(const DiracPlan& d, int ste
   return d.acoef_rows + size_t(row) * 3 * (kMaxDeg + 1);
 }
 
+// mdt: Mass(y) c^2 dt (mass_half c^2 dt for a constant mass)
 __device__ __forceinline__ DiracPot dirac_pot_factor(const DiracPlan& d, const double* __restrict__ ac, double y,
-                                                     double s) {
+                                                     double s, double mdt) {
   constexpr int K = kMaxDeg + 1;
   const double a1 = d.adeg0 >= 0 ? poly(ac, d.adeg0, y) : 0.0;
   const double a2 = d.adeg1 >= 0 ? poly(ac + K, d.adeg1, y) : 0.0;
   const double a3 = d.adeg2 >= 0 ? poly(ac + 2 * K, d.adeg2, y) : 0.0;
-  const double mdt = d.mass_phase;  // mass_half c^2 dt
   const double F = sqrt(mdt * mdt + (a1 * a1 + a2 * a2 + a3 * a3) * d.dt * d.dt);
   double sn, cs;
   sincos_fp64(F, &sn, &cs);
@@ -456,7 +456,10 @@ __device__ __forceinline__ void dirac_pleft_body(const DiracPlan& d, int tile, c
       if (d.gp_coeff != 0.0) ph += d.dt * d.gp_coeff * gmem(d.G)[size_t(i) * X + col];
       const double r = (d.theta_decay ? gmem(d.theta_decay)[i] : 1.0) * mask;
       const cplx<double> e = expmi(r, ph);
-      const DiracPot u = dirac_pot_factor(d, ac, x - 0.5 * th, -1.0);
+      // F(x - theta/2) = (sm + dm) / 2 of the mass tables (MassPotential variant)
+      const size_t e_ = size_t(i) * X + col;
+      const double mdt = d.mass_dm ? 0.5 * (gmem(d.mass_sm)[e_] + gmem(d.mass_dm)[e_]) : d.mass_phase;
+      const DiracPot u = dirac_pot_factor(d, ac, x - 0.5 * th, -1.0, mdt);
       const cplx<double> w0 = widen(v[0][j]), w1 = widen(v[1][j]), w2 = widen(v[2][j]), w3 = widen(v[3][j]);
       const cplx<double> mu13{-u.u13.x, -u.u13.y};
       const cplx<double> o0 = cadd(cmul(u.u11, w0), cadd(cmul(u.u13, w2), cmul(u.u14, w3)));
@@ -511,7 +514,10 @@ __device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, 
     for (int j = 0; j < R; ++j) {
       const int i = t + j * TT;
       const double th = d.dtheta * double(fidx(i, N));
-      const DiracPot u = dirac_pot_factor(d, ac, x + 0.5 * th, 1.0);
+      // F(x + theta/2) = (sm - dm) / 2 of the mass tables (MassPotential variant)
+      const size_t e_ = size_t(i) * X + col;
+      const double mdt = d.mass_dm ? 0.5 * (gmem(d.mass_sm)[e_] - gmem(d.mass_dm)[e_]) : d.mass_phase;
+      const DiracPot u = dirac_pot_factor(d, ac, x + 0.5 * th, 1.0, mdt);
       const cplx<double> w0 = widen(v[0][j]), w1 = widen(v[1][j]), w2 = widen(v[2][j]), w3 = widen(v[3][j]);
       const cplx<double> mu13{-u.u13.x, -u.u13.y};
       // (W U)_{ij} = sum_k W_ik U_kj with U31 = U13, U32 = U14, U41 = U23, U42 = U24 = -U13

@@ -106,8 +106,6 @@ class WignerDirac4x4:
             gamma, damping, gp_coeff, step_filter = 0.0, None, 0.0, True
             if kinetic_mass_half is None:
                 kinetic_mass_half = 0.00001  # mass_half of GPU_WignerDiracKvN2D_4x4.py:1961
-        if max(a_deg) >= 0 and mass_potential is not None:
-            raise NotImplementedError("a position-dependent mass together with a vector potential")
         if gamma != 0.0 and damping not in (None, "CaldeiraLegget", "ODM"):
             raise NotImplementedError(f"dampingModel {damping!r} is not supported ('CaldeiraLegget' or 'ODM')")
         if gamma > 0.0 and damping == "ODM":
