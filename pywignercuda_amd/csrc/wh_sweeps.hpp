@@ -68,6 +68,48 @@ __device__ __noinline__ double eval_expr(const double* __restrict__ code, int n,
   return st[0];
 }
 
+// The same stack machine inlined, for lanes that own a slice of LDS: the top of the stack in a
+// register, the values below it at st[s * LS] (LS lanes share the slice, so that they touch
+// consecutive words).  Used between two transforms, when the team's LDS row is free.
+// The program is read through the constant address space: its addresses are uniform, so
+// the ops come from scalar loads rather than a vector memory round trip each.
+template <int LS>
+__device__ __forceinline__ double eval_expr_lds(cptr<double> code, int n, double x, double t, double* st) {
+  double tos = 0.0;
+  int sp = 0;  // values on the stack (the top one in tos)
+  for (int i = 0; i < n; ++i) {
+    const int op = int(code[2 * i]);
+    const double v = code[2 * i + 1];
+    if (op <= 2) {
+      if (sp > 0) st[(sp - 1) * LS] = tos;
+      tos = op == 0 ? v : op == 1 ? x : t;
+      ++sp;
+    } else if (op == 3) {
+      tos = -tos;
+    } else if (op <= 8) {
+      const double a = st[(sp - 2) * LS];
+      --sp;
+      tos = op == 4 ? a + tos : op == 5 ? a - tos : op == 6 ? a * tos : op == 7 ? a / tos : pow(a, tos);
+    } else {
+      switch (op) {
+        case 9: tos = exp(tos); break;
+        case 10: tos = sin(tos); break;
+        case 11: tos = cos(tos); break;
+        case 12: tos = tan(tos); break;
+        case 13: tos = sqrt(tos); break;
+        case 14: tos = fabs(tos); break;
+        case 15: tos = tanh(tos); break;
+        case 16: tos = sinh(tos); break;
+        case 17: tos = cosh(tos); break;
+        case 18: tos = log(tos); break;
+        case 19: tos = atan(tos); break;
+        default: tos = tos < 0.0 ? 0.0 : 1.0; break;  // Heaviside
+      }
+    }
+  }
+  return tos;
+}
+
 __device__ __forceinline__ double poly(const double* __restrict__ c, int deg, double x) {
   double r = c[deg];
   for (int n = deg - 1; n >= 0; --n) r = fma(r, x, c[n]);
@@ -457,17 +499,48 @@ struct PSweep {
 
   // potential step multiplier of time index `step` (expPotential; GP phase optional),
   // times the real factor `scale` (the 1/N of the inverse transform that follows).
-  // EXPR: the kernel also evaluates expression programs (pot_deg = kPotExpr); the call to
-  // the evaluator makes the compiler save the live state around it, so only the kernels
-  // launched for plans with such a V carry it
+  // EXPR: the kernel also evaluates expression programs (pot_deg = kPotExpr); only the
+  // kernels launched for plans with such a V carry the evaluator.  Wave teams evaluate it
+  // inline with the stack in their own LDS row (team_lds, free between two transforms);
+  // block-wide teams call the out-of-line evaluator.
   template <bool EXPR>
   __device__ __forceinline__ static void potential(cplx<T> (&v)[R], const SweepArgs& a,
                                                    const Member& m, int t_in, int col_in, int step,
-                                                   double scale) {
+                                                   double scale, cplx<E>* team_lds) {
     // opaque lane indices: recomputing theta and x -/+ hbar theta/2 per call is cheaper
     // than keeping them live across the transforms between two calls
     const int t = opaque_v(t_in), col = opaque_v(col_in);
     const double x = m.dx * double(fidx(col, m.X));
+    if constexpr (EXPR && kWave) {
+      if (m.pot_deg == kPotExpr) {
+        // kExprStack slots per lane inside the team's row
+        static_assert(kExprStack * TT <= LSZ * int(sizeof(cplx<E>) / sizeof(double)), "expression stack in LDS");
+        double* st = reinterpret_cast<double*>(team_lds) + t;
+        const double smask = (m.xmask ? gmem(m.xmask)[col] : 1.0) * scale;
+        const bool gpe = (a.flags & kGP) != 0;
+        const double hd = a.hbar_half * m.dtheta, tt = double(step) * m.pot_t_step;
+        Sync::sync();  // the transform's last reads of the row are done
+#pragma unroll 1
+        for (int j = 0; j < R; ++j) {
+          const int i = t + j * TT;
+          const double u = hd * double(fidx(i, N));
+          double phi = a.vphase * (eval_expr_lds<TT>(cmem(m.pot_expr), m.pot_expr_len, x - u, tt, st) -
+                                   eval_expr_lds<TT>(cmem(m.pot_expr), m.pot_expr_len, x + u, tt, st));
+          if (gpe) phi -= a.gp_phase * gmem(m.G)[size_t(i) * m.X + col];
+          const cplx<double> M = expmi(smask, phi);
+          static_for<0, R>([&](auto J) {
+            constexpr int jj = decltype(J)::value;
+            if (jj == j) v[jj] = apply(v[jj], M);
+          });
+        }
+        Sync::sync();  // the stack is dead before the next transform writes the row
+        if (m.theta_decay) {
+#pragma unroll
+          for (int j = 0; j < R; ++j) v[j] = apply(v[j], cplx<double>{gmem(m.theta_decay)[t + j * TT], 0.0});
+        }
+        return;
+      }
+    }
     const int row = step < m.pot_rows ? step : m.pot_rows - 1;
     const cptr<double> cf = cmem(m.pot) + size_t(row) * (kMaxDeg + 1);
     const double mask = (m.xmask ? gmem(m.xmask)[col] : 1.0) * scale;
@@ -595,7 +668,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
   } else {
     team_fft<T, N, R, -1, View, Sync>(v, tc, view_c, twsrc);
   }
-  PS::template potential<EXPR>(v, a, m, tc, col, step, 1.0 / N);
+  PS::template potential<EXPR>(v, a, m, tc, col, step, 1.0 / N, view_c.base);
   team_fft<T, N, R, +1, View, Sync>(v, tc, view_c, twsrc);
 
   if constexpr (CL) {
@@ -692,7 +765,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
 
   if (a.flags & kMerge) {
     team_fft<T, N, R, -1, View, Sync>(v, tc, view_c, twsrc);
-    PS::template potential<EXPR>(v, a, m, tc, col, step + 1, 1.0 / N);
+    PS::template potential<EXPR>(v, a, m, tc, col, step + 1, 1.0 / N, view_c.base);
     team_fft<T, N, R, +1, View, Sync>(v, tc, view_c, twsrc);
   }
 

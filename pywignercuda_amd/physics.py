@@ -152,6 +152,21 @@ def compile_expression(expr: str, var: str = "x") -> np.ndarray:
                 return float(_EXPR_NP[name](args[0]))
         return None
 
+    def emit_pow(base, expo):
+        """pow(base, k) of a leaf base (x, t) and an integer 1 <= k <= 4 as k - 1 products
+        (within an ulp of pow, and without its cost on the device); else the pow op."""
+        k = fold(expo)
+        leaf = isinstance(base, ast.Name) and base.id in (var, "t")
+        if leaf and k is not None and float(k).is_integer() and 1 <= k <= 4:
+            for _ in range(int(k)):
+                emit(base)
+            for _ in range(int(k) - 1):
+                code.append((EXPR_OPS["mul"], 0.0))
+            return
+        emit(base)
+        emit(expo)
+        code.append((EXPR_OPS["pow"], 0.0))
+
     def emit(node):
         v = fold(node)
         if v is not None:
@@ -168,17 +183,17 @@ def compile_expression(expr: str, var: str = "x") -> np.ndarray:
             emit(node.operand)
             if isinstance(node.op, ast.USub):
                 code.append((EXPR_OPS["neg"], 0.0))
-        elif isinstance(node, ast.BinOp) and isinstance(node.op, (ast.Add, ast.Sub, ast.Mult, ast.Div, ast.Pow)):
+        elif isinstance(node, ast.BinOp) and isinstance(node.op, ast.Pow):
+            emit_pow(node.left, node.right)
+        elif isinstance(node, ast.BinOp) and isinstance(node.op, (ast.Add, ast.Sub, ast.Mult, ast.Div)):
             emit(node.left)
             emit(node.right)
-            code.append((EXPR_OPS[{ast.Add: "add", ast.Sub: "sub", ast.Mult: "mul", ast.Div: "div",
-                                   ast.Pow: "pow"}[type(node.op)]], 0.0))
+            code.append((EXPR_OPS[{ast.Add: "add", ast.Sub: "sub", ast.Mult: "mul", ast.Div: "div"}[type(node.op)]],
+                         0.0))
         elif isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and not node.keywords:
             name = "fabs" if node.func.id == "abs" else node.func.id
             if name == "pow" and len(node.args) == 2:
-                emit(node.args[0])
-                emit(node.args[1])
-                code.append((EXPR_OPS["pow"], 0.0))
+                emit_pow(node.args[0], node.args[1])
             elif name in _EXPR_FUNCS and len(node.args) == 1:
                 emit(node.args[0])
                 code.append((EXPR_OPS[name], 0.0))
