@@ -55,8 +55,8 @@ extern "C" {
 #define WH_POT_EXPR (-2)  /* pot_deg value: V(t, x) given as an expression program (below) */
 
 /* Expression programs (pot_deg = WH_POT_EXPR): V(t, x) as WH_EXPR_MAX or fewer (op, value)
-   pairs of doubles for a stack machine, evaluated per cell by the p sweep at time
-   t = step * pot_t_step -- the in-kernel counterpart of the reference splicing its Potential
+   pairs of doubles for a stack machine, evaluated per cell on the device before each p sweep
+   at time t = step * pot_t_step -- the counterpart of the reference splicing its Potential
    string into expPotential (GPU_Wigner2D_FFT.py:88-94, 869-873).  Ops: push value / x / t,
    negate, the binary + - * / pow (second operand on top), and the functions exp sin cos tan
    sqrt fabs tanh sinh cosh log atan Heaviside (0 below 0, else 1, as the reference's device
@@ -182,8 +182,9 @@ typedef struct {
      (clamped to pot_rows-1); pot_rows = 1 for a time-independent V.
      pot_deg = WH_POT_TABLE: V is not a polynomial; the p sweep reads the Bopp
      difference from pot_dv instead (pot_coef is then unused).
-     pot_deg = WH_POT_EXPR: V(t, x) is the program pot_expr (pot_coef unused); the p
-     sweeps of such a plan run p variant 0 */
+     pot_deg = WH_POT_EXPR: V(t, x) is the program pot_expr (pot_coef unused); before each
+     p sweep the library tabulates its Bopp difference at the sweep's time index (two
+     [P][X] FP64 tables of device memory per such member) */
   int pot_deg;
   int pot_rows;
   const double* pot_coef;
@@ -349,9 +350,7 @@ int wh_ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_s
 /*
  * Launch variants (index into the kernel variant tables; -1 keeps the current
  * choice): x sweep, p sweep, p sweep with Caldeira-Leggett.  Rejected with
- * WH_INVALID_VALUE if a member's grid does not fit the variant.  A plan with a
- * member whose V is an expression program (WH_POT_EXPR) keeps p variant 0: its p
- * choices are ignored.
+ * WH_INVALID_VALUE if a member's grid does not fit the variant.
  */
 int wh_ensemble_set_variants(wh_ensemble* e, int xvar, int pvar, int pclvar);
 /* Number of variants of each table and the (radix, lanes) of variant i.  The last

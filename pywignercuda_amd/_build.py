@@ -16,7 +16,7 @@ CSRC = PKG / "csrc"
 OUT = PKG / "_build"
 LIB = PKG / "libwignerhip.so"
 SOURCES = ["wh_sweeps_f64.hip", "wh_sweeps_f32.hip", "wh_sweeps_f32w.hip", "wh_misc.hip", "wh_capi.cpp", "wh_adjoint.hip", "wh_dirac.hip",
-           "wh_odm.hip", "wh_bloch.hip", "wh_noalloc.hip"]
+           "wh_odm.hip", "wh_bloch.hip", "wh_noalloc.hip", "wh_expr.hip"]
 ARCH = os.environ.get("WH_OFFLOAD_ARCH", "gfx950")
 
 

@@ -28,14 +28,16 @@
 namespace wh {
 hipError_t launch_xsweep_f64(int variant, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_xsweep_f32(int variant, const SweepArgs& a, int nwork, hipStream_t s);
-hipError_t launch_psweep_f64(int variant, bool cl, bool expr, const SweepArgs& a, int nwork, hipStream_t s);
-hipError_t launch_psweep_f32(int variant, bool cl, bool expr, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_psweep_f64(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_psweep_f32(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_expr_table(const Member* members, int member, int cells, int depth, const int* step,
+                             int step_add, int use_arg, int ntables, double hbar_half, hipStream_t s);
 hipError_t launch_gpshift(const Member* members, const unsigned* work, int nwork, hipStream_t s);
 hipError_t launch_colsum(int prec, const Member* members, const unsigned* work, int nwork, hipStream_t s);
 hipError_t launch_observe_noalloc(int prec, Member m, hipStream_t s);
 int noalloc_max_x();
 hipError_t launch_xsweep_f32w(int variant, const SweepArgs& a, int nwork, hipStream_t s);
-hipError_t launch_psweep_f32w(int variant, bool cl, bool expr, const SweepArgs& a, int nwork, hipStream_t s);
+hipError_t launch_psweep_f32w(int variant, bool cl, const SweepArgs& a, int nwork, hipStream_t s);
 hipError_t launch_reduce(const ReduceArgs& a, int nmembers, int* done, hipStream_t s);
 hipError_t launch_colpart_reduce(const Member* members, int nmembers, int xradix, int xblock, const int* step,
                                  hipStream_t s);
@@ -389,7 +391,7 @@ struct wh_ensemble {
   DevBuf mem_dev;
   std::map<int, std::unique_ptr<TwSet>> tw;  // by size
   int xvar = 0, pvar = 0, pclvar = 0;
-  bool expr = false;  // a member's V is an expression program: p sweeps run kExprPVariant
+  bool expr = false;  // a member's V is an expression program: tables before each p sweep
   // p work lists: one for the plain p variant and one for the Caldeira-Leggett p variant
   // (a tile spans block * R / P columns, which differs between the two variant tables;
   // a CL plan also runs plain p sweeps: the merged Strang first half step, Strang + GP
@@ -423,11 +425,12 @@ namespace {
 
 // An expression program: length, op codes and the stack depth it reaches (the device
 // evaluator trusts all three).
-int check_expr(const double* code, int n, int member) {
+// peak (optional): the deepest stack the program reaches.
+int check_expr(const double* code, int n, int member, int* peak = nullptr) {
   if (!code || n < 1 || n > WH_EXPR_MAX)
     return fail(WH_INVALID_DESCRIPTOR, "member %d: expression program %p of %d ops (1..%d)", member,
                 (const void*)code, n, WH_EXPR_MAX);
-  int depth = 0;
+  int depth = 0, top = 0;
   for (int i = 0; i < n; ++i) {
     const double opd = code[2 * i];
     // range first (NaN fails it), then integrality
@@ -438,8 +441,10 @@ int check_expr(const double* code, int n, int member) {
     else if (op >= WH_EXPR_ADD && op <= WH_EXPR_POW) --depth;
     if (depth < 1 || depth > WH_EXPR_STACK)
       return fail(WH_INVALID_DESCRIPTOR, "member %d: expression stack depth %d at op %d", member, depth, i);
+    top = depth > top ? depth : top;
   }
   if (depth != 1) return fail(WH_INVALID_DESCRIPTOR, "member %d: expression leaves %d values", member, depth);
+  if (peak) *peak = top;
   return WH_SUCCESS;
 }
 
@@ -604,8 +609,13 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
     if (md.pot_deg == WH_POT_TABLE || md.pot_deg == WH_POT_EXPR) {
       const double zero_row[WH_MAX_DEG + 1] = {};
       WH_TRY(b->pot.upload(zero_row, sizeof(zero_row)));
-      if (md.pot_deg == WH_POT_TABLE) WH_TRY(b->pot_dv.upload(md.pot_dv, X * P * sizeof(double)));
-      else WH_TRY(b->pot_expr.upload(md.pot_expr, size_t(md.pot_expr_len) * 2 * sizeof(double)));
+      if (md.pot_deg == WH_POT_TABLE) {
+        WH_TRY(b->pot_dv.upload(md.pot_dv, X * P * sizeof(double)));
+      } else {
+        // two [P][X] tables, filled on the device before each p sweep (time index, next index)
+        WH_TRY(b->pot_expr.upload(md.pot_expr, size_t(md.pot_expr_len) * 2 * sizeof(double)));
+        WH_TRY(b->pot_dv.alloc(2 * X * P * sizeof(double)));
+      }
     } else {
       WH_TRY(b->pot.upload(md.pot_coef, size_t(md.pot_rows) * (WH_MAX_DEG + 1) * sizeof(double)));
     }
@@ -664,8 +674,10 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
     m.pot_rows = md.pot_deg < 0 ? 1 : md.pot_rows;
     m.pot_expr = b->pot_expr.as<double>();
     m.pot_expr_len = md.pot_deg == WH_POT_EXPR ? md.pot_expr_len : 0;
+    m.pot_expr_depth = 0;
+    if (md.pot_deg == WH_POT_EXPR) WH_TRY(check_expr(md.pot_expr, md.pot_expr_len, i, &m.pot_expr_depth));
     m.pot_t_step = md.pot_t_step;
-    m.pot_deg = md.pot_deg;
+    m.pot_deg = md.pot_deg == WH_POT_EXPR ? WH_POT_TABLE : md.pot_deg;  // the sweeps read its tables
     m.dx = md.dx;
     m.dp = md.dp;
     m.dtheta = md.dtheta;
@@ -694,13 +706,7 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
   WH_TRY(pick(wh::kXVariants, wh::kNumX, true, e->xvar));
   WH_TRY(pick(wh::kPVariants, wh::kNumP, false, e->pvar));
   WH_TRY(pick(wh::kPCLVariants, wh::kNumPCL, false, e->pclvar, true));
-  for (int i = 0; i < n; ++i) e->expr = e->expr || e->mem[i].pot_deg == WH_POT_EXPR;
-  if (e->expr) {
-    if (!fits_all(e.get(), wh::kPVariants[wh::kExprPVariant], false) ||
-        !fits_all(e.get(), wh::kPCLVariants[wh::kExprPVariant], false, true))
-      return fail(WH_INVALID_SIZE, "a grid does not fit the p variant of expression-program potentials");
-    e->pvar = e->pclvar = wh::kExprPVariant;
-  }
+  for (int i = 0; i < n; ++i) e->expr = e->expr || e->mem[i].pot_expr_len > 0;
   if (gp) {
     const wh::SweepVariant g{16, 256, 0};
     for (int i = 0; i < n; ++i) {
@@ -790,9 +796,22 @@ int psweep(wh_ensemble* e, int flags, double frac, int step_add, hipStream_t s) 
   if (!same_variant(launched, cl ? e->built_pcl : e->built_p))
     return fail(WH_INTERNAL_ERROR, "p work list cut for another variant");
   a.work = wl.buf.as<unsigned>();
-  WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_psweep_f64(v, cl, e->expr, a, wl.n, s)
-            : e->d.precision == WH_C64_WIDE ? wh::launch_psweep_f32w(v, cl, e->expr, a, wl.n, s)
-                                      : wh::launch_psweep_f32(v, cl, e->expr, a, wl.n, s),
+  if (e->expr) {
+    // expression-program members: their [P][X] tables at the time index this sweep reads
+    // (and the next one for the merged Strang half step)
+    const bool arg = (flags & wh::kStepArg) != 0;
+    for (int i = 0; i < e->n; ++i) {
+      const wh::Member& m = e->mem[i];
+      if (m.pot_expr_len <= 0) continue;
+      WH_LAUNCH(wh::launch_expr_table(e->mem_dev.as<wh::Member>(), i, m.X * m.P, m.pot_expr_depth,
+                                      e->step.as<int>(), step_add, arg ? 1 : 0, (flags & wh::kMerge) ? 2 : 1,
+                                      a.hbar_half, s),
+                "expr_table");
+    }
+  }
+  WH_LAUNCH(e->d.precision == WH_C128      ? wh::launch_psweep_f64(v, cl, a, wl.n, s)
+            : e->d.precision == WH_C64_WIDE ? wh::launch_psweep_f32w(v, cl, a, wl.n, s)
+                                      : wh::launch_psweep_f32(v, cl, a, wl.n, s),
             "psweep");
   return WH_SUCCESS;
 }
@@ -1058,7 +1077,6 @@ extern "C" int wh_ensemble_set_variants(wh_ensemble* e, int xvar, int pvar, int 
   if (e->gp() && xvar >= 0 && xvar != e->xvar)
     return fail(WH_INVALID_VALUE, "the x variant of a Gross-Pitaevskii plan is fixed at creation");
   if (xvar >= 0) e->xvar = xvar;
-  if (e->expr) pvar = pclvar = -1;  // expression-program plans keep their p variant
   if (pvar >= 0) e->pvar = pvar;
   if (pclvar >= 0) e->pclvar = pclvar;
   return build_work_lists(e);

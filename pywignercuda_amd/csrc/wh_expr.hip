@@ -1,0 +1,109 @@
+// Tables of a time-dependent potential given as an expression program (pot_deg WH_POT_EXPR
+// on the host): before each p sweep, the Bopp difference V(x - hbar theta/2, t) -
+// V(x + hbar theta/2, t) of every cell at the sweep's time index (and the next one, for the
+// merged Strang half step) -- the per-cell evaluation of expPotential (GPU_Wigner2D_FFT.py:
+// 88-94) as a kernel of its own, so that the p sweep reads a table as for a time-independent
+// non-polynomial V and keeps its register budget.
+#include <hip/hip_runtime.h>
+
+#include "wh_sweeps.hpp"
+
+namespace wh {
+
+constexpr int kExprBlock = 256;
+
+// V(x1, t) - V(x2, t) in one pass over the program: both stacks advance together (the same
+// ops), so each op is dispatched once and the two evaluations overlap.  The tops of the
+// stacks live in registers; slot s below them: st[s * kExprBlock] (x1) and
+// st[off + s * kExprBlock] (x2), off = (depth - 1) * kExprBlock.
+__device__ __forceinline__ double eval_expr_diff(cptr<double> code, int n, double x1, double x2, double t,
+                                                 double* st, int off) {
+  constexpr int LS = kExprBlock;
+  const int OFF = off;
+  double a = 0.0, b = 0.0;  // tops of the two stacks
+  int sp = 0;
+  for (int i = 0; i < n; ++i) {
+    const int op = int(code[2 * i]);
+    const double v = code[2 * i + 1];
+    if (op <= 2) {
+      if (sp > 0) {
+        st[(sp - 1) * LS] = a;
+        st[OFF + (sp - 1) * LS] = b;
+      }
+      a = op == 0 ? v : op == 1 ? x1 : t;
+      b = op == 0 ? v : op == 1 ? x2 : t;
+      ++sp;
+    } else if (op == 3) {
+      a = -a;
+      b = -b;
+    } else if (op <= 8) {
+      const double p = st[(sp - 2) * LS], q = st[OFF + (sp - 2) * LS];
+      --sp;
+      switch (op) {
+        case 4: a = p + a; b = q + b; break;
+        case 5: a = p - a; b = q - b; break;
+        case 6: a = p * a; b = q * b; break;
+        case 7: a = p / a; b = q / b; break;
+        default: a = pow(p, a); b = pow(q, b); break;
+      }
+    } else {
+      switch (op) {
+        case 9: a = exp(a); b = exp(b); break;
+        case 10: a = sin(a); b = sin(b); break;
+        case 11: a = cos(a); b = cos(b); break;
+        case 12: a = tan(a); b = tan(b); break;
+        case 13: a = sqrt(a); b = sqrt(b); break;
+        case 14: a = fabs(a); b = fabs(b); break;
+        case 15: a = tanh(a); b = tanh(b); break;
+        case 16: a = sinh(a); b = sinh(b); break;
+        case 17: a = cosh(a); b = cosh(b); break;
+        case 18: a = log(a); b = log(b); break;
+        case 19: a = atan(a); b = atan(b); break;
+        default: a = a < 0.0 ? 0.0 : 1.0; b = b < 0.0 ? 0.0 : 1.0; break;  // Heaviside
+      }
+    }
+  }
+  return a - b;
+}
+
+// Table k (k < ntables) of member `member` at time index step + k, where step is step_add
+// (use_arg) or the device counter plus step_add, as the p sweep reads it.  One thread per cell,
+// grid-stride; the stack slots of the threads interleave in LDS.
+// depth: the deepest stack of the program (the LDS holds 2 (depth - 1) slots per thread).
+__global__ void __launch_bounds__(kExprBlock) expr_table_kernel(const Member* __restrict__ members, int member,
+                                                                int depth, const int* __restrict__ step,
+                                                                int step_add, int use_arg, int ntables,
+                                                                double hbar_half) {
+  extern __shared__ double st[];
+  const int off = (depth > 1 ? depth - 1 : 1) * kExprBlock;
+  const Member& m = members[member];
+  const int X = m.X, P = m.P;
+  const size_t n = size_t(X) * size_t(P);
+  const int s0 = use_arg ? step_add : *step + step_add;
+  const double hd = hbar_half * m.dtheta;
+  double* tab = const_cast<double*>(m.pot_dv);
+  const cptr<double> code = cmem(m.pot_expr);
+  const int len = m.pot_expr_len;
+  const int lx = __ffs(X) - 1;  // X is a power of two
+  const int cells = int(n);     // at most 2048^2
+  for (int c = int(blockIdx.x) * kExprBlock + int(threadIdx.x); c < cells; c += int(gridDim.x) * kExprBlock) {
+    const int i = c >> lx, col = c & (X - 1);
+    const double x = m.dx * double(fidx(col, X)), u = hd * double(fidx(i, P));
+    for (int k = 0; k < ntables; ++k) {
+      const double t = double(s0 + k) * m.pot_t_step;
+      tab[size_t(k) * n + size_t(c)] = eval_expr_diff(code, len, x - u, x + u, t, st + threadIdx.x, off);
+    }
+  }
+}
+
+hipError_t launch_expr_table(const Member* members, int member, int cells, int depth, const int* step,
+                             int step_add, int use_arg, int ntables, double hbar_half, hipStream_t s) {
+  if (depth < 1 || depth > kExprStack) return hipErrorInvalidValue;
+  const int blocks = (cells + kExprBlock - 1) / kExprBlock;
+  const size_t lds = size_t(2) * size_t(depth > 1 ? depth - 1 : 1) * kExprBlock * sizeof(double);
+  hipLaunchKernelGGL(expr_table_kernel, dim3(blocks < 8192 ? blocks : 8192), dim3(kExprBlock), lds, s, members,
+                     member, depth, step, step_add, use_arg, ntables, hbar_half);
+  return hipGetLastError();
+}
+
+}  // namespace wh

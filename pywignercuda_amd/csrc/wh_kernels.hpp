@@ -88,7 +88,7 @@ struct Member {
   double* gcols;               // [X][kMaxDeg+1]
   // V(t, x) as an expression program (pot_deg = kPotExpr): (op, value) pairs, t = step * pot_t_step
   const double* pot_expr;
-  int pot_expr_len, pad3;
+  int pot_expr_len, pot_expr_depth;  // ops; the deepest stack the program reaches
   double pot_t_step;
   int obs_base, pad0, pad1, pad2;
 };

@@ -29,50 +29,11 @@
 namespace wh {
 
 // ------------------------------------------------------------------ helpers
-// V(t, x) of an expression program (the op codes of include/wignerhip.h, WH_EXPR_*): a
-// stack machine over (op, value) pairs; the host checks the program (length, stack depth)
-// when the plan is created.  Device math functions of the HIP runtime.
-__device__ __noinline__ double eval_expr(const double* __restrict__ code, int n, double x, double t) {
-  double st[kExprStack];
-  int sp = 0;
-  for (int i = 0; i < n; ++i) {
-    const int op = int(code[2 * i]);
-    const double v = code[2 * i + 1];
-    if (op <= 2) {
-      st[sp++] = op == 0 ? v : op == 1 ? x : t;
-    } else if (op <= 3) {
-      st[sp - 1] = -st[sp - 1];
-    } else if (op <= 8) {
-      const double b = st[--sp], a = st[sp - 1];
-      st[sp - 1] = op == 4 ? a + b : op == 5 ? a - b : op == 6 ? a * b : op == 7 ? a / b : pow(a, b);
-    } else {
-      const double a = st[sp - 1];
-      double r;
-      switch (op) {
-        case 9: r = exp(a); break;
-        case 10: r = sin(a); break;
-        case 11: r = cos(a); break;
-        case 12: r = tan(a); break;
-        case 13: r = sqrt(a); break;
-        case 14: r = fabs(a); break;
-        case 15: r = tanh(a); break;
-        case 16: r = sinh(a); break;
-        case 17: r = cosh(a); break;
-        case 18: r = log(a); break;
-        case 19: r = atan(a); break;
-        default: r = a < 0.0 ? 0.0 : 1.0; break;  // Heaviside
-      }
-      st[sp - 1] = r;
-    }
-  }
-  return st[0];
-}
-
-// The same stack machine inlined, for lanes that own a slice of LDS: the top of the stack in a
-// register, the values below it at st[s * LS] (LS lanes share the slice, so that they touch
-// consecutive words).  Used between two transforms, when the team's LDS row is free.
-// The program is read through the constant address space: its addresses are uniform, so
-// the ops come from scalar loads rather than a vector memory round trip each.
+// V(t, x) of an expression program (the op codes of include/wignerhip.h, WH_EXPR_*; the host
+// checks length, op codes and stack depth when the plan is created): a stack machine with the
+// top of the stack in a register and the values below it at st[s * LS] (LS threads interleave
+// their slots in LDS).  The program is read through the constant address space: its addresses
+// are uniform, so the ops come from scalar loads.  Device math functions of the HIP runtime.
 template <int LS>
 __device__ __forceinline__ double eval_expr_lds(cptr<double> code, int n, double x, double t, double* st) {
   double tos = 0.0;
@@ -499,48 +460,15 @@ struct PSweep {
 
   // potential step multiplier of time index `step` (expPotential; GP phase optional),
   // times the real factor `scale` (the 1/N of the inverse transform that follows).
-  // EXPR: the kernel also evaluates expression programs (pot_deg = kPotExpr); only the
-  // kernels launched for plans with such a V carry the evaluator.  Wave teams evaluate it
-  // inline with the stack in their own LDS row (team_lds, free between two transforms);
-  // block-wide teams call the out-of-line evaluator.
-  template <bool EXPR>
+  // tab: which [P][X] table a member with an expression program reads (0: time index `step`,
+  // 1: `step` + 1, the merged Strang half step; expr_table_kernel fills them before the sweep)
   __device__ __forceinline__ static void potential(cplx<T> (&v)[R], const SweepArgs& a,
                                                    const Member& m, int t_in, int col_in, int step,
-                                                   double scale, cplx<E>* team_lds) {
+                                                   double scale, int tab) {
     // opaque lane indices: recomputing theta and x -/+ hbar theta/2 per call is cheaper
     // than keeping them live across the transforms between two calls
     const int t = opaque_v(t_in), col = opaque_v(col_in);
     const double x = m.dx * double(fidx(col, m.X));
-    if constexpr (EXPR && kWave) {
-      if (m.pot_deg == kPotExpr) {
-        // kExprStack slots per lane inside the team's row
-        static_assert(kExprStack * TT <= LSZ * int(sizeof(cplx<E>) / sizeof(double)), "expression stack in LDS");
-        double* st = reinterpret_cast<double*>(team_lds) + t;
-        const double smask = (m.xmask ? gmem(m.xmask)[col] : 1.0) * scale;
-        const bool gpe = (a.flags & kGP) != 0;
-        const double hd = a.hbar_half * m.dtheta, tt = double(step) * m.pot_t_step;
-        Sync::sync();  // the transform's last reads of the row are done
-#pragma unroll 1
-        for (int j = 0; j < R; ++j) {
-          const int i = t + j * TT;
-          const double u = hd * double(fidx(i, N));
-          double phi = a.vphase * (eval_expr_lds<TT>(cmem(m.pot_expr), m.pot_expr_len, x - u, tt, st) -
-                                   eval_expr_lds<TT>(cmem(m.pot_expr), m.pot_expr_len, x + u, tt, st));
-          if (gpe) phi -= a.gp_phase * gmem(m.G)[size_t(i) * m.X + col];
-          const cplx<double> M = expmi(smask, phi);
-          static_for<0, R>([&](auto J) {
-            constexpr int jj = decltype(J)::value;
-            if (jj == j) v[jj] = apply(v[jj], M);
-          });
-        }
-        Sync::sync();  // the stack is dead before the next transform writes the row
-        if (m.theta_decay) {
-#pragma unroll
-          for (int j = 0; j < R; ++j) v[j] = apply(v[j], cplx<double>{gmem(m.theta_decay)[t + j * TT], 0.0});
-        }
-        return;
-      }
-    }
     const int row = step < m.pot_rows ? step : m.pot_rows - 1;
     const cptr<double> cf = cmem(m.pot) + size_t(row) * (kMaxDeg + 1);
     const double mask = (m.xmask ? gmem(m.xmask)[col] : 1.0) * scale;
@@ -584,12 +512,9 @@ struct PSweep {
         double phi;
         if (m.pot_deg >= 0) {
           phi = bopp(hd * double(fidx(i, N)));
-        } else if (EXPR && m.pot_deg == kPotExpr) {
-          const double u = hd * double(fidx(i, N)), tt = double(step) * m.pot_t_step;
-          phi = a.vphase * (eval_expr(m.pot_expr, m.pot_expr_len, x - u, tt) -
-                            eval_expr(m.pot_expr, m.pot_expr_len, x + u, tt));
         } else {
-          phi = a.vphase * gmem(m.pot_dv)[size_t(i) * m.X + col];
+          const size_t toff = (tab && m.pot_expr_len > 0) ? size_t(m.X) * size_t(m.P) : 0;
+          phi = a.vphase * gmem(m.pot_dv)[toff + size_t(i) * m.X + col];
         }
         if (gp) phi -= a.gp_phase * gmem(m.G)[size_t(i) * m.X + col];
         v[j] = apply(v[j], expmi(mask, phi));
@@ -624,7 +549,7 @@ constexpr int kPTailBytes = 64 * 12 * 8;
 // teams the first exchange of the first transform moves each column to one team of
 // TT consecutive lanes (column cc = tid / TT, team index tc = tid % TT) and the store
 // moves it back through LDS; otherwise the load layout is kept throughout.
-template <typename S, typename T, int N, int R, int BLOCK, bool CL, bool TP, bool EXPR, typename E>
+template <typename S, typename T, int N, int R, int BLOCK, bool CL, bool TP, typename E>
 __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m, int tile,
                                             cplx<E>* lds) {
   using PS = PSweep<T, N, R, BLOCK, E>;
@@ -668,7 +593,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
   } else {
     team_fft<T, N, R, -1, View, Sync>(v, tc, view_c, twsrc);
   }
-  PS::template potential<EXPR>(v, a, m, tc, col, step, 1.0 / N, view_c.base);
+  PS::potential(v, a, m, tc, col, step, 1.0 / N, 0);
   team_fft<T, N, R, +1, View, Sync>(v, tc, view_c, twsrc);
 
   if constexpr (CL) {
@@ -765,7 +690,7 @@ __device__ __forceinline__ void psweep_body(const SweepArgs& a, const Member& m,
 
   if (a.flags & kMerge) {
     team_fft<T, N, R, -1, View, Sync>(v, tc, view_c, twsrc);
-    PS::template potential<EXPR>(v, a, m, tc, col, step + 1, 1.0 / N, view_c.base);
+    PS::potential(v, a, m, tc, col, step + 1, 1.0 / N, 1);
     team_fft<T, N, R, +1, View, Sync>(v, tc, view_c, twsrc);
   }
 
@@ -847,20 +772,20 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TP ?
 #undef BODY
 }
 
-template <typename S, typename T, int R, int BLOCK, bool CL, bool TP, bool EXPR>
+template <typename S, typename T, int R, int BLOCK, bool CL, bool TP>
 __device__ __forceinline__ void psweep_entry(const SweepArgs& a) {
   using E = LdsElem<S, T, TP>;
   __shared__ cplx<E> lds[lds_elems<BLOCK, R>() + kPTailBytes / sizeof(cplx<E>)];
   const unsigned w = a.work[blockIdx.x];
   const Member& m = a.members[w >> 16];
   const int tile = int(w & 0xffffu);
-#define BODY(n) psweep_body<S, T, n, R, BLOCK, CL, TP, EXPR, E>(a, m, tile, lds)
+#define BODY(n) psweep_body<S, T, n, R, BLOCK, CL, TP, E>(a, m, tile, lds)
   WH_DISPATCH_TO(m.P, BODY, TP && !CL ? kTpMaxN : 2048)
 #undef BODY
 }
-template <typename S, typename T, int R, int BLOCK, bool CL, bool TP, bool EXPR = false>
+template <typename S, typename T, int R, int BLOCK, bool CL, bool TP>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(CL ? 1 : TP ? 3 : 2))) psweep_kernel(SweepArgs a) {
-  psweep_entry<S, T, R, BLOCK, CL, TP, EXPR>(a);
+  psweep_entry<S, T, R, BLOCK, CL, TP>(a);
 }
 
 template <int R, int BLOCK>

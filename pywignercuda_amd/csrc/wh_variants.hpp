@@ -21,10 +21,6 @@ constexpr int kNumX = sizeof(kXVariants) / sizeof(kXVariants[0]);
 constexpr int kNumP = sizeof(kPVariants) / sizeof(kPVariants[0]);
 constexpr int kNumPCL = sizeof(kPCLVariants) / sizeof(kPCLVariants[0]);
 
-// the p variant (plain and Caldeira-Leggett) of plans whose V is an expression program:
-// it fits every supported grid side
-constexpr int kExprPVariant = 0;
-
 // largest grid side of the plain (x and p) throughput forms
 constexpr int kTpMaxN = 1024;
 
