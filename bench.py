@@ -431,7 +431,7 @@ def main():
     L.wh_variant_info(table, vi, ctypes.byref(rr), ctypes.byref(bb))
     tp = vi == L.wh_variant_info(table, -1, None, None) - 1
     kname = (f"psweep_kernel<double, double, {rr.value}, {bb.value}, {'true' if cl else 'false'}, "
-             f"{'true' if tp else 'false'}, false>")  # last: no expression-program potential
+             f"{'true' if tp else 'false'}>")
     # the dominant kernels alone first (on a copy of the state): their launches also bring the
     # GPU out of idle before the headline run, as in a running application
     Wk = W.clone()
