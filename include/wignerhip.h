@@ -228,9 +228,13 @@ typedef struct {
   int odm_channels;
   double odm_weight1, odm_weight2;
   /* pot_deg = WH_POT_EXPR: the program of V(t, x) (pot_expr_len pairs, host memory) and the
-     time per step index (the host dt, as the reference's t = tIndex*dt) */
+     time per step index (the host dt, as the reference's t = tIndex*dt).  pot_expr_kvn = 1:
+     the program is dV/dx(t, x) and the p sweep takes the classical KvN phase
+     -dt dV/dx(t, x) theta instead of the Bopp difference (exp_PotentialKvN,
+     GPU_Wigner2D_FFT.py:27-75) */
   const double* pot_expr;
   int pot_expr_len;
+  int pot_expr_kvn;
   double pot_t_step;
 } wh_wigner2d_desc;
 
@@ -333,6 +337,7 @@ typedef struct {
   double odm_sign;
   const double* pot_expr; /* pot_deg = WH_POT_EXPR: as in wh_wigner2d_desc */
   int pot_expr_len;
+  int pot_expr_kvn;
   double pot_t_step;
 } wh_member_desc;
 

@@ -408,8 +408,8 @@ class GPU_FokkerPlank2D_FFT(Propagator_Base):
     exp(i dt dV(x) theta) exp(-D_Theta theta^2 dt) and the edge mask of exp_PotentialKvN
     (:27-75) in place of the Bopp difference of V, the Caldeira-Leggett dissipator when
     gammaDamping != 0 and dampingFunction == 'CaldeiraLeggett', and the renormalisation when
-    gammaDamping != 0 and normalization == 'Wigner' (:2403-2410).  dPotentialString must not
-    depend on t.  The fields the reference reads off ``self`` (dampingFunction,
+    gammaDamping != 0 and normalization == 'Wigner' (:2403-2410).  A dPotentialString that
+    depends on t is evaluated per cell and step on the device.  The fields the reference reads off ``self`` (dampingFunction,
     fp_Damping_String, SetTimeTrack) are set as for GPU_Wigner2D_FFT; no kinematicString is
     needed."""
 

@@ -114,7 +114,7 @@ class Wigner2DDesc(ctypes.Structure):
         ("dxdp", c_double), ("inv_2m", c_double), ("dx_host", c_double),
         ("pot_dv", _DP), ("odm_g", _DP), ("odm_x", _DP), ("odm_th", _DP), ("odm_sign", c_double),
         ("odm_channels", c_int), ("odm_weight1", c_double), ("odm_weight2", c_double),
-        ("pot_expr", _DP), ("pot_expr_len", c_int), ("pot_t_step", c_double),
+        ("pot_expr", _DP), ("pot_expr_len", c_int), ("pot_expr_kvn", c_int), ("pot_t_step", c_double),
     ]
 
 
@@ -142,7 +142,7 @@ class MemberDesc(ctypes.Structure):
         ("x_host", _DP), ("p_host", _DP), ("dv_host", _DP), ("v0_host", _DP),
         ("dxdp", c_double), ("inv_2m", c_double), ("dx_host", c_double), ("gp_rho_factor", c_double),
         ("pot_dv", _DP), ("odm_g", _DP), ("odm_x", _DP), ("odm_th", _DP), ("odm_sign", c_double),
-        ("pot_expr", _DP), ("pot_expr_len", c_int), ("pot_t_step", c_double),
+        ("pot_expr", _DP), ("pot_expr_len", c_int), ("pot_expr_kvn", c_int), ("pot_t_step", c_double),
     ]
 
 

@@ -675,6 +675,7 @@ int ensemble_create(const wh_ensemble_desc* desc, const wh_member_desc* members,
     m.pot_expr = b->pot_expr.as<double>();
     m.pot_expr_len = md.pot_deg == WH_POT_EXPR ? md.pot_expr_len : 0;
     m.pot_expr_depth = 0;
+    m.pot_expr_kvn = md.pot_deg == WH_POT_EXPR && md.pot_expr_kvn ? 1 : 0;
     if (md.pot_deg == WH_POT_EXPR) WH_TRY(check_expr(md.pot_expr, md.pot_expr_len, i, &m.pot_expr_depth));
     m.pot_t_step = md.pot_t_step;
     m.pot_deg = md.pot_deg == WH_POT_EXPR ? WH_POT_TABLE : md.pot_deg;  // the sweeps read its tables
@@ -1120,6 +1121,7 @@ extern "C" int wh_wigner2d_create(const wh_wigner2d_desc* d, wh_wigner2d** out) 
   m.pot_dv = d->pot_dv;
   m.pot_expr = d->pot_expr;
   m.pot_expr_len = d->pot_expr_len;
+  m.pot_expr_kvn = d->pot_expr_kvn;
   m.pot_t_step = d->pot_t_step;
   m.odm_g = d->odm_g;
   m.odm_x = d->odm_x;

@@ -16,8 +16,9 @@ constexpr int kExprBlock = 256;
 // ops), so each op is dispatched once and the two evaluations overlap.  The tops of the
 // stacks live in registers; slot s below them: st[s * kExprBlock] (x1) and
 // st[off + s * kExprBlock] (x2), off = (depth - 1) * kExprBlock.
+// single: return V(x1, t) alone.
 __device__ __forceinline__ double eval_expr_diff(cptr<double> code, int n, double x1, double x2, double t,
-                                                 double* st, int off) {
+                                                 double* st, int off, bool single) {
   constexpr int LS = kExprBlock;
   const int OFF = off;
   double a = 0.0, b = 0.0;  // tops of the two stacks
@@ -63,7 +64,7 @@ __device__ __forceinline__ double eval_expr_diff(cptr<double> code, int n, doubl
       }
     }
   }
-  return a - b;
+  return single ? a : a - b;
 }
 
 // Table k (k < ntables) of member `member` at time index step + k, where step is step_add
@@ -84,6 +85,7 @@ __global__ void __launch_bounds__(kExprBlock) expr_table_kernel(const Member* __
   double* tab = const_cast<double*>(m.pot_dv);
   const cptr<double> code = cmem(m.pot_expr);
   const int len = m.pot_expr_len;
+  const bool kvn = m.pot_expr_kvn != 0;
   const int lx = __ffs(X) - 1;  // X is a power of two
   const int cells = int(n);     // at most 2048^2
   for (int c = int(blockIdx.x) * kExprBlock + int(threadIdx.x); c < cells; c += int(gridDim.x) * kExprBlock) {
@@ -91,7 +93,10 @@ __global__ void __launch_bounds__(kExprBlock) expr_table_kernel(const Member* __
     const double x = m.dx * double(fidx(col, X)), u = hd * double(fidx(i, P));
     for (int k = 0; k < ntables; ++k) {
       const double t = double(s0 + k) * m.pot_t_step;
-      tab[size_t(k) * n + size_t(c)] = eval_expr_diff(code, len, x - u, x + u, t, st + threadIdx.x, off);
+      // Bopp difference V(x - u) - V(x + u), or the KvN phase -2 u dV/dx(x) (u = hbar theta / 2)
+      tab[size_t(k) * n + size_t(c)] =
+          kvn ? -2.0 * u * eval_expr_diff(code, len, x, x, t, st + threadIdx.x, off, true)
+              : eval_expr_diff(code, len, x - u, x + u, t, st + threadIdx.x, off, false);
     }
   }
 }

@@ -90,7 +90,9 @@ struct Member {
   const double* pot_expr;
   int pot_expr_len, pot_expr_depth;  // ops; the deepest stack the program reaches
   double pot_t_step;
-  int obs_base, pad0, pad1, pad2;
+  int obs_base;
+  int pot_expr_kvn;            // the program is dV/dx: the table holds the KvN phase -hbar theta dV/dx
+  int pad1, pad2;
 };
 
 struct SweepArgs {

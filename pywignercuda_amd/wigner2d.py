@@ -109,6 +109,7 @@ class _Tables:
 
         self.pot_dv = None
         self.pot_expr = None
+        self.pot_expr_kvn = 0
         self.pot_t_step = float(dt)
         self.x_mask = None
         if mask:
@@ -121,12 +122,16 @@ class _Tables:
             # table read by the p sweep: phase = dt/hbar * pot_dv, pot_dv = -hbar theta dV(x)
             if scheme != "lie" or pot_table is not None or not dpotential:
                 raise ValueError("the KvN phase belongs to the Lie scheme and needs dpotential")
-            if depends_on_time(dpotential, "x"):
-                raise NotImplementedError("a time-dependent dPotential in the KvN (Fokker-Planck) propagator")
-            dvk = np.broadcast_to(np.asarray(host_eval(dpotential, t=0.0, x=kx), dtype=np.float64), kx.shape)
-            self.pot_dv = np.ascontiguousarray(-self.k_hbar * ktheta[:, None] * dvk[None, :])
-            self.time_dependent = False
-            deg, tab = WH_POT_TABLE, np.zeros((1, WH_MAX_DEG + 1))
+            self.time_dependent = depends_on_time(dpotential, "x")
+            if self.time_dependent:
+                # dPotential(t, x) as a program: the device tabulates -hbar theta dV/dx(t, x) per step
+                self.pot_expr, self.pot_expr_kvn = compile_expression(dpotential, "x"), 1
+                deg = WH_POT_EXPR
+            else:
+                dvk = np.broadcast_to(np.asarray(host_eval(dpotential, t=0.0, x=kx), dtype=np.float64), kx.shape)
+                self.pot_dv = np.ascontiguousarray(-self.k_hbar * ktheta[:, None] * dvk[None, :])
+                deg = WH_POT_TABLE
+            tab = np.zeros((1, WH_MAX_DEG + 1))
         elif pot_table is not None:
             # explicit device coefficients: the degree is the highest non-zero column
             tab = np.ascontiguousarray(pot_table, dtype=np.float64)
@@ -140,7 +145,7 @@ class _Tables:
                 deg, tab = coefficient_table(potential, "x", times)
             except NonPolynomialPhysics:
                 if self.time_dependent:
-                    # arbitrary V(t, x): a program the p sweep evaluates per cell at t = step * dt, as
+                    # arbitrary V(t, x): a program evaluated per cell on the device at t = step * dt, as
                     # expPotential evaluates the spliced string (FFT.py:88-94); NonPolynomialPhysics for
                     # syntax the device evaluator does not have
                     self.pot_expr = compile_expression(potential, "x")
@@ -216,6 +221,7 @@ class _Tables:
         d.pot_dv = self.ptr(self.pot_dv)
         d.pot_expr = self.ptr(self.pot_expr)
         d.pot_expr_len = 0 if self.pot_expr is None else int(self.pot_expr.shape[0])
+        d.pot_expr_kvn = self.pot_expr_kvn
         d.pot_t_step = self.pot_t_step
         d.odm_g, d.odm_x, d.odm_th = self.ptr(self.odm_g), self.ptr(self.odm_x), self.ptr(self.odm_th)
         d.odm_sign = self.odm_sign
@@ -314,8 +320,8 @@ class Wigner2D(_PlanBase):
     pot_table   : explicit [rows, 9] coefficient table for the device (the string
                   is still used for the host observables).
     kvn         : classical Koopman-von Neumann (Fokker-Planck) potential step
-                  exp(i dt dV(x) theta) in place of the Bopp difference of V
-                  (GPU_FokkerPlank2D_FFT, FFT.py:27-75, 2242-2459); Lie scheme, time-independent dV.
+                  exp(i dt dV(t, x) theta) in place of the Bopp difference of V
+                  (GPU_FokkerPlank2D_FFT, FFT.py:27-75, 2242-2459); Lie scheme.
     renormalize : divide by dXdP sum(W) after each step; None = when gamma != 0.
     variants    : (x, p, p+CL) kernel variant indices; None = tuned choice.
     """

@@ -259,10 +259,11 @@ def test_kvn_phase_tabulated_like_oracle():
     m = np.cos(phase) - 1j * np.sin(phase)
     m = m * t.theta_decay[:, None] * t.x_mask[None, :]
     np.testing.assert_allclose(m, orc.potential_multiplier(0.0), rtol=0, atol=1e-14)
-    with pytest.raises(NotImplementedError):
-        _Tables(X, P, 12.0, 9.0, hbar, 1.0, dt, potential="x*x", dpotential="2*x*cos(t)", scheme="lie", D_theta=0,
-                D_lambda=0, gamma=0, damping=None, fp_damping=None, gp_coeff=0, gp_rho_factor=None, mask=True,
-                truncate_constants=True, time_rows=None, pot_table=None, kvn=True)
+    # a time-dependent dPotential: its program, tabulated on the device per step in the KvN form
+    t3 = _Tables(X, P, 12.0, 9.0, hbar, 1.0, dt, potential="x*x", dpotential="2*x*cos(t)", scheme="lie", D_theta=0,
+                 D_lambda=0, gamma=0, damping=None, fp_damping=None, gp_coeff=0, gp_rho_factor=None, mask=True,
+                 truncate_constants=True, time_rows=None, pot_table=None, kvn=True)
+    assert t3.pot_deg == -2 and t3.pot_expr_kvn == 1 and t3.time_dependent and not t3.time_table
 
 
 @pytest.mark.parametrize("expr", [

@@ -324,6 +324,32 @@ def test_kvn_fokker_planck(cuda, precision, bound):
         check_obs(obs, obs_ref)
 
 
+def test_kvn_fokker_planck_driven(cuda):
+    """KvN (Fokker-Planck) with a time-dependent force dPotential(t, x) (a driven double well):
+    the device tabulates -hbar theta dV/dx(t, x) per step, against the KvN oracle."""
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    X = P = 256
+    dt = 0.005
+    hip = Wigner2D(X, P, 10.0, 10.0, 1.0, 1.0, dt, potential="0.1*pow(x,4) - x*x",
+                   dpotential="0.4*pow(x,3) - 2*x + 0.5*sin(3*t)", D_theta=0.0143, gamma=0.07,
+                   damping="CaldeiraLeggett", fp_damping=" p*p/sqrt( p*p + 0.25  ) ", mask=True,
+                   truncate_constants=True, kvn=True, device=cuda)
+    assert hip.tables.pot_expr_kvn == 1
+    orc = ow.Wigner2DOracle(X, P, 10.0, 10.0, 1.0, 1.0, dt, D_theta=0.0143, gamma=0.07,
+                            potential=lambda t, x: 0.1 * x ** 4 - x * x,
+                            dpotential=lambda t, x: 0.4 * x ** 3 - 2 * x + 0.5 * np.sin(3 * t),
+                            fp_damping=lambda p: p * p / np.sqrt(p * p + 0.25), damping="CaldeiraLeggett",
+                            mask=True, truncate_constants=True, kvn=True, time_dependent=True)
+    W0 = ow.gaussian_wigner(orc.g, x0=1.5, p0=1.0, s=0.8)
+    W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 40)
+    err = rel_l2(W, W_ref)
+    print(f"KvN driven: rel L2 {err:.3e}")
+    assert err < 1e-10
+    # the dV record is the reference's dPotential(0, x) average (FFT.py:2310-2311)
+    check_obs(obs, obs_ref)
+
+
 @pytest.mark.parametrize("gp,hbar", [(0.0, 1.0), (0.5, 0.8)])
 def test_lie_odm_fp64(cuda, gp, hbar):
     """ODM Lindblad dissipator (order 1) after the V step, with and without the GP term."""
