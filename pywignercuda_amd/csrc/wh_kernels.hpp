@@ -17,7 +17,7 @@ constexpr int kExprMax = 64, kExprStack = 16;
 constexpr int kNCol = 5;    // column sums: Re S0, Im S0, sum p W, sum p^2 W, sum min(W,0)
 constexpr int kNObs = 16;   // per-step observable record
 constexpr int kColsumCols = 4;     // columns per column-sum tile (64 row groups of 4 lanes)
-constexpr int kScaleChunk = 2048;  // elements per block of the scale kernel
+constexpr int kScaleChunk = 512;   // elements per block of the scale kernel
 
 enum ObsIndex {
   kObsX = 0, kObsX2, kObsP, kObsP2, kObsXP, kObsDV, kObsPDV, kObsXDV, kObsH, kObsNeg,

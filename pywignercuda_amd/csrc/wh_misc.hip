@@ -191,10 +191,23 @@ __global__ void __launch_bounds__(256) scale_kernel(const Member* members, const
   const size_t chunk = size_t(w & 0xffffu) * kScaleChunk;
   const size_t n = size_t(m.X) * size_t(m.P);
   cplx<T>* W = reinterpret_cast<cplx<T>*>(m.W);
+  // all of this thread's loads in flight before the factor arrives and the stores start
+  static_assert(kScaleChunk % 256 == 0, "whole elements per thread");
+  constexpr int kPer = kScaleChunk / 256;
+  cplx<T> v[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const size_t i = chunk + threadIdx.x + size_t(k) * 256;
+    if (i < n) v[k] = W[i];
+  }
   const double sr = m.scale[0], si = m.scale[1];
-  for (size_t i = chunk + threadIdx.x; i < chunk + kScaleChunk && i < n; i += 256) {
-    const double x = double(W[i].x), y = double(W[i].y);
-    W[i] = {T(x * sr - y * si), T(x * si + y * sr)};
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const size_t i = chunk + threadIdx.x + size_t(k) * 256;
+    if (i < n) {
+      const double x = double(v[k].x), y = double(v[k].y);
+      W[i] = {T(x * sr - y * si), T(x * si + y * sr)};
+    }
   }
 }
 
