@@ -504,8 +504,9 @@ typedef struct {
   int a_deg[3];
   double a_coef[3][WH_MAX_DEG + 1];
   /* ODM damping (dampingModel 'ODM': DampingODM_Kernel, GPU_WignerDirac2D_4x4.py:952-1195,
-     applied between the potential factors and Theta_To_P, :3608-3615); needs A = 0 and
-     gamma_dt = 0 (the two damping models exclude each other) */
+     applied between the potential factors and Theta_To_P, :3608-3615); needs gamma_dt = 0
+     (the two damping models exclude each other).  With A the factor sweeps leave the
+     planes in the theta picture and the ODM sweep transforms them back */
   int odm;                    /* 1 = on */
   double odm_gamma;           /* gammaDamping (kernel argument, full precision) */
   double lambda_bar;          /* lambdaBar (kernel constant) */

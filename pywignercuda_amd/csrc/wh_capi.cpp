@@ -1354,8 +1354,6 @@ int dirac_validate(const wh_dirac_desc* d) {
                   "the KvN variant takes V0 only (no vector potential, position-dependent mass, GP or damping)");
   }
   if (d->odm) {
-    if (d->a_deg[0] >= 0 || d->a_deg[1] >= 0 || d->a_deg[2] >= 0)
-      return fail(WH_INVALID_DESCRIPTOR, "ODM damping needs A1 = A2 = A3 = 0");
     if (d->gamma_dt != 0.0) return fail(WH_INVALID_DESCRIPTOR, "ODM and Caldeira-Leggett damping exclude each other");
     if (!(d->lambda_bar > 0.0)) return fail(WH_INVALID_DESCRIPTOR, "ODM damping needs lambda_bar > 0");
   }
@@ -1367,7 +1365,11 @@ int dirac_validate(const wh_dirac_desc* d) {
 int dirac_potential_sweep(wh_dirac* e, hipStream_t s) {
   const auto* dp = e->plan_dev.as<wh::DiracPlan>();
   const int prec = e->d.precision;
-  if (e->host.odm)
+  if (e->host.odm && e->host.general) {
+    // A and ODM: the two factor sweeps stay in the theta picture, the ODM sweep transforms back
+    WH_LAUNCH(wh::launch_dirac_pgeneral(prec, false, dp, e->pgblocks, s), "dirac_pgeneral");
+    WH_LAUNCH(wh::launch_dirac_podm(prec, dp, e->pgblocks, s), "dirac_podm");
+  } else if (e->host.odm)
     WH_LAUNCH(wh::launch_dirac_podm(prec, dp, e->pgblocks, s), "dirac_podm");
   else if (e->host.general)
     WH_LAUNCH(wh::launch_dirac_pgeneral(prec, e->d.gamma_dt != 0.0, dp, e->pgblocks, s), "dirac_pgeneral");

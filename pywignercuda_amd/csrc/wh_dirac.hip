@@ -528,17 +528,23 @@ __device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, 
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
+      const int k = 4 * ir + q;
+      const gptr<cplx<S>> W = W0 + size_t(k) * plane;
+      if (d.odm) {  // ODM damping follows in the theta picture (dirac_podm_body with A)
+#pragma unroll
+        for (int j = 0; j < R; ++j) store_stream(at_off(W + size_t(j * TT) * X, boff), to_arith<S>(v[q][j]));
+        continue;
+      }
       team_fft<T, N, R, +1>(v[q], t, view, tw);
 #pragma unroll
       for (int j = 0; j < R; ++j) v[q][j] = cscale(v[q][j], inv);
       if constexpr (CL) dirac_cl2<T, N, R>(v[q], d, t, view, tw);
-      const int k = 4 * ir + q;
       dirac_accumulate<T, R, TT>(d, k, v[q], t, acc, trre);
-      const gptr<cplx<S>> W = W0 + size_t(k) * plane;
 #pragma unroll
       for (int j = 0; j < R; ++j) store_stream(at_off(W + size_t(j * TT) * X, boff), to_arith<S>(v[q][j]));
     }
   }
+  if (d.odm) return;  // dirac_podm_body transforms back and writes the column sums
 #pragma unroll
   for (int j = 0; j < R; ++j) acc[kDcNeg] += trre[j] < 0.0 ? trre[j] : 0.0;
   dirac_store_colsums<BLOCK, C>(d, tile, acc, lds);
@@ -595,7 +601,8 @@ __device__ __forceinline__ void dirac_podm_body(const DiracPlan& d, int tile, cp
       const gptr<cplx<S>> W = W0 + size_t(kp[q]) * plane;
 #pragma unroll
       for (int j = 0; j < R; ++j) v[q][j] = to_arith<T>(gload(at_off(W + size_t(j * TT) * X, boff)));
-      team_fft<T, N, R, -1>(v[q], t, view, tw);
+      // with A the potential sweeps left the planes in the theta picture, factors applied
+      if (!d.general) team_fft<T, N, R, -1>(v[q], t, view, tw);
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -621,7 +628,7 @@ __device__ __forceinline__ void dirac_podm_body(const DiracPlan& d, int tile, cp
       }
       cplx<double> w[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) w[q] = cmul(widen(v[q][j]), expmi(r, ph + m[q]));
+      for (int q = 0; q < 4; ++q) w[q] = d.general ? widen(v[q][j]) : cmul(widen(v[q][j]), expmi(r, ph + m[q]));
       // ODM mixing of the quad (a = w0, b = w1, c = w2, d = w3)
       const double kt = d.odm_k * th;  // mc t g theta
       double st, ct;

@@ -109,8 +109,6 @@ class WignerDirac4x4:
         if gamma != 0.0 and damping not in (None, "CaldeiraLegget", "ODM"):
             raise NotImplementedError(f"dampingModel {damping!r} is not supported ('CaldeiraLegget' or 'ODM')")
         if gamma > 0.0 and damping == "ODM":
-            if max(a_deg) >= 0:
-                raise NotImplementedError("ODM damping together with a vector potential")
             if lambda_bar is None:
                 raise ValueError("ODM damping needs lambdaBar")
         self.precision = precision

@@ -314,6 +314,34 @@ def test_dirac_odm_matches_oracle(cuda, precision, mass_potential):
         assert err < 1e-5
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_dirac_odm_with_vector_potential(cuda, precision):
+    """ODM damping together with a vector potential (and a position-dependent mass): the two
+    factor sweeps leave the planes in the theta picture and the ODM sweep mixes the quads and
+    transforms back (D44.py:3576-3615), against the oracle."""
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    vs, vf, dvf = CASES["klein"]
+    pots = (vs, "0.05*x", " -x ", "0.1 + 0.02*pow(x,2)")
+    hip = WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=pots, D_theta=0.001, gamma=0.05,
+                         damping="ODM", lambda_bar=4.0, energy=True, mass_potential="1. + 0.1*x*x",
+                         kinetic_mass_half=0.0001, precision=precision, device=cuda)
+    orc = Dirac4x4Oracle(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01,
+                         [vf, lambda t, x: 0.05 * x, lambda t, x: -x, lambda t, x: 0.1 + 0.02 * x * x],
+                         dpotential0=dvf, D_theta=0.001, gamma=0.05, damping="ODM", lambda_bar=4.0,
+                         compute_energy=True, mass_potential=lambda t, x: 1.0 + 0.1 * x * x,
+                         kinetic_mass_half=0.0001)
+    W0 = wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 3.0, 1.0, Xg, 1.0, 1.0), orc)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 25)
+    err = rel_l2(W, W_ref)
+    print(f"Dirac ODM + A {precision}: rel L2 {err:.3e}")
+    if precision == "fp64":
+        assert err < 1e-10
+        check_obs(obs, rec)
+    else:
+        assert err < 1e-5
+
+
 def _odm_notebook(tmp_path, V0, steps, pX, D_Theta, epsilon):
     from pywignercuda_amd.GPU_WignerDirac2D_4x4 import GPU_WignerDirac2D_4x4
 
