@@ -3,7 +3,8 @@
 // V(x + hbar theta/2, t) of every cell at the sweep's time index (and the next one, for the
 // merged Strang half step) -- the per-cell evaluation of expPotential (GPU_Wigner2D_FFT.py:
 // 88-94) as a kernel of its own, so that the p sweep reads a table as for a time-independent
-// non-polynomial V and keeps its register budget.
+// non-polynomial V and keeps its register budget.  With pot_expr_kvn the program is dV/dx and
+// the table holds the classical KvN phase -hbar theta dV/dx(t, x) (exp_PotentialKvN, :27-75).
 #include <hip/hip_runtime.h>
 
 #include "wh_sweeps.hpp"
@@ -12,8 +13,10 @@ namespace wh {
 
 constexpr int kExprBlock = 256;
 
-// V(x1, t) - V(x2, t) in one pass over the program: both stacks advance together (the same
-// ops), so each op is dispatched once and the two evaluations overlap.  The tops of the
+// V(x1, t) - V(x2, t) of a program (op codes WH_EXPR_* of include/wignerhip.h; the library
+// checks length, op codes and stack depth when the plan is created) in one pass: both stacks
+// advance together (the same ops), so each op is dispatched once and the two evaluations
+// overlap.  Ops are read through the constant address space (uniform: scalar loads).  The tops of the
 // stacks live in registers; slot s below them: st[s * kExprBlock] (x1) and
 // st[off + s * kExprBlock] (x2), off = (depth - 1) * kExprBlock.
 // single: return V(x1, t) alone.

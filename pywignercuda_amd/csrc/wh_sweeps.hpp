@@ -29,48 +29,6 @@
 namespace wh {
 
 // ------------------------------------------------------------------ helpers
-// V(t, x) of an expression program (the op codes of include/wignerhip.h, WH_EXPR_*; the host
-// checks length, op codes and stack depth when the plan is created): a stack machine with the
-// top of the stack in a register and the values below it at st[s * LS] (LS threads interleave
-// their slots in LDS).  The program is read through the constant address space: its addresses
-// are uniform, so the ops come from scalar loads.  Device math functions of the HIP runtime.
-template <int LS>
-__device__ __forceinline__ double eval_expr_lds(cptr<double> code, int n, double x, double t, double* st) {
-  double tos = 0.0;
-  int sp = 0;  // values on the stack (the top one in tos)
-  for (int i = 0; i < n; ++i) {
-    const int op = int(code[2 * i]);
-    const double v = code[2 * i + 1];
-    if (op <= 2) {
-      if (sp > 0) st[(sp - 1) * LS] = tos;
-      tos = op == 0 ? v : op == 1 ? x : t;
-      ++sp;
-    } else if (op == 3) {
-      tos = -tos;
-    } else if (op <= 8) {
-      const double a = st[(sp - 2) * LS];
-      --sp;
-      tos = op == 4 ? a + tos : op == 5 ? a - tos : op == 6 ? a * tos : op == 7 ? a / tos : pow(a, tos);
-    } else {
-      switch (op) {
-        case 9: tos = exp(tos); break;
-        case 10: tos = sin(tos); break;
-        case 11: tos = cos(tos); break;
-        case 12: tos = tan(tos); break;
-        case 13: tos = sqrt(tos); break;
-        case 14: tos = fabs(tos); break;
-        case 15: tos = tanh(tos); break;
-        case 16: tos = sinh(tos); break;
-        case 17: tos = cosh(tos); break;
-        case 18: tos = log(tos); break;
-        case 19: tos = atan(tos); break;
-        default: tos = tos < 0.0 ? 0.0 : 1.0; break;  // Heaviside
-      }
-    }
-  }
-  return tos;
-}
-
 __device__ __forceinline__ double poly(const double* __restrict__ c, int deg, double x) {
   double r = c[deg];
   for (int n = deg - 1; n >= 0; --n) r = fma(r, x, c[n]);
