@@ -92,6 +92,27 @@ def cl1024_reference():
     return W0, W_ref, obs_ref
 
 
+def test_damped_1024_expression_potential(cuda, cl1024_reference):
+    """The damped 1024^2 workload with V given as a time-dependent expression program whose
+    time-dependent part is below FP64 resolution (1e-300 sin(t) e^x): the per-step device
+    tables at full size reproduce the polynomial oracle run."""
+    import torch
+
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    W0, W_ref, obs_ref = cl1024_reference
+    hip = Wigner2D(1024, 1024, 12.0, 12.0, 1.0, 1.0, 0.005,
+                   potential="0.1*pow(x,4) - pow(x,2) + 1e-300*sin(t)*exp(x)", dpotential="0.4*pow(x,3) - 2.0*x",
+                   D_theta=0.1, gamma=0.1, damping="CaldeiraLeggett", fp_damping="fabs(p)", mask=True, device=cuda)
+    assert hip.tables.pot_deg == -2
+    W = hip.to_device(W0)
+    hip.run(W, 21, use_graph=True)
+    torch.cuda.synchronize()
+    err = rel_l2(W.cpu().numpy(), W_ref)
+    print(f"1024^2 damped, expression V: rel-L2 {err:.3e}")
+    assert err < 1e-10
+
+
 def test_damped_1024_fp64(cuda, cl1024_reference):
     """21 Lie iterations (K; V + Caldeira-Leggett order 3; renormalisation) in the tuned kernels,
     graph-replayed as in the benchmark."""
