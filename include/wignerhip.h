@@ -521,6 +521,15 @@ typedef struct {
   int step_filter;
   const double* a_coef_rows;  /* [a_rows][3][WH_MAX_DEG + 1], see a_deg */
   int a_rows;
+  /* pot_deg = WH_POT_EXPR: V0(t, x) that is not a polynomial in x and depends on t, as the
+     expression programs (see WH_EXPR_*) of V0 and of dV0/dx, evaluated on the device at
+     t = time index * pot_t_step: the potential step's Bopp difference before each potential
+     sweep, dV0/dx and V0 of the record rows before each reduce.  Not with kvn. */
+  const double* pot_expr;
+  int pot_expr_len;
+  const double* dpot_expr;
+  int dpot_expr_len;
+  double pot_t_step;
 } wh_dirac_desc;
 
 typedef struct wh_dirac wh_dirac;

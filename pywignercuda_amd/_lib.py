@@ -162,6 +162,8 @@ class DiracDesc(ctypes.Structure):
         ("a_deg", c_int * 3), ("a_coef", (c_double * (WH_MAX_DEG + 1)) * 3),
         ("odm", c_int), ("odm_gamma", c_double), ("lambda_bar", c_double),
         ("kvn", c_int), ("step_filter", c_int), ("a_coef_rows", _DP), ("a_rows", c_int),
+        ("pot_expr", _DP), ("pot_expr_len", c_int), ("dpot_expr", _DP), ("dpot_expr_len", c_int),
+        ("pot_t_step", c_double),
     ]
 
 

@@ -61,6 +61,8 @@ hipError_t launch_dirac_xsweep(int prec, int X, const DiracPlan* d, int nblocks,
 hipError_t launch_dirac_psweep(int prec, int P, bool cl, const DiracPlan* d, int nblocks, hipStream_t s);
 hipError_t launch_dirac_colsum(int prec, const DiracPlan* d, int X, hipStream_t s);
 hipError_t launch_dirac_reduce(const DiracPlan* d, int state_add, int renorm, int anti, int inc, hipStream_t s);
+hipError_t launch_dirac_expr_table(const DiracPlan* d, int cells, int depth, hipStream_t s);
+hipError_t launch_dirac_expr_obs(const DiracPlan* d, int X, int depth, int state_add, hipStream_t s);
 hipError_t launch_dirac_finish(int prec, const DiracPlan* d, int P, hipStream_t s);
 int dirac_x_blocks(int prec, int X, int P);
 int dirac_gp_blocks(int X, int P);
@@ -1304,7 +1306,7 @@ struct wh_dirac {
   wh_dirac_desc d{};
   wh::DiracPlan host{};  // host mirror of the device plan
   DevBuf plan_dev, pot, pot_dv, theta_decay, xmask, x_host, p_host, theta_host, dv0, vsum, mass_dm, mass_sm;
-  DevBuf colsums, rowpart, scale, step, rho, G, lambda_host, acoef_rows;
+  DevBuf colsums, rowpart, scale, step, rho, G, lambda_host, acoef_rows, pot_expr, dpot_expr;
   TwSet twx, twp;
   int xblocks = 0, pblocks = 0, gpblocks = 0, pgblocks = 0;
   hipStream_t capture_stream = nullptr;
@@ -1333,6 +1335,10 @@ int dirac_validate(const wh_dirac_desc* d) {
   if (d->pot_deg == WH_POT_TABLE) {
     if (!d->pot_dv || !d->dv0 || !d->vsum)
       return fail(WH_INVALID_DESCRIPTOR, "pot_deg WH_POT_TABLE needs pot_dv, dv0 and vsum");
+  } else if (d->pot_deg == WH_POT_EXPR) {
+    WH_TRY(check_expr(d->pot_expr, d->pot_expr_len, 0));
+    WH_TRY(check_expr(d->dpot_expr, d->dpot_expr_len, 0));
+    if (d->kvn) return fail(WH_INVALID_DESCRIPTOR, "the KvN variant takes a polynomial or time-independent V0");
   } else if (d->pot_deg < 0 || d->pot_deg > WH_MAX_DEG || d->pot_rows < 1 || !d->pot_coef) {
     return fail(WH_INVALID_DESCRIPTOR, "potential deg %d rows %d coef %p", d->pot_deg, d->pot_rows,
                 (const void*)d->pot_coef);
@@ -1362,9 +1368,24 @@ int dirac_validate(const wh_dirac_desc* d) {
 
 // The potential half of a step: A = 0 (diagonal factors), vector potentials (two sweeps) or
 // A = 0 with ODM damping (one sweep over plane quads).
+// The record row and normalisation of the state at time index step + state_add; with a V0
+// program, its dV0/dx and V0 at that time first.
+int dirac_reduce(wh_dirac* e, int state_add, int renorm, int anti, int inc, hipStream_t s) {
+  const auto* dp = e->plan_dev.as<wh::DiracPlan>();
+  if (e->host.pot_expr_len > 0) {
+    const int depth = std::max(e->host.pot_expr_depth, e->host.dpot_expr_depth);
+    WH_LAUNCH(wh::launch_dirac_expr_obs(dp, e->d.x_dim, depth, state_add, s), "dirac_expr_obs");
+  }
+  WH_LAUNCH(wh::launch_dirac_reduce(dp, state_add, renorm, anti, inc, s), "dirac_reduce");
+  return WH_SUCCESS;
+}
+
 int dirac_potential_sweep(wh_dirac* e, hipStream_t s) {
   const auto* dp = e->plan_dev.as<wh::DiracPlan>();
   const int prec = e->d.precision;
+  if (e->host.pot_expr_len > 0)  // V0(t, x) program: the Bopp difference at this sweep's time index
+    WH_LAUNCH(wh::launch_dirac_expr_table(dp, e->d.x_dim * e->d.p_dim, e->host.pot_expr_depth, s),
+              "dirac_expr_table");
   if (e->host.odm && e->host.general) {
     // A and ODM: the two factor sweeps stay in the theta picture, the ODM sweep transforms back
     WH_LAUNCH(wh::launch_dirac_pgeneral(prec, false, dp, e->pgblocks, s), "dirac_pgeneral");
@@ -1391,7 +1412,7 @@ int dirac_iteration(wh_dirac* e, hipStream_t s) {
     WH_LAUNCH(wh::launch_dirac_fsweep(prec, e->d.x_dim, dp, e->xblocks, s), "dirac_fsweep");
     WH_LAUNCH(wh::launch_dirac_colsum(storage_code(prec), dp, e->d.x_dim, s), "dirac_colsum");
   }
-  WH_LAUNCH(wh::launch_dirac_reduce(dp, 1, 1, 1, 1, s), "dirac_reduce");
+  WH_TRY(dirac_reduce(e, 1, 1, 1, 1, s));
   return WH_SUCCESS;
 }
 
@@ -1461,6 +1482,15 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
     WH_TRY(e->pot_dv.upload(d.pot_dv, X * P * sizeof(double)));
     WH_TRY(e->dv0.upload(d.dv0, X * sizeof(double)));
     WH_TRY(e->vsum.upload(d.vsum, X * sizeof(double)));
+  } else if (d.pot_deg == WH_POT_EXPR) {
+    // the tables are filled on the device from the programs
+    const double zero_row[WH_MAX_DEG + 1] = {};
+    WH_TRY(e->pot.upload(zero_row, sizeof(zero_row)));
+    WH_TRY(e->pot_dv.alloc(X * P * sizeof(double)));
+    WH_TRY(e->dv0.alloc(X * sizeof(double)));
+    WH_TRY(e->vsum.alloc(X * sizeof(double)));
+    WH_TRY(e->pot_expr.upload(d.pot_expr, size_t(d.pot_expr_len) * 2 * sizeof(double)));
+    WH_TRY(e->dpot_expr.upload(d.dpot_expr, size_t(d.dpot_expr_len) * 2 * sizeof(double)));
   } else {
     WH_TRY(e->pot.upload(d.pot_coef, size_t(d.pot_rows) * (WH_MAX_DEG + 1) * sizeof(double)));
   }
@@ -1540,8 +1570,17 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   h.twP_lo = e->twp.lo.p;
   h.X = d.x_dim;
   h.P = d.p_dim;
-  h.pot_rows = d.pot_deg == WH_POT_TABLE ? 1 : d.pot_rows;
-  h.pot_deg = d.pot_deg;
+  h.pot_rows = d.pot_deg == WH_POT_TABLE || d.pot_deg == WH_POT_EXPR ? 1 : d.pot_rows;
+  h.pot_deg = d.pot_deg == WH_POT_EXPR ? WH_POT_TABLE : d.pot_deg;  // the sweeps read the tables
+  if (d.pot_deg == WH_POT_EXPR) {
+    h.pot_expr = e->pot_expr.as<double>();
+    h.dpot_expr = e->dpot_expr.as<double>();
+    h.pot_expr_len = d.pot_expr_len;
+    h.dpot_expr_len = d.dpot_expr_len;
+    WH_TRY(check_expr(d.pot_expr, d.pot_expr_len, 0, &h.pot_expr_depth));
+    WH_TRY(check_expr(d.dpot_expr, d.dpot_expr_len, 0, &h.dpot_expr_depth));
+    h.pot_t_step = d.pot_t_step;
+  }
   h.energy = d.energy;
   h.antiparticle = d.antiparticle;
   h.dx = d.dx;
@@ -1621,7 +1660,7 @@ extern "C" int wh_dirac_run(wh_dirac* e, void* W, int first_step, int n_steps, d
   WH_TRY(dirac_bind(e, W, obs, first_step, first_step, false, s));
   const auto* dp = e->plan_dev.as<wh::DiracPlan>();
   WH_LAUNCH(wh::launch_dirac_colsum(storage_code(e->d.precision), dp, e->d.x_dim, s), "dirac_colsum");
-  WH_LAUNCH(wh::launch_dirac_reduce(dp, 0, 0, 0, 0, s), "dirac_reduce");
+  WH_TRY(dirac_reduce(e, 0, 0, 0, 0, s));
   if (use_graph) {
     hipGraphExec_t exec = nullptr;
     WH_TRY(dirac_ensure_graph(e, s, &exec));  // ready for a later, longer run

@@ -862,9 +862,9 @@ __global__ void __launch_bounds__(kRedThreads) dirac_reduce_kernel(const DiracPl
       dv = gmem(d.dv0)[col];
       vs = gmem(d.vsum)[col];
     }
-    // DiracEnergy adds Potential_1..3 like Potential_0 (:2801-2804); with a V0 table, vsum holds
-    // them already unless they depend on time
-    if (d.general && (d.pot_deg >= 0 || d.a_rows > 0)) {
+    // DiracEnergy adds Potential_1..3 like Potential_0 (:2801-2804); a V0 table from the host
+    // holds them already unless they depend on time (one from V0's program holds V0 alone)
+    if (d.general && (d.pot_deg >= 0 || d.a_rows > 0 || d.pot_expr_len > 0)) {
       const double* ac = dirac_acoef(d, tstate);
       if (d.adeg0 >= 0) vs += poly(ac, d.adeg0, x);
       if (d.adeg1 >= 0) vs += poly(ac + (kMaxDeg + 1), d.adeg1, x);

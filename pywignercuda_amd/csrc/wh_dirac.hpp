@@ -88,6 +88,12 @@ struct DiracPlan {
   // clamped); a_rows = 0: acoef above
   const double* acoef_rows;
   int a_rows, pad2;
+  // V0(t, x) and dV0/dx(t, x) as expression programs (the host's pot_deg WH_POT_EXPR; pot_deg
+  // is the table form here): pot_dv before each potential sweep, dv0 and vsum before each reduce
+  const double* pot_expr;
+  const double* dpot_expr;
+  int pot_expr_len, dpot_expr_len, pot_expr_depth, dpot_expr_depth;
+  double pot_t_step;
 };
 
 }  // namespace wh

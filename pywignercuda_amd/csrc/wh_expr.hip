@@ -7,6 +7,7 @@
 // the table holds the classical KvN phase -hbar theta dV/dx(t, x) (exp_PotentialKvN, :27-75).
 #include <hip/hip_runtime.h>
 
+#include "wh_dirac.hpp"
 #include "wh_sweeps.hpp"
 
 namespace wh {
@@ -102,6 +103,62 @@ __global__ void __launch_bounds__(kExprBlock) expr_table_kernel(const Member* __
               : eval_expr_diff(code, len, x - u, x + u, t, st + threadIdx.x, off, false);
     }
   }
+}
+
+// Wigner-Dirac V0(t, x) programs: the potential step's Bopp difference V0(x - theta/2) -
+// V0(x + theta/2) (no hbar: GPU_WignerDirac2D_4x4.py:408, 568) at the sweep's time index.
+__global__ void __launch_bounds__(kExprBlock) dirac_expr_table_kernel(const DiracPlan* __restrict__ dp) {
+  extern __shared__ double st[];
+  const DiracPlan& d = *dp;
+  const int depth = d.pot_expr_depth;
+  const int off = (depth > 1 ? depth - 1 : 1) * kExprBlock;
+  const int X = d.X, P = d.P;
+  const int lx = __ffs(X) - 1;
+  const int cells = X * P;
+  const double t = double(*d.step) * d.pot_t_step;
+  double* tab = const_cast<double*>(d.pot_dv);
+  const cptr<double> code = cmem(d.pot_expr);
+  for (int c = int(blockIdx.x) * kExprBlock + int(threadIdx.x); c < cells; c += int(gridDim.x) * kExprBlock) {
+    const int i = c >> lx, col = c & (X - 1);
+    const double x = d.dx * double(fidx(col, X)), u = 0.5 * d.dtheta * double(fidx(i, P));
+    tab[c] = eval_expr_diff(code, d.pot_expr_len, x - u, x + u, t, st + threadIdx.x, off, false);
+  }
+}
+
+// dV0/dx and V0 of the record row the next reduce writes (time index step + state_add).
+__global__ void __launch_bounds__(kExprBlock) dirac_expr_obs_kernel(const DiracPlan* __restrict__ dp,
+                                                                    int state_add) {
+  extern __shared__ double st[];
+  const DiracPlan& d = *dp;
+  const int depth = d.pot_expr_depth > d.dpot_expr_depth ? d.pot_expr_depth : d.dpot_expr_depth;
+  const int off = (depth > 1 ? depth - 1 : 1) * kExprBlock;
+  const double t = double(*d.step + state_add) * d.pot_t_step;
+  for (int col = int(blockIdx.x) * kExprBlock + int(threadIdx.x); col < d.X; col += int(gridDim.x) * kExprBlock) {
+    const double x = d.dx * double(fidx(col, d.X));
+    const_cast<double*>(d.dv0)[col] =
+        eval_expr_diff(cmem(d.dpot_expr), d.dpot_expr_len, x, x, t, st + threadIdx.x, off, true);
+    const_cast<double*>(d.vsum)[col] =
+        eval_expr_diff(cmem(d.pot_expr), d.pot_expr_len, x, x, t, st + threadIdx.x, off, true);
+  }
+}
+
+static size_t expr_lds(int depth) {
+  return size_t(2) * size_t(depth > 1 ? depth - 1 : 1) * kExprBlock * sizeof(double);
+}
+
+hipError_t launch_dirac_expr_table(const DiracPlan* d, int cells, int depth, hipStream_t s) {
+  if (depth < 1 || depth > kExprStack) return hipErrorInvalidValue;
+  const int blocks = (cells + kExprBlock - 1) / kExprBlock;
+  hipLaunchKernelGGL(dirac_expr_table_kernel, dim3(blocks < 8192 ? blocks : 8192), dim3(kExprBlock),
+                     expr_lds(depth), s, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_dirac_expr_obs(const DiracPlan* d, int X, int depth, int state_add, hipStream_t s) {
+  if (depth < 1 || depth > kExprStack) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(dirac_expr_obs_kernel, dim3((X + kExprBlock - 1) / kExprBlock), dim3(kExprBlock),
+                     expr_lds(depth), s, d, state_add);
+  return hipGetLastError();
 }
 
 hipError_t launch_expr_table(const Member* members, int member, int cells, int depth, const int* step,

@@ -23,8 +23,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import DIRAC_OBS, DIRAC_OBS_REAL, WH_DNOBS, WH_MAX_DEG, WH_POT_TABLE
-from .physics import NonPolynomialPhysics, coefficient_table, depends_on_time, host_eval
+from ._lib import DIRAC_OBS, DIRAC_OBS_REAL, WH_DNOBS, WH_MAX_DEG, WH_POT_EXPR, WH_POT_TABLE
+from .physics import NonPolynomialPhysics, coefficient_table, compile_expression, depends_on_time, host_eval
 from .wigner2d import PRECISIONS, fft_order_axis, host_axis, percent_f
 
 
@@ -169,20 +169,30 @@ class WignerDirac4x4:
             d.pot_deg, d.pot_rows = deg, tab.shape[0]
             d.pot_coef = ptr(self.pot_table)
         except NonPolynomialPhysics:
-            if self.time_dependent:
-                raise NonPolynomialPhysics(f"{V0!r}: a time-dependent V0 must be a polynomial in x")
             self.pot_table = None
-            u = 0.5 * ktheta[:, None]  # x -/+ theta/2 (no hbar: GPU_WignerDirac2D_4x4.py:408, 568)
-            vm = host_eval(V0, t=0.0, x=kx[None, :] - u)
-            vp = host_eval(V0, t=0.0, x=kx[None, :] + u)
-            d.pot_deg, d.pot_rows = WH_POT_TABLE, 1
-            d.pot_dv = ptr(np.broadcast_to(np.asarray(vm - vp, dtype=np.float64), (self.P, self.X)))
-            dv0 = host_eval(_sympy_derivative(V0) + " + 0.*x", t=0.0, x=kx)
-            d.dv0 = ptr(np.broadcast_to(np.asarray(dv0, dtype=np.float64), kx.shape))
-            # V0 + A1 + A2 + A3 of the energy record; time-dependent A are added per step on the device
-            vs = sum(np.broadcast_to(np.asarray(host_eval(v, t=0.0, x=kx), dtype=np.float64), kx.shape)
-                     for v in (potentials[:1] if a_time else potentials))
-            d.vsum = ptr(vs)
+            if self.time_dependent:
+                # V0(t, x) and dV0/dx(t, x) as programs the device evaluates per step (the
+                # reference splices both strings into its kernels, :1964-1966)
+                if kvn:
+                    raise NotImplementedError("the KvN variant with a time-dependent non-polynomial V0")
+                self._pot_expr = compile_expression(V0, "x")
+                self._dpot_expr = compile_expression(_sympy_derivative(V0), "x")
+                d.pot_deg, d.pot_rows = WH_POT_EXPR, 1
+                d.pot_expr, d.pot_expr_len = ptr(self._pot_expr), self._pot_expr.shape[0]
+                d.dpot_expr, d.dpot_expr_len = ptr(self._dpot_expr), self._dpot_expr.shape[0]
+                d.pot_t_step = self.dt
+            else:
+                u = 0.5 * ktheta[:, None]  # x -/+ theta/2 (no hbar: GPU_WignerDirac2D_4x4.py:408, 568)
+                vm = host_eval(V0, t=0.0, x=kx[None, :] - u)
+                vp = host_eval(V0, t=0.0, x=kx[None, :] + u)
+                d.pot_deg, d.pot_rows = WH_POT_TABLE, 1
+                d.pot_dv = ptr(np.broadcast_to(np.asarray(vm - vp, dtype=np.float64), (self.P, self.X)))
+                dv0 = host_eval(_sympy_derivative(V0) + " + 0.*x", t=0.0, x=kx)
+                d.dv0 = ptr(np.broadcast_to(np.asarray(dv0, dtype=np.float64), kx.shape))
+                # V0 + A1 + A2 + A3 of the energy record; time-dependent A are added per step on the device
+                vs = sum(np.broadcast_to(np.asarray(host_eval(v, t=0.0, x=kx), dtype=np.float64), kx.shape)
+                         for v in (potentials[:1] if a_time else potentials))
+                d.vsum = ptr(vs)
         k_D = tr(D_theta)
         d.theta_decay = ptr(np.exp(-k_dt * k_D * ktheta * ktheta)) if k_D != 0.0 else None
         if absorb:

@@ -270,6 +270,30 @@ def test_dirac_time_dependent_vector_potential(cuda, case, precision):
         hip.run(hip.to_device(W0), 80)
 
 
+@pytest.mark.parametrize("with_a", [False, True])
+def test_dirac_time_dependent_expression_v0(cuda, with_a):
+    """A time-dependent non-polynomial V0(t, x) (a modulated Klein step): the device tabulates
+    its Bopp difference before each potential sweep and dV0/dx, V0 before each record row,
+    from the programs of V0 and of its sympy derivative, against the oracle (FP64)."""
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    vs = "1.5*0.5*( 1. + tanh((x-5.)/0.25) )*(1. + 0.3*sin(2*t))"
+    vf = lambda t, x: 0.75 * (1.0 + np.tanh((x - 5.0) / 0.25)) * (1.0 + 0.3 * np.sin(2 * t))  # noqa: E731
+    dvf = lambda t, x: 0.75 / (0.25 * np.cosh((x - 5.0) / 0.25) ** 2) * (1.0 + 0.3 * np.sin(2 * t))  # noqa: E731
+    a_s = ("0.05*x", " -x ", "0.1 + 0.02*pow(x,2)") if with_a else ("0.", "0.", "0.")
+    a_f = [lambda t, x: 0.05 * x, lambda t, x: -x, lambda t, x: 0.1 + 0.02 * x * x] if with_a else [zero] * 3
+    hip = WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=(vs,) + a_s, energy=True, D_theta=0.01,
+                         device=cuda)
+    orc = Dirac4x4Oracle(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, [vf] + a_f, dpotential0=dvf, compute_energy=True,
+                         D_theta=0.01)
+    W0 = wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 3.0, 1.0, Xg, 1.0, 1.0), orc)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 20)
+    err = rel_l2(W, W_ref)
+    print(f"time-dependent V0 program (A: {with_a}): rel L2 {err:.3e}")
+    assert err < 1e-10
+    check_obs(obs, rec)
+
+
 def test_dirac_single_sweeps_match_oracle_factors(cuda):
     """wh_dirac_kinetic / wh_dirac_potential (one sweep each, no normalisation) against the
     oracle's kinetic and potential steps between the transforms."""
