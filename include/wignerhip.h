@@ -530,6 +530,12 @@ typedef struct {
   const double* dpot_expr;
   int dpot_expr_len;
   double pot_t_step;
+  /* Mass(t, x) that depends on t, as an expression program (instead of mass_dm/mass_sm):
+     the library refills both mass tables on the device before each potential sweep, with
+     F = Mass(x, t) c dt's kernel constants c and dt, t = time index * mass_t_step */
+  const double* mass_expr;
+  int mass_expr_len;
+  double mass_t_step;
 } wh_dirac_desc;
 
 typedef struct wh_dirac wh_dirac;

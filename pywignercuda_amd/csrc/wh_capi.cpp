@@ -63,6 +63,7 @@ hipError_t launch_dirac_colsum(int prec, const DiracPlan* d, int X, hipStream_t 
 hipError_t launch_dirac_reduce(const DiracPlan* d, int state_add, int renorm, int anti, int inc, hipStream_t s);
 hipError_t launch_dirac_expr_table(const DiracPlan* d, int cells, int depth, hipStream_t s);
 hipError_t launch_dirac_expr_obs(const DiracPlan* d, int X, int depth, int state_add, hipStream_t s);
+hipError_t launch_dirac_mass_table(const DiracPlan* d, int cells, int depth, hipStream_t s);
 hipError_t launch_dirac_finish(int prec, const DiracPlan* d, int P, hipStream_t s);
 int dirac_x_blocks(int prec, int X, int P);
 int dirac_gp_blocks(int X, int P);
@@ -1306,7 +1307,7 @@ struct wh_dirac {
   wh_dirac_desc d{};
   wh::DiracPlan host{};  // host mirror of the device plan
   DevBuf plan_dev, pot, pot_dv, theta_decay, xmask, x_host, p_host, theta_host, dv0, vsum, mass_dm, mass_sm;
-  DevBuf colsums, rowpart, scale, step, rho, G, lambda_host, acoef_rows, pot_expr, dpot_expr;
+  DevBuf colsums, rowpart, scale, step, rho, G, lambda_host, acoef_rows, pot_expr, dpot_expr, mass_expr;
   TwSet twx, twp;
   int xblocks = 0, pblocks = 0, gpblocks = 0, pgblocks = 0;
   hipStream_t capture_stream = nullptr;
@@ -1346,6 +1347,11 @@ int dirac_validate(const wh_dirac_desc* d) {
   if (!d->x_host || !d->p_host || !d->theta_host) return fail(WH_INVALID_DESCRIPTOR, "host grids missing");
   if ((d->mass_dm == nullptr) != (d->mass_sm == nullptr))
     return fail(WH_INVALID_DESCRIPTOR, "mass_dm and mass_sm go together");
+  if (d->mass_expr || d->mass_expr_len) {
+    if (d->mass_dm) return fail(WH_INVALID_DESCRIPTOR, "mass tables and a mass program exclude each other");
+    if (d->kvn) return fail(WH_INVALID_DESCRIPTOR, "the KvN variant takes no mass function");
+    WH_TRY(check_expr(d->mass_expr, d->mass_expr_len, 0));
+  }
   for (int k = 0; k < 3; ++k)
     if (d->a_deg[k] < -1 || d->a_deg[k] > WH_MAX_DEG)
       return fail(WH_INVALID_DESCRIPTOR, "vector potential %d: degree %d", k + 1, d->a_deg[k]);
@@ -1386,6 +1392,9 @@ int dirac_potential_sweep(wh_dirac* e, hipStream_t s) {
   if (e->host.pot_expr_len > 0)  // V0(t, x) program: the Bopp difference at this sweep's time index
     WH_LAUNCH(wh::launch_dirac_expr_table(dp, e->d.x_dim * e->d.p_dim, e->host.pot_expr_depth, s),
               "dirac_expr_table");
+  if (e->host.mass_expr_len > 0)  // Mass(t, x) program: the mass tables at this sweep's time index
+    WH_LAUNCH(wh::launch_dirac_mass_table(dp, e->d.x_dim * e->d.p_dim, e->host.mass_expr_depth, s),
+              "dirac_mass_table");
   if (e->host.odm && e->host.general) {
     // A and ODM: the two factor sweeps stay in the theta picture, the ODM sweep transforms back
     WH_LAUNCH(wh::launch_dirac_pgeneral(prec, false, dp, e->pgblocks, s), "dirac_pgeneral");
@@ -1499,6 +1508,10 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   if (d.mass_dm) {
     WH_TRY(e->mass_dm.upload(d.mass_dm, X * P * sizeof(double)));
     WH_TRY(e->mass_sm.upload(d.mass_sm, X * P * sizeof(double)));
+  } else if (d.mass_expr_len > 0) {  // tables refilled on the device from the program
+    WH_TRY(e->mass_dm.alloc(X * P * sizeof(double)));
+    WH_TRY(e->mass_sm.alloc(X * P * sizeof(double)));
+    WH_TRY(e->mass_expr.upload(d.mass_expr, size_t(d.mass_expr_len) * 2 * sizeof(double)));
   }
   WH_TRY(e->x_host.upload(d.x_host, X * sizeof(double)));
   WH_TRY(e->p_host.upload(d.p_host, P * sizeof(double)));
@@ -1536,6 +1549,14 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   h.vsum = e->vsum.as<double>();
   h.mass_dm = e->mass_dm.as<double>();
   h.mass_sm = e->mass_sm.as<double>();
+  if (d.mass_expr_len > 0) {
+    h.mass_expr = e->mass_expr.as<double>();
+    h.mass_expr_len = d.mass_expr_len;
+    WH_TRY(check_expr(d.mass_expr, d.mass_expr_len, 0, &h.mass_expr_depth));
+    h.mass_c = d.c;
+    h.mass_dt = d.dt;
+    h.mass_t_step = d.mass_t_step;
+  }
   h.rho = e->rho.as<double>();
   h.G = e->G.as<double>();
   h.lambda_host = e->lambda_host.as<double>();

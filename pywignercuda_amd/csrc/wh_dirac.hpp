@@ -94,6 +94,10 @@ struct DiracPlan {
   const double* dpot_expr;
   int pot_expr_len, dpot_expr_len, pot_expr_depth, dpot_expr_depth;
   double pot_t_step;
+  // Mass(t, x) as an expression program: mass_dm / mass_sm refilled before each potential sweep
+  const double* mass_expr;
+  int mass_expr_len, mass_expr_depth;
+  double mass_c, mass_dt, mass_t_step;
 };
 
 }  // namespace wh

@@ -20,9 +20,11 @@ constexpr int kExprBlock = 256;
 // overlap.  Ops are read through the constant address space (uniform: scalar loads).  The tops of the
 // stacks live in registers; slot s below them: st[s * kExprBlock] (x1) and
 // st[off + s * kExprBlock] (x2), off = (depth - 1) * kExprBlock.
-// single: return V(x1, t) alone.
-__device__ __forceinline__ double eval_expr_diff(cptr<double> code, int n, double x1, double x2, double t,
-                                                 double* st, int off, bool single) {
+struct ExprPair {
+  double a, b;  // V(x1, t), V(x2, t)
+};
+__device__ __forceinline__ ExprPair eval_expr_pair(cptr<double> code, int n, double x1, double x2, double t,
+                                                   double* st, int off) {
   constexpr int LS = kExprBlock;
   const int OFF = off;
   double a = 0.0, b = 0.0;  // tops of the two stacks
@@ -68,7 +70,14 @@ __device__ __forceinline__ double eval_expr_diff(cptr<double> code, int n, doubl
       }
     }
   }
-  return single ? a : a - b;
+  return {a, b};
+}
+
+// single: return V(x1, t) alone.
+__device__ __forceinline__ double eval_expr_diff(cptr<double> code, int n, double x1, double x2, double t,
+                                                 double* st, int off, bool single) {
+  const ExprPair r = eval_expr_pair(code, n, x1, x2, t, st, off);
+  return single ? r.a : r.a - r.b;
 }
 
 // Table k (k < ntables) of member `member` at time index step + k, where step is step_add
@@ -142,6 +151,28 @@ __global__ void __launch_bounds__(kExprBlock) dirac_expr_obs_kernel(const DiracP
   }
 }
 
+// Mass(t, x) program of the MassPotential variant: the tables F(x - theta/2) -/+ F(x + theta/2),
+// F = Mass c^2 dt (kernel constants, in the host's order of products), at the sweep's time index.
+__global__ void __launch_bounds__(kExprBlock) dirac_mass_table_kernel(const DiracPlan* __restrict__ dp) {
+  extern __shared__ double st[];
+  const DiracPlan& d = *dp;
+  const int depth = d.mass_expr_depth;
+  const int off = (depth > 1 ? depth - 1 : 1) * kExprBlock;
+  const int X = d.X, P = d.P;
+  const int lx = __ffs(X) - 1;
+  const int cells = X * P;
+  const double t = double(*d.step) * d.mass_t_step;
+  const cptr<double> code = cmem(d.mass_expr);
+  for (int c = int(blockIdx.x) * kExprBlock + int(threadIdx.x); c < cells; c += int(gridDim.x) * kExprBlock) {
+    const int i = c >> lx, col = c & (X - 1);
+    const double x = d.dx * double(fidx(col, X)), u = 0.5 * d.dtheta * double(fidx(i, P));
+    const ExprPair m = eval_expr_pair(code, d.mass_expr_len, x - u, x + u, t, st + threadIdx.x, off);
+    const double fm = m.a * d.mass_c * d.mass_c * d.mass_dt, fp = m.b * d.mass_c * d.mass_c * d.mass_dt;
+    const_cast<double*>(d.mass_dm)[c] = fm - fp;
+    const_cast<double*>(d.mass_sm)[c] = fm + fp;
+  }
+}
+
 static size_t expr_lds(int depth) {
   return size_t(2) * size_t(depth > 1 ? depth - 1 : 1) * kExprBlock * sizeof(double);
 }
@@ -150,6 +181,14 @@ hipError_t launch_dirac_expr_table(const DiracPlan* d, int cells, int depth, hip
   if (depth < 1 || depth > kExprStack) return hipErrorInvalidValue;
   const int blocks = (cells + kExprBlock - 1) / kExprBlock;
   hipLaunchKernelGGL(dirac_expr_table_kernel, dim3(blocks < 8192 ? blocks : 8192), dim3(kExprBlock),
+                     expr_lds(depth), s, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_dirac_mass_table(const DiracPlan* d, int cells, int depth, hipStream_t s) {
+  if (depth < 1 || depth > kExprStack) return hipErrorInvalidValue;
+  const int blocks = (cells + kExprBlock - 1) / kExprBlock;
+  hipLaunchKernelGGL(dirac_mass_table_kernel, dim3(blocks < 8192 ? blocks : 8192), dim3(kExprBlock),
                      expr_lds(depth), s, d);
   return hipGetLastError();
 }

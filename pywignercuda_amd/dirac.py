@@ -146,10 +146,12 @@ class WignerDirac4x4:
         k_c = tr(c)
         d.mass, d.c = tr(mass), k_c
         d.mass_half = tr(mass / 2.0 if kinetic_mass_half is None else kinetic_mass_half)
-        if mass_potential is not None:
+        if mass_potential is not None and depends_on_time(mass_potential, "x"):
+            # Mass(t, x) as a program: the device refills the mass tables before each potential sweep
+            self._mass_expr = compile_expression(mass_potential, "x")
+            d.mass_expr, d.mass_expr_len, d.mass_t_step = ptr(self._mass_expr), self._mass_expr.shape[0], self.dt
+        elif mass_potential is not None:
             # Mass(t, x) of the potential factors (GPU_WignerDirac2D_MassPotential_4x4.py:422-427, 585-591)
-            if depends_on_time(mass_potential, "x"):
-                raise NotImplementedError(f"Mass_String {mass_potential!r}: a time-dependent mass is not supported")
             u = 0.5 * ktheta[:, None]
             shape = (self.P, self.X)
             Fm = np.broadcast_to(np.asarray(host_eval(mass_potential, t=0.0, x=kx[None, :] - u), dtype=np.float64),

@@ -337,8 +337,8 @@ def test_dirac_plan_rejects_unsupported_physics():
 
     with pytest.raises(NotImplementedError):
         WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=("0.", "0.", " tanh(x) ", "0."))
-    with pytest.raises(NotImplementedError):  # a time-dependent mass function
-        WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, mass_potential="1.0 + 0.1*x*x*cos(t)")
+    with pytest.raises(NotImplementedError):  # a vector potential that is not a polynomial in x
+        WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=("0.", "tanh(x)*cos(t)", "0.", "0."))
     with pytest.raises(NotImplementedError):
         WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, gamma=0.1, damping="Lindblad")
     with pytest.raises(NotImplementedError):  # KvN takes Potential_0 only

@@ -200,6 +200,30 @@ def test_dirac_mass_potential_with_vector_potential(cuda):
     check_obs(obs, rec)
 
 
+@pytest.mark.parametrize("with_a", [False, True])
+def test_dirac_time_dependent_mass(cuda, with_a):
+    """Mass(t, x) that depends on t (MassPotential variant, D44M.py:387-427): the device refills
+    the mass tables F(x -/+ theta/2) before each potential sweep from the program, with and
+    without a vector potential, against the oracle."""
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    mass_s = "0.5 + 0.4*tanh(x/2.)*cos(3*t)"
+    mass_f = lambda t, x: 0.5 + 0.4 * np.tanh(x / 2.0) * np.cos(3 * t)  # noqa: E731
+    vs, vf, dvf = CASES["ramp"]
+    a_s = ("0.05*x", " -x ", "0.1 + 0.02*pow(x,2)") if with_a else ("0.", "0.", "0.")
+    a_f = [lambda t, x: 0.05 * x, lambda t, x: -x, lambda t, x: 0.1 + 0.02 * x * x] if with_a else [zero] * 3
+    hip = WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=(vs,) + a_s, energy=True,
+                         mass_potential=mass_s, kinetic_mass_half=0.0001, device=cuda)
+    orc = Dirac4x4Oracle(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, [vf] + a_f, dpotential0=dvf, compute_energy=True,
+                         mass_potential=mass_f, kinetic_mass_half=0.0001)
+    W0 = wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 3.0, 1.0, Xg, 1.0, 1.0), orc)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 20)
+    err = rel_l2(W, W_ref)
+    print(f"time-dependent mass (A: {with_a}): rel L2 {err:.3e}")
+    assert err < 1e-10
+    check_obs(obs, rec)
+
+
 def test_dirac_gross_pitaevskii(cuda):
     """ConstantB_GP0.02-style nonlinearity (A = 0 here): phase dt a Re(B- - B+) from the trace density."""
     from pywignercuda_amd.dirac import WignerDirac4x4
