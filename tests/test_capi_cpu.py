@@ -324,7 +324,8 @@ def test_dirac_descriptor_validation_before_device():
         L.wh_dirac_destroy(h)
     for field, value, code in [("x_dim", 96, 8), ("precision", 4, 3), ("pot_deg", -1, 10), ("c", 0.0, 10),
                                ("struct_size", 8, 10), ("a_rows", -1, 10), ("a_rows", 3, 10),
-                               ("pot_deg", -2, 10)]:  # WH_POT_EXPR without programs
+                               ("pot_deg", -2, 10),  # WH_POT_EXPR without programs
+                               ("mass_expr_len", 3, 10)]:  # a mass program without its ops
         old = getattr(d, field)
         setattr(d, field, value)
         assert L.wh_dirac_create(ctypes.byref(d), ctypes.byref(h)) == code, (field, L.wh_last_error())
