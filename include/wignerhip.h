@@ -536,6 +536,13 @@ typedef struct {
   const double* mass_expr;
   int mass_expr_len;
   double mass_t_step;
+  /* vector potentials that are not polynomials in x: the expression programs of Potential_1..3
+     (a_expr_len[k] = 0: identically zero; a_deg[k] = 0 for a non-zero one so that the two
+     factor sweeps run), evaluated on the device at x -/+ theta/2 before each potential sweep
+     and at x for the energy of each record row, t = time index * a_t_step */
+  const double* a_expr[3];
+  int a_expr_len[3];
+  double a_t_step;
 } wh_dirac_desc;
 
 typedef struct wh_dirac wh_dirac;

@@ -11,9 +11,9 @@ step, observables reduced on the device, copied once per saved frame).
 
 Extra attributes read if present: precision ('fp64' default or 'fp32'),
 truncateConstants (default True, the reference's '%f' splicing), useGraph
-(default True).  Not implemented (NotImplementedError): vector potentials that
-are not polynomials in x (their coefficients may depend on t) and a Mass_String
-that depends on t.  GPU_WignerDiracKvN2D_4x4 (GPU_WignerDiracKvN2D_4x4.py) is the KvN
+(default True).  Physics strings (Potential_0..3, Mass_String) may depend on t and
+need not be polynomials in x; strings calling C functions outside the expression
+programs' ops raise NotImplementedError.  GPU_WignerDiracKvN2D_4x4 (GPU_WignerDiracKvN2D_4x4.py) is the KvN
 variant of the same class.
 """
 from __future__ import annotations

@@ -164,6 +164,7 @@ class DiracDesc(ctypes.Structure):
         ("kvn", c_int), ("step_filter", c_int), ("a_coef_rows", _DP), ("a_rows", c_int),
         ("pot_expr", _DP), ("pot_expr_len", c_int), ("dpot_expr", _DP), ("dpot_expr_len", c_int),
         ("pot_t_step", c_double), ("mass_expr", _DP), ("mass_expr_len", c_int), ("mass_t_step", c_double),
+        ("a_expr", _DP * 3), ("a_expr_len", c_int * 3), ("a_t_step", c_double),
     ]
 
 

@@ -64,6 +64,8 @@ hipError_t launch_dirac_reduce(const DiracPlan* d, int state_add, int renorm, in
 hipError_t launch_dirac_expr_table(const DiracPlan* d, int cells, int depth, hipStream_t s);
 hipError_t launch_dirac_expr_obs(const DiracPlan* d, int X, int depth, int state_add, hipStream_t s);
 hipError_t launch_dirac_mass_table(const DiracPlan* d, int cells, int depth, hipStream_t s);
+hipError_t launch_dirac_a_table(const DiracPlan* d, int cells, int depth, hipStream_t s);
+hipError_t launch_dirac_a_sum(const DiracPlan* d, int X, int depth, int state_add, hipStream_t s);
 hipError_t launch_dirac_finish(int prec, const DiracPlan* d, int P, hipStream_t s);
 int dirac_x_blocks(int prec, int X, int P);
 int dirac_gp_blocks(int X, int P);
@@ -1308,6 +1310,7 @@ struct wh_dirac {
   wh::DiracPlan host{};  // host mirror of the device plan
   DevBuf plan_dev, pot, pot_dv, theta_decay, xmask, x_host, p_host, theta_host, dv0, vsum, mass_dm, mass_sm;
   DevBuf colsums, rowpart, scale, step, rho, G, lambda_host, acoef_rows, pot_expr, dpot_expr, mass_expr;
+  DevBuf a_expr[3], a_tab, asum;
   TwSet twx, twp;
   int xblocks = 0, pblocks = 0, gpblocks = 0, pgblocks = 0;
   hipStream_t capture_stream = nullptr;
@@ -1347,6 +1350,12 @@ int dirac_validate(const wh_dirac_desc* d) {
   if (!d->x_host || !d->p_host || !d->theta_host) return fail(WH_INVALID_DESCRIPTOR, "host grids missing");
   if ((d->mass_dm == nullptr) != (d->mass_sm == nullptr))
     return fail(WH_INVALID_DESCRIPTOR, "mass_dm and mass_sm go together");
+  for (int k = 0; k < 3; ++k)
+    if (d->a_expr_len[k] != 0 || d->a_expr[k]) {
+      WH_TRY(check_expr(d->a_expr[k], d->a_expr_len[k], k + 1));
+      if (d->a_deg[k] != 0)
+        return fail(WH_INVALID_DESCRIPTOR, "vector potential %d: a program needs a_deg 0", k + 1);
+    }
   if (d->mass_expr || d->mass_expr_len) {
     if (d->mass_dm) return fail(WH_INVALID_DESCRIPTOR, "mass tables and a mass program exclude each other");
     if (d->kvn) return fail(WH_INVALID_DESCRIPTOR, "the KvN variant takes no mass function");
@@ -1382,6 +1391,8 @@ int dirac_reduce(wh_dirac* e, int state_add, int renorm, int anti, int inc, hipS
     const int depth = std::max(e->host.pot_expr_depth, e->host.dpot_expr_depth);
     WH_LAUNCH(wh::launch_dirac_expr_obs(dp, e->d.x_dim, depth, state_add, s), "dirac_expr_obs");
   }
+  if (e->host.asum)
+    WH_LAUNCH(wh::launch_dirac_a_sum(dp, e->d.x_dim, e->host.a_expr_depth, state_add, s), "dirac_a_sum");
   WH_LAUNCH(wh::launch_dirac_reduce(dp, state_add, renorm, anti, inc, s), "dirac_reduce");
   return WH_SUCCESS;
 }
@@ -1392,6 +1403,8 @@ int dirac_potential_sweep(wh_dirac* e, hipStream_t s) {
   if (e->host.pot_expr_len > 0)  // V0(t, x) program: the Bopp difference at this sweep's time index
     WH_LAUNCH(wh::launch_dirac_expr_table(dp, e->d.x_dim * e->d.p_dim, e->host.pot_expr_depth, s),
               "dirac_expr_table");
+  if (e->host.a_tab)  // vector potential programs at this sweep's time index
+    WH_LAUNCH(wh::launch_dirac_a_table(dp, e->d.x_dim * e->d.p_dim, e->host.a_expr_depth, s), "dirac_a_table");
   if (e->host.mass_expr_len > 0)  // Mass(t, x) program: the mass tables at this sweep's time index
     WH_LAUNCH(wh::launch_dirac_mass_table(dp, e->d.x_dim * e->d.p_dim, e->host.mass_expr_depth, s),
               "dirac_mass_table");
@@ -1549,6 +1562,24 @@ extern "C" int wh_dirac_create(const wh_dirac_desc* desc, wh_dirac** out) {
   h.vsum = e->vsum.as<double>();
   h.mass_dm = e->mass_dm.as<double>();
   h.mass_sm = e->mass_sm.as<double>();
+  if (d.a_expr_len[0] > 0 || d.a_expr_len[1] > 0 || d.a_expr_len[2] > 0) {
+    // vector potential programs: A_k at x -/+ theta/2 per sweep, their sum per record row
+    WH_TRY(e->a_tab.alloc(6 * X * P * sizeof(double)));
+    WH_TRY(e->asum.alloc(X * sizeof(double)));
+    h.a_tab = e->a_tab.as<double>();
+    h.asum = e->asum.as<double>();
+    h.a_expr_depth = 1;
+    for (int k = 0; k < 3; ++k) {
+      if (d.a_expr_len[k] <= 0) continue;
+      WH_TRY(e->a_expr[k].upload(d.a_expr[k], size_t(d.a_expr_len[k]) * 2 * sizeof(double)));
+      h.a_expr[k] = e->a_expr[k].as<double>();
+      h.a_expr_len[k] = d.a_expr_len[k];
+      int depth = 0;
+      WH_TRY(check_expr(d.a_expr[k], d.a_expr_len[k], k + 1, &depth));
+      h.a_expr_depth = std::max(h.a_expr_depth, depth);
+    }
+    h.a_t_step = d.a_t_step;
+  }
   if (d.mass_expr_len > 0) {
     h.mass_expr = e->mass_expr.as<double>();
     h.mass_expr_len = d.mass_expr_len;

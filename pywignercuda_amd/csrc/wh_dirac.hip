@@ -398,13 +398,25 @@ __device__ __forceinline__ const double* dirac_acoef(const DiracPlan& d, int ste
   return d.acoef_rows + size_t(row) * 3 * (kMaxDeg + 1);
 }
 
-// mdt: Mass(y) c^2 dt (mass_half c^2 dt for a constant mass)
-__device__ __forceinline__ DiracPot dirac_pot_factor(const DiracPlan& d, const double* __restrict__ ac, double y,
-                                                     double s, double mdt) {
+// A1..A3 at y = x -/+ theta/2 of cell e (side 0: left, 1: right): the tables of the programs
+// or the polynomial coefficients ac.
+struct DiracA {
+  double a1, a2, a3;
+};
+__device__ __forceinline__ DiracA dirac_avals(const DiracPlan& d, const double* __restrict__ ac, double y,
+                                              size_t e, int side) {
+  if (d.a_tab) {
+    const size_t n = size_t(d.P) * size_t(d.X);
+    return {d.a_tab[size_t(side) * n + e], d.a_tab[size_t(2 + side) * n + e], d.a_tab[size_t(4 + side) * n + e]};
+  }
   constexpr int K = kMaxDeg + 1;
-  const double a1 = d.adeg0 >= 0 ? poly(ac, d.adeg0, y) : 0.0;
-  const double a2 = d.adeg1 >= 0 ? poly(ac + K, d.adeg1, y) : 0.0;
-  const double a3 = d.adeg2 >= 0 ? poly(ac + 2 * K, d.adeg2, y) : 0.0;
+  return {d.adeg0 >= 0 ? poly(ac, d.adeg0, y) : 0.0, d.adeg1 >= 0 ? poly(ac + K, d.adeg1, y) : 0.0,
+          d.adeg2 >= 0 ? poly(ac + 2 * K, d.adeg2, y) : 0.0};
+}
+
+// mdt: Mass(y) c^2 dt (mass_half c^2 dt for a constant mass)
+__device__ __forceinline__ DiracPot dirac_pot_factor(const DiracPlan& d, DiracA av, double s, double mdt) {
+  const double a1 = av.a1, a2 = av.a2, a3 = av.a3;
   const double F = sqrt(mdt * mdt + (a1 * a1 + a2 * a2 + a3 * a3) * d.dt * d.dt);
   double sn, cs;
   sincos_fp64(F, &sn, &cs);
@@ -459,7 +471,7 @@ __device__ __forceinline__ void dirac_pleft_body(const DiracPlan& d, int tile, c
       // F(x - theta/2) = (sm + dm) / 2 of the mass tables (MassPotential variant)
       const size_t e_ = size_t(i) * X + col;
       const double mdt = d.mass_dm ? 0.5 * (gmem(d.mass_sm)[e_] + gmem(d.mass_dm)[e_]) : d.mass_phase;
-      const DiracPot u = dirac_pot_factor(d, ac, x - 0.5 * th, -1.0, mdt);
+      const DiracPot u = dirac_pot_factor(d, dirac_avals(d, ac, x - 0.5 * th, e_, 0), -1.0, mdt);
       const cplx<double> w0 = widen(v[0][j]), w1 = widen(v[1][j]), w2 = widen(v[2][j]), w3 = widen(v[3][j]);
       const cplx<double> mu13{-u.u13.x, -u.u13.y};
       const cplx<double> o0 = cadd(cmul(u.u11, w0), cadd(cmul(u.u13, w2), cmul(u.u14, w3)));
@@ -517,7 +529,7 @@ __device__ __forceinline__ void dirac_pright_body(const DiracPlan& d, int tile, 
       // F(x + theta/2) = (sm - dm) / 2 of the mass tables (MassPotential variant)
       const size_t e_ = size_t(i) * X + col;
       const double mdt = d.mass_dm ? 0.5 * (gmem(d.mass_sm)[e_] - gmem(d.mass_dm)[e_]) : d.mass_phase;
-      const DiracPot u = dirac_pot_factor(d, ac, x + 0.5 * th, 1.0, mdt);
+      const DiracPot u = dirac_pot_factor(d, dirac_avals(d, ac, x + 0.5 * th, e_, 1), 1.0, mdt);
       const cplx<double> w0 = widen(v[0][j]), w1 = widen(v[1][j]), w2 = widen(v[2][j]), w3 = widen(v[3][j]);
       const cplx<double> mu13{-u.u13.x, -u.u13.y};
       // (W U)_{ij} = sum_k W_ik U_kj with U31 = U13, U32 = U14, U41 = U23, U42 = U24 = -U13
@@ -864,7 +876,9 @@ __global__ void __launch_bounds__(kRedThreads) dirac_reduce_kernel(const DiracPl
     }
     // DiracEnergy adds Potential_1..3 like Potential_0 (:2801-2804); a V0 table from the host
     // holds them already unless they depend on time (one from V0's program holds V0 alone)
-    if (d.general && (d.pot_deg >= 0 || d.a_rows > 0 || d.pot_expr_len > 0)) {
+    if (d.asum) {  // A1..A3 programs: their sum at this row's time
+      vs += gmem(d.asum)[col];
+    } else if (d.general && (d.pot_deg >= 0 || d.a_rows > 0 || d.pot_expr_len > 0)) {
       const double* ac = dirac_acoef(d, tstate);
       if (d.adeg0 >= 0) vs += poly(ac, d.adeg0, x);
       if (d.adeg1 >= 0) vs += poly(ac + (kMaxDeg + 1), d.adeg1, x);

@@ -98,6 +98,13 @@ struct DiracPlan {
   const double* mass_expr;
   int mass_expr_len, mass_expr_depth;
   double mass_c, mass_dt, mass_t_step;
+  // A1..A3 as expression programs: a_tab [3][2][P][X] = A_k(x - theta/2), A_k(x + theta/2) at the
+  // potential sweep's time index; asum [X] = sum_k A_k(x) at the record row's (null: polynomials)
+  const double* a_expr[3];
+  int a_expr_len[3], a_expr_depth;
+  const double* a_tab;
+  const double* asum;
+  double a_t_step;
 };
 
 }  // namespace wh

@@ -173,6 +173,47 @@ __global__ void __launch_bounds__(kExprBlock) dirac_mass_table_kernel(const Dira
   }
 }
 
+// Potential_1..3 programs (vector potentials that are not polynomials in x): A_k(x -/+ theta/2)
+// of every cell at the potential sweep's time index, for the two factor sweeps.
+__global__ void __launch_bounds__(kExprBlock) dirac_a_table_kernel(const DiracPlan* __restrict__ dp) {
+  extern __shared__ double st[];
+  const DiracPlan& d = *dp;
+  const int depth = d.a_expr_depth;
+  const int off = (depth > 1 ? depth - 1 : 1) * kExprBlock;
+  const int X = d.X, P = d.P;
+  const int lx = __ffs(X) - 1;
+  const int cells = X * P;
+  const size_t n = size_t(cells);
+  const double t = double(*d.step) * d.a_t_step;
+  double* tab = const_cast<double*>(d.a_tab);
+  for (int c = int(blockIdx.x) * kExprBlock + int(threadIdx.x); c < cells; c += int(gridDim.x) * kExprBlock) {
+    const int i = c >> lx, col = c & (X - 1);
+    const double x = d.dx * double(fidx(col, X)), u = 0.5 * d.dtheta * double(fidx(i, P));
+    for (int k = 0; k < 3; ++k) {
+      ExprPair a{0.0, 0.0};
+      if (d.a_expr_len[k] > 0) a = eval_expr_pair(cmem(d.a_expr[k]), d.a_expr_len[k], x - u, x + u, t, st + threadIdx.x, off);
+      tab[size_t(2 * k) * n + size_t(c)] = a.a;
+      tab[size_t(2 * k + 1) * n + size_t(c)] = a.b;
+    }
+  }
+}
+
+// sum_k A_k(x) of the record row the next reduce writes (time index step + state_add).
+__global__ void __launch_bounds__(kExprBlock) dirac_a_sum_kernel(const DiracPlan* __restrict__ dp, int state_add) {
+  extern __shared__ double st[];
+  const DiracPlan& d = *dp;
+  const int depth = d.a_expr_depth;
+  const int off = (depth > 1 ? depth - 1 : 1) * kExprBlock;
+  const double t = double(*d.step + state_add) * d.a_t_step;
+  for (int col = int(blockIdx.x) * kExprBlock + int(threadIdx.x); col < d.X; col += int(gridDim.x) * kExprBlock) {
+    const double x = d.dx * double(fidx(col, d.X));
+    double s = 0.0;
+    for (int k = 0; k < 3; ++k)
+      if (d.a_expr_len[k] > 0) s += eval_expr_pair(cmem(d.a_expr[k]), d.a_expr_len[k], x, x, t, st + threadIdx.x, off).a;
+    const_cast<double*>(d.asum)[col] = s;
+  }
+}
+
 static size_t expr_lds(int depth) {
   return size_t(2) * size_t(depth > 1 ? depth - 1 : 1) * kExprBlock * sizeof(double);
 }
@@ -190,6 +231,21 @@ hipError_t launch_dirac_mass_table(const DiracPlan* d, int cells, int depth, hip
   const int blocks = (cells + kExprBlock - 1) / kExprBlock;
   hipLaunchKernelGGL(dirac_mass_table_kernel, dim3(blocks < 8192 ? blocks : 8192), dim3(kExprBlock),
                      expr_lds(depth), s, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_dirac_a_table(const DiracPlan* d, int cells, int depth, hipStream_t s) {
+  if (depth < 1 || depth > kExprStack) return hipErrorInvalidValue;
+  const int blocks = (cells + kExprBlock - 1) / kExprBlock;
+  hipLaunchKernelGGL(dirac_a_table_kernel, dim3(blocks < 8192 ? blocks : 8192), dim3(kExprBlock), expr_lds(depth),
+                     s, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_dirac_a_sum(const DiracPlan* d, int X, int depth, int state_add, hipStream_t s) {
+  if (depth < 1 || depth > kExprStack) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(dirac_a_sum_kernel, dim3((X + kExprBlock - 1) / kExprBlock), dim3(kExprBlock),
+                     expr_lds(depth), s, d, state_add);
   return hipGetLastError();
 }
 

@@ -89,16 +89,29 @@ class WignerDirac4x4:
         a_time = any(not _is_zero(potentials[k]) and depends_on_time(potentials[k], "x") for k in (1, 2, 3))
         a_times = np.arange(int(time_rows) if time_rows else 4096) * float(dt) if a_time else [0.0]
         a_rows = np.zeros((len(a_times), 3, WH_MAX_DEG + 1))
+        a_nonpoly = False
         for k in (1, 2, 3):
             if _is_zero(potentials[k]):
                 continue
             try:
                 deg, tab = coefficient_table(potentials[k], "x", a_times)
-            except NonPolynomialPhysics as exc:
-                raise NotImplementedError(
-                    f"Potential_{k}_String = {potentials[k]!r}: vector potentials must be polynomials in x") from exc
+            except NonPolynomialPhysics:
+                a_nonpoly = True
+                continue
             a_deg[k - 1], a_coef[k - 1] = deg, tab[0]
             a_rows[:, k - 1, :] = tab
+        # a vector potential that is not a polynomial in x: all of them as programs the device
+        # evaluates per cell and step (the reference splices the strings, :417-442, 575-601)
+        self._a_expr = None
+        if a_nonpoly:
+            try:
+                self._a_expr = [None if _is_zero(potentials[k]) else compile_expression(potentials[k], "x")
+                                for k in (1, 2, 3)]
+            except NonPolynomialPhysics as exc:
+                raise NotImplementedError(f"vector potentials {potentials[1:]!r}: {exc}") from exc
+            a_deg = [-1 if p is None else 0 for p in self._a_expr]
+            a_coef[:] = 0.0
+            a_time = False
         if kvn:
             # GPU_WignerDiracKvN2D_4x4: V0 only, no damping or GP terms in its loop (:3409-3690)
             if max(a_deg) >= 0 or mass_potential is not None:
@@ -164,6 +177,11 @@ class WignerDirac4x4:
         self.a_table = a_rows if a_time else None
         if a_time:
             d.a_coef_rows, d.a_rows = ptr(a_rows), a_rows.shape[0]
+        if self._a_expr is not None:
+            for k, prog in enumerate(self._a_expr):
+                if prog is not None:
+                    d.a_expr[k], d.a_expr_len[k] = ptr(prog), prog.shape[0]
+            d.a_t_step = self.dt
         try:
             times = np.arange(int(time_rows) if time_rows else 4096) * self.dt if self.time_dependent else [0.0]
             deg, tab = coefficient_table(V0, "x", times)
@@ -193,7 +211,7 @@ class WignerDirac4x4:
                 d.dv0 = ptr(np.broadcast_to(np.asarray(dv0, dtype=np.float64), kx.shape))
                 # V0 + A1 + A2 + A3 of the energy record; time-dependent A are added per step on the device
                 vs = sum(np.broadcast_to(np.asarray(host_eval(v, t=0.0, x=kx), dtype=np.float64), kx.shape)
-                         for v in (potentials[:1] if a_time else potentials))
+                         for v in (potentials[:1] if (a_time or self._a_expr is not None) else potentials))
                 d.vsum = ptr(vs)
         k_D = tr(D_theta)
         d.theta_decay = ptr(np.exp(-k_dt * k_D * ktheta * ktheta)) if k_D != 0.0 else None

@@ -318,6 +318,33 @@ def test_dirac_time_dependent_expression_v0(cuda, with_a):
     check_obs(obs, rec)
 
 
+@pytest.mark.parametrize("case,mass", [("klein", False), ("ramp", True)])
+def test_dirac_non_polynomial_vector_potential(cuda, case, mass):
+    """Vector potentials that are not polynomials in x (a localised, driven A1 and a saturating
+    A2): their programs are tabulated at x -/+ theta/2 before each potential sweep and summed
+    at x for the energy record (D44.py:417-442, 575-601, 2801-2804), against the oracle; with
+    a position-dependent mass in the second case."""
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    vs, vf, dvf = CASES[case]
+    pots = (vs, "0.3*exp(-x*x/8.)*cos(2*t)", " -2*tanh(x/4.) ", "0.")
+    extra_h = dict(mass_potential="1. + 0.1*x*x", kinetic_mass_half=0.0001) if mass else {}
+    extra_o = dict(mass_potential=lambda t, x: 1.0 + 0.1 * x * x, kinetic_mass_half=0.0001) if mass else {}
+    hip = WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=pots, energy=True, D_theta=0.01,
+                         device=cuda, **extra_h)
+    assert hip.vector_potential
+    orc = Dirac4x4Oracle(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01,
+                         [vf, lambda t, x: 0.3 * np.exp(-x * x / 8.0) * np.cos(2 * t),
+                          lambda t, x: -2.0 * np.tanh(x / 4.0), zero],
+                         dpotential0=dvf, compute_energy=True, D_theta=0.01, **extra_o)
+    W0 = wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 3.0, 1.0, Xg, 1.0, 1.0), orc)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 20)
+    err = rel_l2(W, W_ref)
+    print(f"non-polynomial A ({case}, mass {mass}): rel L2 {err:.3e}")
+    assert err < 1e-10
+    check_obs(obs, rec)
+
+
 def test_dirac_single_sweeps_match_oracle_factors(cuda):
     """wh_dirac_kinetic / wh_dirac_potential (one sweep each, no normalisation) against the
     oracle's kinetic and potential steps between the transforms."""
