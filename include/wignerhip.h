@@ -214,7 +214,8 @@ typedef struct {
      for pot_deg = WH_POT_TABLE (time-independent V given as an arbitrary expression,
      Potential(t,x) spliced into expPotential, FFT.py:88-94, 869-873) */
   const double* pot_dv;
-  /* ODM Lindblad dissipator (damping = 2; Lindbladian_ODM_Order1, FFT.py:1516-1634, 1907-1910):
+  /* ODM Lindblad dissipator (damping = 2; Lindbladian_ODM_Order1, FFT.py:1516-1634, 1907-1910;
+     with WH_SCHEME_STRANG after the second half step, GP.py:1958-1968):
      odm_g [4][P][X] = g(p + hbar lambda/2), g(p - hbar lambda/2) and both times sign(p -/+ ...),
      g(q) = sqrt(L_material f(|q| + hbar/(2 L_material))) on host grids (SetA_ODM, :1196-1244);
      odm_x [2][X] = cos 2x, sin 2x; odm_th [2][P] = cos(hbar theta), sin(hbar theta) */
@@ -315,7 +316,7 @@ typedef struct {
   double dt, hbar, mass;
   int kinetic_poly, kinetic_deg;
   double kin_coef[WH_MAX_DEG + 1];
-  int damping;     /* 0 none, 1 Caldeira-Leggett order 3, 2 ODM (Lie scheme) */
+  int damping;     /* 0 none, 1 Caldeira-Leggett order 3, 2 ODM */
   double gamma_dt;
   int renormalize;
   double gp_coeff;

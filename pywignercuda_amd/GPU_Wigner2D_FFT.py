@@ -258,8 +258,6 @@ class Propagator_Base:
         else:
             if self.gammaDamping != 0.0 and self.dampingFunction not in ("CaldeiraLeggett", "ODM"):
                 raise NotImplementedError(f"dampingFunction {self.dampingFunction!r} is not implemented")
-            if self.gammaDamping != 0.0 and self.dampingFunction == "ODM" and scheme != "lie":
-                raise NotImplementedError("ODM damping is part of GPU_Wigner2D_FFT (Lie scheme) only")
             damping = self.dampingFunction if self.gammaDamping != 0.0 else None
         return Wigner2D(
             self.X_gridDIM, self.P_gridDIM, self.X_amplitude, self.P_amplitude, self.hBar, self.mass, self.dt,

@@ -490,8 +490,6 @@ int validate_common(const wh_ensemble_desc& d) {
     return fail(WH_INVALID_DESCRIPTOR, "dt/hbar/mass: %g %g %g", d.dt, d.hbar, d.mass);
   if (d.damping < 0 || d.damping > 2)
     return fail(WH_INVALID_DESCRIPTOR, "damping %d (0 none, 1 Caldeira-Leggett, 2 ODM)", d.damping);
-  if (d.damping == 2 && d.scheme != WH_SCHEME_LIE)
-    return fail(WH_INVALID_DESCRIPTOR, "the ODM dissipator is part of the Lie scheme (GPU_Wigner2D_FFT)");
   if (d.damping == 2 && (d.odm_channels < 0 || d.odm_channels > 2))
     return fail(WH_INVALID_DESCRIPTOR, "odm_channels %d (0 or 1: one channel, 2: two)", d.odm_channels);
   return WH_SUCCESS;
@@ -881,6 +879,31 @@ int iteration(wh_ensemble* e, bool merge, bool first, hipStream_t s) {
                                 e->work_colsum.n, s),
               "colsum");
     WH_TRY(reduce(e, renorm, 1, e->gp(), 1, true, s));
+  } else if (d.scheme == WH_SCHEME_STRANG && e->odm()) {
+    // V/2; K; V/2; W += L_ODM(W); column sums; renormalise (GPU_Wigner2D_GPitaevskii.Run,
+    // GP.py:1942-1968): the dissipator sits between the two half steps of consecutive
+    // iterations, so they are not merged
+    if (e->gp()) WH_TRY(gpshift(e, s));
+    WH_TRY(psweep(e, wh::kScaleOnLoad | (e->gp() ? wh::kGP : 0), 0.5, 0, s));
+    if (e->gp()) {
+      WH_TRY(xsweep(e, wh::kColPart, s));
+      WH_LAUNCH(wh::launch_colpart_reduce(e->mem_dev.as<wh::Member>(), e->n, wh::kXVariants[e->xvar].R,
+                                          wh::kXVariants[e->xvar].block, e->step.as<int>(), s),
+                "colpart_reduce");
+      WH_TRY(gpshift(e, s));
+    } else {
+      WH_TRY(xsweep(e, 0, s));
+    }
+    WH_TRY(psweep(e, e->gp() ? wh::kGP : 0, 0.5, 0, s));
+    for (int ch = 0; ch < (d.odm_channels == 2 ? 2 : 1); ++ch)
+      WH_LAUNCH(wh::launch_odm(d.precision, e->mem_dev.as<wh::Member>(), e->work_odm_x.buf.as<unsigned>(),
+                               e->work_odm_x.n, e->work_odm_p.buf.as<unsigned>(), e->work_odm_p.n,
+                               ch == 0 ? d.odm_weight1 : d.odm_weight2, ch == 0 ? 1.0 : -1.0, s),
+                "odm");
+    WH_LAUNCH(wh::launch_colsum(storage_code(d.precision), e->mem_dev.as<wh::Member>(),
+                                e->work_colsum.buf.as<unsigned>(), e->work_colsum.n, s),
+              "colsum");
+    WH_TRY(reduce(e, renorm, 1, e->gp(), 1, true, s));
   } else if (d.scheme == WH_SCHEME_LIE && e->gp()) {
     WH_TRY(gpshift(e, s));
     WH_TRY(xsweep(e, wh::kScaleOnLoad, s));
@@ -1018,7 +1041,7 @@ int ensemble_run(wh_ensemble* e, void* const* states, int first_step, int n_step
   // 1024^2 instead of 13: its device-scope fences write back and invalidate the L2 per block.)
   WH_TRY(reduce(e, normalize_initial, d.scheme == WH_SCHEME_LIE, e->gp(), 1, true, s, true,
                 d.scheme == WH_SCHEME_LIE ? first_step - 1 : first_step, first_step));
-  const bool merged_strang = d.scheme == WH_SCHEME_STRANG && !e->gp();
+  const bool merged_strang = d.scheme == WH_SCHEME_STRANG && !e->gp() && !e->odm();
   if (n_steps > 0 && merged_strang) {
     // first half step of the first iteration, with the pending factor folded in
     WH_TRY(psweep(e, wh::kScaleOnLoad, 0.5, 0, s));

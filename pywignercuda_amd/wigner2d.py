@@ -172,8 +172,6 @@ class _Tables:
         self.odm_g = self.odm_x = self.odm_th = None
         self.odm_sign = float(odm_sign)
         if damping == "ODM" and gamma != 0.0:
-            if scheme != "lie":
-                raise ValueError("the ODM dissipator belongs to the Lie scheme (GPU_Wigner2D_FFT)")
             if L_material is None or fp_damping is None:
                 raise ValueError("ODM damping needs L_material and fp_damping")
             h, L = float(hbar), float(L_material)
@@ -306,7 +304,8 @@ class Wigner2D(_PlanBase):
                   GPU_Wigner2D_GPitaevskii).
     kinetic     : ignored by the Lie scheme (the reference hard-wires dt*lambda*p/m).
     damping     : None, 'CaldeiraLeggett' (order-3 composite) or 'ODM' (Lindblad dissipator of
-                  order 1, Lie scheme; needs L_material and fp_damping; odm_sign -1 pumps).
+                  order 1, after K; V in the Lie scheme and after V/2; K; V/2 in the Strang
+                  scheme, GP.py:1958-1968; needs L_material and fp_damping; odm_sign -1 pumps).
     odm_weights : ODM channel weights: None = one channel of weight 1 (GPU_Wigner2D_FFT);
                   (w1, w2) = GPU_Wigner2D_ODM2 (W += L(W) with (w1, +1), then (w2, -1)).
     gp_coeff    : Gross-Pitaevskii coefficient a_GP (0 = off).

@@ -364,6 +364,24 @@ def test_lie_odm_fp64(cuda, gp, hbar):
     check_obs(obs, obs_ref)
 
 
+@pytest.mark.parametrize("gp,hbar", [(0.0, 1.0), (0.5, 0.8)])
+def test_strang_odm_fp64(cuda, gp, hbar):
+    """GPU_Wigner2D_GPitaevskii with dampingFunction 'ODM' (GP.py:1942-1968): V/2; K; V/2;
+    Lindbladian_ODM_Order1; renormalisation -- the half steps of consecutive iterations are not
+    merged around the dissipator.  With and without the GP term, against the oracle."""
+    fp_str = " p*p/sqrt( p*p + 0.5  ) "
+    fp_fn = lambda p: p * p / np.sqrt(p * p + 0.5)  # noqa: E731
+    hip, orc = make_pair(cuda, scheme="strang", amp=12.0, dt=0.005, gamma=0.07 / 4.0, damping="ODM", fp_str=fp_str,
+                         fp_fn=fp_fn, truncate=True, gp=gp, L_material=3.0, hbar=hbar,
+                         pot=("0.5*pow(x,2)", "x", lambda t, x: 0.5 * x * x, lambda t, x: x))
+    W0 = ow.harmonic_oscillator_wigner(orc.g, 0, 1.0, 0.0, 3.0)
+    W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 30)
+    err = rel_l2(W, W_ref)
+    print(f"Strang ODM gp={gp}: rel L2 {err:.3e}")
+    assert err < 1e-10
+    check_obs(obs, obs_ref)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_lie_odm2_two_channels(cuda, precision):
     """GPU_Wigner2D_ODM2 (GPU_Wigner2D_FFT.py:1973-2230): W += L(W) with (weight 1, sign 1), then
