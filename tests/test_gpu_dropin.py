@@ -12,6 +12,7 @@ Sources (raw notebook JSON lines):
   Caldeira + GP          instances/Wigner2D/Wigner2D_FFT_Caldeira_GPitaevskii_X2.ipynb:999
   GP Strang              instances/Wigner2D_GPitaevskii/Wigner2D_GPitaevskii_X2.ipynb:405
   ODM X2                 instances/Wigner2D/Wigner2D_FFT_ODM_X2_D0.ipynb:453
+  (GP + ODM has no notebook: that drop-in run is checked against the oracle instead)
   Fokker-Planck X2       instances/FokkerPlanck2D/Damping_XX_PInit3_Omega1_Gamma0.07_DTheta0.0143.ipynb
                          (cells 12 and 15 outputs)
 """
@@ -107,6 +108,45 @@ def test_gpitaevskii_strang_notebook(cuda, tmp_path):
     assert inst.Run() == 0
     _check(inst, -0.00105811219803, 0.318230122578)
     assert inst.ProbabilitySquare_average.shape == (4001,)
+
+
+def test_gpitaevskii_odm_dropin_matches_oracle(cuda, tmp_path):
+    """GPU_Wigner2D_GPitaevskii with dampingFunction 'ODM' (GP.py:1958-1968: Lindbladian_ODM_Order1
+    after the second half step, then renormalisation).  No notebook records this combination, so
+    the drop-in class's W_end and record are checked against the oracle's Strang run with the same
+    '%f'-spliced constants and absorbing mask: 512^2, amplitudes 12, dt 0.005, 30 steps, a_GP 0.5,
+    gamma 0.07/4, L_material 3, epsilon 0.5, HO ground state at p = 3."""
+    from oracle import wigner2d as ow
+    from pywignercuda_amd.GPU_Wigner2D_GPitaevskii import GPU_Wigner2D_GPitaevskii
+
+    class frame(GPU_Wigner2D_GPitaevskii):
+        def __init__(self):
+            self.dampingFunction = "ODM"
+            self.L_material = 3.0
+            self.GPitaevskiiCoeff = 0.5
+            self.omega = 1.0
+            self.fp_Damping_String = " p*p/sqrt( p*p + 0.5  ) "
+            self.SetTimeTrack(0.005, 30, 10, fileName=str(tmp_path / "gp_odm.hdf5"))
+            GPU_Wigner2D_GPitaevskii.__init__(self, 512, 512, 12, 12, 1.0, 1.0, 0.0, 0.0, 0.07 / 4.0,
+                                              "0.5*pow(x,2)", "x", "0.5*p*p")
+
+    inst = frame()
+    inst.W_init = inst.Wigner_HarmonicOscillator(0, inst.omega, 0.0, 3.0)
+    assert inst.Run() == 0
+    orc = ow.Wigner2DOracle(512, 512, 12.0, 12.0, 1.0, 1.0, 0.005, gamma=0.07 / 4.0,
+                            potential=lambda t, x: 0.5 * x * x, dpotential=lambda t, x: x,
+                            kinetic=lambda p: 0.5 * p * p,
+                            fp_damping=lambda p: p * p / np.sqrt(p * p + 0.5), damping="ODM",
+                            gp_coeff=0.5, scheme="strang", mask=True, truncate_constants=True,
+                            L_material=3.0)
+    W_ref, obs_ref, _ = orc.run(np.asarray(inst.W_init, dtype=np.complex128), 30)
+    err = np.linalg.norm(inst.W_end - W_ref.real) / np.linalg.norm(W_ref)
+    print(f"GP + ODM drop-in: rel L2 {err:.3e}")
+    assert err < 1e-10
+    np.testing.assert_allclose(inst.X_average, obs_ref["X"], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(inst.P_average, obs_ref["P"], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(inst.Hamiltonian_average, obs_ref["H"], rtol=1e-10, atol=1e-12)
+    assert abs(inst.W_end.sum() * inst.dX * inst.dP - 1.0) < 1e-9
 
 
 def test_dropin_output_file_and_attributes(cuda, tmp_path):
