@@ -249,6 +249,62 @@ class Propagator_Base:
         self.W_init = f["W_init"] + 0j
         self.W_end = f["W_end"] + 0j
 
+    def _ehrenfest_series(self, name, fileName):
+        """The Ehrenfest_*_FromFile readers (GPU_Wigner2D_FFT.py:944-1048): (t, record) with
+        t = k dt, k = 0..timeSteps.  They read '/Ehrenfest/<name>_average'; for X, X2, P, P2
+        and XP a file written by Run holds '<name>_Ehrenfest' instead (:1939-1943), which is
+        read when the first is absent."""
+        f = _io.read(fileName or self.fileName)
+        key = f"Ehrenfest/{name}_average"
+        if key not in f:
+            key = f"Ehrenfest/{name}_Ehrenfest"
+        return np.arange(int(f["timeSteps"]) + 1) * float(f["dt"]), f[key]
+
+    def Ehrenfest_X_FromFile(self, fileName=None):
+        return self._ehrenfest_series("X", fileName)
+
+    def Ehrenfest_P_FromFile(self, fileName=None):
+        return self._ehrenfest_series("P", fileName)
+
+    def Ehrenfest_X2_FromFile(self, fileName=None):
+        return self._ehrenfest_series("X2", fileName)
+
+    def Ehrenfest_P2_FromFile(self, fileName=None):
+        return self._ehrenfest_series("P2", fileName)
+
+    def Ehrenfest_XP_FromFile(self, fileName=None):
+        return self._ehrenfest_series("XP", fileName)
+
+    def Ehrenfest_Hamiltonian_FromFile(self, fileName=None):
+        return self._ehrenfest_series("Hamiltonian", fileName)
+
+    def WignerMarginal_Probability_x(self, W):
+        """Position marginal: sum over the p axis (axis 0) times dP (GPU_Wigner2D_FFT.py:1050-1051)."""
+        return np.sum(W, axis=0) * self.dP
+
+    def WignerMarginal_Probability_p(self, W):
+        """Momentum marginal: sum over the x axis (axis 1) times dX (GPU_Wigner2D_FFT.py:1053-1054)."""
+        return np.sum(W, axis=1) * self.dX
+
+    def OperatorX_XBasis(self):
+        """diag(X_range) as complex128 (GPU_Wigner2D_FFT.py:1187-1193)."""
+        return np.diag(self.X_range).astype(np.complex128)
+
+    def OperatorP_XBasis(self):
+        """The momentum operator on the X grid (GPU_Wigner2D_FFT.py:1159-1185): the matrix of
+        -i d/dx for the band-limited interpolant through the grid points, at half-integer
+        indices j = n - (N-1)/2.  For k = j_n - j_m (an integer, nonzero off the diagonal):
+        P[n, m] = i pi / (dX N^2) * (sin(pi k) cos(pi k/N) - N cos(pi k) sin(pi k/N)) / sin^2(pi k/N),
+        zero on the diagonal.  It equals sum_q p_q |q><q| over the plane waves of the
+        half-integer momenta p_q = 2 pi q / (N dX), q = -(N-1)/2 .. (N-1)/2."""
+        N = self.X_gridDIM
+        j = (2.0 * np.arange(N) - N + 1.0) / 2.0
+        k = j[:, None] - j[None, :]
+        off = k != 0
+        s = np.sin(np.pi * np.where(off, k, 1.0) / N)
+        val = (np.sin(np.pi * k) * np.cos(np.pi * k / N) - N * np.cos(np.pi * k) * s) / s ** 2
+        return np.pi * 1j / (self.dX * N ** 2) * np.where(off, val, 0.0)
+
     # ---------------------------------------------------------------- device plan
     def _make_plan(self, scheme, gp_coeff, kinetic, odm_weights=None, kvn=False, renormalize=None):
         if kvn:

@@ -412,3 +412,49 @@ def test_observe_noalloc_validation_before_device():
     bad.v0_host = None
     assert L.wh_wigner2d_observe_noalloc(ctypes.byref(bad), W, obs, None) == 10
     assert L.wh_wigner2d_observe_noalloc(ctypes.byref(d), None, obs, None) == 4
+
+
+def test_dropin_host_helpers(tmp_path):
+    """Off-path host helpers of Propagator_Base (GPU_Wigner2D_FFT.py:944-1054, 1159-1193), no
+    device: OperatorP_XBasis equals the spectral construction sum_q p_q |q><q| over the
+    half-integer momenta p_q = 2 pi q / (N dX), q = -(N-1)/2 .. (N-1)/2 (so it is Hermitian and
+    differentiates those plane waves exactly); the marginals integrate to the norm and are the
+    x / p densities of a product state; the Ehrenfest readers return k*dt and the run file's
+    record whether it is stored as '<name>_average' or as Run's '<name>_Ehrenfest'."""
+    from pywignercuda_amd import _io
+    from pywignercuda_amd.GPU_Wigner2D_FFT import GPU_Wigner2D_FFT
+
+    o = GPU_Wigner2D_FFT.__new__(GPU_Wigner2D_FFT)
+    o.SetPhaseSpaceBox2D(32, 16, 5.0, 4.0)
+    N, dX = o.X_gridDIM, o.dX
+    q = (2.0 * np.arange(N) - N + 1.0) / 2.0
+    x, p = q * dX, 2.0 * np.pi * q / (N * dX)
+    P_spec = (np.exp(1j * np.outer(x, p)) * p) @ np.exp(-1j * np.outer(p, x)) / N
+    P = o.OperatorP_XBasis()
+    np.testing.assert_allclose(P, P_spec, atol=1e-13)
+    np.testing.assert_allclose(P, P.conj().T, atol=1e-13)
+    wave = np.exp(1j * p[N // 2 + 3] * o.X_range)
+    np.testing.assert_allclose(P @ wave, p[N // 2 + 3] * wave, atol=1e-12)
+    np.testing.assert_allclose(np.diag(o.OperatorX_XBasis()), o.X_range)
+
+    fx = np.exp(-(o.X_range - 1.0) ** 2)
+    gp = np.exp(-0.5 * o.P_range ** 2)
+    W = gp[:, None] * fx[None, :]
+    W /= W.sum() * o.dX * o.dP
+    px, pp = o.WignerMarginal_Probability_x(W), o.WignerMarginal_Probability_p(W)
+    assert px.shape == (32,) and pp.shape == (16,)
+    np.testing.assert_allclose(px.sum() * o.dX, 1.0)
+    np.testing.assert_allclose(pp.sum() * o.dP, 1.0)
+    np.testing.assert_allclose(px / px.max(), fx / fx.max())
+    np.testing.assert_allclose(pp / pp.max(), gp / gp.max())
+
+    f = _io.RunFile(str(tmp_path / "rec"))
+    f["dt"], f["timeSteps"] = 0.25, 4
+    f["/Ehrenfest/X_Ehrenfest"] = np.arange(5.0)
+    f["/Ehrenfest/Hamiltonian_average"] = np.arange(5.0) + 10
+    f.close()
+    o.fileName = str(tmp_path / "rec")
+    t, X = o.Ehrenfest_X_FromFile()
+    np.testing.assert_allclose(t, [0.0, 0.25, 0.5, 0.75, 1.0])
+    np.testing.assert_allclose(X, np.arange(5.0))
+    np.testing.assert_allclose(o.Ehrenfest_Hamiltonian_FromFile(str(tmp_path / "rec"))[1], np.arange(5.0) + 10)
