@@ -20,7 +20,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from ._io import RunFile
+from ._io import RunFile, read
 from .dirac import WignerDirac4x4
 from .physics import host_eval
 
@@ -120,6 +120,19 @@ class GPU_WignerDirac2D_4x4:
     def SpinorNorm(self, Psi):
         return sum(np.sum(np.abs(Psi[k]) ** 2) for k in range(4)) * self.dX * self.dP
 
+    def _majorana(self, Psi, sign):
+        """(Psi + sign * C Psi) / 2 with the charge conjugate C Psi = (-Psi4*, Psi3*, Psi2*, -Psi1*)
+        of the standard representation (MajoranaSpinorPlus / Minus, D44.py:2337-2355)."""
+        Psi = np.asarray(Psi) + 0j
+        conj = np.array([-Psi[3].conj(), Psi[2].conj(), Psi[1].conj(), -Psi[0].conj()])
+        return 0.5 * (Psi + sign * conj)
+
+    def MajoranaSpinorPlus(self, Psi):
+        return self._majorana(Psi, 1.0)
+
+    def MajoranaSpinorMinus(self, Psi):
+        return self._majorana(Psi, -1.0)
+
     # ------------------------------------------------------------------ host transforms (:2535-2556)
     def Fourier_X_To_Lambda(self, W):
         return np.fft.fft(W, axis=1)
@@ -204,6 +217,17 @@ class GPU_WignerDirac2D_4x4:
                 Wij = self.fftshift(W[4 * i + j])
                 f.create_dataset(f"W_real_{i + 1}{j + 1}/{t}", data=np.real(Wij))
                 f.create_dataset(f"W_imag_{i + 1}{j + 1}/{t}", data=np.imag(Wij))
+
+    def Load_Density(self, fileName, n):
+        """Frame n of a run saved with frameSaveMode 'Density' (D44.py:2915-2922)."""
+        return read(fileName)[str(n)]
+
+    def Load_WignerFunction(self, fileName, n):
+        """Frame n of a run saved with frameSaveMode 'Wigner_4X4' as a [4][4][P][X] array
+        (D44.py:2925-2951)."""
+        f = read(fileName)
+        return np.array([[f[f"W_real_{i}{j}/{n}"] + 1j * f[f"W_imag_{i}{j}/{n}"] for j in range(1, 5)]
+                         for i in range(1, 5)])
 
     def _save_frame(self, f, t_index, W_dev):
         W = W_dev.cpu().numpy().astype(np.complex128)

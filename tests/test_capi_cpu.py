@@ -458,3 +458,36 @@ def test_dropin_host_helpers(tmp_path):
     np.testing.assert_allclose(t, [0.0, 0.25, 0.5, 0.75, 1.0])
     np.testing.assert_allclose(X, np.arange(5.0))
     np.testing.assert_allclose(o.Ehrenfest_Hamiltonian_FromFile(str(tmp_path / "rec"))[1], np.arange(5.0) + 10)
+
+
+def test_dirac_dropin_host_helpers(tmp_path):
+    """GPU_WignerDirac2D_4x4 host helpers, no device.  MajoranaSpinorPlus/Minus (D44.py:2337-2355)
+    split Psi into the even and odd parts of the antiunitary charge conjugation C, C^2 = 1; C maps
+    the reference's own particle-up spinor at px onto i times its antiparticle-up spinor at -px.
+    Load_Density / Load_WignerFunction (D44.py:2915-2951) read back what Run's frame writers store."""
+    from pywignercuda_amd import _io
+    from pywignercuda_amd.GPU_WignerDirac2D_4x4 import GPU_WignerDirac2D_4x4
+
+    o = GPU_WignerDirac2D_4x4.__new__(GPU_WignerDirac2D_4x4)
+    o.mass, o.c = 1.0, 137.036
+    X = np.linspace(-5.0, 5.0, 64, endpoint=False)
+    rng = np.random.default_rng(7)
+    Psi = rng.normal(size=(4, 64)) + 1j * rng.normal(size=(4, 64))
+    plus, minus = o.MajoranaSpinorPlus(Psi), o.MajoranaSpinorMinus(Psi)
+    np.testing.assert_allclose(plus + minus, Psi)
+    C = lambda s: 2.0 * o.MajoranaSpinorPlus(s) - s  # noqa: E731  (C s = 2 Plus(s) - s)
+    np.testing.assert_allclose(C(C(Psi)), Psi, atol=1e-14)
+    np.testing.assert_allclose(C(plus), plus, atol=1e-14)
+    np.testing.assert_allclose(C(minus), -minus, atol=1e-14)
+    up = o.GaussianSpinor_ParticleUp(0.5, 3.0, 1.2, X)
+    np.testing.assert_allclose(C(up), 1j * o.GaussianSpinor_AntiParticleUp(0.5, -3.0, 1.2, X), atol=1e-10)
+
+    W = rng.normal(size=(16, 8, 4)) + 1j * rng.normal(size=(16, 8, 4))
+    f = _io.RunFile(str(tmp_path / "d"))
+    o.save_Density(f, 3, W)
+    o.save_WignerFunction(f, 5, W)
+    f.close()
+    np.testing.assert_allclose(o.Load_Density(str(tmp_path / "d"), 3), np.real(W[0] + W[5] + W[10] + W[15]))
+    W44 = o.Load_WignerFunction(str(tmp_path / "d"), 5)
+    assert W44.shape == (4, 4, 8, 4)
+    np.testing.assert_allclose(W44[2, 1], np.fft.fftshift(W[9]))
