@@ -461,16 +461,20 @@ class WignerEnsemble(_PlanBase):
     """Many Wigner functions (uneven grid sizes and potentials) propagated together.
 
     members : list of dicts with the per-member keys X, P, X_amp, P_amp, potential,
-              and optionally dpotential, pot_table, D_theta, D_lambda, fp_damping, mask.
-    The remaining keyword arguments are shared by all members, as for Wigner2D.
+              and optionally dpotential, pot_table, D_theta, D_lambda, fp_damping, mask,
+              L_material and odm_sign (damping 'ODM').
+    The remaining keyword arguments (odm_weights included) are shared by all members, as
+    for Wigner2D.
     All sweeps of all members are one launch each; the step loop is graph-captured.
     """
 
     def __init__(self, members, hbar, mass, dt, *, scheme="lie", gamma=0.0, damping=None,
                  kinetic=None, gp_coeff=0.0, truncate_constants=False, precision="fp64",
-                 time_rows=None, device=None, variants=None):
+                 time_rows=None, device=None, variants=None, odm_weights=None):
         import torch
 
+        if odm_weights is not None and not 1 <= len(odm_weights) <= 2:
+            raise ValueError("odm_weights: one or two Lindblad channel weights")
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         self.precision = precision
@@ -488,7 +492,8 @@ class WignerEnsemble(_PlanBase):
                 D_lambda=m.get("D_lambda", 0.0), gamma=gamma, damping=damping,
                 fp_damping=m.get("fp_damping"), gp_coeff=gp_coeff, gp_rho_factor=m.get("gp_rho_factor"),
                 mask=m.get("mask", False), truncate_constants=truncate_constants, time_rows=time_rows,
-                pot_table=m.get("pot_table")))
+                pot_table=m.get("pot_table"), L_material=m.get("L_material"),
+                odm_sign=m.get("odm_sign", 1.0)))
         self.n = len(self.tables)
         self.shapes = [(t.P, t.X) for t in self.tables]
         self.time_dependent = any(t.time_dependent for t in self.tables)
@@ -497,7 +502,8 @@ class WignerEnsemble(_PlanBase):
         d = _lib.EnsembleDesc()
         d.struct_size = ctypes.sizeof(_lib.EnsembleDesc)
         _common(d, precision=precision, scheme=scheme, t=self.tables[0], dt=dt, gamma=gamma,
-                damping=damping, kinetic=kinetic, gp_coeff=gp_coeff)
+                damping=damping, kinetic=kinetic, gp_coeff=gp_coeff,
+                odm_weights=odm_weights if damping == "ODM" else None)
         arr = (_lib.MemberDesc * self.n)()
         for i, t in enumerate(self.tables):
             arr[i].struct_size = ctypes.sizeof(_lib.MemberDesc)

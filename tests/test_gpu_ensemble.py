@@ -81,6 +81,46 @@ def test_ensemble_with_an_expression_potential_member(cuda):
         np.testing.assert_allclose(obs[i, : steps + 1, 8], obs_ref["H"], rtol=1e-10, atol=1e-12)
 
 
+@pytest.mark.parametrize("scheme", ["lie", "strang"])
+def test_uneven_ensemble_with_odm_damping(cuda, scheme):
+    """ODM damping (Lindbladian_ODM_Order1, GPU_Wigner2D_FFT.py:1516-1634; after the second half
+    step in the Strang scheme, GP.py:1958-1968) over uneven members with their own L_material,
+    one launch per ODM sweep for all of them; every member against its own oracle."""
+    import torch
+
+    from pywignercuda_amd.wigner2d import WignerEnsemble
+
+    fp_str = " p*p/sqrt( p*p + 0.5  ) "
+    members = [_member(128, 0.1, 1.0, 1.0, 1.0), _member(256, 0.05, 0.5, -2.0, 0.8),
+               _member(128, 0.15, 1.5, -1.0, 0.9)]
+    for m, L in zip(members, (3.0, 2.0, 4.0)):
+        m.update(fp_damping=fp_str, L_material=L)
+    dt, steps, gamma = 0.01, 20, 0.05
+    ens = WignerEnsemble(members, 1.0, 1.0, dt, scheme=scheme, kinetic="0.5*p*p", gamma=gamma,
+                         damping="ODM", device=cuda)
+    refs = []
+    for m in members:
+        a, b = m["a"], m["b"]
+        orc = ow.Wigner2DOracle(m["X"], m["P"], m["X_amp"], m["P_amp"], 1.0, 1.0, dt, scheme=scheme,
+                                potential=lambda t, x, a=a, b=b: a * x ** 4 - b * x ** 2,
+                                dpotential=lambda t, x, a=a, b=b: 4 * a * x ** 3 - 2 * b * x,
+                                kinetic=lambda p: 0.5 * p * p, mask=False, gamma=gamma, damping="ODM",
+                                fp_damping=lambda p: p * p / np.sqrt(p * p + 0.5), L_material=m["L_material"])
+        W0 = ow.gaussian_wigner(orc.g, x0=m["x0"], s=m["s"])
+        W_ref, obs_ref, _ = orc.run(W0, steps)
+        refs.append((W0, W_ref, obs_ref))
+    states = ens.states_to_device([r[0] for r in refs])
+    obs = ens.run(states, steps + 1, use_graph=True).cpu().numpy()
+    torch.cuda.synchronize()
+    for i, (W0, W_ref, obs_ref) in enumerate(refs):
+        W = states[i].cpu().numpy()
+        err = np.linalg.norm(W - W_ref) / np.linalg.norm(W_ref)
+        print(f"ODM {scheme} member {i}: rel L2 {err:.3e}")
+        assert err < 1e-10, (i, err)
+        np.testing.assert_allclose(obs[i, : steps + 1, 0], obs_ref["X"], rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(obs[i, : steps + 1, 8], obs_ref["H"], rtol=1e-10, atol=1e-12)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32fast"])
 def test_every_variant_matches_oracle(cuda, prec):
     import ctypes
