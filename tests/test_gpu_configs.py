@@ -8,6 +8,7 @@ against the oracle on the same seeded inputs).
 * the seeded uneven ensemble (sides 128..1024, V = a x^4 - b x^2) in the tuned kernels and in
   the per-size throughput kernels, FP64 and FP32, each member against its own oracle; and the
   full 256-member, 1000-step FP32-vs-FP64 error;
+* the largest member side, 2048^2 (Lie with Caldeira-Leggett, Strang), against the oracle;
 * the Wigner-Dirac 4x4 step at 512^2 x 16 planes (soft Coulomb V0 = -2/sqrt(x^2+1), dt = 0.002,
   positive-energy Gaussian spinor x0 = -2, px = 6) in FP64 and FP32.
 
@@ -125,6 +126,41 @@ def test_damped_1024_fp64(cuda, cl1024_reference):
     torch.cuda.synchronize()
     err = rel_l2(W.cpu().numpy(), W_ref)
     print(f"1024^2 damped fp64: rel-L2 {err:.3e}")
+    assert err < 1e-10
+    check_obs(obs, obs_ref)
+
+
+@pytest.mark.parametrize("scheme", ["lie", "strang"])
+def test_largest_side_2048(cuda, scheme):
+    """The largest grid side a member may have (2048^2, x,p in [-16,16)): the Lie step with the
+    Caldeira-Leggett composite (gamma 0.1, D_theta 0.1, sigma(p) = |p|, V = 0.1x^4 - x^2) and the
+    Strang step (V = 0.5x^2 + 0.1x^4, merged half steps), 5 iterations each, FP64, against the
+    oracle."""
+    import torch
+
+    from pywignercuda_amd.wigner2d import Wigner2D
+
+    args = (2048, 2048, 16.0, 16.0, 1.0, 1.0, 0.005)
+    if scheme == "lie":
+        hip = Wigner2D(*args, potential="0.1*pow(x,4) - pow(x,2)", dpotential="0.4*pow(x,3) - 2.0*x",
+                       D_theta=0.1, gamma=0.1, damping="CaldeiraLeggett", fp_damping="fabs(p)", mask=True,
+                       device=cuda)
+        orc = ow.Wigner2DOracle(*args, D_theta=0.1, gamma=0.1, potential=lambda t, x: 0.1 * x ** 4 - x ** 2,
+                                dpotential=lambda t, x: 0.4 * x ** 3 - 2.0 * x, fp_damping=lambda p: np.abs(p),
+                                damping="CaldeiraLeggett", scheme="lie", mask=True)
+    else:
+        hip = Wigner2D(*args, potential="0.5*pow(x,2)+0.1*pow(x,4)", dpotential="x+0.4*pow(x,3)",
+                       kinetic="0.5*p*p", scheme="strang", mask=True, device=cuda)
+        orc = ow.Wigner2DOracle(*args, potential=lambda t, x: 0.5 * x ** 2 + 0.1 * x ** 4,
+                                dpotential=lambda t, x: x + 0.4 * x ** 3, kinetic=lambda p: 0.5 * p * p,
+                                scheme="strang", mask=True)
+    W0 = ow.cat_state_wigner(orc.g, a=2.0)
+    W_ref, obs_ref, _ = orc.run(W0, 4)
+    W = hip.to_device(W0)
+    obs = hip.run(W, 5, use_graph=False).cpu().numpy()[:5]
+    torch.cuda.synchronize()
+    err = rel_l2(W.cpu().numpy(), W_ref)
+    print(f"2048^2 {scheme}: rel-L2 {err:.3e}")
     assert err < 1e-10
     check_obs(obs, obs_ref)
 
