@@ -10,7 +10,8 @@ against the oracle on the same seeded inputs).
   full 256-member, 1000-step FP32-vs-FP64 error;
 * the largest member side, 2048^2 (Lie with Caldeira-Leggett, Strang), against the oracle;
 * the Wigner-Dirac 4x4 step at 512^2 x 16 planes (soft Coulomb V0 = -2/sqrt(x^2+1), dt = 0.002,
-  positive-energy Gaussian spinor x0 = -2, px = 6) in FP64 and FP32.
+  positive-energy Gaussian spinor x0 = -2, px = 6) in FP64 and FP32, and at 1024^2 and 2048^2
+  in FP64.
 
 Bars: FP64 relative L2 <= 1e-10 in W and the observables within 1e-10 relative (1e-12
 absolute near 0); FP32 state forms relative L2 <= 1e-5.
@@ -378,3 +379,33 @@ def test_dirac_512_soft_coulomb(cuda, dirac512_reference, precision):
               "negativity", "antiparticle"]:
         ref = np.array([r[k] for r in rec])
         np.testing.assert_allclose(o[k][:steps], ref, rtol=rtol, atol=atol, err_msg=k)
+
+
+@pytest.mark.parametrize("side,steps", [(1024, 2), (2048, 1)])
+def test_dirac_large_sides(cuda, side, steps):
+    """The Wigner-Dirac step at the larger sides the library accepts (1024^2 and 2048^2 x 16
+    planes; radix-8 teams of one or two rows per block), soft Coulomb V0 as at 512^2, FP64
+    against the oracle."""
+    import torch
+
+    from oracle.dirac4x4 import Dirac4x4Oracle, gaussian_spinor_particle_up, wigner_from_spinor
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    zero = lambda t, x: 0.0 * x  # noqa: E731
+    orc = Dirac4x4Oracle(side, side, 18.0, 16.0, 1.0, 1.0, 0.002,
+                         [lambda t, x: -2.0 / np.sqrt(x * x + 1.0), zero, zero, zero],
+                         dpotential0=lambda t, x: 2.0 * x / (x * x + 1.0) ** 1.5, compute_energy=True)
+    W0 = wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 6.0, 1.0, Xg, 1.0, 1.0), orc)
+    W_ref, rec = orc.run(W0, steps)
+    hip = WignerDirac4x4(side, side, 18.0, 16.0, 1.0, 1.0, 0.002, potentials=("-2/sqrt(x*x+1)", "0.", "0.", "0."),
+                         energy=True, device=cuda)
+    W = hip.to_device(W0)
+    obs = hip.run(W, steps, first_step=1, use_graph=False).cpu().numpy()
+    torch.cuda.synchronize()
+    err = rel_l2(W.cpu().numpy().reshape(4, 4, side, side), W_ref)
+    print(f"Dirac {side}^2: rel-L2 {err:.3e}")
+    assert err < 1e-10
+    o = WignerDirac4x4.observables(obs)
+    for k in ["X", "P", "XX", "PP", "Alpha1", "energy", "antiparticle"]:
+        ref = np.array([r[k] for r in rec])
+        np.testing.assert_allclose(o[k][:steps], ref, rtol=1e-10, atol=1e-12, err_msg=k)
