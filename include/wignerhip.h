@@ -514,7 +514,7 @@ typedef struct {
   /* the KvN variant (GPU_WignerDiracKvN2D_4x4.py): kinetic factors at q = +/- lambda/2 without
      the mass term on the diagonal (:183-189, 283-289; mass_half = 1e-5 there, :1961), potential
      step e^{-dt D_Theta theta^2 + i dt V0'(x) theta} on every plane (KvN_XTheta_Kernel, :356-420;
-     V0' from the polynomial rows or dv0), the step filter always on with projectors at p
+     V0' from the polynomial rows, dv0 or the program dpot_expr), the step filter always on with projectors at p
      (:775, 3599-3640); V0 only (no A, mass function, GP or damping) */
   int kvn;
   /* antiparticle step filter (antiParticleStepFiltering, :3636-3678): after the potential half,
@@ -525,7 +525,8 @@ typedef struct {
   /* pot_deg = WH_POT_EXPR: V0(t, x) that is not a polynomial in x and depends on t, as the
      expression programs (see WH_EXPR_*) of V0 and of dV0/dx, evaluated on the device at
      t = time index * pot_t_step: the potential step's Bopp difference before each potential
-     sweep, dV0/dx and V0 of the record rows before each reduce.  Not with kvn. */
+     sweep (with kvn: dV0/dx of the KvN phase), dV0/dx and V0 of the record rows before
+     each reduce. */
   const double* pot_expr;
   int pot_expr_len;
   const double* dpot_expr;

@@ -1365,7 +1365,6 @@ int dirac_validate(const wh_dirac_desc* d) {
   } else if (d->pot_deg == WH_POT_EXPR) {
     WH_TRY(check_expr(d->pot_expr, d->pot_expr_len, 0));
     WH_TRY(check_expr(d->dpot_expr, d->dpot_expr_len, 0));
-    if (d->kvn) return fail(WH_INVALID_DESCRIPTOR, "the KvN variant takes a polynomial or time-independent V0");
   } else if (d->pot_deg < 0 || d->pot_deg > WH_MAX_DEG || d->pot_rows < 1 || !d->pot_coef) {
     return fail(WH_INVALID_DESCRIPTOR, "potential deg %d rows %d coef %p", d->pot_deg, d->pot_rows,
                 (const void*)d->pot_coef);
@@ -1423,9 +1422,15 @@ int dirac_reduce(wh_dirac* e, int state_add, int renorm, int anti, int inc, hipS
 int dirac_potential_sweep(wh_dirac* e, hipStream_t s) {
   const auto* dp = e->plan_dev.as<wh::DiracPlan>();
   const int prec = e->d.precision;
-  if (e->host.pot_expr_len > 0)  // V0(t, x) program: the Bopp difference at this sweep's time index
+  if (e->host.pot_expr_len > 0 && e->d.kvn) {
+    // KvN: the potential factor reads dV0/dx(t, x) per column; refresh it (and V0) at this
+    // sweep's time index
+    const int depth = std::max(e->host.pot_expr_depth, e->host.dpot_expr_depth);
+    WH_LAUNCH(wh::launch_dirac_expr_obs(dp, e->d.x_dim, depth, 0, s), "dirac_expr_obs");
+  } else if (e->host.pot_expr_len > 0) {  // V0(t, x) program: the Bopp difference at this sweep's time index
     WH_LAUNCH(wh::launch_dirac_expr_table(dp, e->d.x_dim * e->d.p_dim, e->host.pot_expr_depth, s),
               "dirac_expr_table");
+  }
   if (e->host.a_tab)  // vector potential programs at this sweep's time index
     WH_LAUNCH(wh::launch_dirac_a_table(dp, e->d.x_dim * e->d.p_dim, e->host.a_expr_depth, s), "dirac_a_table");
   if (e->host.mass_expr_len > 0)  // Mass(t, x) program: the mass tables at this sweep's time index

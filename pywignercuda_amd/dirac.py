@@ -193,8 +193,6 @@ class WignerDirac4x4:
             if self.time_dependent:
                 # V0(t, x) and dV0/dx(t, x) as programs the device evaluates per step (the
                 # reference splices both strings into its kernels, :1964-1966)
-                if kvn:
-                    raise NotImplementedError("the KvN variant with a time-dependent non-polynomial V0")
                 self._pot_expr = compile_expression(V0, "x")
                 self._dpot_expr = compile_expression(_sympy_derivative(V0), "x")
                 d.pot_deg, d.pot_rows = WH_POT_EXPR, 1

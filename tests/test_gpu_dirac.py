@@ -507,6 +507,29 @@ def test_kvn_matches_oracle(cuda, case):
     check_obs(obs, rec)
 
 
+def test_kvn_time_dependent_expression_v0(cuda):
+    """GPU_WignerDiracKvN2D_4x4 with a time-dependent non-polynomial V0 (a modulated Klein step):
+    the device evaluates the program of dV0/dx at each potential sweep's time index for the
+    KvN phase e^{i dt V0'(t, x) theta} (KvN_XTheta_Kernel, D44K.py:356-420), against the oracle."""
+    from pywignercuda_amd.dirac import WignerDirac4x4
+
+    vs = "1.5*0.5*( 1. + tanh((x-5.)/0.25) )*(1. + 0.3*sin(2*t))"
+    vf = lambda t, x: 0.75 * (1.0 + np.tanh((x - 5.0) / 0.25)) * (1.0 + 0.3 * np.sin(2 * t))  # noqa: E731
+    dvf = lambda t, x: 0.75 / (0.25 * np.cosh((x - 5.0) / 0.25) ** 2) * (1.0 + 0.3 * np.sin(2 * t))  # noqa: E731
+    hip = WignerDirac4x4(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, potentials=(vs, "0.", "0.", "0."), D_theta=0.01,
+                         energy=True, kvn=True, device=cuda)
+    orc = Dirac4x4Oracle(128, 128, 18.0, 16.0, 1.0, 1.0, 0.01, [vf, zero, zero, zero], dpotential0=dvf,
+                         D_theta=0.01, compute_energy=True, kvn=True)
+    W0 = orc.filter_particles(
+        wigner_from_spinor(lambda Xg: gaussian_spinor_particle_up(-2.0, 3.0, 1.0, Xg, 1.0, 1.0), orc))
+    W0 = W0 / orc.norm(W0)
+    W, obs, W_ref, rec = run_both(hip, orc, W0, 20)
+    err = rel_l2(W, W_ref)
+    print(f"KvN time-dependent V0 program: rel L2 {err:.3e}")
+    assert err < 1e-10
+    check_obs(obs, rec)
+
+
 def test_kvn_x1_notebook(cuda, tmp_path):
     """instances/WignerDiracKvN2D_4x4/X1.ipynb: 512^2, amplitudes 14, m = c = 1, dt = 0.01, 800 steps,
     V0 = 1.5 x, spinor at x0 = -1, px = 5, s = 1 filtered to positive energy (FilterElectrons) and
