@@ -414,6 +414,22 @@ def test_lie_odm_fp32(cuda):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp32fast"])
+def test_strang_odm_complex64(cuda, precision):
+    """The Strang scheme with ODM damping and the GP term (GP.py:1942-1968) in both complex64
+    state forms, 100 steps against the FP64 oracle (bars 1e-5 for 'fp32', as the Lie ODM FP32
+    test, and 1e-4 for 'fp32fast', whose transforms run in FP32)."""
+    fp_fn = lambda p: p * p / np.sqrt(p * p + 0.5)  # noqa: E731
+    hip, orc = make_pair(cuda, scheme="strang", amp=12.0, dt=0.005, gamma=0.07 / 4.0, damping="ODM",
+                         fp_str=" p*p/sqrt( p*p + 0.5  ) ", fp_fn=fp_fn, L_material=3.0, precision=precision,
+                         gp=0.5, pot=("0.5*pow(x,2)", "x", lambda t, x: 0.5 * x * x, lambda t, x: x))
+    W0 = ow.harmonic_oscillator_wigner(orc.g, 0, 1.0, 0.0, 3.0)
+    W, obs, W_ref, obs_ref, _ = run_both(hip, orc, W0, 100)
+    err = rel_l2(W.astype(np.complex128), W_ref)
+    print(f"Strang ODM {precision}: rel L2 {err:.3e}")
+    assert err < (1e-5 if precision == "fp32" else 1e-4)
+
+
 def test_single_sweeps_match_oracle_factors(cuda):
     """wh_wigner2d_kinetic / wh_wigner2d_potential (caller-composed loops): one K factor and
     one V factor at time index 7 of a driven potential V = 0.5 x^2 + 0.3 sin(t) x^3, against
