@@ -258,6 +258,20 @@ def _uneven_one(prec):
     return run
 
 
+def damped_expression(steps, warmup, dev):
+    """The damped 1024^2 step with a time-dependent non-polynomial potential
+    V = 0.1x^4 - x^2 + 0.05 sin(t) e^{-x^2}: an expression program tabulated on the device
+    before each p sweep (one expr_table_kernel launch per step)."""
+    c = cfg2("fp64")
+    c["potential"] = "0.1*pow(x,4) - pow(x,2) + 0.05*sin(t)*exp(-x*x)"
+    c["dpotential"] = "0.4*pow(x,3) - 2.0*x - 0.1*x*sin(t)*exp(-x*x)"
+    prop = make(c, dev)
+    W = prop.to_device(initial_state(prop, "cat"))
+    wall, _ = timed_run(prop, W, steps, warmup)
+    cells = prop.X * prop.P
+    return {"cell_steps_per_s": cells * steps / wall, "steps": steps, "us_per_step": wall / steps * 1e6}
+
+
 # extra workloads runnable alone (--workload NAME): (runner(steps, warmup, dev), description)
 SIDE_WORKLOADS = {
     "batch-fp64": (lambda st, w, d: hbm_batch("fp64", 64, st, w, d), "64 x 1024^2 plain quartic Lie step, FP64"),
@@ -268,6 +282,8 @@ SIDE_WORKLOADS = {
     "uneven-fp32": (_uneven_one("fp32"), "256-member uneven ensemble (sides 128..1024, seeded), 'fp32'"),
     "uneven-fp32fast": (_uneven_one("fp32fast"), "256-member uneven ensemble, 'fp32fast'"),
     "uneven-fp64": (_uneven_one("fp64"), "256-member uneven ensemble, FP64"),
+    "damped-expr": (damped_expression, "damped 1024^2 Lie step, FP64, V(t, x) = 0.1x^4 - x^2 + 0.05 sin(t) e^{-x^2} "
+                    "as a device-evaluated expression program"),
     "dirac-fp64": (lambda st, w, d: dirac_workload("fp64", st, d), "Wigner-Dirac 4x4, 512^2 x 16 planes, FP64"),
     "dirac-fp32": (lambda st, w, d: dirac_workload("fp32", st, d), "Wigner-Dirac 4x4, 512^2 x 16 planes, FP32"),
     "adjoint": (lambda st, w, d: adjoint_workload(st, w, d),
