@@ -430,6 +430,25 @@ def test_strang_odm_complex64(cuda, precision):
     assert err < (1e-5 if precision == "fp32" else 1e-4)
 
 
+def test_strang_odm_graph_replay_matches_direct_launches(cuda):
+    """The Strang + ODM iteration (with GP) captured into the step graph and replayed (a run of
+    80 steps: four 16-iteration chunks) gives the state and record of the same run launched
+    directly, bit for bit."""
+    import torch
+
+    fp_fn = lambda p: p * p / np.sqrt(p * p + 0.5)  # noqa: E731
+    hip, orc = make_pair(cuda, scheme="strang", amp=12.0, dt=0.005, gamma=0.07 / 4.0, damping="ODM",
+                         fp_str=" p*p/sqrt( p*p + 0.5  ) ", fp_fn=fp_fn, L_material=3.0, gp=0.5,
+                         pot=("0.5*pow(x,2)", "x", lambda t, x: 0.5 * x * x, lambda t, x: x))
+    W0 = ow.harmonic_oscillator_wigner(orc.g, 0, 1.0, 0.0, 3.0)
+    Wa, Wb = hip.to_device(W0), hip.to_device(W0)
+    oa = hip.run(Wa, 80, use_graph=True).cpu().numpy()
+    ob = hip.run(Wb, 80, use_graph=False).cpu().numpy()
+    torch.cuda.synchronize()
+    assert np.array_equal(Wa.cpu().numpy(), Wb.cpu().numpy())
+    assert np.array_equal(oa, ob)
+
+
 def test_single_sweeps_match_oracle_factors(cuda):
     """wh_wigner2d_kinetic / wh_wigner2d_potential (caller-composed loops): one K factor and
     one V factor at time index 7 of a driven potential V = 0.5 x^2 + 0.3 sin(t) x^3, against
