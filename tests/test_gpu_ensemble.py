@@ -95,7 +95,7 @@ def test_uneven_ensemble_with_odm_damping(cuda, scheme):
                _member(128, 0.15, 1.5, -1.0, 0.9)]
     for m, L in zip(members, (3.0, 2.0, 4.0)):
         m.update(fp_damping=fp_str, L_material=L)
-    dt, steps, gamma = 0.01, 20, 0.05
+    dt, steps, gamma = 0.01, 40, 0.05  # 41 iterations: the step graph's 16-iteration chunks replay
     ens = WignerEnsemble(members, 1.0, 1.0, dt, scheme=scheme, kinetic="0.5*p*p", gamma=gamma,
                          damping="ODM", device=cuda)
     refs = []
